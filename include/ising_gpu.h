@@ -1,0 +1,129 @@
+/*
+ * ising_gpu.h -- C ABI of the MI355X multispin Metropolis / parallel-tempering
+ * engine (gfx950, one GPU).
+ *
+ * Conventions
+ *   - Every call returns int32 status: IGE_OK (0) or a negative IGE_ERR_*.
+ *     ige_last_error() returns the message of the calling thread's last
+ *     failure.  No C++ exception crosses this boundary.
+ *   - The library owns device buffers; the caller owns host buffers.
+ *   - A state is externally synchronised: one host thread at a time.  Distinct
+ *     states run on distinct HIP streams and may be driven concurrently.
+ *   - A "cell" (slot) holds 64 replicas of the model, one bit each, packed as
+ *     one uint64 per site ("MSC word": bit r = replica r).
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   ige_model_from_pairs    ProblemIR::from_pairs      src/engine_v2/ir.rs:30-69
+ *   ige_model_coloring      ProblemIR::greedy_coloring src/engine_v2/ir.rs:114-139
+ *   ige_model_get_info      ProblemIR::is_integral + DecisionEngine backend choice
+ *                                                      src/engine_v2/ir.rs:143-148,
+ *                                                      src/engine_v2/decision.rs:115-124
+ *   ige_state_create        SparseBitSlice::new        src/engine_v2/backends/sparse_bitslice.rs:42-81
+ *                           QuantumField::new + random init
+ *                                                      src/solver/types.rs:23-58,
+ *                                                      src/solver/ultimate.rs:271-275
+ *   ige_run                 MetropolisSweep::apply     src/engine_v2/operators/metropolis_sweep.rs:95-146
+ *                           engine::step (sweep + DEO) src/solver/engine.rs:573-819
+ *   ige_read_energies       SpinState::energies_into   src/engine_v2/backends/sparse_bitslice.rs:273-278
+ *   ige_read_fields         SparseBitSlice fields ledger
+ *                                                      src/engine_v2/backends/sparse_bitslice.rs:26-27
+ *   ige_recompute_energies  SpinState::audit / energy_of
+ *                                                      src/engine_v2/backends/sparse_bitslice.rs:114-137,317-346
+ *   ige_read_spins          SpinState::spin / extract_into
+ *                                                      src/engine_v2/backends/sparse_bitslice.rs:280-289
+ *   ige_read_stats          Report{accepted,proposed} + StepScratch swap statistics
+ *                                                      src/engine_v2/operator.rs:34-52,
+ *                                                      src/solver/engine.rs:454-557
+ */
+#ifndef ISING_GPU_H
+#define ISING_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IGE_OK 0
+#define IGE_ERR_INVALID (-1)     /* bad argument or inconsistent input      */
+#define IGE_ERR_HIP (-2)         /* a HIP runtime call failed               */
+#define IGE_ERR_NOMEM (-3)       /* device or host allocation failed        */
+#define IGE_ERR_UNSUPPORTED (-4) /* model/kernel combination not supported  */
+#define IGE_ERR_RANGE (-5)       /* value outside the exact integer range   */
+
+#define IGE_KERNEL_AUTO 0      /* multispin when eligible, else generic     */
+#define IGE_KERNEL_GENERIC 1   /* lane = replica, any integer CSR model     */
+#define IGE_KERNEL_MULTISPIN 2 /* lane = site, bit-sliced +-J arithmetic    */
+
+#define IGE_RUN_EXCHANGE 1u /* DEO replica exchange after every sweep       */
+
+typedef struct ige_ctx ige_ctx;
+typedef struct ige_model ige_model;
+typedef struct ige_state ige_state;
+
+typedef struct {
+    uint32_t n;
+    uint32_t num_colors;
+    uint64_t nnz;
+    uint32_t max_degree;
+    int32_t multispin_eligible; /* 1 when every |q_ij| == w0 and the Ising field is 0 */
+    int32_t coupling_magnitude; /* w0 when eligible, else 0                        */
+    int32_t reserved;
+    int64_t max_local_field; /* max_i |lin_i| + sum_j |q_ij|: bound on |dE|      */
+} ige_model_info;
+
+typedef struct {
+    uint64_t sweeps;   /* sweeps run on this state                      */
+    uint64_t accepted; /* accepted single-spin proposals                */
+    uint64_t proposed; /* proposals (= sweeps * n * 64 * cells)         */
+    uint32_t phase;    /* DEO phase of the next exchange (0 even, 1 odd) */
+    uint32_t kernel;   /* IGE_KERNEL_GENERIC or IGE_KERNEL_MULTISPIN    */
+} ige_stats;
+
+/* ---- context ---- */
+int32_t ige_context_create(int32_t device, ige_ctx **out);
+int32_t ige_context_destroy(ige_ctx *ctx);
+const char *ige_last_error(void);
+const char *ige_version(void);
+
+/* ---- model (integer QUBO: E = offset + sum lin_i x_i + sum_{i<j} q_ij x_i x_j) ---- */
+int32_t ige_model_from_pairs(ige_ctx *ctx, uint32_t n, int64_t offset, const int32_t *linear,
+                             uint64_t npairs, const uint32_t *pi, const uint32_t *pj,
+                             const int32_t *q, ige_model **out);
+int32_t ige_model_destroy(ige_model *model);
+int32_t ige_model_get_info(const ige_model *model, ige_model_info *info);
+int32_t ige_model_coloring(const ige_model *model, uint32_t *colors /* n */);
+int32_t ige_model_csr(const ige_model *model, uint32_t *row_ptr /* n+1 */,
+                      uint32_t *col_idx /* nnz */, int32_t *w /* nnz */);
+
+/* ---- replica state ---- */
+/* init_words: [cells][n] in site order, or NULL for the seeded default. */
+int32_t ige_state_create(ige_ctx *ctx, const ige_model *model, uint32_t cells, uint64_t seed,
+                         const uint64_t *init_words, uint32_t kernel, ige_state **out);
+int32_t ige_state_destroy(ige_state *state);
+/* beta[t] for temperature index t (cells entries, finite, >= 0). */
+int32_t ige_set_ladder(ige_state *state, const double *beta);
+/* beta_schedule: NULL (use the ladder) or [sweeps][cells]; the ladder is left
+ * at the last row.  Asynchronous on the state's stream. */
+int32_t ige_run(ige_state *state, uint32_t sweeps, uint32_t flags, const double *beta_schedule);
+int32_t ige_synchronize(ige_state *state);
+
+int32_t ige_read_spins(ige_state *state, uint64_t *words /* [cells][n] */);
+int32_t ige_read_energies(ige_state *state, int64_t *energies /* [cells][64] */);
+int32_t ige_read_temp_of(ige_state *state, uint32_t *temp_of /* [cells][64] */);
+int32_t ige_read_fields(ige_state *state, int32_t *fields /* [cells][n][64] */);
+int32_t ige_recompute_energies(ige_state *state, int64_t *energies /* [cells][64] */);
+int32_t ige_read_stats(ige_state *state, ige_stats *stats, uint64_t *pair_attempts,
+                       uint64_t *pair_accepts /* cells-1 each, or NULL */);
+
+/* Per-launch timing of the sweep kernels with HIP events on the state's
+ * stream (off by default).  ms_total is the summed duration of `launches`
+ * sweep-kernel launches since profiling was enabled. */
+int32_t ige_profile_enable(ige_state *state, int32_t on);
+int32_t ige_profile_read(ige_state *state, double *ms_total, uint64_t *launches,
+                         double *ms_other, uint64_t *other_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,280 @@
+"""ctypes binding of the C ABI in include/ising_gpu.h.
+
+The shared library is built in-tree (``ising_engine_amd/libising_gpu.so``).
+There is no CPU fallback: if the library is missing or cannot be loaded this
+module raises, so a run can never silently measure or test something else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libising_gpu.so")
+
+IGE_OK = 0
+IGE_ERR_INVALID = -1
+IGE_ERR_HIP = -2
+IGE_ERR_NOMEM = -3
+IGE_ERR_UNSUPPORTED = -4
+IGE_ERR_RANGE = -5
+
+KERNEL_AUTO = 0
+KERNEL_GENERIC = 1
+KERNEL_MULTISPIN = 2
+RUN_EXCHANGE = 1
+
+EXPORTED = (
+    "ige_context_create", "ige_context_destroy", "ige_last_error", "ige_version",
+    "ige_model_from_pairs", "ige_model_destroy", "ige_model_get_info",
+    "ige_model_coloring", "ige_model_csr", "ige_state_create", "ige_state_destroy",
+    "ige_set_ladder", "ige_run", "ige_synchronize", "ige_read_spins",
+    "ige_read_energies", "ige_read_temp_of", "ige_read_fields",
+    "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
+    "ige_profile_read",
+)
+
+
+class ModelInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", c_uint32), ("num_colors", c_uint32), ("nnz", c_uint64),
+        ("max_degree", c_uint32), ("multispin_eligible", c_int32),
+        ("coupling_magnitude", c_int32), ("reserved", c_int32),
+        ("max_local_field", c_int64),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("sweeps", c_uint64), ("accepted", c_uint64), ("proposed", c_uint64),
+        ("phase", c_uint32), ("kernel", c_uint32),
+    ]
+
+
+class IgeError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"ige error {code}: {message}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(POINTER(ctype))
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load and type the library once; raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = c_void_p
+    sig = {
+        "ige_context_create": (c_int32, [c_int32, POINTER(vp)]),
+        "ige_context_destroy": (c_int32, [vp]),
+        "ige_last_error": (c_char_p, []),
+        "ige_version": (c_char_p, []),
+        "ige_model_from_pairs": (c_int32, [vp, c_uint32, c_int64, POINTER(c_int32), c_uint64,
+                                           POINTER(c_uint32), POINTER(c_uint32),
+                                           POINTER(c_int32), POINTER(vp)]),
+        "ige_model_destroy": (c_int32, [vp]),
+        "ige_model_get_info": (c_int32, [vp, POINTER(ModelInfo)]),
+        "ige_model_coloring": (c_int32, [vp, POINTER(c_uint32)]),
+        "ige_model_csr": (c_int32, [vp, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]),
+        "ige_state_create": (c_int32, [vp, vp, c_uint32, c_uint64, POINTER(c_uint64), c_uint32,
+                                       POINTER(vp)]),
+        "ige_state_destroy": (c_int32, [vp]),
+        "ige_set_ladder": (c_int32, [vp, POINTER(c_double)]),
+        "ige_run": (c_int32, [vp, c_uint32, c_uint32, POINTER(c_double)]),
+        "ige_synchronize": (c_int32, [vp]),
+        "ige_read_spins": (c_int32, [vp, POINTER(c_uint64)]),
+        "ige_read_energies": (c_int32, [vp, POINTER(c_int64)]),
+        "ige_read_temp_of": (c_int32, [vp, POINTER(c_uint32)]),
+        "ige_read_fields": (c_int32, [vp, POINTER(c_int32)]),
+        "ige_recompute_energies": (c_int32, [vp, POINTER(c_int64)]),
+        "ige_read_stats": (c_int32, [vp, POINTER(Stats), POINTER(c_uint64), POINTER(c_uint64)]),
+        "ige_profile_enable": (c_int32, [vp, c_int32]),
+        "ige_profile_read": (c_int32, [vp, POINTER(c_double), POINTER(c_uint64),
+                                       POINTER(c_double), POINTER(c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != IGE_OK:
+        msg = load().ige_last_error().decode(errors="replace")
+        raise IgeError(code, msg)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        lib = load()
+        self._lib = lib
+        self.handle = c_void_p()
+        check(lib.ige_context_create(device, ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            self._lib.ige_context_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model:
+    """Integer QUBO model on the device (ProblemIR::from_pairs semantics)."""
+
+    def __init__(self, ctx: Context, n: int, offset: int, linear, pi, pj, q):
+        self._lib = ctx._lib
+        self.ctx = ctx
+        linear = np.ascontiguousarray(linear, dtype=np.int32)
+        pi = np.ascontiguousarray(pi, dtype=np.uint32)
+        pj = np.ascontiguousarray(pj, dtype=np.uint32)
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        if linear.shape != (n,):
+            raise ValueError("linear must have length n")
+        if not (pi.shape == pj.shape == q.shape):
+            raise ValueError("pair arrays must have equal length")
+        self.handle = c_void_p()
+        check(self._lib.ige_model_from_pairs(
+            ctx.handle, n, int(offset), _ptr(linear, c_int32), len(q), _ptr(pi, c_uint32),
+            _ptr(pj, c_uint32), _ptr(q, c_int32), ctypes.byref(self.handle)))
+        self.info = ModelInfo()
+        check(self._lib.ige_model_get_info(self.handle, ctypes.byref(self.info)))
+        self.n = n
+        self.offset = int(offset)
+
+    def coloring(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.uint32)
+        check(self._lib.ige_model_coloring(self.handle, _ptr(out, c_uint32)))
+        return out
+
+    def csr(self):
+        rp = np.empty(self.n + 1, dtype=np.uint32)
+        ci = np.empty(self.info.nnz, dtype=np.uint32)
+        w = np.empty(self.info.nnz, dtype=np.int32)
+        check(self._lib.ige_model_csr(self.handle, _ptr(rp, c_uint32), _ptr(ci, c_uint32),
+                                      _ptr(w, c_int32)))
+        return rp, ci, w
+
+    def close(self):
+        if self.handle:
+            self._lib.ige_model_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class State:
+    """`cells` slots x 64 replicas of a model, resident on the device."""
+
+    def __init__(self, model: Model, cells: int, seed: int, init_words=None,
+                 kernel: int = KERNEL_AUTO):
+        self._lib = model._lib
+        self.model = model
+        self.cells = int(cells)
+        self.n = model.n
+        self.handle = c_void_p()
+        iw = None
+        if init_words is not None:
+            iw = np.ascontiguousarray(init_words, dtype=np.uint64).reshape(self.cells * self.n)
+        check(self._lib.ige_state_create(model.ctx.handle, model.handle, self.cells,
+                                         int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                         _ptr(iw, c_uint64), kernel, ctypes.byref(self.handle)))
+
+    def set_ladder(self, beta) -> None:
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        if b.shape != (self.cells,):
+            raise ValueError("ladder needs one beta per cell")
+        check(self._lib.ige_set_ladder(self.handle, _ptr(b, c_double)))
+
+    def run(self, sweeps: int, exchange: bool = False, beta_schedule=None) -> None:
+        sched = None
+        if beta_schedule is not None:
+            sched = np.ascontiguousarray(beta_schedule, dtype=np.float64)
+            if sched.shape != (sweeps, self.cells):
+                raise ValueError("beta_schedule must be [sweeps][cells]")
+        self._sched_keepalive = sched
+        check(self._lib.ige_run(self.handle, int(sweeps), RUN_EXCHANGE if exchange else 0,
+                                _ptr(sched, c_double)))
+
+    def synchronize(self) -> None:
+        check(self._lib.ige_synchronize(self.handle))
+
+    def spins(self) -> np.ndarray:
+        out = np.empty((self.cells, self.n), dtype=np.uint64)
+        check(self._lib.ige_read_spins(self.handle, _ptr(out, c_uint64)))
+        return out
+
+    def energies(self) -> np.ndarray:
+        out = np.empty((self.cells, 64), dtype=np.int64)
+        check(self._lib.ige_read_energies(self.handle, _ptr(out, c_int64)))
+        return out
+
+    def temp_of(self) -> np.ndarray:
+        out = np.empty((self.cells, 64), dtype=np.uint32)
+        check(self._lib.ige_read_temp_of(self.handle, _ptr(out, c_uint32)))
+        return out
+
+    def fields(self) -> np.ndarray:
+        out = np.empty((self.cells, self.n, 64), dtype=np.int32)
+        check(self._lib.ige_read_fields(self.handle, _ptr(out, c_int32)))
+        return out
+
+    def recompute_energies(self) -> np.ndarray:
+        out = np.empty((self.cells, 64), dtype=np.int64)
+        check(self._lib.ige_recompute_energies(self.handle, _ptr(out, c_int64)))
+        return out
+
+    def stats(self):
+        st = Stats()
+        pa = np.zeros(max(self.cells - 1, 1), dtype=np.uint64)
+        pc = np.zeros(max(self.cells - 1, 1), dtype=np.uint64)
+        check(self._lib.ige_read_stats(self.handle, ctypes.byref(st), _ptr(pa, c_uint64),
+                                       _ptr(pc, c_uint64)))
+        return st, pa[: self.cells - 1], pc[: self.cells - 1]
+
+    def profile(self, on: bool) -> None:
+        check(self._lib.ige_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_read(self):
+        ms = c_double()
+        n = c_uint64()
+        ms2 = c_double()
+        n2 = c_uint64()
+        check(self._lib.ige_profile_read(self.handle, ctypes.byref(ms), ctypes.byref(n),
+                                         ctypes.byref(ms2), ctypes.byref(n2)))
+        return ms.value, n.value, ms2.value, n2.value
+
+    def close(self):
+        if self.handle:
+            self._lib.ige_state_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,567 @@
+// api.cpp -- the C ABI declared in include/ising_gpu.h.
+//
+// Host side of the engine: owns device buffers, uploads the model in
+// colour-ordered position space, and sequences the per-sweep launches
+// (threshold refresh, one sweep launch per colour class, DEO exchange) on
+// the state's own stream.  No C++ exception crosses the ABI.
+#include "../../include/ising_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ige_math.h"
+#include "kernels.hpp"
+#include "problem_ir.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure : std::runtime_error {
+    int32_t code;
+    Failure(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw Failure(e == hipErrorOutOfMemory ? IGE_ERR_NOMEM : IGE_ERR_HIP,
+                      std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int32_t guarded(F&& f) {
+    try {
+        f();
+        return IGE_OK;
+    } catch (const Failure& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const ige::RangeError& e) {
+        g_last_error = e.what();
+        return IGE_ERR_RANGE;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return IGE_ERR_INVALID;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return IGE_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return IGE_ERR_HIP;
+    }
+}
+
+void require(bool ok, const std::string& msg) {
+    if (!ok) throw Failure(IGE_ERR_INVALID, msg);
+}
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        count = n;
+        if (n) check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    }
+    void upload(const T* src, size_t n, hipStream_t st) {
+        if (n) check(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+    }
+    void download(T* dst, size_t n, hipStream_t st) const {
+        if (n) check(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyDeviceToHost, st), "download");
+    }
+};
+
+}  // namespace
+
+struct ige_ctx {
+    int device = 0;
+};
+
+struct ige_model {
+    ige_ctx* ctx = nullptr;
+    ige::ProblemIR ir;
+    std::vector<uint32_t> colors;
+    ige::ColorOrder co;
+    ige::MultispinShape ms;
+    int64_t max_local = 0;
+    uint32_t dmax = 0;
+    DevBuf<uint32_t> rp, ci, order, sgn;
+    DevBuf<int32_t> wt, lin;
+    ige::DevModel view() const {
+        ige::DevModel m;
+        m.n = ir.n;
+        m.rp = rp.p;
+        m.ci = ci.p;
+        m.wt = wt.p;
+        m.lin = lin.p;
+        m.order = order.p;
+        m.sgn = sgn.p;
+        m.w0 = ms.w0;
+        m.dmax = dmax;
+        m.offset = ir.offset;
+        return m;
+    }
+};
+
+struct ige_state {
+    ige_ctx* ctx = nullptr;
+    const ige_model* model = nullptr;
+    uint32_t cells = 0;
+    uint64_t seed = 0;
+    uint32_t kernel = IGE_KERNEL_GENERIC;
+    hipStream_t stream = nullptr;
+    uint32_t sweep = 0;  // sweeps run so far: the counter the uniforms are keyed on
+    uint32_t phase = 0;
+    uint32_t dtab = 0;
+    bool ladder_set = false;
+    std::vector<double> ladder;
+    DevBuf<uint64_t> spins;
+    DevBuf<int64_t> energy, scratch_e;
+    DevBuf<uint32_t> temp_of, slot_of, thr;
+    DevBuf<unsigned long long> counters, qplanes;  // counters: accepted, then pair stats
+    DevBuf<double> beta;                             // current ladder (+ schedule rows)
+    DevBuf<int32_t> fields;
+    bool profile = false;
+    double prof_ms = 0.0, prof_other_ms = 0.0;
+    uint64_t prof_launches = 0, prof_other_launches = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ige::DevState view() {
+        ige::DevState s;
+        s.cells = cells;
+        s.spins = spins.p;
+        s.energy = energy.p;
+        s.temp_of = temp_of.p;
+        s.slot_of = slot_of.p;
+        s.accepted = counters.p;
+        s.pair_stats = counters.p + 1;
+        return s;
+    }
+    ~ige_state() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace {
+
+void set_device(const ige_ctx* ctx) { check(hipSetDevice(ctx->device), "hipSetDevice"); }
+
+void upload_model(ige_model* m) {
+    const ige::ProblemIR& ir = m->ir;
+    const uint32_t n = ir.n;
+    const auto& co = m->co;
+    std::vector<uint32_t> rp(static_cast<size_t>(n) + 1, 0), ci(ir.nnz()), sgn(n, 0);
+    std::vector<int32_t> wt(ir.nnz()), lin(n);
+    uint64_t e_out = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        const uint32_t site = co.order[p];
+        lin[p] = ir.linear[site];
+        uint32_t k = 0;
+        for (uint32_t e = ir.row_ptr[site]; e < ir.row_ptr[site + 1]; ++e, ++k) {
+            ci[e_out] = co.pos_of[ir.col_idx[e]];
+            wt[e_out] = ir.weights[e];
+            if (ir.weights[e] < 0 && k < 32) sgn[p] |= 1u << k;
+            ++e_out;
+        }
+        rp[p + 1] = static_cast<uint32_t>(e_out);
+    }
+    m->rp.alloc(rp.size());
+    m->ci.alloc(ci.size());
+    m->wt.alloc(wt.size());
+    m->lin.alloc(lin.size());
+    m->order.alloc(n);
+    m->sgn.alloc(n);
+    m->rp.upload(rp.data(), rp.size(), nullptr);
+    m->ci.upload(ci.data(), ci.size(), nullptr);
+    m->wt.upload(wt.data(), wt.size(), nullptr);
+    m->lin.upload(lin.data(), lin.size(), nullptr);
+    m->order.upload(co.order.data(), n, nullptr);
+    m->sgn.upload(sgn.data(), n, nullptr);
+    check(hipStreamSynchronize(nullptr), "model upload");
+}
+
+void finish_model(ige_model* m) {
+    if (m->ir.n == 0) throw std::invalid_argument("model needs n >= 1");
+    m->max_local = m->ir.max_local_field();
+    if (m->max_local > 0x7FFFFFFFll)
+        throw ige::RangeError("max local field exceeds int32; fields would not be exact");
+    m->dmax = m->ir.max_degree();
+    m->colors = m->ir.greedy_coloring();
+    m->co = ige::color_order(m->colors);
+    m->ms = ige::multispin_shape(m->ir);
+    upload_model(m);
+}
+
+void refresh_thresholds(ige_state* s, const double* dev_beta) {
+    const auto m = s->model->view();
+    auto st = s->view();
+    if (s->kernel == IGE_KERNEL_GENERIC)
+        check(ige::launch_thr_table(dev_beta, s->thr.p, s->cells, s->dtab, s->stream), "thr");
+    else
+        check(ige::launch_qplanes(m, st, dev_beta, s->qplanes.p, s->stream), "qplanes");
+}
+
+void recompute_energy_into_state(ige_state* s) {
+    const auto m = s->model->view();
+    auto st = s->view();
+    check(ige::launch_energy(m, st, s->energy.p, s->stream), "energy");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ige_last_error(void) { return g_last_error.c_str(); }
+
+const char* ige_version(void) { return "ising_engine_amd 0.1 gfx950"; }
+
+int32_t ige_context_create(int32_t device, ige_ctx** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        int count = 0;
+        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        require(device >= 0 && device < count, "device index out of range");
+        auto* c = new ige_ctx;
+        c->device = device;
+        *out = c;
+    });
+}
+
+int32_t ige_context_destroy(ige_ctx* ctx) {
+    delete ctx;
+    return IGE_OK;
+}
+
+int32_t ige_model_from_pairs(ige_ctx* ctx, uint32_t n, int64_t offset, const int32_t* linear,
+                             uint64_t npairs, const uint32_t* pi, const uint32_t* pj,
+                             const int32_t* q, ige_model** out) {
+    return guarded([&] {
+        require(ctx && out && linear, "NULL argument");
+        require(npairs == 0 || (pi && pj && q), "NULL pair arrays");
+        set_device(ctx);
+        auto m = std::make_unique<ige_model>();
+        m->ctx = ctx;
+        m->ir = ige::ProblemIR::from_pairs(n, offset, linear, npairs, pi, pj, q);
+        finish_model(m.get());
+        *out = m.release();
+    });
+}
+
+int32_t ige_model_destroy(ige_model* model) {
+    delete model;
+    return IGE_OK;
+}
+
+int32_t ige_model_get_info(const ige_model* m, ige_model_info* info) {
+    return guarded([&] {
+        require(m && info, "NULL argument");
+        std::memset(info, 0, sizeof(*info));
+        info->n = m->ir.n;
+        info->num_colors = m->co.num_colors;
+        info->nnz = m->ir.nnz();
+        info->max_degree = m->dmax;
+        info->multispin_eligible = m->ms.eligible ? 1 : 0;
+        info->coupling_magnitude = m->ms.w0;
+        info->max_local_field = m->max_local;
+    });
+}
+
+int32_t ige_model_coloring(const ige_model* m, uint32_t* colors) {
+    return guarded([&] {
+        require(m && colors, "NULL argument");
+        std::memcpy(colors, m->colors.data(), m->colors.size() * sizeof(uint32_t));
+    });
+}
+
+int32_t ige_model_csr(const ige_model* m, uint32_t* row_ptr, uint32_t* col_idx, int32_t* w) {
+    return guarded([&] {
+        require(m && row_ptr && (m->ir.nnz() == 0 || (col_idx && w)), "NULL argument");
+        std::memcpy(row_ptr, m->ir.row_ptr.data(), m->ir.row_ptr.size() * sizeof(uint32_t));
+        std::memcpy(col_idx, m->ir.col_idx.data(), m->ir.nnz() * sizeof(uint32_t));
+        std::memcpy(w, m->ir.weights.data(), m->ir.nnz() * sizeof(int32_t));
+    });
+}
+
+int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, uint64_t seed,
+                         const uint64_t* init_words, uint32_t kernel, ige_state** out) {
+    return guarded([&] {
+        require(ctx && model && out, "NULL argument");
+        require(cells >= 1 && cells <= 65536, "cells must be in [1, 65536]");
+        require(kernel <= IGE_KERNEL_MULTISPIN, "unknown kernel id");
+        if (kernel == IGE_KERNEL_MULTISPIN && !model->ms.eligible)
+            throw Failure(IGE_ERR_UNSUPPORTED,
+                          "multispin kernel needs |q_ij| == w0, zero Ising field, degree <= 31");
+        set_device(ctx);
+        auto s = std::make_unique<ige_state>();
+        s->ctx = ctx;
+        s->model = model;
+        s->cells = cells;
+        s->seed = seed;
+        s->kernel = kernel == IGE_KERNEL_AUTO
+                        ? (model->ms.eligible ? IGE_KERNEL_MULTISPIN : IGE_KERNEL_GENERIC)
+                        : kernel;
+        check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&s->ev0), "hipEventCreate");
+        check(hipEventCreate(&s->ev1), "hipEventCreate");
+        const uint32_t n = model->ir.n;
+        s->spins.alloc(static_cast<size_t>(cells) * n);
+        s->energy.alloc(static_cast<size_t>(cells) * 64);
+        s->temp_of.alloc(static_cast<size_t>(cells) * 64);
+        s->slot_of.alloc(static_cast<size_t>(cells) * 64);
+        s->counters.alloc(1 + 2 * static_cast<size_t>(cells));
+        s->beta.alloc(cells);
+        if (s->kernel == IGE_KERNEL_GENERIC) {
+            s->dtab = static_cast<uint32_t>(model->max_local);
+            s->thr.alloc(static_cast<size_t>(cells) * (s->dtab + 1));
+        } else {
+            s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * 33);
+        }
+        std::vector<unsigned long long> zeros(s->counters.count, 0);
+        s->counters.upload(zeros.data(), zeros.size(), s->stream);
+        auto st = s->view();
+        check(ige::launch_identity_perm(st, s->stream), "perm");
+        if (init_words) {
+            std::vector<uint64_t> w(static_cast<size_t>(cells) * n);
+            for (uint32_t c = 0; c < cells; ++c)
+                for (uint32_t p = 0; p < n; ++p)
+                    w[static_cast<size_t>(c) * n + p] =
+                        init_words[static_cast<size_t>(c) * n + model->co.order[p]];
+            s->spins.upload(w.data(), w.size(), s->stream);
+            check(hipStreamSynchronize(s->stream), "init upload");
+        } else {
+            check(ige::launch_init_spins(model->view(), st, seed, s->stream), "init");
+        }
+        recompute_energy_into_state(s.get());
+        check(hipStreamSynchronize(s->stream), "state create");
+        *out = s.release();
+    });
+}
+
+int32_t ige_state_destroy(ige_state* s) {
+    if (s) (void)hipSetDevice(s->ctx->device);
+    delete s;
+    return IGE_OK;
+}
+
+int32_t ige_set_ladder(ige_state* s, const double* beta) {
+    return guarded([&] {
+        require(s && beta, "NULL argument");
+        for (uint32_t t = 0; t < s->cells; ++t)
+            require(std::isfinite(beta[t]) && beta[t] >= 0.0, "beta must be finite and >= 0");
+        set_device(s->ctx);
+        s->ladder.assign(beta, beta + s->cells);
+        s->beta.upload(s->ladder.data(), s->cells, s->stream);
+        s->ladder_set = true;
+        refresh_thresholds(s, s->beta.p);
+        check(hipStreamSynchronize(s->stream), "set ladder");
+    });
+}
+
+int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sched) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        require(sched != nullptr || s->ladder_set, "no ladder set and no beta schedule given");
+        require((flags & ~IGE_RUN_EXCHANGE) == 0, "unknown flag");
+        require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull,
+                "sweep counter would exceed 2^32-1");
+        set_device(s->ctx);
+        const bool exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
+        DevBuf<double> sched_dev;
+        if (sched) {
+            const size_t total = static_cast<size_t>(sweeps) * s->cells;
+            for (size_t i = 0; i < total; ++i)
+                require(std::isfinite(sched[i]) && sched[i] >= 0.0,
+                        "beta must be finite and >= 0");
+            sched_dev.alloc(total);
+            sched_dev.upload(sched, total, s->stream);
+        }
+        const auto m = s->model->view();
+        auto st = s->view();
+        const auto& cs = s->model->co.class_start;
+        for (uint32_t k = 0; k < sweeps; ++k) {
+            const double* beta = sched ? sched_dev.p + static_cast<size_t>(k) * s->cells : s->beta.p;
+            // Thresholds depend on beta and, for the multispin planes, on
+            // which temperature each (slot, lane) holds after the last swap.
+            if (sched || (s->kernel == IGE_KERNEL_MULTISPIN && exch && k > 0))
+                refresh_thresholds(s, beta);
+            if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
+            for (uint32_t c = 0; c < s->model->co.num_colors; ++c) {
+                if (s->kernel == IGE_KERNEL_GENERIC)
+                    check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
+                                                    s->thr.p, s->dtab, s->stream),
+                          "sweep_generic");
+                else
+                    check(ige::launch_sweep_multispin(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
+                                                      s->qplanes.p, s->stream),
+                          "sweep_multispin");
+            }
+            if (s->profile) {
+                check(hipEventRecord(s->ev1, s->stream), "event");
+                check(hipEventSynchronize(s->ev1), "event sync");
+                float ms = 0.f;
+                check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
+                s->prof_ms += ms;
+                s->prof_launches += s->model->co.num_colors;
+            }
+            if (exch) {
+                if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
+                check(ige::launch_exchange(st, beta, s->phase, s->sweep, s->seed, s->stream),
+                      "exchange");
+                if (s->profile) {
+                    check(hipEventRecord(s->ev1, s->stream), "event");
+                    check(hipEventSynchronize(s->ev1), "event sync");
+                    float ms = 0.f;
+                    check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
+                    s->prof_other_ms += ms;
+                    s->prof_other_launches += 1;
+                }
+                s->phase ^= 1u;
+            }
+            s->sweep += 1;
+        }
+        if (sched) {
+            // Leave the ladder at the schedule's last row.
+            const double* last = sched + static_cast<size_t>(sweeps - 1) * s->cells;
+            if (sweeps > 0) {
+                s->ladder.assign(last, last + s->cells);
+                s->beta.upload(s->ladder.data(), s->cells, s->stream);
+                s->ladder_set = true;
+            }
+            check(hipStreamSynchronize(s->stream), "run");  // sched_dev is freed here
+            if (sweeps > 0) refresh_thresholds(s, s->beta.p);
+        } else if (s->kernel == IGE_KERNEL_MULTISPIN && exch && sweeps > 0) {
+            refresh_thresholds(s, s->beta.p);
+        }
+    });
+}
+
+int32_t ige_synchronize(ige_state* s) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        set_device(s->ctx);
+        check(hipStreamSynchronize(s->stream), "synchronize");
+    });
+}
+
+int32_t ige_read_spins(ige_state* s, uint64_t* words) {
+    return guarded([&] {
+        require(s && words, "NULL argument");
+        set_device(s->ctx);
+        const uint32_t n = s->model->ir.n;
+        std::vector<uint64_t> w(static_cast<size_t>(s->cells) * n);
+        s->spins.download(w.data(), w.size(), s->stream);
+        check(hipStreamSynchronize(s->stream), "read spins");
+        const auto& order = s->model->co.order;
+        for (uint32_t c = 0; c < s->cells; ++c)
+            for (uint32_t p = 0; p < n; ++p)
+                words[static_cast<size_t>(c) * n + order[p]] = w[static_cast<size_t>(c) * n + p];
+    });
+}
+
+int32_t ige_read_energies(ige_state* s, int64_t* e) {
+    return guarded([&] {
+        require(s && e, "NULL argument");
+        set_device(s->ctx);
+        s->energy.download(e, static_cast<size_t>(s->cells) * 64, s->stream);
+        check(hipStreamSynchronize(s->stream), "read energies");
+    });
+}
+
+int32_t ige_read_temp_of(ige_state* s, uint32_t* t) {
+    return guarded([&] {
+        require(s && t, "NULL argument");
+        set_device(s->ctx);
+        s->temp_of.download(t, static_cast<size_t>(s->cells) * 64, s->stream);
+        check(hipStreamSynchronize(s->stream), "read temp_of");
+    });
+}
+
+int32_t ige_read_fields(ige_state* s, int32_t* out) {
+    return guarded([&] {
+        require(s && out, "NULL argument");
+        set_device(s->ctx);
+        const size_t total = static_cast<size_t>(s->cells) * s->model->ir.n * 64;
+        DevBuf<int32_t> f;
+        f.alloc(total);
+        check(ige::launch_fields(s->model->view(), s->view(), f.p, s->stream), "fields");
+        f.download(out, total, s->stream);
+        check(hipStreamSynchronize(s->stream), "read fields");
+    });
+}
+
+int32_t ige_recompute_energies(ige_state* s, int64_t* out) {
+    return guarded([&] {
+        require(s && out, "NULL argument");
+        set_device(s->ctx);
+        DevBuf<int64_t> e;
+        e.alloc(static_cast<size_t>(s->cells) * 64);
+        check(ige::launch_energy(s->model->view(), s->view(), e.p, s->stream), "energy");
+        e.download(out, e.count, s->stream);
+        check(hipStreamSynchronize(s->stream), "recompute energies");
+    });
+}
+
+int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
+                       uint64_t* pair_accepts) {
+    return guarded([&] {
+        require(s && stats, "NULL argument");
+        set_device(s->ctx);
+        std::vector<unsigned long long> c(s->counters.count);
+        s->counters.download(c.data(), c.size(), s->stream);
+        check(hipStreamSynchronize(s->stream), "read stats");
+        stats->sweeps = s->sweep;
+        stats->accepted = c[0];
+        stats->proposed = static_cast<uint64_t>(s->sweep) * s->model->ir.n * 64ull * s->cells;
+        stats->phase = s->phase;
+        stats->kernel = s->kernel;
+        for (uint32_t t = 0; t + 1 < s->cells; ++t) {
+            if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];
+            if (pair_accepts) pair_accepts[t] = c[2 + 2 * t];
+        }
+    });
+}
+
+int32_t ige_profile_enable(ige_state* s, int32_t on) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        s->profile = on != 0;
+        s->prof_ms = s->prof_other_ms = 0.0;
+        s->prof_launches = s->prof_other_launches = 0;
+    });
+}
+
+int32_t ige_profile_read(ige_state* s, double* ms_total, uint64_t* launches, double* ms_other,
+                         uint64_t* other_launches) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        if (ms_total) *ms_total = s->prof_ms;
+        if (launches) *launches = s->prof_launches;
+        if (ms_other) *ms_other = s->prof_other_ms;
+        if (other_launches) *other_launches = s->prof_other_launches;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,495 @@
+// kernels.hip -- gfx950 kernels of the sweep / exchange hot path.
+//
+// Wave64 throughout.  Two sweep kernels share one acceptance rule
+// (dE <= 0, or u < metropolis_threshold(beta, dE)) and differ in how a wave
+// maps onto the MSC words:
+//   sweep_generic    lane = replica.  The wave walks the positions of one
+//                    colour class for one slot; every word it touches is
+//                    wave-uniform, so CSR rows and neighbour words arrive
+//                    through scalar loads and a neighbour's spin is applied
+//                    with the word itself as the lane mask (v_cndmask on an
+//                    SGPR pair).  Any integer CSR model.
+//   sweep_multispin  lane = site.  Each lane owns one u64 word and evaluates
+//                    all 64 replicas with bitwise arithmetic (+-J models).
+#include <hip/hip_runtime.h>
+
+#include "ige_math.h"
+#include "kernels.hpp"
+
+namespace ige {
+
+namespace {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kBlock = 256;  // 4 waves
+constexpr uint32_t kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ bool lane_bit(uint64_t word) {
+    return __builtin_amdgcn_inverse_ballot_w64(word);
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Threshold table: one thread per (t, d).
+// ---------------------------------------------------------------------------
+__global__ void thr_table_kernel(const double* __restrict__ beta, uint32_t* __restrict__ thr,
+                                 uint32_t cells, uint32_t dtab) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t row = (uint64_t)dtab + 1;
+    if (i >= row * cells) return;
+    const uint32_t t = (uint32_t)(i / row);
+    const uint32_t d = (uint32_t)(i % row);
+    thr[i] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta[t], (int64_t)d);
+}
+
+// ---------------------------------------------------------------------------
+// Generic sweep, lane = replica.  Each wave owns positions [p0, p1) of the
+// colour class for one slot.  Uniforms: word view, one Philox call per lane
+// per aligned group of four positions.
+// ---------------------------------------------------------------------------
+template <bool kLdsThr>
+__global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
+    const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr, uint32_t n, uint32_t cells,
+    const uint32_t* __restrict__ temp_of, const uint32_t* __restrict__ thr, uint32_t dtab,
+    int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
+    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    extern __shared__ uint32_t lds_thr[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    if (kLdsThr) {
+        const uint32_t total = cells * (dtab + 1);
+        for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_thr[i] = thr[i];
+        __syncthreads();
+    }
+    const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t slot = wave / waves_per_slot;
+    const uint32_t p0 = c_begin + (wave % waves_per_slot) * chunk;
+    if (slot >= cells || p0 >= c_end) return;
+    const uint32_t p1 = min(p0 + chunk, c_end);
+
+    const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
+    uint64_t* W = spins_wr + (uint64_t)slot * n;
+    const uint32_t trow = temp_of[slot * kWave + lane] * (dtab + 1);
+
+    int64_t eacc = 0;
+    uint32_t nacc = 0;
+    U32x4 u{};
+    uint32_t group = 0xFFFFFFFFu;
+    for (uint32_t p = p0; p < p1; ++p) {
+        if ((p >> 2) != group) {
+            group = p >> 2;
+            u = philox4x32_10(group, slot, sweep, (kTagWord << 24) | lane, key0, key1);
+        }
+        const uint32_t q = p & 3u;
+        const uint32_t uu = q == 0 ? u.v[0] : q == 1 ? u.v[1] : q == 2 ? u.v[2] : u.v[3];
+        const uint64_t x = S[p];
+        int32_t h = lin[p];
+        const uint32_t e1 = rp[p + 1];
+        for (uint32_t e = rp[p]; e < e1; ++e) {
+            const int32_t w = wt[e];
+            h += lane_bit(S[ci[e]]) ? w : 0;
+        }
+        const int32_t de = lane_bit(x) ? -h : h;
+        bool acc = de <= 0;
+        if (!acc) {
+            const uint32_t th = kLdsThr ? lds_thr[trow + (uint32_t)de] : thr[trow + (uint32_t)de];
+            acc = uu < th;
+        }
+        const uint64_t m = ballot(acc);
+        if (lane == 0) W[p] = x ^ m;
+        eacc += acc ? de : 0;
+        nacc += (uint32_t)__builtin_popcountll(m);
+    }
+    atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
+              (unsigned long long)eacc);
+    if (lane == 0) atomicAdd(accepted, (unsigned long long)nacc);
+}
+
+// ---------------------------------------------------------------------------
+// DEO replica exchange: one wave per adjacent pair (t, t+1), t = phase mod 2;
+// lane r decides for replica lane r and swaps temperature indices.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWave) void exchange_kernel(
+    const double* __restrict__ beta, const int64_t* __restrict__ energy, uint32_t* temp_of,
+    uint32_t* slot_of, unsigned long long* pair_stats, uint32_t cells, uint32_t phase,
+    uint32_t sweep, uint32_t key0, uint32_t key1) {
+    const uint32_t r = threadIdx.x;
+    const uint32_t t = phase + 2u * blockIdx.x;
+    if (t + 1 >= cells) return;
+    const uint32_t s1 = slot_of[t * kWave + r];
+    const uint32_t s2 = slot_of[(t + 1) * kWave + r];
+    const double delta = (beta[t] - beta[t + 1]) *
+                         (double)(energy[s1 * kWave + r] - energy[s2 * kWave + r]);
+    const U32x4 u = philox4x32_10(t, r >> 2, sweep, kTagExch << 24, key0, key1);
+    const uint32_t q = r & 3u;
+    const uint32_t uu = q == 0 ? u.v[0] : q == 1 ? u.v[1] : q == 2 ? u.v[2] : u.v[3];
+    const bool acc = delta >= 0.0 || uu < exchange_threshold(delta);
+    if (acc) {
+        slot_of[t * kWave + r] = s2;
+        slot_of[(t + 1) * kWave + r] = s1;
+        temp_of[s1 * kWave + r] = t + 1;
+        temp_of[s2 * kWave + r] = t;
+    }
+    const uint64_t m = ballot(acc);
+    if (r == 0) {
+        pair_stats[2 * t] += kWave;
+        pair_stats[2 * t + 1] += (unsigned long long)__builtin_popcountll(m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multispin sweep, lane = site.  Model restriction (checked on the host):
+// |q_ij| = w0 for every bond and lin_i = -1/2 sum_j q_ij, so that
+//   dE_i(replica r) = w0 (k_r - d_i / 2),
+//   k_r = #{ j : bit r of (x_i ^ x_j ^ [q_ij < 0]) }.
+// A lane counts k for its 64 replicas in B bit planes, accepts dE <= 0
+// outright, and resolves u_r < thr_r for the rest by comparing the uniform's
+// bit planes (Philox, plane view) against the threshold's bit planes from
+// most to least significant, stopping once every replica is decided.
+// Per-replica energy changes are summed in bit-sliced counters and turned
+// into per-replica integers by one 64x64 bit transpose per counter plane.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kQStride = 33;  // 32 threshold bit planes + nonzero mask
+
+// 64x64 bit-matrix transpose across the wave: lane l holds row l on entry,
+// column l on exit (bit k of the result = bit l of row k).
+__device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                               0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const uint32_t j = 32u >> s;
+        const uint64_t m = masks[s];
+        const uint64_t y = __shfl_xor(x, (int)j, 64);
+        x = (lane & j) ? ((x & ~m) | ((y & ~m) >> j)) : ((x & m) | ((y & m) << j));
+    }
+    return x;
+}
+
+// Sum over the wave's lanes of the per-lane bit-sliced counter acc[0..W-1],
+// returned in lane r for replica r.
+template <int W>
+__device__ __forceinline__ int64_t wave_column_sum(const uint64_t (&acc)[W], uint32_t lane) {
+    int64_t tot = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b)
+        tot += (int64_t)__builtin_popcountll(wave_transpose(acc[b], lane)) << b;
+    return tot;
+}
+
+template <int B, int W>
+__global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const uint32_t* __restrict__ sgn, const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr,
+    uint32_t n, uint32_t dmax, int32_t w0, const unsigned long long* __restrict__ qplanes,
+    int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
+    uint32_t per_lane, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    constexpr int LMAX = 1 << (B - 1);  // positive levels k in (d/2, d], d < 2^B
+    extern __shared__ unsigned long long lds_q[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t slot = blockIdx.y;
+    {
+        const uint32_t total = (dmax + 1) * kQStride;
+        const unsigned long long* src = qplanes + (uint64_t)slot * total;
+        for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_q[i] = src[i];
+        __syncthreads();
+    }
+    const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t base = c_begin + wave * kWave * per_lane;
+    if (base >= c_end) return;
+    const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
+    uint64_t* Wr = spins_wr + (uint64_t)slot * n;
+
+    uint64_t kacc[W], dacc[W];
+#pragma unroll
+    for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
+    uint32_t nacc = 0;
+
+    for (uint32_t it = 0; it < per_lane; ++it) {
+        const uint32_t p = base + it * kWave + lane;
+        if (p >= c_end) break;
+        const uint64_t x = S[p];
+        const uint32_t e0 = rp[p], d = rp[p + 1] - e0;
+        const uint32_t sg = sgn[p];
+        uint64_t c[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) c[b] = 0;
+        for (uint32_t e = 0; e < d; ++e) {
+            uint64_t carry = x ^ S[ci[e0 + e]] ^ (((sg >> e) & 1u) ? ~0ull : 0ull);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const uint64_t s = c[b] ^ carry;
+                carry &= c[b];
+                c[b] = s;
+            }
+        }
+        // Level masks for k = floor(d/2) + 1 + j, j < LMAX, k <= d; m = 2k - d.
+        uint64_t lev[LMAX];
+        uint32_t mrow[LMAX];
+        uint64_t pos = 0, und = 0;
+#pragma unroll
+        for (int j = 0; j < LMAX; ++j) {
+            const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
+            uint64_t eq = k <= d ? ~0ull : 0ull;
+#pragma unroll
+            for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
+            lev[j] = eq;
+            const uint32_t mm = k <= d ? 2 * k - d : 0;
+            mrow[j] = mm * kQStride;
+            pos |= eq;
+            und |= eq & lds_q[mrow[j] + 32];
+        }
+        uint64_t lt = 0, P0 = 0, P1 = 0;
+        for (uint32_t b = 0; b < 32 && und != 0; ++b) {
+            if ((b & 1u) == 0) {
+                const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | (b >> 1), key0,
+                                              key1);
+                P0 = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
+                P1 = (uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32);
+            }
+            const uint64_t P = (b & 1u) ? P1 : P0;
+            uint64_t T = 0;
+#pragma unroll
+            for (int j = 0; j < LMAX; ++j) T |= lev[j] & lds_q[mrow[j] + b];
+            lt |= und & T & ~P;
+            und &= ~(P ^ T);
+        }
+        const uint64_t flip = ~pos | lt;
+        Wr[p] = x ^ flip;
+        nacc += (uint32_t)__builtin_popcountll(flip);
+        // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
+        uint64_t ck = 0, cd = 0;
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const uint64_t ak = b < B ? (c[b] & flip) : 0ull;
+            const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
+            const uint64_t sk = kacc[b] ^ ak ^ ck;
+            ck = (kacc[b] & ak) | (ck & (kacc[b] ^ ak));
+            kacc[b] = sk;
+            const uint64_t sd = dacc[b] ^ ad ^ cd;
+            cd = (dacc[b] & ad) | (cd & (dacc[b] ^ ad));
+            dacc[b] = sd;
+        }
+    }
+    const int64_t ksum = wave_column_sum<W>(kacc, lane);
+    const int64_t dsum = wave_column_sum<W>(dacc, lane);
+    const int64_t de = ((int64_t)w0 * (2 * ksum - dsum)) / 2;
+    atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
+              (unsigned long long)de);
+    // Wave total of accepted flips.
+    for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
+    if (lane == 0) atomicAdd(accepted, (unsigned long long)nacc);
+}
+
+// One wave per (slot, m), m in [0, dmax]: lane r holds the threshold of the
+// temperature that (slot, r) currently carries.
+__global__ __launch_bounds__(kWave) void qplanes_kernel(const double* __restrict__ beta,
+                                                        const uint32_t* __restrict__ temp_of,
+                                                        unsigned long long* qplanes,
+                                                        uint32_t dmax, int32_t w0) {
+    const uint32_t r = threadIdx.x;
+    const uint32_t m = blockIdx.x;
+    const uint32_t slot = blockIdx.y;
+    const int64_t de = ((int64_t)m * w0) / 2;
+    const uint32_t th = m == 0 ? 0u : metropolis_threshold(beta[temp_of[slot * kWave + r]], de);
+    unsigned long long* out = qplanes + ((uint64_t)slot * (dmax + 1) + m) * kQStride;
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 32; ++b) {
+        const uint64_t plane = ballot((th >> (31u - b)) & 1u);
+        if (r == b) mine = plane;
+    }
+    const uint64_t nz = ballot(th != 0);
+    if (r < 32) out[r] = mine;
+    if (r == 32) out[32] = nz;
+}
+
+__global__ void identity_perm_kernel(uint32_t* temp_of, uint32_t* slot_of, uint32_t cells) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * kWave) return;
+    temp_of[i] = i / kWave;
+    slot_of[i] = i / kWave;
+}
+
+__global__ void init_spins_kernel(const uint32_t* __restrict__ order, uint64_t* spins, uint32_t n,
+                                  uint32_t cells, uint32_t key0, uint32_t key1) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n * cells) return;
+    const uint32_t slot = (uint32_t)(i / n);
+    const uint32_t p = (uint32_t)(i % n);
+    const U32x4 o = philox4x32_10(order[p], slot, 0u, kTagInit << 24, key0, key1);
+    spins[i] = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
+}
+
+// Energy from scratch: one block per slot, lane = replica; accumulates
+// A = sum_i x_i (2 lin_i + sum_j q_ij x_j) so that E = offset + A/2 exactly.
+__global__ __launch_bounds__(kBlock) void energy_kernel(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
+    const uint64_t* __restrict__ spins, uint32_t n, int64_t offset, int64_t* out) {
+    __shared__ int64_t part[kBlock];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wv = uniform(threadIdx.x >> 6);
+    const uint32_t slot = blockIdx.x;
+    const uint64_t* S = spins + (uint64_t)slot * n;
+    int64_t a = 0;
+    for (uint32_t p = wv; p < n; p += kWavesPerBlock) {
+        const uint64_t x = S[p];
+        int64_t f = 2 * (int64_t)lin[p];
+        for (uint32_t e = rp[p]; e < rp[p + 1]; ++e) f += lane_bit(S[ci[e]]) ? wt[e] : 0;
+        a += lane_bit(x) ? f : 0;
+    }
+    part[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        int64_t tot = 0;
+        for (uint32_t w = 0; w < kWavesPerBlock; ++w) tot += part[w * kWave + lane];
+        out[slot * kWave + lane] = offset + tot / 2;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void fields_kernel(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
+    const uint32_t* __restrict__ order, const uint64_t* __restrict__ spins, uint32_t n,
+    uint32_t cells, int32_t* out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t w = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    if (w >= (uint64_t)n * cells) return;
+    const uint32_t slot = (uint32_t)(w / n);
+    const uint32_t p = (uint32_t)(w % n);
+    const uint64_t* S = spins + (uint64_t)slot * n;
+    int32_t h = lin[p];
+    for (uint32_t e = rp[p]; e < rp[p + 1]; ++e) h += lane_bit(S[ci[e]]) ? wt[e] : 0;
+    out[((uint64_t)slot * n + order[p]) * kWave + lane] = h;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_thr_table(const double* beta, uint32_t* thr, uint32_t cells, uint32_t dtab,
+                            hipStream_t stream) {
+    const uint64_t total = ((uint64_t)dtab + 1) * cells;
+    hipLaunchKernelGGL(thr_table_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, stream,
+                       beta, thr, cells, dtab);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                uint32_t c_end, uint32_t sweep, uint64_t seed,
+                                const uint32_t* thr, uint32_t dtab, hipStream_t stream) {
+    if (c_end <= c_begin) return hipSuccess;
+    const uint32_t len = c_end - c_begin;
+    // Aim for >= 16K waves per launch (256 CUs x 4 SIMDs x 16), at least 16
+    // positions per wave so the Philox groups amortise.
+    uint64_t chunk = ((uint64_t)len * s.cells + 16383) / 16384;
+    if (chunk < 16) chunk = 16;
+    if (chunk > 4096) chunk = 4096;
+    const uint32_t waves_per_slot = ceil_div(len, chunk);
+    const uint64_t waves = (uint64_t)waves_per_slot * s.cells;
+    const dim3 grid(ceil_div(waves, kWavesPerBlock));
+    const uint64_t lds_bytes = (uint64_t)s.cells * (dtab + 1) * sizeof(uint32_t);
+    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
+    if (lds_bytes <= 32768) {
+        hipLaunchKernelGGL(sweep_generic_kernel<true>, grid, dim3(kBlock), (uint32_t)lds_bytes,
+                           stream, m.rp, m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells,
+                           s.temp_of, thr, dtab, s.energy, s.accepted, c_begin, c_end,
+                           (uint32_t)chunk, waves_per_slot, sweep, key0, key1);
+    } else {
+        hipLaunchKernelGGL(sweep_generic_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
+                           m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, thr,
+                           dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
+                           waves_per_slot, sweep, key0, key1);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                  uint32_t c_end, uint32_t sweep, uint64_t seed,
+                                  const unsigned long long* qplanes, hipStream_t stream) {
+    if (c_end <= c_begin) return hipSuccess;
+    const uint32_t len = c_end - c_begin;
+    // Sites per lane: enough waves to fill the chip (~8K), at most 32 so the
+    // bit-sliced energy counters stay narrow.
+    uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 8192 - 1) /
+                        ((uint64_t)kWave * 8192);
+    if (per_lane < 1) per_lane = 1;
+    if (per_lane > 32) per_lane = 32;
+    const uint32_t waves = ceil_div(len, kWave * per_lane);
+    const dim3 grid(ceil_div(waves, kWavesPerBlock), s.cells);
+    const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
+    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
+#define IGE_MS_LAUNCH(B, W)                                                                      \
+    hipLaunchKernelGGL((sweep_multispin_kernel<B, W>), grid, dim3(kBlock), lds, stream, m.rp,   \
+                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,    \
+                       s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1)
+    if (m.dmax < 8) {
+        IGE_MS_LAUNCH(3, 8);
+    } else if (m.dmax < 16) {
+        IGE_MS_LAUNCH(4, 12);
+    } else if (m.dmax < 32) {
+        IGE_MS_LAUNCH(5, 12);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef IGE_MS_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
+                          unsigned long long* qplanes, hipStream_t stream) {
+    hipLaunchKernelGGL(qplanes_kernel, dim3(m.dmax + 1, s.cells), dim3(kWave), 0, stream, beta,
+                       s.temp_of, qplanes, m.dmax, m.w0);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t phase, uint32_t sweep,
+                           uint64_t seed, hipStream_t stream) {
+    if (s.cells < 2) return hipSuccess;
+    const uint32_t pairs = (s.cells - phase) / 2;  // t = phase, phase+2, ... with t+1 < cells
+    if (pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(exchange_kernel, dim3(pairs), dim3(kWave), 0, stream, beta, s.energy,
+                       s.temp_of, s.slot_of, s.pair_stats, s.cells, phase, sweep, (uint32_t)seed,
+                       (uint32_t)(seed >> 32));
+    return hipGetLastError();
+}
+
+hipError_t launch_identity_perm(const DevState& s, hipStream_t stream) {
+    hipLaunchKernelGGL(identity_perm_kernel, dim3(ceil_div((uint64_t)s.cells * kWave, kBlock)),
+                       dim3(kBlock), 0, stream, s.temp_of, s.slot_of, s.cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_spins(const DevModel& m, const DevState& s, uint64_t seed,
+                             hipStream_t stream) {
+    const uint64_t total = (uint64_t)m.n * s.cells;
+    hipLaunchKernelGGL(init_spins_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, stream,
+                       m.order, s.spins, m.n, s.cells, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const DevModel& m, const DevState& s, int64_t* out, hipStream_t stream) {
+    hipLaunchKernelGGL(energy_kernel, dim3(s.cells), dim3(kBlock), 0, stream, m.rp, m.ci, m.wt,
+                       m.lin, s.spins, m.n, m.offset, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fields(const DevModel& m, const DevState& s, int32_t* out, hipStream_t stream) {
+    const uint64_t waves = (uint64_t)m.n * s.cells;
+    hipLaunchKernelGGL(fields_kernel, dim3(ceil_div(waves, kWavesPerBlock)), dim3(kBlock), 0,
+                       stream, m.rp, m.ci, m.wt, m.lin, m.order, s.spins, m.n, s.cells, out);
+    return hipGetLastError();
+}
+
+}  // namespace ige

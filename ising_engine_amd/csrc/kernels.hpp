@@ -1,0 +1,68 @@
+// kernels.hpp -- device data views and host-side launchers for the gfx950
+// kernels.  All launchers are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ige {
+
+// Model in colour-ordered position space: position p holds site order[p];
+// colour class c occupies positions [class_start[c], class_start[c+1]).
+struct DevModel {
+    uint32_t n = 0;
+    const uint32_t* rp = nullptr;   // n+1 row pointers (position space)
+    const uint32_t* ci = nullptr;   // nnz neighbour positions
+    const int32_t* wt = nullptr;    // nnz couplings q_ij
+    const int32_t* lin = nullptr;   // n linear terms
+    const uint32_t* order = nullptr;  // n: position -> site
+    const uint32_t* sgn = nullptr;  // n: bit e set iff entry e of the row has q < 0
+    int32_t w0 = 0;                 // common |q| for the multispin kernel
+    uint32_t dmax = 0;              // max degree
+    int64_t offset = 0;
+};
+
+struct DevState {
+    uint32_t cells = 0;
+    uint64_t* spins = nullptr;      // [cells][n] MSC words, position order
+    int64_t* energy = nullptr;      // [cells][64]
+    uint32_t* temp_of = nullptr;    // [cells][64] temperature index of (slot, lane)
+    uint32_t* slot_of = nullptr;    // [cells][64] slot of (temperature, lane)
+    unsigned long long* accepted = nullptr;    // 1 counter
+    unsigned long long* pair_stats = nullptr;  // [cells-1][2] attempts, accepts
+};
+
+// thr[t*(dtab+1) + d] = metropolis_threshold(beta[t], d) for d in [1, dtab].
+hipError_t launch_thr_table(const double* beta, uint32_t* thr, uint32_t cells, uint32_t dtab,
+                            hipStream_t stream);
+
+// Generic sweep of one colour class (lane = replica).
+hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                uint32_t c_end, uint32_t sweep, uint64_t seed,
+                                const uint32_t* thr, uint32_t dtab, hipStream_t stream);
+
+// Multispin sweep of one colour class (lane = site, bit-sliced +-J).
+// qplanes: [cells][dmax+1][33] threshold bit planes, see launch_qplanes.
+hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                  uint32_t c_end, uint32_t sweep, uint64_t seed,
+                                  const unsigned long long* qplanes, hipStream_t stream);
+
+// qplanes[(slot*(dmax+1) + m)*33 + b], b < 32: bit r = bit (31-b) of
+// metropolis_threshold(beta[temp_of[slot][r]], m*w0/2); b == 32: bit r set
+// iff that threshold is nonzero.
+hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
+                          unsigned long long* qplanes, hipStream_t stream);
+
+// DEO exchange of the pairs (t, t+1) with t = phase mod 2.
+hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t phase, uint32_t sweep,
+                           uint64_t seed, hipStream_t stream);
+
+hipError_t launch_init_spins(const DevModel& m, const DevState& s, uint64_t seed,
+                             hipStream_t stream);
+// Exact per-(slot, lane) energies from scratch into `out` ([cells][64]).
+hipError_t launch_energy(const DevModel& m, const DevState& s, int64_t* out, hipStream_t stream);
+// Local fields, out[(slot*n + site)*64 + lane] (site order).
+hipError_t launch_fields(const DevModel& m, const DevState& s, int32_t* out, hipStream_t stream);
+hipError_t launch_identity_perm(const DevState& s, hipStream_t stream);
+
+}  // namespace ige

@@ -1,0 +1,425 @@
+/*
+ * oracle.c -- CPU restatement of the reference sweep/exchange path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py, never by the product.  Each function cites
+ * the reference lines it restates (paths relative to the reference checkout).
+ *
+ * Build with -ffp-contract=off: or_det_exp must round exactly like the
+ * device code, which is compiled the same way.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as   */
+/* easy as 1, 2, 3", SC'11).  Multipliers and Weyl key increments are the    */
+/* published constants; pinned by the paper's known-answer vectors in        */
+/* tests/test_oracle_kat.py.                                                  */
+/* ------------------------------------------------------------------------- */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += PHILOX_W0;
+            k1 += PHILOX_W1;
+        }
+        uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Stream tags in the top byte of counter word 3. */
+#define TAG_WORD 1u
+#define TAG_PLANE 2u
+#define TAG_EXCH 3u
+#define TAG_INIT 4u
+
+static void philox_seeded(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                          uint32_t out[4]) {
+    uint32_t ctr[4] = {c0, c1, c2, c3};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    or_philox4x32_10(ctr, key, out);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Deterministic exp for x <= 0: Cody-Waite reduction by ln2 with the fdlibm  */
+/* split constants, degree-13 Taylor polynomial in Horner form, exact power   */
+/* of two scaling.  Results below exp(-40) are returned as 0 (they map to a   */
+/* zero threshold anyway).  Pinned against libm exp in the tests.             */
+/* ------------------------------------------------------------------------- */
+double or_det_exp(double x) {
+    if (!(x > -40.0)) return 0.0;
+    if (x >= 0.0) return 1.0;
+    const double inv_ln2 = 1.44269504088896338700e+00;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    double kd = rint(x * inv_ln2);
+    double r = (x - kd * ln2_hi) - kd * ln2_lo;
+    double p = 1.0 / 6227020800.0;           /* 1/13! */
+    p = p * r + 1.0 / 479001600.0;           /* 1/12! */
+    p = p * r + 1.0 / 39916800.0;            /* 1/11! */
+    p = p * r + 1.0 / 3628800.0;             /* 1/10! */
+    p = p * r + 1.0 / 362880.0;              /* 1/9!  */
+    p = p * r + 1.0 / 40320.0;               /* 1/8!  */
+    p = p * r + 1.0 / 5040.0;                /* 1/7!  */
+    p = p * r + 1.0 / 720.0;                 /* 1/6!  */
+    p = p * r + 1.0 / 120.0;                 /* 1/5!  */
+    p = p * r + 1.0 / 24.0;                  /* 1/4!  */
+    p = p * r + 1.0 / 6.0;                   /* 1/3!  */
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)kd);
+}
+
+static uint32_t prob_to_u32(double e) {
+    double v = e * 4294967296.0;
+    if (v >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
+uint32_t or_threshold(double beta, int64_t de) {
+    return prob_to_u32(or_det_exp(-(beta * (double)de)));
+}
+
+uint32_t or_exchange_threshold(double delta) { return prob_to_u32(or_det_exp(delta)); }
+
+/* ------------------------------------------------------------------------- */
+/* ProblemIR (src/engine_v2/ir.rs)                                            */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t col;
+    int64_t q;
+} entry_t;
+
+static int cmp_entry(const void *a, const void *b) {
+    uint32_t x = ((const entry_t *)a)->col, y = ((const entry_t *)b)->col;
+    return (x > y) - (x < y);
+}
+
+/* ir.rs:30-69: both triangles, rows by ascending column, duplicates summed,
+ * zero sums dropped.  The reference asserts i < j < n; here that is -1. */
+int64_t or_ir_from_pairs(uint32_t n, uint64_t npairs, const uint32_t *pi, const uint32_t *pj,
+                         const int64_t *q, uint32_t *row_ptr, uint32_t *col_idx, int64_t *w) {
+    uint64_t *cnt = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+    for (uint64_t k = 0; k < npairs; ++k) {
+        if (!(pi[k] < pj[k] && pj[k] < n)) {
+            free(cnt);
+            return -1;
+        }
+        cnt[pi[k] + 1]++;
+        cnt[pj[k] + 1]++;
+    }
+    for (uint32_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    entry_t *adj = (entry_t *)malloc(sizeof(entry_t) * (size_t)(2 * npairs + 1));
+    uint64_t *fill = (uint64_t *)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+    memcpy(fill, cnt, sizeof(uint64_t) * ((size_t)n + 1));
+    for (uint64_t k = 0; k < npairs; ++k) {
+        adj[fill[pi[k]]++] = (entry_t){pj[k], q[k]};
+        adj[fill[pj[k]]++] = (entry_t){pi[k], q[k]};
+    }
+    int64_t nnz = 0;
+    row_ptr[0] = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        entry_t *row = adj + cnt[i];
+        uint64_t len = cnt[i + 1] - cnt[i];
+        qsort(row, (size_t)len, sizeof(entry_t), cmp_entry);
+        uint64_t k = 0;
+        while (k < len) {
+            uint32_t j = row[k].col;
+            int64_t acc = 0;
+            while (k < len && row[k].col == j) acc += row[k++].q;
+            if (acc != 0) {
+                col_idx[nnz] = j;
+                w[nnz] = acc;
+                nnz++;
+            }
+        }
+        row_ptr[i + 1] = (uint32_t)nnz;
+    }
+    free(adj);
+    free(fill);
+    free(cnt);
+    return nnz;
+}
+
+/* ir.rs:85-103: E = offset + sum_i lin_i x_i + 1/2 sum_i x_i sum_row q x_j.
+ * The pair sum over both triangles is even, so /2 is exact in integers. */
+int64_t or_ir_energy(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                     const int64_t *w, const int64_t *linear, int64_t offset, const uint8_t *x) {
+    int64_t lin = 0, pair = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!x[i]) continue;
+        lin += linear[i];
+        for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+            if (x[col_idx[e]]) pair += w[e];
+    }
+    return offset + lin + pair / 2;
+}
+
+/* ir.rs:114-139: ascending sites, smallest colour not used by an already
+ * coloured neighbour. */
+void or_greedy_coloring(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                        uint32_t *colors) {
+    uint32_t *seen = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 2));
+    for (uint32_t i = 0; i < n + 2; ++i) seen[i] = UINT32_MAX;
+    for (uint32_t i = 0; i < n; ++i) colors[i] = UINT32_MAX;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+            uint32_t c = colors[col_idx[e]];
+            if (c != UINT32_MAX && c <= n) seen[c] = i;
+        }
+        uint32_t c = 0;
+        while (seen[c] == i) c++;
+        colors[i] = c;
+    }
+    free(seen);
+}
+
+/* metropolis_sweep.rs:40-58: colour 0 sites ascending, then colour 1, ... */
+uint32_t or_color_order(uint32_t n, const uint32_t *colors, uint32_t *order,
+                        uint32_t *class_start) {
+    uint32_t ncol = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (colors[i] + 1 > ncol) ncol = colors[i] + 1;
+    uint32_t pos = 0;
+    for (uint32_t c = 0; c < ncol; ++c) {
+        class_start[c] = pos;
+        for (uint32_t i = 0; i < n; ++i)
+            if (colors[i] == c) order[pos++] = i;
+    }
+    class_start[ncol] = pos;
+    return ncol;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Fields and energies (sparse_bitslice.rs:88-137)                            */
+/* ------------------------------------------------------------------------- */
+static inline int bit(uint64_t word, uint32_t lane) { return (int)((word >> lane) & 1u); }
+
+static int64_t field_of(const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+                        const int64_t *linear, const uint64_t *slot_words, uint32_t site,
+                        uint32_t lane) {
+    int64_t h = linear[site];
+    for (uint32_t e = row_ptr[site]; e < row_ptr[site + 1]; ++e)
+        if (bit(slot_words[col_idx[e]], lane)) h += w[e];
+    return h;
+}
+
+void or_fields(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+               const int64_t *linear, uint32_t cells, const uint64_t *words, int64_t *fields) {
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t r = 0; r < 64; ++r)
+                fields[((uint64_t)s * n + i) * 64 + r] =
+                    field_of(row_ptr, col_idx, w, linear, words + (uint64_t)s * n, i, r);
+}
+
+void or_energies(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+                 const int64_t *linear, int64_t offset, uint32_t cells, const uint64_t *words,
+                 int64_t *energies) {
+    uint8_t *x = (uint8_t *)malloc((size_t)n + 1);
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t r = 0; r < 64; ++r) {
+            for (uint32_t i = 0; i < n; ++i) x[i] = (uint8_t)bit(words[(uint64_t)s * n + i], r);
+            energies[s * 64 + r] = or_ir_energy(n, row_ptr, col_idx, w, linear, offset, x);
+        }
+    free(x);
+}
+
+void or_init_words(uint32_t n, uint32_t cells, uint64_t seed, uint64_t *words) {
+    uint32_t o[4];
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t i = 0; i < n; ++i) {
+            philox_seeded(seed, i, s, 0, TAG_INIT << 24, o);
+            words[(uint64_t)s * n + i] = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+        }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Uniforms for one (position, slot, sweep): 64 lanes.                        */
+/* ------------------------------------------------------------------------- */
+static void uniforms_word(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t sweep,
+                          uint32_t u[64]) {
+    uint32_t o[4];
+    for (uint32_t r = 0; r < 64; ++r) {
+        philox_seeded(seed, pos >> 2, slot, sweep, (TAG_WORD << 24) | r, o);
+        u[r] = o[pos & 3u];
+    }
+}
+
+static void uniforms_plane(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t sweep,
+                           uint32_t u[64]) {
+    uint64_t plane[32];
+    uint32_t o[4];
+    for (uint32_t j = 0; j < 16; ++j) {
+        philox_seeded(seed, pos, slot, sweep, (TAG_PLANE << 24) | j, o);
+        plane[2 * j] = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+        plane[2 * j + 1] = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
+    }
+    for (uint32_t r = 0; r < 64; ++r) {
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < 32; ++b) v |= (uint32_t)bit(plane[b], r) << (31u - b);
+        u[r] = v;
+    }
+}
+
+static uint32_t uniform_exchange(uint64_t seed, uint32_t t, uint32_t lane, uint32_t sweep) {
+    uint32_t o[4];
+    philox_seeded(seed, t, lane >> 2, sweep, TAG_EXCH << 24, o);
+    return o[lane & 3u];
+}
+
+/* ------------------------------------------------------------------------- */
+/* The run: metropolis_sweep.rs:95-146 in colour order, then engine.rs:713-818 */
+/* ------------------------------------------------------------------------- */
+int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+           const int64_t *linear, uint32_t cells, uint64_t seed, uint32_t view, uint32_t exchange,
+           uint32_t sweep0, uint32_t phase0, uint32_t sweeps, const double *betas,
+           uint32_t betas_per_sweep, uint64_t *words, uint32_t *temp_of, int64_t *energies,
+           uint64_t *stats) {
+    if (cells == 0 || view > 1) return -1;
+    for (uint32_t i = 0; i < cells * 64u; ++i)
+        if (temp_of[i] >= cells) return -1;
+    uint32_t *colors = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    uint32_t *cstart = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 2));
+    uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
+    or_greedy_coloring(n, row_ptr, col_idx, colors);
+    uint32_t ncol = or_color_order(n, colors, order, cstart);
+    uint32_t u[64];
+
+    for (uint32_t k = 0; k < sweeps; ++k) {
+        uint32_t sweep = sweep0 + k;
+        const double *beta = betas + (betas_per_sweep ? (uint64_t)k * cells : 0);
+        for (uint32_t c = 0; c < ncol; ++c) {
+            for (uint32_t pos = cstart[c]; pos < cstart[c + 1]; ++pos) {
+                uint32_t site = order[pos];
+                for (uint32_t s = 0; s < cells; ++s) {
+                    uint64_t *sw = words + (uint64_t)s * n;
+                    if (view == OR_VIEW_WORD)
+                        uniforms_word(seed, pos, s, sweep, u);
+                    else
+                        uniforms_plane(seed, pos, s, sweep, u);
+                    uint64_t flip = 0;
+                    for (uint32_t r = 0; r < 64; ++r) {
+                        int64_t h = field_of(row_ptr, col_idx, w, linear, sw, site, r);
+                        int64_t de = bit(sw[site], r) ? -h : h;
+                        double b = beta[temp_of[s * 64 + r]];
+                        int accept = (de <= 0) || (u[r] < or_threshold(b, de));
+                        if (accept) {
+                            flip |= 1ull << r;
+                            energies[s * 64 + r] += de;
+                        }
+                    }
+                    sw[site] ^= flip;
+                    stats[0] += (uint64_t)__builtin_popcountll(flip);
+                    stats[1] += 64;
+                }
+            }
+        }
+        if (exchange && cells >= 2) {
+            uint32_t phase = (phase0 + k) & 1u;
+            for (uint32_t s = 0; s < cells; ++s)
+                for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
+            for (uint32_t t = phase; t + 1 < cells; t += 2) {
+                double dbeta = beta[t] - beta[t + 1];
+                for (uint32_t r = 0; r < 64; ++r) {
+                    uint32_t s1 = slot_of[t * 64 + r], s2 = slot_of[(t + 1) * 64 + r];
+                    double delta = dbeta * (double)(energies[s1 * 64 + r] - energies[s2 * 64 + r]);
+                    uint32_t uu = uniform_exchange(seed, t, r, sweep);
+                    int accept = (delta >= 0.0) || (uu < or_exchange_threshold(delta));
+                    stats[2 + 2 * t] += 1;
+                    if (accept) {
+                        stats[3 + 2 * t] += 1;
+                        temp_of[s1 * 64 + r] = t + 1;
+                        temp_of[s2 * 64 + r] = t;
+                        slot_of[t * 64 + r] = s2;
+                        slot_of[(t + 1) * 64 + r] = s1;
+                    }
+                }
+            }
+        }
+    }
+    free(colors);
+    free(order);
+    free(cstart);
+    free(slot_of);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ladders                                                                   */
+/* ------------------------------------------------------------------------- */
+/* Rust's f64::powi lowers to compiler-rt __powidf2 (binary exponentiation). */
+static double powi_rt(double a, int b) {
+    int recip = b < 0;
+    double r = 1.0;
+    while (1) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+/* decision.rs:133-142 */
+void or_geometric_ladder(uint32_t k, double t_hi, double t_lo, double *temps) {
+    double hi = t_hi > t_lo ? t_hi : t_lo;
+    double lo = t_hi < t_lo ? t_hi : t_lo;
+    if (lo < 1e-6) lo = 1e-6;
+    if (k <= 1) {
+        temps[0] = hi;
+        return;
+    }
+    double ratio = pow(lo / hi, 1.0 / ((double)k - 1.0));
+    for (uint32_t i = 0; i < k; ++i) temps[i] = hi * powi_rt(ratio, (int)i);
+}
+
+/* schedule.rs:21-43 over the CSR rows (both triangles, full q). */
+void or_suggested_range(uint32_t n, const uint32_t *row_ptr, const int64_t *w,
+                        const int64_t *linear, double *t_max, double *t_min) {
+    double max_local = 0.0, min_nonzero = INFINITY;
+    for (uint32_t i = 0; i < n; ++i) {
+        double local = fabs((double)linear[i]);
+        if (linear[i] != 0) min_nonzero = fmin(min_nonzero, fabs((double)linear[i]));
+        for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+            local += fabs((double)w[e]);
+            if (w[e] != 0) min_nonzero = fmin(min_nonzero, fabs((double)w[e]));
+        }
+        if (local > max_local) max_local = local;
+    }
+    if (max_local == 0.0 || !isfinite(min_nonzero)) {
+        *t_max = 1.0;
+        *t_min = 0.1;
+        return;
+    }
+    double tmax = max_local / 0.69314718055994530942;
+    double tmin = min_nonzero / log(100.0);
+    if (tmax * 0.5 < tmin) tmin = tmax * 0.5;
+    *t_max = tmax;
+    *t_min = tmin;
+}
