@@ -1,0 +1,179 @@
+"""Benchmark: spin-flip attempts/s of the sweep + replica-exchange hot path on one MI355X.
+
+Workload (default): 3D +-J Edwards-Anderson cubic lattice L=64 periodic
+(N=262144, 6 neighbours), couplings +-1, 32 temperatures geometric T 0.5 -> 2.0,
+64 replicas per temperature, one DEO swap attempt after every sweep.
+A step = one Metropolis sweep of all 32 x 64 replicas + one exchange.
+
+Prints ONE JSON line.  Inputs are resident in HBM before timing starts.
+The roofline figure uses HIP events recorded on the state's own stream
+around the sweep launches (ige_profile_*), averaged per launch.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_attempt(cells: int, deg: float, w_b: int = 1, w_l: int = 1) -> float:
+    """Algorithmic HBM bytes per attempt: per cell-site read+write of the u64 word (16 B),
+    plus the model stream once per site shared by all cells (row pointer 4 B, deg x
+    (4 B column + w_b coupling), w_l linear)."""
+    return (16.0 * cells + 4 + deg * (4 + w_b) + w_l) / (64.0 * cells)
+
+
+def cpu_baseline(qb, cells, ladder, budget_s: float):
+    """The reference's production sweep (engine.rs::step port, OpenMP over cells) on a
+    bounded sample of the same workload."""
+    import ctypes
+    from ctypes import POINTER, c_double, c_int, c_int8, c_uint32, c_uint64
+    from oracle import binding as ob
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "libcpu_baseline.so"))
+    lib.cb_run.argtypes = [c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double),
+                           POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, c_uint64,
+                           c_int, POINTER(c_int8), POINTER(c_double), POINTER(c_int)]
+    lib.cb_run.restype = c_int
+    csr = ob.CSR.from_qubo(qb)
+    n = qb.n
+    rp = csr.row_ptr
+    ci = csr.col_idx
+    w = csr.w.astype(np.float64)
+    lin = csr.linear.astype(np.float64)
+    temps = np.ascontiguousarray(1.0 / np.asarray(ladder, dtype=np.float64))
+    rng = np.random.default_rng(5)
+    spins = rng.integers(0, 2, size=(cells, n, 64), dtype=np.int8)
+    energies = np.zeros((cells, 64), dtype=np.float64)
+    phase = c_int(0)
+    threads = max(1, min(cells, 16, os.cpu_count() or 1))
+    P = lambda a, t: a.ctypes.data_as(POINTER(t))
+    args = lambda steps: (n, P(rp, c_uint32), P(ci, c_uint32), P(w, c_double), P(lin, c_double),
+                          cells, P(temps, c_double), steps, 7, threads, P(spins, c_int8),
+                          P(energies, c_double), ctypes.byref(phase))
+    lib.cb_run(*args(1))  # warm-up step
+    t0 = time.perf_counter()
+    lib.cb_run(*args(1))
+    one = time.perf_counter() - t0
+    steps = max(1, int(budget_s / max(one, 1e-6)))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        lib.cb_run(*args(max(1, steps // 3)))
+        times.append((time.perf_counter() - t0) / max(1, steps // 3))
+    per_step = float(np.median(times))
+    attempts = n * 64 * cells
+    return dict(value=attempts / per_step, unit="spin-flip attempts/s", cores=threads,
+                kind="port",
+                sample=f"engine.rs::step port (i8 lanes, f64 dE, xoshiro256++x8, fast_exp, DEO), "
+                       f"same 3D L={round(n ** (1 / 3))} {cells}-temperature workload, "
+                       f"{3 * max(1, steps // 3)} steps, median of 3 repetitions")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--temps", type=int, default=32)
+    ap.add_argument("--kernel", choices=["auto", "generic", "multispin"], default="auto")
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    from ising_engine_amd import _lib, instances
+
+    kernel = {"auto": _lib.KERNEL_AUTO, "generic": _lib.KERNEL_GENERIC,
+              "multispin": _lib.KERNEL_MULTISPIN}[args.kernel]
+    wl = instances.workload_3d_pt(L=args.L, temps=args.temps)
+    qb, cells, ladder = wl["qubo"], wl["cells"], wl["ladder"]
+    log(f"[bench] building {wl['name']}: n={qb.n} bonds={len(qb.q)} cells={cells}")
+    ctx = _lib.Context(0)
+    model = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+    st = _lib.State(model, cells, 0x5EED, kernel=kernel)
+    st.set_ladder(ladder)
+    kname = {1: "generic", 2: "multispin"}[st.stats()[0].kernel]
+    log(f"[bench] kernel={kname} colours={model.info.num_colors}")
+
+    st.run(args.warmup, exchange=True)
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.run(args.steps, exchange=True)
+    st.synchronize()
+    dt = time.perf_counter() - t0
+    attempts = args.steps * qb.n * 64 * cells
+    value = attempts / dt
+    log(f"[bench] {args.steps} steps in {dt:.3f} s -> {value:.4e} attempts/s")
+
+    # Parity guard on the timed state: tracked energies equal a recompute.
+    if not np.array_equal(st.energies(), st.recompute_energies()):
+        raise SystemExit("tracked energies drifted from recompute")
+
+    # Per-launch kernel timing with HIP events on the state's stream.
+    st.profile(True)
+    st.run(args.profile_steps, exchange=True)
+    st.synchronize()
+    ms, launches, ms_x, n_x = st.profile_read()
+    st.profile(False)
+    avg_launch_s = ms / 1e3 / launches
+    attempts_per_launch = qb.n * 64 * cells / model.info.num_colors
+    deg = 2.0 * len(qb.q) / qb.n
+    bpa = bytes_per_attempt(cells, deg)
+    achieved = attempts_per_launch * bpa / avg_launch_s / 1e9
+    log(f"[bench] sweep launch avg {avg_launch_s * 1e6:.1f} us over {launches} launches; "
+        f"exchange avg {ms_x / max(n_x, 1) * 1e3:.1f} us; {bpa:.4f} B/attempt")
+    roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                    kernel=f"sweep_{kname}_kernel",
+                    avg_launch_us=round(avg_launch_s * 1e6, 2),
+                    bytes_per_attempt=round(bpa, 4))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(qb, cells, ladder, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu['value']:.4e} attempts/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "spin-flip attempts/s (sweep + DEO replica exchange), 1 MI355X",
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "L": args.L, "n": qb.n, "temperatures": cells,
+                   "replicas_per_temperature": 64, "exchange": "DEO every sweep",
+                   "kernel": kname, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

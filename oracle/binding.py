@@ -59,6 +59,8 @@ def load():
                            c_uint32, c_uint32, c_uint32, c_uint32, dp, c_uint32, u64p, u32p,
                            i64p, u64p]
     lib.or_run.restype = c_int
+    lib.or_exchange.argtypes = [c_uint32, c_uint64, c_uint32, c_uint32, dp, i64p, u32p, u64p]
+    lib.or_exchange.restype = None
     lib.or_geometric_ladder.argtypes = [c_uint32, c_double, c_double, dp]
     lib.or_geometric_ladder.restype = None
     lib.or_suggested_range.argtypes = [c_uint32, u32p, i64p, i64p, dp, dp]
@@ -201,6 +203,14 @@ class OracleState:
         self.sweep += sweeps
         if exchange and self.cells >= 2:
             self.phase = (self.phase + sweeps) & 1
+
+
+def exchange(cells, seed, sweep, phase, beta, energies, temp_of, stats):
+    """One DEO exchange in place on temp_of / stats."""
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    energies = np.ascontiguousarray(energies, dtype=np.int64)
+    load().or_exchange(cells, seed & 0xFFFFFFFFFFFFFFFF, sweep, phase, _p(beta, c_double),
+                       _p(energies, c_int64), _p(temp_of, c_uint32), _p(stats, c_uint64))
 
 
 def geometric_ladder(k, t_hi, t_lo):

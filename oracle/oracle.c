@@ -292,6 +292,36 @@ static uint32_t uniform_exchange(uint64_t seed, uint32_t t, uint32_t lane, uint3
     return o[lane & 3u];
 }
 
+/* engine.rs:713-789 + :818: pairs (t, t+1) with t = phase mod 2; lane r of
+ * temperature t against lane r of temperature t+1; accept iff
+ * delta = (beta_t - beta_{t+1}) (E_t - E_{t+1}) >= 0 or u < exp(delta).  The
+ * configurations stay in their slots and the two lanes swap temperatures. */
+void or_exchange(uint32_t cells, uint64_t seed, uint32_t sweep, uint32_t phase,
+                 const double *beta, const int64_t *energies, uint32_t *temp_of,
+                 uint64_t *stats) {
+    uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
+    for (uint32_t t = phase; t + 1 < cells; t += 2) {
+        double dbeta = beta[t] - beta[t + 1];
+        for (uint32_t r = 0; r < 64; ++r) {
+            uint32_t s1 = slot_of[t * 64 + r], s2 = slot_of[(t + 1) * 64 + r];
+            double delta = dbeta * (double)(energies[s1 * 64 + r] - energies[s2 * 64 + r]);
+            uint32_t uu = uniform_exchange(seed, t, r, sweep);
+            int accept = (delta >= 0.0) || (uu < or_exchange_threshold(delta));
+            stats[2 + 2 * t] += 1;
+            if (accept) {
+                stats[3 + 2 * t] += 1;
+                temp_of[s1 * 64 + r] = t + 1;
+                temp_of[s2 * 64 + r] = t;
+                slot_of[t * 64 + r] = s2;
+                slot_of[(t + 1) * 64 + r] = s1;
+            }
+        }
+    }
+    free(slot_of);
+}
+
 /* ------------------------------------------------------------------------- */
 /* The run: metropolis_sweep.rs:95-146 in colour order, then engine.rs:713-818 */
 /* ------------------------------------------------------------------------- */
@@ -306,7 +336,6 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
     uint32_t *colors = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
     uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
     uint32_t *cstart = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 2));
-    uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
     or_greedy_coloring(n, row_ptr, col_idx, colors);
     uint32_t ncol = or_color_order(n, colors, order, cstart);
     uint32_t u[64];
@@ -340,33 +369,12 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
                 }
             }
         }
-        if (exchange && cells >= 2) {
-            uint32_t phase = (phase0 + k) & 1u;
-            for (uint32_t s = 0; s < cells; ++s)
-                for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
-            for (uint32_t t = phase; t + 1 < cells; t += 2) {
-                double dbeta = beta[t] - beta[t + 1];
-                for (uint32_t r = 0; r < 64; ++r) {
-                    uint32_t s1 = slot_of[t * 64 + r], s2 = slot_of[(t + 1) * 64 + r];
-                    double delta = dbeta * (double)(energies[s1 * 64 + r] - energies[s2 * 64 + r]);
-                    uint32_t uu = uniform_exchange(seed, t, r, sweep);
-                    int accept = (delta >= 0.0) || (uu < or_exchange_threshold(delta));
-                    stats[2 + 2 * t] += 1;
-                    if (accept) {
-                        stats[3 + 2 * t] += 1;
-                        temp_of[s1 * 64 + r] = t + 1;
-                        temp_of[s2 * 64 + r] = t;
-                        slot_of[t * 64 + r] = s2;
-                        slot_of[(t + 1) * 64 + r] = s1;
-                    }
-                }
-            }
-        }
+        if (exchange && cells >= 2)
+            or_exchange(cells, seed, sweep, (phase0 + k) & 1u, beta, energies, temp_of, stats);
     }
     free(colors);
     free(order);
     free(cstart);
-    free(slot_of);
     return 0;
 }
 

@@ -94,6 +94,11 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
            uint32_t sweeps, const double *betas, uint32_t betas_per_sweep,
            uint64_t *words, uint32_t *temp_of, int64_t *energies, uint64_t *stats);
 
+/* One DEO exchange (engine.rs:713-818) on tracked energies; stats as or_run. */
+void or_exchange(uint32_t cells, uint64_t seed, uint32_t sweep, uint32_t phase,
+                 const double *beta, const int64_t *energies, uint32_t *temp_of,
+                 uint64_t *stats);
+
 /* ---- ladders (reference helpers) ---- */
 void or_geometric_ladder(uint32_t k, double t_hi, double t_lo, double *temps);
 void or_suggested_range(uint32_t n, const uint32_t *row_ptr, const int64_t *w,
