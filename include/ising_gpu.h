@@ -24,6 +24,8 @@
  *                                                      src/solver/ultimate.rs:271-275
  *   ige_run                 MetropolisSweep::apply     src/engine_v2/operators/metropolis_sweep.rs:95-146
  *                           engine::step (sweep + DEO) src/solver/engine.rs:573-819
+ *   ige_exchange            ReplicaExchange::apply     src/engine_v2/operators/replica_exchange.rs:61-109
+ *                           (DEO schedule of engine.rs:713-818)
  *   ige_read_energies       SpinState::energies_into   src/engine_v2/backends/sparse_bitslice.rs:273-278
  *   ige_read_fields         SparseBitSlice fields ledger
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:26-27
@@ -73,10 +75,11 @@ typedef struct {
 } ige_model_info;
 
 typedef struct {
-    uint64_t sweeps;   /* sweeps run on this state                      */
-    uint64_t accepted; /* accepted single-spin proposals                */
+    uint64_t sweeps;    /* sweeps run on this state                      */
+    uint64_t exchanges; /* DEO exchange rounds run on this state         */
+    uint64_t accepted;  /* accepted single-spin proposals                */
     uint64_t proposed; /* proposals (= sweeps * n * 64 * cells)         */
-    uint32_t phase;    /* DEO phase of the next exchange (0 even, 1 odd) */
+    uint32_t phase;    /* DEO phase of the next exchange (= exchanges & 1) */
     uint32_t kernel;   /* IGE_KERNEL_GENERIC or IGE_KERNEL_MULTISPIN    */
 } ige_stats;
 
@@ -106,6 +109,10 @@ int32_t ige_set_ladder(ige_state *state, const double *beta);
 /* beta_schedule: NULL (use the ladder) or [sweeps][cells]; the ladder is left
  * at the last row.  Asynchronous on the state's stream. */
 int32_t ige_run(ige_state *state, uint32_t sweeps, uint32_t flags, const double *beta_schedule);
+/* `rounds` DEO exchanges without sweeps, on the current ladder and tracked
+ * energies.  Exchange number k uses pairs (t, t+1) with t = k mod 2 and is
+ * keyed on k, whether it runs here or after a sweep in ige_run. */
+int32_t ige_exchange(ige_state *state, uint32_t rounds);
 int32_t ige_synchronize(ige_state *state);
 
 int32_t ige_read_spins(ige_state *state, uint64_t *words /* [cells][n] */);

@@ -30,7 +30,7 @@ EXPORTED = (
     "ige_context_create", "ige_context_destroy", "ige_last_error", "ige_version",
     "ige_model_from_pairs", "ige_model_destroy", "ige_model_get_info",
     "ige_model_coloring", "ige_model_csr", "ige_state_create", "ige_state_destroy",
-    "ige_set_ladder", "ige_run", "ige_synchronize", "ige_read_spins",
+    "ige_set_ladder", "ige_run", "ige_exchange", "ige_synchronize", "ige_read_spins",
     "ige_read_energies", "ige_read_temp_of", "ige_read_fields",
     "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
     "ige_profile_read",
@@ -48,7 +48,8 @@ class ModelInfo(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [
-        ("sweeps", c_uint64), ("accepted", c_uint64), ("proposed", c_uint64),
+        ("sweeps", c_uint64), ("exchanges", c_uint64), ("accepted", c_uint64),
+        ("proposed", c_uint64),
         ("phase", c_uint32), ("kernel", c_uint32),
     ]
 
@@ -95,6 +96,7 @@ def load() -> ctypes.CDLL:
         "ige_state_destroy": (c_int32, [vp]),
         "ige_set_ladder": (c_int32, [vp, POINTER(c_double)]),
         "ige_run": (c_int32, [vp, c_uint32, c_uint32, POINTER(c_double)]),
+        "ige_exchange": (c_int32, [vp, c_uint32]),
         "ige_synchronize": (c_int32, [vp]),
         "ige_read_spins": (c_int32, [vp, POINTER(c_uint64)]),
         "ige_read_energies": (c_int32, [vp, POINTER(c_int64)]),
@@ -219,6 +221,9 @@ class State:
         self._sched_keepalive = sched
         check(self._lib.ige_run(self.handle, int(sweeps), RUN_EXCHANGE if exchange else 0,
                                 _ptr(sched, c_double)))
+
+    def exchange(self, rounds: int) -> None:
+        check(self._lib.ige_exchange(self.handle, int(rounds)))
 
     def synchronize(self) -> None:
         check(self._lib.ige_synchronize(self.handle))

@@ -128,8 +128,8 @@ struct ige_state {
     uint64_t seed = 0;
     uint32_t kernel = IGE_KERNEL_GENERIC;
     hipStream_t stream = nullptr;
-    uint32_t sweep = 0;  // sweeps run so far: the counter the uniforms are keyed on
-    uint32_t phase = 0;
+    uint32_t sweep = 0;   // sweeps run so far: the counter sweep uniforms are keyed on
+    uint32_t xround = 0;  // exchanges run so far: keys exchange uniforms, phase = xround & 1
     uint32_t dtab = 0;
     bool ladder_set = false;
     std::vector<double> ladder;
@@ -385,8 +385,9 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         require(s != nullptr, "NULL state");
         require(sched != nullptr || s->ladder_set, "no ladder set and no beta schedule given");
         require((flags & ~IGE_RUN_EXCHANGE) == 0, "unknown flag");
-        require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull,
-                "sweep counter would exceed 2^32-1");
+        require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull &&
+                    static_cast<uint64_t>(s->xround) + sweeps <= 0xFFFFFFFFull,
+                "sweep or exchange counter would exceed 2^32-1");
         set_device(s->ctx);
         const bool exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
         DevBuf<double> sched_dev;
@@ -428,8 +429,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             }
             if (exch) {
                 if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
-                check(ige::launch_exchange(st, beta, s->phase, s->sweep, s->seed, s->stream),
-                      "exchange");
+                check(ige::launch_exchange(st, beta, s->xround, s->seed, s->stream), "exchange");
                 if (s->profile) {
                     check(hipEventRecord(s->ev1, s->stream), "event");
                     check(hipEventSynchronize(s->ev1), "event sync");
@@ -438,7 +438,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                     s->prof_other_ms += ms;
                     s->prof_other_launches += 1;
                 }
-                s->phase ^= 1u;
+                s->xround += 1;
             }
             s->sweep += 1;
         }
@@ -455,6 +455,23 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         } else if (s->kernel == IGE_KERNEL_MULTISPIN && exch && sweeps > 0) {
             refresh_thresholds(s, s->beta.p);
         }
+    });
+}
+
+int32_t ige_exchange(ige_state* s, uint32_t rounds) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        require(s->ladder_set, "no ladder set");
+        require(static_cast<uint64_t>(s->xround) + rounds <= 0xFFFFFFFFull,
+                "exchange counter would exceed 2^32-1");
+        if (s->cells < 2 || rounds == 0) return;
+        set_device(s->ctx);
+        auto st = s->view();
+        for (uint32_t k = 0; k < rounds; ++k) {
+            check(ige::launch_exchange(st, s->beta.p, s->xround, s->seed, s->stream), "exchange");
+            s->xround += 1;
+        }
+        if (s->kernel == IGE_KERNEL_MULTISPIN) refresh_thresholds(s, s->beta.p);
     });
 }
 
@@ -533,9 +550,10 @@ int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
         s->counters.download(c.data(), c.size(), s->stream);
         check(hipStreamSynchronize(s->stream), "read stats");
         stats->sweeps = s->sweep;
+        stats->exchanges = s->xround;
         stats->accepted = c[0];
         stats->proposed = static_cast<uint64_t>(s->sweep) * s->model->ir.n * 64ull * s->cells;
-        stats->phase = s->phase;
+        stats->phase = s->xround & 1u;
         stats->kernel = s->kernel;
         for (uint32_t t = 0; t + 1 < s->cells; ++t) {
             if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];

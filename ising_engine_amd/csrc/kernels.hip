@@ -114,21 +114,22 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// DEO replica exchange: one wave per adjacent pair (t, t+1), t = phase mod 2;
-// lane r decides for replica lane r and swaps temperature indices.
+// DEO replica exchange number `round`: one wave per adjacent pair (t, t+1)
+// with t = round mod 2; lane r decides for replica lane r and swaps
+// temperature indices.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void exchange_kernel(
     const double* __restrict__ beta, const int64_t* __restrict__ energy, uint32_t* temp_of,
-    uint32_t* slot_of, unsigned long long* pair_stats, uint32_t cells, uint32_t phase,
-    uint32_t sweep, uint32_t key0, uint32_t key1) {
+    uint32_t* slot_of, unsigned long long* pair_stats, uint32_t cells, uint32_t round,
+    uint32_t key0, uint32_t key1) {
     const uint32_t r = threadIdx.x;
-    const uint32_t t = phase + 2u * blockIdx.x;
+    const uint32_t t = (round & 1u) + 2u * blockIdx.x;
     if (t + 1 >= cells) return;
     const uint32_t s1 = slot_of[t * kWave + r];
     const uint32_t s2 = slot_of[(t + 1) * kWave + r];
     const double delta = (beta[t] - beta[t + 1]) *
                          (double)(energy[s1 * kWave + r] - energy[s2 * kWave + r]);
-    const U32x4 u = philox4x32_10(t, r >> 2, sweep, kTagExch << 24, key0, key1);
+    const U32x4 u = philox4x32_10(t, r >> 2, round, kTagExch << 24, key0, key1);
     const uint32_t q = r & 3u;
     const uint32_t uu = q == 0 ? u.v[0] : q == 1 ? u.v[1] : q == 2 ? u.v[2] : u.v[3];
     const bool acc = delta >= 0.0 || uu < exchange_threshold(delta);
@@ -454,13 +455,14 @@ hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* be
     return hipGetLastError();
 }
 
-hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t phase, uint32_t sweep,
-                           uint64_t seed, hipStream_t stream) {
+hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round, uint64_t seed,
+                           hipStream_t stream) {
     if (s.cells < 2) return hipSuccess;
+    const uint32_t phase = round & 1u;
     const uint32_t pairs = (s.cells - phase) / 2;  // t = phase, phase+2, ... with t+1 < cells
     if (pairs == 0) return hipSuccess;
     hipLaunchKernelGGL(exchange_kernel, dim3(pairs), dim3(kWave), 0, stream, beta, s.energy,
-                       s.temp_of, s.slot_of, s.pair_stats, s.cells, phase, sweep, (uint32_t)seed,
+                       s.temp_of, s.slot_of, s.pair_stats, s.cells, round, (uint32_t)seed,
                        (uint32_t)(seed >> 32));
     return hipGetLastError();
 }

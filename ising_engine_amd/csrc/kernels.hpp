@@ -53,9 +53,9 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
 hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
                           unsigned long long* qplanes, hipStream_t stream);
 
-// DEO exchange of the pairs (t, t+1) with t = phase mod 2.
-hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t phase, uint32_t sweep,
-                           uint64_t seed, hipStream_t stream);
+// DEO exchange number `round`: pairs (t, t+1) with t = round mod 2.
+hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round, uint64_t seed,
+                           hipStream_t stream);
 
 hipError_t launch_init_spins(const DevModel& m, const DevState& s, uint64_t seed,
                              hipStream_t stream);

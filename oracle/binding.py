@@ -59,7 +59,7 @@ def load():
                            c_uint32, c_uint32, c_uint32, c_uint32, dp, c_uint32, u64p, u32p,
                            i64p, u64p]
     lib.or_run.restype = c_int
-    lib.or_exchange.argtypes = [c_uint32, c_uint64, c_uint32, c_uint32, dp, i64p, u32p, u64p]
+    lib.or_exchange.argtypes = [c_uint32, c_uint64, c_uint32, dp, i64p, u32p, u64p]
     lib.or_exchange.restype = None
     lib.or_geometric_ladder.argtypes = [c_uint32, c_double, c_double, dp]
     lib.or_geometric_ladder.restype = None
@@ -182,7 +182,7 @@ class OracleState:
         self.temp_of = np.repeat(np.arange(self.cells, dtype=np.uint32)[:, None], 64, axis=1).copy()
         self.energy = csr.energies(self.words)
         self.sweep = 0
-        self.phase = 0
+        self.xround = 0
         self.stats = np.zeros(2 + 2 * max(self.cells - 1, 0), dtype=np.uint64)
 
     def run(self, sweeps: int, exchange: bool, ladder=None, schedule=None):
@@ -194,7 +194,7 @@ class OracleState:
             per = 0
         n, rp, ci, w = self.csr._args()
         rc = load().or_run(n, rp, ci, w, _p(self.csr.linear, c_int64), self.cells, self.seed,
-                           self.view, 1 if exchange else 0, self.sweep, self.phase, sweeps,
+                           self.view, 1 if exchange else 0, self.sweep, self.xround, sweeps,
                            _p(betas, c_double), per, _p(self.words, c_uint64),
                            _p(self.temp_of, c_uint32), _p(self.energy, c_int64),
                            _p(self.stats, c_uint64))
@@ -202,14 +202,23 @@ class OracleState:
             raise ValueError("or_run rejected its arguments")
         self.sweep += sweeps
         if exchange and self.cells >= 2:
-            self.phase = (self.phase + sweeps) & 1
+            self.xround += sweeps
+
+    def exchange(self, rounds: int, ladder):
+        """Exchange-only rounds (ReplicaExchange::apply) on the tracked energies."""
+        if self.cells < 2:
+            return
+        for _ in range(rounds):
+            exchange(self.cells, self.seed, self.xround, ladder, self.energy, self.temp_of,
+                     self.stats)
+            self.xround += 1
 
 
-def exchange(cells, seed, sweep, phase, beta, energies, temp_of, stats):
-    """One DEO exchange in place on temp_of / stats."""
+def exchange(cells, seed, round_, beta, energies, temp_of, stats):
+    """Exchange number round_ (phase round_ & 1) in place on temp_of / stats."""
     beta = np.ascontiguousarray(beta, dtype=np.float64)
     energies = np.ascontiguousarray(energies, dtype=np.int64)
-    load().or_exchange(cells, seed & 0xFFFFFFFFFFFFFFFF, sweep, phase, _p(beta, c_double),
+    load().or_exchange(cells, seed & 0xFFFFFFFFFFFFFFFF, round_, _p(beta, c_double),
                        _p(energies, c_int64), _p(temp_of, c_uint32), _p(stats, c_uint64))
 
 

@@ -286,19 +286,20 @@ static void uniforms_plane(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t 
     }
 }
 
-static uint32_t uniform_exchange(uint64_t seed, uint32_t t, uint32_t lane, uint32_t sweep) {
+static uint32_t uniform_exchange(uint64_t seed, uint32_t t, uint32_t lane, uint32_t round) {
     uint32_t o[4];
-    philox_seeded(seed, t, lane >> 2, sweep, TAG_EXCH << 24, o);
+    philox_seeded(seed, t, lane >> 2, round, TAG_EXCH << 24, o);
     return o[lane & 3u];
 }
 
-/* engine.rs:713-789 + :818: pairs (t, t+1) with t = phase mod 2; lane r of
+/* engine.rs:713-789 + :818: exchange number `round` uses the pairs (t, t+1)
+ * with t = round mod 2 (the phase toggles after every exchange); lane r of
  * temperature t against lane r of temperature t+1; accept iff
  * delta = (beta_t - beta_{t+1}) (E_t - E_{t+1}) >= 0 or u < exp(delta).  The
  * configurations stay in their slots and the two lanes swap temperatures. */
-void or_exchange(uint32_t cells, uint64_t seed, uint32_t sweep, uint32_t phase,
-                 const double *beta, const int64_t *energies, uint32_t *temp_of,
-                 uint64_t *stats) {
+void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *beta,
+                 const int64_t *energies, uint32_t *temp_of, uint64_t *stats) {
+    const uint32_t phase = round & 1u;
     uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
     for (uint32_t s = 0; s < cells; ++s)
         for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
@@ -307,7 +308,7 @@ void or_exchange(uint32_t cells, uint64_t seed, uint32_t sweep, uint32_t phase,
         for (uint32_t r = 0; r < 64; ++r) {
             uint32_t s1 = slot_of[t * 64 + r], s2 = slot_of[(t + 1) * 64 + r];
             double delta = dbeta * (double)(energies[s1 * 64 + r] - energies[s2 * 64 + r]);
-            uint32_t uu = uniform_exchange(seed, t, r, sweep);
+            uint32_t uu = uniform_exchange(seed, t, r, round);
             int accept = (delta >= 0.0) || (uu < or_exchange_threshold(delta));
             stats[2 + 2 * t] += 1;
             if (accept) {
@@ -327,7 +328,7 @@ void or_exchange(uint32_t cells, uint64_t seed, uint32_t sweep, uint32_t phase,
 /* ------------------------------------------------------------------------- */
 int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
            const int64_t *linear, uint32_t cells, uint64_t seed, uint32_t view, uint32_t exchange,
-           uint32_t sweep0, uint32_t phase0, uint32_t sweeps, const double *betas,
+           uint32_t sweep0, uint32_t xround0, uint32_t sweeps, const double *betas,
            uint32_t betas_per_sweep, uint64_t *words, uint32_t *temp_of, int64_t *energies,
            uint64_t *stats) {
     if (cells == 0 || view > 1) return -1;
@@ -370,7 +371,7 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
             }
         }
         if (exchange && cells >= 2)
-            or_exchange(cells, seed, sweep, (phase0 + k) & 1u, beta, energies, temp_of, stats);
+            or_exchange(cells, seed, xround0 + k, beta, energies, temp_of, stats);
     }
     free(colors);
     free(order);

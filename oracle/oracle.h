@@ -79,7 +79,8 @@ enum { OR_VIEW_WORD = 0, OR_VIEW_PLANE = 1 };
 
 /*
  * Run `sweeps` colour-ordered Metropolis sweeps over `cells` slots of 64
- * lanes, optionally followed by one DEO exchange per sweep.
+ * lanes, optionally followed by one DEO exchange per sweep; sweep0 and
+ * xround0 are the sweep and exchange counters the uniforms are keyed on.
  *   words      in/out [cells][n], site order; bit r = lane r
  *   temp_of    in/out [cells][64], temperature index held by (slot, lane)
  *   energies   in/out [cells][64], tracked int64 energies
@@ -90,14 +91,14 @@ enum { OR_VIEW_WORD = 0, OR_VIEW_PLANE = 1 };
  */
 int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
            const int64_t *w, const int64_t *linear, uint32_t cells, uint64_t seed,
-           uint32_t view, uint32_t exchange, uint32_t sweep0, uint32_t phase0,
+           uint32_t view, uint32_t exchange, uint32_t sweep0, uint32_t xround0,
            uint32_t sweeps, const double *betas, uint32_t betas_per_sweep,
            uint64_t *words, uint32_t *temp_of, int64_t *energies, uint64_t *stats);
 
-/* One DEO exchange (engine.rs:713-818) on tracked energies; stats as or_run. */
-void or_exchange(uint32_t cells, uint64_t seed, uint32_t sweep, uint32_t phase,
-                 const double *beta, const int64_t *energies, uint32_t *temp_of,
-                 uint64_t *stats);
+/* Exchange number `round` (phase = round & 1; engine.rs:713-818) on tracked
+ * energies; stats as or_run. */
+void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *beta,
+                 const int64_t *energies, uint32_t *temp_of, uint64_t *stats);
 
 /* ---- ladders (reference helpers) ---- */
 void or_geometric_ladder(uint32_t k, double t_hi, double t_lo, double *temps);

@@ -59,7 +59,7 @@ CASES = [
      _lib.KERNEL_MULTISPIN, ob.VIEW_PLANE, True, 10, False),
     ("er_n300_d12_multispin", lambda: instances.erdos_renyi_pm_j(300, 12, 8), 1,
      _lib.KERNEL_MULTISPIN, ob.VIEW_PLANE, False, 8, False),
-    ("er_n200_d22_multispin", lambda: instances.erdos_renyi_pm_j(200, 22, 9), 1,
+    ("er_n200_d18_multispin", lambda: instances.erdos_renyi_pm_j(200, 18, 10), 1,
      _lib.KERNEL_MULTISPIN, ob.VIEW_PLANE, False, 6, False),
 ]
 
@@ -178,3 +178,25 @@ def test_errors_are_reported(ctx):
     with pytest.raises(_lib.IgeError) as ei:
         make(ctx, big)
     assert ei.value.code == _lib.IGE_ERR_RANGE
+
+
+@pytest.mark.parametrize("kernel,view", [(_lib.KERNEL_MULTISPIN, ob.VIEW_PLANE),
+                                         (_lib.KERNEL_GENERIC, ob.VIEW_WORD)])
+def test_exchange_only_rounds_interleaved(ctx, kernel, view):
+    """ReplicaExchange::apply as its own operator (replica_exchange.rs:61-109): exchange-only
+    rounds between sweeps continue the same DEO phase and keys as fused exchanges."""
+    qb = instances.hypercubic_pm_j(4, 3, 21)
+    csr = ob.CSR.from_qubo(qb)
+    model = make(ctx, qb)
+    ladder = instances.geometric_betas(2.5, 0.3, 6)
+    st = _lib.State(model, 6, 31, kernel=kernel)
+    ost = ob.OracleState(csr, 6, 31, view=view)
+    st.set_ladder(ladder)
+    for sweeps, rounds, fused in [(3, 2, True), (2, 3, False), (4, 1, True), (1, 5, False)]:
+        st.run(sweeps, exchange=fused)
+        ost.run(sweeps, exchange=fused, ladder=ladder)
+        st.exchange(rounds)
+        ost.exchange(rounds, ladder)
+        compare(st, ost, csr, fields=False)
+    s = st.stats()[0]
+    assert s.exchanges == ost.xround and s.phase == ost.xround & 1

@@ -263,7 +263,7 @@ def test_exchange_exact_rates():
     temp_of = np.array([[0] * 64, [1] * 64], dtype=np.uint32)
     stats = np.zeros(4, dtype=np.uint64)
     e = np.array([[0] * 64, [10] * 64], dtype=np.int64)  # hot E=0, cold E=10
-    ob.exchange(2, 1, 0, 0, beta, e, temp_of, stats)
+    ob.exchange(2, 1, 0, beta, e, temp_of, stats)
     assert stats[3] == 64 and np.all(temp_of == [[1] * 64, [0] * 64])
     acc = 0
     tries = 0
@@ -272,7 +272,7 @@ def test_exchange_exact_rates():
     for sweep in range(400):
         temp_of = np.array([[0] * 64, [1] * 64], dtype=np.uint32)
         stats = np.zeros(4, dtype=np.uint64)
-        ob.exchange(2, 7, sweep, 0, beta, e, temp_of, stats)
+        ob.exchange(2, 7, 2 * sweep, beta, e, temp_of, stats)
         acc += int(stats[3])
         tries += int(stats[2])
     p = math.exp(-1.0)
