@@ -86,7 +86,7 @@ def cpu_baseline(qb, cells, ladder, budget_s: float):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--temps", type=int, default=32)

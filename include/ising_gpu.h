@@ -123,6 +123,48 @@ int32_t ige_recompute_energies(ige_state *state, int64_t *energies /* [cells][64
 int32_t ige_read_stats(ige_state *state, ige_stats *stats, uint64_t *pair_attempts,
                        uint64_t *pair_accepts /* cells-1 each, or NULL */);
 
+/* ---- batches: thousands of independent instances in one persistent launch ---- */
+typedef struct ige_batch ige_batch;
+
+typedef struct {
+    uint32_t n;
+    int64_t offset;
+    const int32_t *linear; /* n                                    */
+    uint64_t npairs;
+    const uint32_t *pi;    /* npairs, pi < pj                      */
+    const uint32_t *pj;
+    const int32_t *q;
+    uint64_t seed;
+    uint32_t sweeps;       /* anneal length                        */
+    double beta_lo;        /* schedule of ige_anneal_schedule      */
+    double beta_hi;
+} ige_instance_desc;
+
+#define IGE_BATCH_PERSISTENT 0u /* one launch, device work queue           */
+#define IGE_BATCH_PLAIN 1u      /* per instance, per sweep, per colour launches */
+
+/* beta_k = beta_hi * exp(-(sweeps-1-k) c), c = ln(beta_hi/beta_lo)/(sweeps-1),
+ * exp evaluated deterministically; geometric from beta_lo to beta_hi.  A batch
+ * instance follows exactly this schedule, so ige_run with it on a one-cell
+ * state of the same seed reproduces the instance bit for bit. */
+int32_t ige_anneal_schedule(double beta_lo, double beta_hi, uint32_t sweeps, double *beta_out);
+int32_t ige_batch_create(ige_ctx *ctx, uint32_t count, const ige_instance_desc *desc,
+                         ige_batch **out);
+int32_t ige_batch_destroy(ige_batch *batch);
+/* Back to the seeded initial spins, exact energies, zero counters. */
+int32_t ige_batch_reset(ige_batch *batch);
+/* Sweeps [k0, k1) of every instance's schedule (clamped to its length).
+ * waves: workgroup size in wavefronts per instance (1, 2, 4, 8), 0 = tuned. */
+int32_t ige_batch_run(ige_batch *batch, uint32_t mode, uint32_t waves, uint32_t k0, uint32_t k1);
+/* Times the persistent kernel at 1, 2, 4, 8 waves per instance on
+ * `probe_sweeps` sweeps, keeps the fastest, then resets the batch. */
+int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *waves_out,
+                           double *ms_out /* 4 entries, or NULL */);
+int32_t ige_batch_synchronize(ige_batch *batch);
+int32_t ige_batch_info(const ige_batch *batch, uint64_t *total_sites, uint64_t *total_attempts);
+int32_t ige_batch_read(ige_batch *batch, uint32_t index, uint64_t *words /* n, site order */,
+                       int64_t *energies /* 64 */, uint64_t *accepted);
+
 /* Per-launch timing of the sweep kernels with HIP events on the state's
  * stream (off by default).  ms_total is the summed duration of `launches`
  * sweep-kernel launches since profiling was enabled. */

@@ -33,8 +33,13 @@ EXPORTED = (
     "ige_set_ladder", "ige_run", "ige_exchange", "ige_synchronize", "ige_read_spins",
     "ige_read_energies", "ige_read_temp_of", "ige_read_fields",
     "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
-    "ige_profile_read",
+    "ige_profile_read", "ige_anneal_schedule", "ige_batch_create", "ige_batch_destroy",
+    "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
+    "ige_batch_info", "ige_batch_read",
 )
+
+BATCH_PERSISTENT = 0
+BATCH_PLAIN = 1
 
 
 class ModelInfo(ctypes.Structure):
@@ -51,6 +56,15 @@ class Stats(ctypes.Structure):
         ("sweeps", c_uint64), ("exchanges", c_uint64), ("accepted", c_uint64),
         ("proposed", c_uint64),
         ("phase", c_uint32), ("kernel", c_uint32),
+    ]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", c_uint32), ("offset", c_int64), ("linear", POINTER(c_int32)),
+        ("npairs", c_uint64), ("pi", POINTER(c_uint32)), ("pj", POINTER(c_uint32)),
+        ("q", POINTER(c_int32)), ("seed", c_uint64), ("sweeps", c_uint32),
+        ("beta_lo", c_double), ("beta_hi", c_double),
     ]
 
 
@@ -105,6 +119,16 @@ def load() -> ctypes.CDLL:
         "ige_recompute_energies": (c_int32, [vp, POINTER(c_int64)]),
         "ige_read_stats": (c_int32, [vp, POINTER(Stats), POINTER(c_uint64), POINTER(c_uint64)]),
         "ige_profile_enable": (c_int32, [vp, c_int32]),
+        "ige_anneal_schedule": (c_int32, [c_double, c_double, c_uint32, POINTER(c_double)]),
+        "ige_batch_create": (c_int32, [vp, c_uint32, POINTER(InstanceDesc), POINTER(vp)]),
+        "ige_batch_destroy": (c_int32, [vp]),
+        "ige_batch_reset": (c_int32, [vp]),
+        "ige_batch_run": (c_int32, [vp, c_uint32, c_uint32, c_uint32, c_uint32]),
+        "ige_batch_autotune": (c_int32, [vp, c_uint32, POINTER(c_uint32), POINTER(c_double)]),
+        "ige_batch_synchronize": (c_int32, [vp]),
+        "ige_batch_info": (c_int32, [vp, POINTER(c_uint64), POINTER(c_uint64)]),
+        "ige_batch_read": (c_int32, [vp, c_uint32, POINTER(c_uint64), POINTER(c_int64),
+                                     POINTER(c_uint64)]),
         "ige_profile_read": (c_int32, [vp, POINTER(c_double), POINTER(c_uint64),
                                        POINTER(c_double), POINTER(c_uint64)]),
     }
@@ -276,6 +300,85 @@ class State:
     def close(self):
         if self.handle:
             self._lib.ige_state_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def anneal_schedule(beta_lo: float, beta_hi: float, sweeps: int) -> np.ndarray:
+    """The deterministic geometric schedule batch instances follow."""
+    out = np.empty(sweeps, dtype=np.float64)
+    check(load().ige_anneal_schedule(float(beta_lo), float(beta_hi), int(sweeps),
+                                     _ptr(out, c_double)))
+    return out
+
+
+class Batch:
+    """Independent instances (one cell of 64 replicas each) in one arena."""
+
+    def __init__(self, ctx: Context, qubos, seeds, sweeps, beta_lo, beta_hi):
+        self._lib = ctx._lib
+        self.ctx = ctx
+        self.count = len(qubos)
+        self.ns = [q.n for q in qubos]
+        descs = (InstanceDesc * self.count)()
+        keep = []
+        for i, qb in enumerate(qubos):
+            lin = np.ascontiguousarray(qb.linear, dtype=np.int32)
+            pi = np.ascontiguousarray(qb.pi, dtype=np.uint32)
+            pj = np.ascontiguousarray(qb.pj, dtype=np.uint32)
+            q = np.ascontiguousarray(qb.q, dtype=np.int32)
+            keep += [lin, pi, pj, q]
+            d = descs[i]
+            d.n = qb.n
+            d.offset = int(qb.offset)
+            d.linear = _ptr(lin, c_int32)
+            d.npairs = len(q)
+            d.pi = _ptr(pi, c_uint32)
+            d.pj = _ptr(pj, c_uint32)
+            d.q = _ptr(q, c_int32)
+            d.seed = int(seeds[i]) & 0xFFFFFFFFFFFFFFFF
+            d.sweeps = int(sweeps[i])
+            d.beta_lo = float(beta_lo[i])
+            d.beta_hi = float(beta_hi[i])
+        self.handle = c_void_p()
+        check(self._lib.ige_batch_create(ctx.handle, self.count, descs, ctypes.byref(self.handle)))
+        ts = c_uint64()
+        ta = c_uint64()
+        check(self._lib.ige_batch_info(self.handle, ctypes.byref(ts), ctypes.byref(ta)))
+        self.total_sites = ts.value
+        self.total_attempts = ta.value
+
+    def reset(self):
+        check(self._lib.ige_batch_reset(self.handle))
+
+    def run(self, mode=BATCH_PERSISTENT, waves=0, k0=0, k1=0xFFFFFFFF):
+        check(self._lib.ige_batch_run(self.handle, mode, waves, k0, k1))
+
+    def autotune(self, probe_sweeps=20):
+        w = c_uint32()
+        ms = (c_double * 4)()
+        check(self._lib.ige_batch_autotune(self.handle, probe_sweeps, ctypes.byref(w), ms))
+        return w.value, list(ms)
+
+    def synchronize(self):
+        check(self._lib.ige_batch_synchronize(self.handle))
+
+    def read(self, i):
+        words = np.empty(self.ns[i], dtype=np.uint64)
+        e = np.empty(64, dtype=np.int64)
+        acc = c_uint64()
+        check(self._lib.ige_batch_read(self.handle, i, _ptr(words, c_uint64), _ptr(e, c_int64),
+                                       ctypes.byref(acc)))
+        return words, e, acc.value
+
+    def close(self):
+        if self.handle:
+            self._lib.ige_batch_destroy(self.handle)
             self.handle = c_void_p()
 
     def __del__(self):

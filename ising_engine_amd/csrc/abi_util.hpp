@@ -1,0 +1,90 @@
+// abi_util.hpp -- shared plumbing of the C ABI implementation: error
+// classes, the exception guard every entry point runs under, and an owning
+// device buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ising_gpu.h"
+#include "problem_ir.hpp"
+
+struct ige_ctx {
+    int device = 0;
+};
+
+namespace ige::abi {
+
+inline thread_local std::string g_last_error;
+
+struct Failure : std::runtime_error {
+    int32_t code;
+    Failure(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw Failure(e == hipErrorOutOfMemory ? IGE_ERR_NOMEM : IGE_ERR_HIP,
+                      std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline void require(bool ok, const std::string& msg) {
+    if (!ok) throw Failure(IGE_ERR_INVALID, msg);
+}
+
+// Runs f, mapping every exception to a status code and a message.
+template <class F>
+int32_t guarded(F&& f) {
+    try {
+        f();
+        return IGE_OK;
+    } catch (const Failure& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const ige::RangeError& e) {
+        g_last_error = e.what();
+        return IGE_ERR_RANGE;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return IGE_ERR_INVALID;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return IGE_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return IGE_ERR_HIP;
+    }
+}
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        count = n;
+        if (n) check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    }
+    void upload(const T* src, size_t n, hipStream_t st) {
+        if (n) check(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+    }
+    void download(T* dst, size_t n, hipStream_t st) const {
+        if (n) check(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyDeviceToHost, st), "download");
+    }
+};
+
+inline int context_device(const ige_ctx* ctx) { return ctx->device; }
+
+}  // namespace ige::abi

@@ -16,84 +16,21 @@
 #include <string>
 #include <vector>
 
+#include "abi_util.hpp"
 #include "ige_math.h"
 #include "kernels.hpp"
 #include "problem_ir.hpp"
 
 namespace {
 
-thread_local std::string g_last_error;
-
-struct Failure : std::runtime_error {
-    int32_t code;
-    Failure(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess)
-        throw Failure(e == hipErrorOutOfMemory ? IGE_ERR_NOMEM : IGE_ERR_HIP,
-                      std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class F>
-int32_t guarded(F&& f) {
-    try {
-        f();
-        return IGE_OK;
-    } catch (const Failure& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const ige::RangeError& e) {
-        g_last_error = e.what();
-        return IGE_ERR_RANGE;
-    } catch (const std::invalid_argument& e) {
-        g_last_error = e.what();
-        return IGE_ERR_INVALID;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return IGE_ERR_NOMEM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return IGE_ERR_HIP;
-    }
-}
-
-void require(bool ok, const std::string& msg) {
-    if (!ok) throw Failure(IGE_ERR_INVALID, msg);
-}
-
-// Owning device buffer.
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t count = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void alloc(size_t n) {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
-        count = n;
-        if (n) check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
-    }
-    void upload(const T* src, size_t n, hipStream_t st) {
-        if (n) check(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, st), "upload");
-    }
-    void download(T* dst, size_t n, hipStream_t st) const {
-        if (n) check(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyDeviceToHost, st), "download");
-    }
-};
+using ige::abi::check;
+using ige::abi::DevBuf;
+using ige::abi::Failure;
+using ige::abi::g_last_error;
+using ige::abi::guarded;
+using ige::abi::require;
 
 }  // namespace
-
-struct ige_ctx {
-    int device = 0;
-};
 
 struct ige_model {
     ige_ctx* ctx = nullptr;

@@ -15,6 +15,7 @@
 
 #include "ige_math.h"
 #include "kernels.hpp"
+#include "sweep_generic.h"
 
 namespace ige {
 
@@ -28,11 +29,7 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-__device__ __forceinline__ bool lane_bit(uint64_t word) {
-    return __builtin_amdgcn_inverse_ballot_w64(word);
-}
-
-__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return wave_ballot(p); }
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
@@ -50,9 +47,8 @@ __global__ void thr_table_kernel(const double* __restrict__ beta, uint32_t* __re
 }
 
 // ---------------------------------------------------------------------------
-// Generic sweep, lane = replica.  Each wave owns positions [p0, p1) of the
-// colour class for one slot.  Uniforms: word view, one Philox call per lane
-// per aligned group of four positions.
+// Generic sweep, lane = replica (generic_positions): each wave owns positions
+// [p0, p1) of the colour class for one slot.
 // ---------------------------------------------------------------------------
 template <bool kLdsThr>
 __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
@@ -81,33 +77,11 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
 
     int64_t eacc = 0;
     uint32_t nacc = 0;
-    U32x4 u{};
-    uint32_t group = 0xFFFFFFFFu;
-    for (uint32_t p = p0; p < p1; ++p) {
-        if ((p >> 2) != group) {
-            group = p >> 2;
-            u = philox4x32_10(group, slot, sweep, (kTagWord << 24) | lane, key0, key1);
-        }
-        const uint32_t q = p & 3u;
-        const uint32_t uu = q == 0 ? u.v[0] : q == 1 ? u.v[1] : q == 2 ? u.v[2] : u.v[3];
-        const uint64_t x = S[p];
-        int32_t h = lin[p];
-        const uint32_t e1 = rp[p + 1];
-        for (uint32_t e = rp[p]; e < e1; ++e) {
-            const int32_t w = wt[e];
-            h += lane_bit(S[ci[e]]) ? w : 0;
-        }
-        const int32_t de = lane_bit(x) ? -h : h;
-        bool acc = de <= 0;
-        if (!acc) {
-            const uint32_t th = kLdsThr ? lds_thr[trow + (uint32_t)de] : thr[trow + (uint32_t)de];
-            acc = uu < th;
-        }
-        const uint64_t m = ballot(acc);
-        if (lane == 0) W[p] = x ^ m;
-        eacc += acc ? de : 0;
-        nacc += (uint32_t)__builtin_popcountll(m);
-    }
+    generic_positions(rp, ci, wt, lin, S, W, p0, p1, slot, sweep, key0, key1, lane,
+                      [&](int32_t de) {
+                          return kLdsThr ? lds_thr[trow + (uint32_t)de] : thr[trow + (uint32_t)de];
+                      },
+                      eacc, nacc);
     atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
               (unsigned long long)eacc);
     if (lane == 0) atomicAdd(accepted, (unsigned long long)nacc);

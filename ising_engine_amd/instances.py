@@ -66,42 +66,76 @@ def hypercubic_pm_j(L: int, dim: int, seed: int) -> Qubo:
     return ising_to_qubo(n, bi, bj, J, name=f"pmJ{dim}d_L{L}")
 
 
+def _er_pairs(n: int, mean_degree: float, rng) -> tuple[np.ndarray, np.ndarray]:
+    """Distinct pairs i < j of G(n, p = mean_degree / (n - 1)), vectorised."""
+    total = n * (n - 1) // 2
+    m = int(rng.binomial(total, min(1.0, mean_degree / max(n - 1, 1))))
+    keys = np.zeros(0, dtype=np.int64)
+    while keys.size < m:
+        a = rng.integers(0, n, size=2 * (m - keys.size) + 16, dtype=np.int64)
+        b = rng.integers(0, n, size=a.size, dtype=np.int64)
+        ok = a != b
+        lo = np.minimum(a[ok], b[ok])
+        hi = np.maximum(a[ok], b[ok])
+        new = lo * n + hi
+        keys = np.concatenate([keys, new[np.isin(new, keys, invert=True)]])
+        keys = keys[np.sort(np.unique(keys, return_index=True)[1])]
+    keys = keys[:m]
+    return (keys // n).astype(np.uint32), (keys % n).astype(np.uint32)
+
+
 def erdos_renyi_pm_j(n: int, mean_degree: float, seed: int) -> Qubo:
     """Erdos-Renyi G(n, p = mean_degree/(n-1)) with J uniform in {-1, +1}."""
     rng = np.random.default_rng(seed)
-    m_target = rng.binomial(n * (n - 1) // 2, mean_degree / (n - 1))
-    # Sample distinct pairs by rejection on a set of encoded pairs.
-    seen = set()
-    bi, bj = [], []
-    while len(bi) < m_target:
-        a = rng.integers(0, n, size=2 * (m_target - len(bi)) + 16)
-        b = rng.integers(0, n, size=a.shape[0])
-        for u, v in zip(a.tolist(), b.tolist()):
-            if u == v:
-                continue
-            lo, hi = (u, v) if u < v else (v, u)
-            key = lo * n + hi
-            if key in seen:
-                continue
-            seen.add(key)
-            bi.append(lo)
-            bj.append(hi)
-            if len(bi) == m_target:
-                break
-    J = rng.choice(np.array([-1, 1], dtype=np.int64), size=len(bi))
-    return ising_to_qubo(n, np.array(bi), np.array(bj), J, name=f"ER_n{n}_d{mean_degree:g}")
+    bi, bj = _er_pairs(n, mean_degree, rng)
+    J = rng.choice(np.array([-1, 1], dtype=np.int64), size=bi.size)
+    return ising_to_qubo(n, bi, bj, J, name=f"ER_n{n}_d{mean_degree:g}")
 
 
 def random_qubo(n: int, mean_degree: float, seed: int, offdiag: int = 100, diag: int = 200) -> Qubo:
     """Sparse QUBO: ER graph, Q_ij int uniform in [-offdiag, offdiag] (zeros redrawn to +1),
     Q_ii uniform in [-diag, diag] as the linear term."""
     rng = np.random.default_rng(seed)
-    er = erdos_renyi_pm_j(n, mean_degree, int(rng.integers(1 << 62)))
-    q = rng.integers(-offdiag, offdiag + 1, size=er.q.shape[0]).astype(np.int64)
+    pi, pj = _er_pairs(n, mean_degree, rng)
+    q = rng.integers(-offdiag, offdiag + 1, size=pi.size).astype(np.int64)
     q[q == 0] = 1
     linear = rng.integers(-diag, diag + 1, size=n).astype(np.int64)
-    return Qubo(n=n, offset=0, linear=linear, pi=er.pi, pj=er.pj, q=q,
+    return Qubo(n=n, offset=0, linear=linear, pi=pi, pj=pj, q=q,
                 name=f"QUBO_n{n}_d{mean_degree:g}")
+
+
+def suggested_temp_range(qb: Qubo) -> tuple[float, float]:
+    """(T_max, T_min) of src/solver/schedule.rs:21-43: T_max = max_i local_i / ln 2 with
+    local_i = |lin_i| + sum_j |q_ij|, T_min = min(min nonzero |coef| / ln 100, T_max / 2)."""
+    local = np.abs(qb.linear).astype(np.float64)
+    aq = np.abs(qb.q).astype(np.float64)
+    np.add.at(local, qb.pi.astype(np.int64), aq)
+    np.add.at(local, qb.pj.astype(np.int64), aq)
+    nz = np.concatenate([np.abs(qb.linear[qb.linear != 0]), aq[aq != 0]])
+    if local.max() == 0 or nz.size == 0:
+        return 1.0, 0.1
+    t_max = local.max() / np.log(2.0)
+    t_min = min(nz.min() / np.log(100.0), t_max * 0.5)
+    return float(t_max), float(t_min)
+
+
+def workload_qubo_batch(count: int = 4096, sweeps: int = 2000, seed: int = 4,
+                        n_lo: int = 64, n_hi: int = 8192) -> dict:
+    """Independent sparse QUBO instances: N log-uniform in [n_lo, n_hi], mean degree 8,
+    Q_ij uniform int in [-100, 100], Q_ii in [-200, 200], 64 replicas each, a geometric anneal
+    between the instance's suggested temperatures."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), size=count))).astype(np.int64)
+    qubos, b_lo, b_hi = [], [], []
+    for i, n in enumerate(ns):
+        qb = random_qubo(int(n), 8, int(rng.integers(1 << 62)))
+        t_max, t_min = suggested_temp_range(qb)
+        qubos.append(qb)
+        b_lo.append(1.0 / t_max)
+        b_hi.append(1.0 / t_min)
+    seeds = rng.integers(1, 1 << 62, size=count)
+    return dict(qubos=qubos, seeds=seeds, sweeps=[sweeps] * count, beta_lo=b_lo, beta_hi=b_hi,
+                name=f"qubo_batch_{count}_n{n_lo}-{n_hi}_d8")
 
 
 def geometric_betas(t_hi: float, t_lo: float, k: int) -> np.ndarray:
