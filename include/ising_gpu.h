@@ -33,6 +33,10 @@
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:114-137,317-346
  *   ige_read_spins          SpinState::spin / extract_into
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:280-289
+ *   ige_server_*            POST /api/v1/solve request loop
+ *                                                      src/bin/server_api.rs:111-213
+ *   ige_batch_*             no single counterpart: many UltimateSolver-style anneals
+ *                           (src/solver/ultimate.rs:234-336) in one launch
  *   ige_read_stats          Report{accepted,proposed} + StepScratch swap statistics
  *                                                      src/engine_v2/operator.rs:34-52,
  *                                                      src/solver/engine.rs:454-557
@@ -164,6 +168,50 @@ int32_t ige_batch_synchronize(ige_batch *batch);
 int32_t ige_batch_info(const ige_batch *batch, uint64_t *total_sites, uint64_t *total_attempts);
 int32_t ige_batch_read(ige_batch *batch, uint32_t index, uint64_t *words /* n, site order */,
                        int64_t *energies /* 64 */, uint64_t *accepted);
+
+/* ---- serving: continuous batching of solve requests ---- */
+/* Request statuses. */
+#define IGE_REQ_QUEUED 0
+#define IGE_REQ_RUNNING 1
+#define IGE_REQ_DONE 2
+#define IGE_REQ_FAILED (-1)
+
+typedef struct ige_server ige_server;
+
+typedef struct {
+    uint64_t id;
+    int32_t status;        /* IGE_REQ_*                                     */
+    uint32_t n;
+    int64_t best_energy;   /* lowest of the 64 replica energies at the end  */
+    uint32_t best_replica;
+    uint32_t reserved;
+    uint64_t accepted;
+    double latency_ms;     /* submit -> result ready                        */
+    double queue_ms;       /* submit -> admitted to the device              */
+} ige_request_info;
+
+typedef struct {
+    uint64_t queued, live, completed, rounds, attempts;
+    uint64_t arena_used, arena_capacity, max_live;
+    double busy_ms;
+} ige_server_counters;
+
+/* Starts the serving loop on a background thread with an HBM arena of
+ * hbm_bytes.  Each round advances every live request by as many sweeps as fit
+ * in about round_ms; waves = workgroup size per request (0 = 8). */
+int32_t ige_server_create(ige_ctx *ctx, uint64_t hbm_bytes, double round_ms, uint32_t waves,
+                          ige_server **out);
+int32_t ige_server_destroy(ige_server *server);
+/* Queues one request (validated like POST /api/v1/solve, server_api.rs:70-99);
+ * thread-safe.  The schedule is that of ige_anneal_schedule. */
+int32_t ige_server_submit(ige_server *server, const ige_instance_desc *request, uint64_t *id);
+/* Status and, once done, the final spins of all 64 replicas (site order) and
+ * their energies.  release != 0 forgets a finished request. */
+int32_t ige_server_poll(ige_server *server, uint64_t id, int32_t release, ige_request_info *info,
+                        uint64_t *words /* n, or NULL */, int64_t *energies /* 64, or NULL */);
+int32_t ige_server_wait(ige_server *server, uint64_t id, double timeout_ms,
+                        ige_request_info *info);
+int32_t ige_server_stats(ige_server *server, ige_server_counters *stats);
 
 /* Per-launch timing of the sweep kernels with HIP events on the state's
  * stream (off by default).  ms_total is the summed duration of `launches`

@@ -35,8 +35,14 @@ EXPORTED = (
     "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
     "ige_profile_read", "ige_anneal_schedule", "ige_batch_create", "ige_batch_destroy",
     "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
-    "ige_batch_info", "ige_batch_read",
+    "ige_batch_info", "ige_batch_read", "ige_server_create", "ige_server_destroy",
+    "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
 )
+
+REQ_QUEUED = 0
+REQ_RUNNING = 1
+REQ_DONE = 2
+REQ_FAILED = -1
 
 BATCH_PERSISTENT = 0
 BATCH_PLAIN = 1
@@ -65,6 +71,22 @@ class InstanceDesc(ctypes.Structure):
         ("npairs", c_uint64), ("pi", POINTER(c_uint32)), ("pj", POINTER(c_uint32)),
         ("q", POINTER(c_int32)), ("seed", c_uint64), ("sweeps", c_uint32),
         ("beta_lo", c_double), ("beta_hi", c_double),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("id", c_uint64), ("status", c_int32), ("n", c_uint32), ("best_energy", c_int64),
+        ("best_replica", c_uint32), ("reserved", c_uint32), ("accepted", c_uint64),
+        ("latency_ms", c_double), ("queue_ms", c_double),
+    ]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [
+        ("queued", c_uint64), ("live", c_uint64), ("completed", c_uint64),
+        ("rounds", c_uint64), ("attempts", c_uint64), ("arena_used", c_uint64),
+        ("arena_capacity", c_uint64), ("max_live", c_uint64), ("busy_ms", c_double),
     ]
 
 
@@ -129,6 +151,13 @@ def load() -> ctypes.CDLL:
         "ige_batch_info": (c_int32, [vp, POINTER(c_uint64), POINTER(c_uint64)]),
         "ige_batch_read": (c_int32, [vp, c_uint32, POINTER(c_uint64), POINTER(c_int64),
                                      POINTER(c_uint64)]),
+        "ige_server_create": (c_int32, [vp, c_uint64, c_double, c_uint32, POINTER(vp)]),
+        "ige_server_destroy": (c_int32, [vp]),
+        "ige_server_submit": (c_int32, [vp, POINTER(InstanceDesc), POINTER(c_uint64)]),
+        "ige_server_poll": (c_int32, [vp, c_uint64, c_int32, POINTER(RequestInfo),
+                                      POINTER(c_uint64), POINTER(c_int64)]),
+        "ige_server_wait": (c_int32, [vp, c_uint64, c_double, POINTER(RequestInfo)]),
+        "ige_server_stats": (c_int32, [vp, POINTER(ServerCounters)]),
         "ige_profile_read": (c_int32, [vp, POINTER(c_double), POINTER(c_uint64),
                                        POINTER(c_double), POINTER(c_uint64)]),
     }
@@ -317,6 +346,79 @@ def anneal_schedule(beta_lo: float, beta_hi: float, sweeps: int) -> np.ndarray:
     return out
 
 
+def _fill_desc(d, qb, seed, sweeps, beta_lo, beta_hi, keep):
+    lin = np.ascontiguousarray(qb.linear, dtype=np.int32)
+    pi = np.ascontiguousarray(qb.pi, dtype=np.uint32)
+    pj = np.ascontiguousarray(qb.pj, dtype=np.uint32)
+    q = np.ascontiguousarray(qb.q, dtype=np.int32)
+    keep += [lin, pi, pj, q]
+    d.n = qb.n
+    d.offset = int(qb.offset)
+    d.linear = _ptr(lin, c_int32)
+    d.npairs = len(q)
+    d.pi = _ptr(pi, c_uint32)
+    d.pj = _ptr(pj, c_uint32)
+    d.q = _ptr(q, c_int32)
+    d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    d.sweeps = int(sweeps)
+    d.beta_lo = float(beta_lo)
+    d.beta_hi = float(beta_hi)
+
+
+class Server:
+    """Continuous-batching request loop (a background thread in the library)."""
+
+    def __init__(self, ctx: Context, hbm_bytes: int = 8 << 30, round_ms: float = 10.0,
+                 waves: int = 0):
+        self._lib = ctx._lib
+        self.ctx = ctx
+        self.handle = c_void_p()
+        check(self._lib.ige_server_create(ctx.handle, int(hbm_bytes), float(round_ms), int(waves),
+                                          ctypes.byref(self.handle)))
+        self.sizes = {}
+
+    def submit(self, qb, seed, sweeps, beta_lo, beta_hi) -> int:
+        d = InstanceDesc()
+        keep = []
+        _fill_desc(d, qb, seed, sweeps, beta_lo, beta_hi, keep)
+        rid = c_uint64()
+        check(self._lib.ige_server_submit(self.handle, ctypes.byref(d), ctypes.byref(rid)))
+        self.sizes[rid.value] = qb.n
+        return rid.value
+
+    def poll(self, rid: int, release: bool = False, want_words: bool = True):
+        info = RequestInfo()
+        n = self.sizes[rid]
+        words = np.empty(n, dtype=np.uint64) if want_words else None
+        e = np.empty(64, dtype=np.int64) if want_words else None
+        check(self._lib.ige_server_poll(self.handle, rid, 1 if release else 0, ctypes.byref(info),
+                                        _ptr(words, c_uint64), _ptr(e, c_int64)))
+        if release and info.status == REQ_DONE:
+            self.sizes.pop(rid, None)
+        return info, words, e
+
+    def wait(self, rid: int, timeout_ms: float = 60000.0):
+        info = RequestInfo()
+        check(self._lib.ige_server_wait(self.handle, rid, float(timeout_ms), ctypes.byref(info)))
+        return info
+
+    def stats(self) -> ServerCounters:
+        st = ServerCounters()
+        check(self._lib.ige_server_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.handle:
+            self._lib.ige_server_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Independent instances (one cell of 64 replicas each) in one arena."""
 
@@ -328,23 +430,7 @@ class Batch:
         descs = (InstanceDesc * self.count)()
         keep = []
         for i, qb in enumerate(qubos):
-            lin = np.ascontiguousarray(qb.linear, dtype=np.int32)
-            pi = np.ascontiguousarray(qb.pi, dtype=np.uint32)
-            pj = np.ascontiguousarray(qb.pj, dtype=np.uint32)
-            q = np.ascontiguousarray(qb.q, dtype=np.int32)
-            keep += [lin, pi, pj, q]
-            d = descs[i]
-            d.n = qb.n
-            d.offset = int(qb.offset)
-            d.linear = _ptr(lin, c_int32)
-            d.npairs = len(q)
-            d.pi = _ptr(pi, c_uint32)
-            d.pj = _ptr(pj, c_uint32)
-            d.q = _ptr(q, c_int32)
-            d.seed = int(seeds[i]) & 0xFFFFFFFFFFFFFFFF
-            d.sweeps = int(sweeps[i])
-            d.beta_lo = float(beta_lo[i])
-            d.beta_hi = float(beta_hi[i])
+            _fill_desc(descs[i], qb, seeds[i], sweeps[i], beta_lo[i], beta_hi[i], keep)
         self.handle = c_void_p()
         check(self._lib.ige_batch_create(ctx.handle, self.count, descs, ctypes.byref(self.handle)))
         ts = c_uint64()

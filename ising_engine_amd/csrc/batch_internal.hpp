@@ -1,0 +1,126 @@
+// batch_internal.hpp -- lowering of independent instances to position space
+// and the persistent-kernel launcher, shared by the batch API and the
+// serving loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/ising_gpu.h"
+#include "abi_util.hpp"
+#include "ige_math.h"
+#include "persistent.hpp"
+#include "problem_ir.hpp"
+
+namespace ige {
+
+static __global__ void thr_value_kernel(double beta, uint32_t* thr, uint32_t dtab) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d <= dtab) thr[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
+}
+
+static __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+}  // namespace ige
+
+
+using ige::abi::check;
+using ige::abi::Failure;
+using ige::abi::require;
+
+namespace ige {
+
+// Host-side description of one instance once lowered to position space.
+struct LoweredInstance {
+    uint32_t n = 0, ncol = 0, sweeps = 0, dtab = 0;
+    uint64_t seed = 0;
+    double beta_hi = 0.0, c = 0.0;
+    int64_t offset = 0;
+    std::vector<uint32_t> rp;   // n+1, relative to the instance's first edge
+    std::vector<uint32_t> ci;   // neighbour positions
+    std::vector<int32_t> wt;
+    std::vector<int32_t> lin;
+    std::vector<uint32_t> cs;   // ncol+1
+    std::vector<uint32_t> order;  // position -> site
+    uint64_t work() const { return (static_cast<uint64_t>(n) + ci.size()) * sweeps; }
+};
+
+inline void check_schedule(double beta_lo, double beta_hi, const std::string& who) {
+    if (!(std::isfinite(beta_lo) && std::isfinite(beta_hi) && beta_lo > 0 && beta_hi >= beta_lo &&
+          std::log(beta_hi / beta_lo) <= 40.0))
+        throw Failure(IGE_ERR_INVALID, who + ": need 0 < beta_lo <= beta_hi <= e^40 beta_lo");
+}
+
+// ProblemIR::from_pairs (ir.rs:30-69), greedy colouring (ir.rs:114-139) and the
+// colour-major position order (metropolis_sweep.rs:40-58) of one request.
+inline LoweredInstance lower_instance(const ige_instance_desc& d, const std::string& who) {
+    require(d.n >= 1 && d.linear && (d.npairs == 0 || (d.pi && d.pj && d.q)),
+            who + ": bad model arrays");
+    require(d.sweeps >= 1, who + ": sweeps must be >= 1");
+    check_schedule(d.beta_lo, d.beta_hi, who);
+    ProblemIR ir = ProblemIR::from_pairs(d.n, d.offset, d.linear, d.npairs, d.pi, d.pj, d.q);
+    const int64_t ml = ir.max_local_field();
+    if (ml > 0x7FFFFFFFll) throw RangeError(who + ": max local field exceeds int32");
+    const ColorOrder co = color_order(ir.greedy_coloring());
+    LoweredInstance L;
+    L.n = d.n;
+    L.ncol = co.num_colors;
+    L.sweeps = d.sweeps;
+    L.dtab = static_cast<uint32_t>(ml);
+    L.seed = d.seed;
+    L.beta_hi = d.beta_hi;
+    L.c = d.sweeps > 1 ? (std::log(d.beta_hi) - std::log(d.beta_lo)) / (d.sweeps - 1) : 0.0;
+    L.offset = d.offset;
+    L.rp.reserve(d.n + 1);
+    L.rp.push_back(0);
+    for (uint32_t p = 0; p < d.n; ++p) {
+        const uint32_t site = co.order[p];
+        L.lin.push_back(ir.linear[site]);
+        for (uint32_t e = ir.row_ptr[site]; e < ir.row_ptr[site + 1]; ++e) {
+            L.ci.push_back(co.pos_of[ir.col_idx[e]]);
+            L.wt.push_back(ir.weights[e]);
+        }
+        L.rp.push_back(static_cast<uint32_t>(L.ci.size()));
+    }
+    L.cs = co.class_start;
+    L.order = co.order;
+    return L;
+}
+
+template <int WAVES>
+void launch_persistent_waves(const InstDesc* insts, const uint32_t* queue, uint32_t count,
+                             unsigned int* next, uint32_t round, int num_cus, hipStream_t st) {
+    int per_cu = 0;
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<WAVES>,
+                                                       WAVES * 64, 0),
+          "occupancy");
+    const uint32_t blocks =
+        std::max(1u, std::min<uint32_t>(count, static_cast<uint32_t>(per_cu * num_cus)));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, next, 0u, 1u);
+    hipLaunchKernelGGL(persistent_anneal_kernel<WAVES>, dim3(blocks), dim3(WAVES * 64), 0, st,
+                       insts, queue, count, next, round);
+    check(hipGetLastError(), "persistent launch");
+}
+
+inline void launch_persistent(uint32_t waves, const InstDesc* insts, const uint32_t* queue,
+                              uint32_t count, unsigned int* next, uint32_t round, int num_cus,
+                              hipStream_t st) {
+    switch (waves) {
+        case 1: launch_persistent_waves<1>(insts, queue, count, next, round, num_cus, st); break;
+        case 2: launch_persistent_waves<2>(insts, queue, count, next, round, num_cus, st); break;
+        case 4: launch_persistent_waves<4>(insts, queue, count, next, round, num_cus, st); break;
+        case 8: launch_persistent_waves<8>(insts, queue, count, next, round, num_cus, st); break;
+        case 16: launch_persistent_waves<16>(insts, queue, count, next, round, num_cus, st); break;
+        default: throw Failure(IGE_ERR_INVALID, "waves per instance must be 1, 2, 4, 8 or 16");
+    }
+}
+
+}  // namespace ige
+

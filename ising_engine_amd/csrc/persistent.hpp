@@ -1,0 +1,167 @@
+// persistent.hpp -- the persistent multi-instance anneal kernel shared by the
+// batch API and the serving loop.
+//
+// Each instance is one cell of 64 replicas with its own CSR (position space),
+// seed and geometric beta schedule.  A workgroup pulls the next instance from
+// a device work queue, advances it by up to `round` sweeps from its stored
+// progress, with workgroup barriers between colour classes, and pulls again:
+// no grid-wide synchronisation.  Per-sweep Metropolis thresholds are rebuilt
+// by the workgroup in LDS (or in a per-instance global scratch table when the
+// field range does not fit).
+//
+// Uniforms are keyed by (instance seed, position, slot 0, sweep), so an
+// instance's trajectory does not depend on the workgroup, the co-resident
+// instances, the round boundaries or the launch mode that ran it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ige_math.h"
+#include "sweep_generic.h"
+
+namespace ige {
+
+struct InstDesc {
+    const uint32_t* rp;    // n+1 absolute indices into ci / wt
+    const uint32_t* ci;    // neighbour positions
+    const int32_t* wt;     // couplings
+    const int32_t* lin;    // n linear terms
+    const uint32_t* cs;    // ncol+1 colour class starts
+    uint64_t* spins;       // n MSC words (position order)
+    int64_t* energy;       // 64 tracked energies
+    unsigned long long* accepted;
+    uint32_t* progress;    // sweeps done so far
+    uint32_t* thr_scratch; // dtab+1 entries, used in kThrGlobal mode
+    uint32_t n, ncol, sweeps, dtab, key0, key1;
+    uint32_t round;        // sweeps to advance in this launch (0: the launch's default)
+    uint32_t thr_mode;     // kThrLds, kThrGlobal or kThrDirect
+    double beta_hi;
+    double c;  // beta_k = beta_hi * det_exp(-(sweeps - 1 - k) * c)
+    int64_t offset;
+};
+
+constexpr uint32_t kLdsThrEntries = 4096;
+// Where a workgroup keeps the per-sweep thresholds thr[d], d <= dtab.
+constexpr uint32_t kThrLds = 0;     // dtab < kLdsThrEntries: table in LDS
+constexpr uint32_t kThrGlobal = 1;  // large range, enough sites to amortise a table
+constexpr uint32_t kThrDirect = 2;  // large range, few sites: evaluate per proposal
+
+__host__ __device__ __forceinline__ uint32_t choose_thr_mode(uint32_t dtab, uint32_t n) {
+    return dtab < kLdsThrEntries ? kThrLds : (dtab <= 32u * n ? kThrGlobal : kThrDirect);
+}
+
+// Largest dE whose threshold can be nonzero at beta: thr(beta, d) = 0 once
+// beta * d > 40 (det_exp returns 0 there); two units of slack keep the bound
+// safe against rounding of 40 / beta.
+__host__ __device__ __forceinline__ uint32_t threshold_limit(double beta, uint32_t dtab) {
+    if (!(beta > 0.0)) return dtab;
+    const double lim = 40.0 / beta;
+    return lim >= (double)dtab ? dtab : min(dtab, (uint32_t)lim + 2u);
+}
+
+__host__ __device__ __forceinline__ double anneal_beta(double beta_hi, double c, uint32_t sweeps,
+                                                       uint32_t k) {
+    return sweeps > 1 ? beta_hi * det_exp(-(double)(sweeps - 1 - k) * c) : beta_hi;
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
+    const InstDesc* __restrict__ insts, const uint32_t* __restrict__ queue, uint32_t count,
+    unsigned int* next, uint32_t round) {
+    __shared__ uint32_t thr_lds[kLdsThrEntries];
+    __shared__ int64_t part[WAVES][64];
+    __shared__ uint32_t s_item;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= count) break;
+        const InstDesc I = insts[queue[item]];
+        const uint32_t mode = I.thr_mode;
+        uint32_t* gthr = I.thr_scratch;
+        const uint32_t k0 = *I.progress;
+        const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
+        int64_t eacc = 0;
+        uint32_t nacc = 0;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const double beta = anneal_beta(I.beta_hi, I.c, I.sweeps, k);
+            const uint32_t dlim = threshold_limit(beta, I.dtab);
+            if (mode != kThrDirect) {
+                uint32_t* tab = mode == kThrLds ? thr_lds : gthr;
+                for (uint32_t d = threadIdx.x; d <= dlim; d += WAVES * 64)
+                    tab[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
+                if (mode == kThrGlobal) __threadfence_block();
+                __syncthreads();
+            }
+            auto thr = [&](int32_t de) -> uint32_t {
+                if ((uint32_t)de > dlim) return 0u;
+                return mode == kThrLds ? thr_lds[de]
+                                       : mode == kThrGlobal ? gthr[de]
+                                                            : metropolis_threshold(beta, (int64_t)de);
+            };
+            for (uint32_t c = 0; c < I.ncol; ++c) {
+                const uint32_t cb = I.cs[c], ce = I.cs[c + 1];
+                if (ce > cb) {
+                    for (uint32_t g = (cb >> 2) + wave; g <= ((ce - 1) >> 2); g += WAVES) {
+                        const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
+                        generic_positions(I.rp, I.ci, I.wt, I.lin, I.spins, I.spins, p0, p1, 0u,
+                                          k, I.key0, I.key1, lane, thr, eacc, nacc);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        part[wave][lane] = eacc;
+        // nacc is already the wave's total (a popcount of the accept ballot).
+        if (lane == 0 && nacc) atomicAdd(I.accepted, (unsigned long long)nacc);
+        __syncthreads();
+        if (wave == 0) {
+            int64_t tot = 0;
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) tot += part[w][lane];
+            I.energy[lane] += tot;
+            if (lane == 0) *I.progress = k1;
+        }
+        __syncthreads();
+    }
+}
+
+// Default initial words (Philox (site, 0, 0, TAG_INIT)) and exact energies,
+// one workgroup per listed instance.
+static __global__ __launch_bounds__(256) void instance_init_kernel(const InstDesc* __restrict__ insts,
+                                                            const uint32_t* __restrict__ list,
+                                                            const uint32_t* const* orders) {
+    const InstDesc I = insts[list[blockIdx.x]];
+    const uint32_t* order = orders[list[blockIdx.x]];
+    for (uint32_t p = threadIdx.x; p < I.n; p += blockDim.x) {
+        const U32x4 o = philox4x32_10(order[p], 0u, 0u, kTagInit << 24, I.key0, I.key1);
+        I.spins[p] = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
+    }
+    if (threadIdx.x == 0) {
+        *I.progress = 0;
+        *I.accepted = 0;
+    }
+}
+
+static __global__ __launch_bounds__(256) void instance_energy_kernel(const InstDesc* __restrict__ insts,
+                                                              const uint32_t* __restrict__ list) {
+    __shared__ int64_t part[4][64];
+    const InstDesc I = insts[list[blockIdx.x]];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int64_t a = 0;
+    for (uint32_t p = wv; p < I.n; p += 4) {
+        int64_t f = 2 * (int64_t)I.lin[p];
+        for (uint32_t e = I.rp[p]; e < I.rp[p + 1]; ++e) f += lane_bit(I.spins[I.ci[e]]) ? I.wt[e] : 0;
+        a += lane_bit(I.spins[p]) ? f : 0;
+    }
+    part[wv][lane] = a;
+    __syncthreads();
+    if (threadIdx.x < 64)
+        I.energy[lane] = I.offset + (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / 2;
+}
+
+}  // namespace ige

@@ -1,0 +1,510 @@
+// server.hip -- continuous-batching serving loop for solve requests.
+//
+// Requests (an integer QUBO, a seed, a sweep budget and an anneal range) are
+// lowered to position space on the submitting thread and queued.  A
+// background loop works in rounds:
+//   1. admit queued requests, first come first served, while their HBM
+//      footprint fits the arena (one device allocation made up front);
+//   2. run one persistent launch in which every live request advances by its
+//      own number of sweeps, sized so that a round takes about the target
+//      round time whatever the request's size;
+//   3. retire finished requests: copy the best replica, the 64 energies and
+//      the counters back and free their arena space.
+// Requests that arrive while others run are admitted at the next round, so
+// their work overlaps on the device with the requests already there.  The
+// uniforms are keyed by (request seed, position, sweep), so a request's
+// result is the same alone and in any mix.
+//
+// Request validation mirrors the reference server's limits
+// (src/bin/server_api.rs:15-17, 70-99): 1 <= n <= 50000, sweeps <= 10000,
+// pair indices in range.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_internal.hpp"
+
+using ige::abi::DevBuf;
+using ige::abi::guarded;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint64_t kAlign = 256;
+constexpr uint32_t kMaxVars = 50000;     // server_api.rs:15
+constexpr uint32_t kMaxSweeps = 10000;   // server_api.rs:16
+
+uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over byte offsets of the device arena.
+class ArenaAllocator {
+   public:
+    explicit ArenaAllocator(uint64_t cap) : cap_(cap) {
+        if (cap) free_[0] = cap;
+    }
+    bool alloc(uint64_t bytes, uint64_t* off) {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[*off + bytes] = rest;
+            used_ += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        bytes = align_up(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint64_t used() const { return used_; }
+    uint64_t capacity() const { return cap_; }
+
+   private:
+    uint64_t cap_;
+    uint64_t used_ = 0;
+    std::map<uint64_t, uint64_t> free_;
+};
+
+// Byte layout of one request inside the arena; each region 256-B aligned.
+struct Layout {
+    uint64_t rp, ci, wt, lin, cs, order, spins, energy, accepted, progress, thr, total;
+    explicit Layout(const ige::LoweredInstance& L) {
+        uint64_t o = 0;
+        auto take = [&](uint64_t bytes) {
+            const uint64_t at = o;
+            o += align_up(bytes);
+            return at;
+        };
+        rp = take((L.n + 1ull) * 4);
+        ci = take(std::max<uint64_t>(L.ci.size(), 1) * 4);
+        wt = take(std::max<uint64_t>(L.wt.size(), 1) * 4);
+        lin = take(L.n * 4ull);
+        cs = take(L.cs.size() * 4ull);
+        order = take(L.n * 4ull);
+        spins = take(L.n * 8ull);
+        energy = take(64 * 8);
+        accepted = take(8);
+        progress = take(4);
+        thr = ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal ? take((L.dtab + 1ull) * 4) : o;
+        total = o;
+    }
+};
+
+struct Request {
+    uint64_t id = 0;
+    ige::LoweredInstance L;
+    Clock::time_point t_submit, t_admit, t_done;
+    int32_t status = IGE_REQ_QUEUED;
+    uint64_t off = 0, bytes = 0;
+    ige::InstDesc D{};
+    uint32_t progress = 0;
+    std::vector<uint8_t> staging;  // host image of the model regions (kept until uploaded)
+    std::vector<uint64_t> words;   // result, site order
+    std::vector<int64_t> energies;
+    uint64_t accepted = 0;
+};
+
+}  // namespace
+
+struct ige_server {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    uint32_t waves = 8;
+    double round_ms = 10.0;
+    uint32_t max_live = 4096;
+    ArenaAllocator arena{0};
+    uint8_t* d_arena = nullptr;
+    DevBuf<ige::InstDesc> d_desc;
+    DevBuf<uint32_t> d_queue, d_list;
+    DevBuf<const uint32_t*> d_orders;
+    DevBuf<unsigned int> d_next;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Request*> live;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::string fatal;
+    std::thread loop;
+
+    // statistics
+    uint64_t rounds = 0, completed = 0, attempts_done = 0, max_live_seen = 0;
+    double busy_ms = 0.0;
+    double attempts_per_ms_per_block = 2.0e4;  // refined from measured rounds
+
+    ~ige_server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (loop.joinable()) loop.join();
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (d_arena) (void)hipFree(d_arena);
+    }
+};
+
+namespace {
+
+void upload_request(ige_server* s, Request& r) {
+    const ige::LoweredInstance& L = r.L;
+    const Layout lay(L);
+    r.staging.assign(lay.spins, 0);  // model regions only: rp .. order
+    auto put = [&](uint64_t at, const void* src, uint64_t bytes) {
+        if (bytes) std::memcpy(r.staging.data() + at, src, bytes);
+    };
+    put(lay.rp, L.rp.data(), L.rp.size() * 4);
+    put(lay.ci, L.ci.data(), L.ci.size() * 4);
+    put(lay.wt, L.wt.data(), L.wt.size() * 4);
+    put(lay.lin, L.lin.data(), L.lin.size() * 4);
+    put(lay.cs, L.cs.data(), L.cs.size() * 4);
+    put(lay.order, L.order.data(), L.order.size() * 4);
+    uint8_t* base = s->d_arena + r.off;
+    check(hipMemcpyAsync(base, r.staging.data(), r.staging.size(), hipMemcpyHostToDevice,
+                         s->stream),
+          "request upload");
+    ige::InstDesc& D = r.D;
+    D.rp = reinterpret_cast<const uint32_t*>(base + lay.rp);
+    D.ci = reinterpret_cast<const uint32_t*>(base + lay.ci);
+    D.wt = reinterpret_cast<const int32_t*>(base + lay.wt);
+    D.lin = reinterpret_cast<const int32_t*>(base + lay.lin);
+    D.cs = reinterpret_cast<const uint32_t*>(base + lay.cs);
+    D.spins = reinterpret_cast<uint64_t*>(base + lay.spins);
+    D.energy = reinterpret_cast<int64_t*>(base + lay.energy);
+    D.accepted = reinterpret_cast<unsigned long long*>(base + lay.accepted);
+    D.progress = reinterpret_cast<uint32_t*>(base + lay.progress);
+    D.thr_mode = ige::choose_thr_mode(L.dtab, L.n);
+    D.thr_scratch =
+        D.thr_mode == ige::kThrGlobal ? reinterpret_cast<uint32_t*>(base + lay.thr) : nullptr;
+    D.n = L.n;
+    D.ncol = L.ncol;
+    D.sweeps = L.sweeps;
+    D.dtab = L.dtab;
+    D.key0 = static_cast<uint32_t>(L.seed);
+    D.key1 = static_cast<uint32_t>(L.seed >> 32);
+    D.round = 1;
+    D.beta_hi = L.beta_hi;
+    D.c = L.c;
+    D.offset = L.offset;
+}
+
+void retire(ige_server* s, Request& r) {
+    const ige::LoweredInstance& L = r.L;
+    std::vector<uint64_t> w(L.n);
+    r.energies.resize(64);
+    check(hipMemcpyAsync(w.data(), r.D.spins, L.n * 8ull, hipMemcpyDeviceToHost, s->stream),
+          "result words");
+    check(hipMemcpyAsync(r.energies.data(), r.D.energy, 64 * 8, hipMemcpyDeviceToHost, s->stream),
+          "result energies");
+    unsigned long long acc = 0;
+    check(hipMemcpyAsync(&acc, r.D.accepted, 8, hipMemcpyDeviceToHost, s->stream), "accepted");
+    check(hipStreamSynchronize(s->stream), "retire");
+    r.words.assign(L.n, 0);
+    for (uint32_t p = 0; p < L.n; ++p) r.words[L.order[p]] = w[p];
+    r.accepted = acc;
+    r.staging.clear();
+    r.staging.shrink_to_fit();
+}
+
+void serve_loop(ige_server* s) {
+    (void)hipSetDevice(s->device);
+    std::vector<ige::InstDesc> descs;
+    std::vector<uint32_t> queue, list;
+    std::vector<const uint32_t*> orders;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    try {
+        for (;;) {
+            std::vector<Request*> admitted;
+            {
+                std::unique_lock<std::mutex> lk(s->mu);
+                s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->live.empty(); });
+                if (s->stop) break;
+                // 1. admission: FIFO while the head fits (no overtaking, no starvation)
+                while (!s->pending.empty() && s->live.size() < s->max_live) {
+                    Request& r = *s->reqs[s->pending.front()];
+                    const Layout lay(r.L);
+                    uint64_t off = 0;
+                    if (!s->arena.alloc(lay.total, &off)) break;
+                    r.off = off;
+                    r.bytes = lay.total;
+                    r.status = IGE_REQ_RUNNING;
+                    r.t_admit = Clock::now();
+                    s->pending.pop_front();
+                    s->live.push_back(&r);
+                    admitted.push_back(&r);
+                }
+                s->max_live_seen = std::max<uint64_t>(s->max_live_seen, s->live.size());
+            }
+            for (Request* r : admitted) upload_request(s, *r);
+            // 2. one round over every live request
+            const size_t L = s->live.size();
+            descs.resize(L);
+            queue.resize(L);
+            uint64_t round_attempts = 0;
+            const double per_block = std::max(1.0, s->attempts_per_ms_per_block) * s->round_ms;
+            for (size_t i = 0; i < L; ++i) {
+                Request& r = *s->live[i];
+                const uint64_t per_sweep = static_cast<uint64_t>(r.L.n) * 64;
+                uint32_t k = static_cast<uint32_t>(std::max(1.0, per_block / per_sweep));
+                k = std::min(k, r.L.sweeps - r.progress);
+                r.D.round = k;
+                descs[i] = r.D;
+                round_attempts += per_sweep * k;
+            }
+            std::vector<std::pair<uint64_t, uint32_t>> order_by_work(L);
+            for (size_t i = 0; i < L; ++i)
+                order_by_work[i] = {static_cast<uint64_t>(s->live[i]->L.n) * descs[i].round,
+                                    static_cast<uint32_t>(i)};
+            std::sort(order_by_work.begin(), order_by_work.end(),
+                      [](const auto& a, const auto& b) { return a.first > b.first; });
+            for (size_t i = 0; i < L; ++i) queue[i] = order_by_work[i].second;
+            if (s->d_desc.count < L) {
+                s->d_desc.alloc(std::max<size_t>(L, 2 * s->d_desc.count));
+                s->d_queue.alloc(s->d_desc.count);
+                s->d_list.alloc(s->d_desc.count);
+                s->d_orders.alloc(s->d_desc.count);
+            }
+            s->d_desc.upload(descs.data(), L, s->stream);
+            s->d_queue.upload(queue.data(), L, s->stream);
+            if (!admitted.empty()) {
+                list.clear();
+                orders.assign(L, nullptr);
+                for (size_t i = 0; i < L; ++i)
+                    if (s->live[i]->status == IGE_REQ_RUNNING && s->live[i]->progress == 0 &&
+                        std::find(admitted.begin(), admitted.end(), s->live[i]) != admitted.end()) {
+                        list.push_back(static_cast<uint32_t>(i));
+                        const Layout lay(s->live[i]->L);
+                        orders[i] = reinterpret_cast<const uint32_t*>(s->d_arena + s->live[i]->off +
+                                                                      lay.order);
+                    }
+                s->d_list.upload(list.data(), list.size(), s->stream);
+                s->d_orders.upload(orders.data(), L, s->stream);
+                hipLaunchKernelGGL(ige::instance_init_kernel, dim3(list.size()), dim3(256), 0,
+                                   s->stream, s->d_desc.p, s->d_list.p, s->d_orders.p);
+                hipLaunchKernelGGL(ige::instance_energy_kernel, dim3(list.size()), dim3(256), 0,
+                                   s->stream, s->d_desc.p, s->d_list.p);
+                check(hipGetLastError(), "request init");
+            }
+            check(hipEventRecord(e0, s->stream), "event");
+            ige::launch_persistent(s->waves, s->d_desc.p, s->d_queue.p, static_cast<uint32_t>(L),
+                                   s->d_next.p, 1, s->num_cus, s->stream);
+            check(hipEventRecord(e1, s->stream), "event");
+            check(hipEventSynchronize(e1), "round");
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, e0, e1), "event");
+            // refine the per-workgroup rate used to size the next round
+            {
+                int per_cu = 0;
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, ige::persistent_anneal_kernel<8>, 512, 0);
+                const double blocks =
+                    std::max(1.0, std::min<double>(L, std::max(1, per_cu) * s->num_cus));
+                if (ms > 0.05)
+                    s->attempts_per_ms_per_block =
+                        0.5 * s->attempts_per_ms_per_block + 0.5 * (round_attempts / ms / blocks);
+            }
+            // 3. retire
+            std::vector<Request*> done;
+            for (Request* r : s->live) {
+                r->progress += r->D.round;
+                if (r->progress >= r->L.sweeps) done.push_back(r);
+            }
+            for (Request* r : done) retire(s, *r);
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                const auto now = Clock::now();
+                for (Request* r : done) {
+                    r->status = IGE_REQ_DONE;
+                    r->t_done = now;
+                    s->arena.release(r->off, r->bytes);
+                    s->completed++;
+                }
+                s->live.erase(std::remove_if(s->live.begin(), s->live.end(),
+                                             [](Request* r) { return r->status == IGE_REQ_DONE; }),
+                              s->live.end());
+                s->rounds++;
+                s->busy_ms += ms;
+                s->attempts_done += round_attempts;
+            }
+            if (!done.empty()) s->cv_done.notify_all();
+        }
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->fatal = e.what();
+        for (Request* r : s->live) r->status = IGE_REQ_FAILED;
+        for (uint64_t id : s->pending) s->reqs[id]->status = IGE_REQ_FAILED;
+        s->live.clear();
+        s->pending.clear();
+        s->cv_done.notify_all();
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+}
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+void fill_info(const Request& r, ige_request_info* info) {
+    std::memset(info, 0, sizeof(*info));
+    info->id = r.id;
+    info->status = r.status;
+    info->n = r.L.n;
+    if (r.status == IGE_REQ_DONE) {
+        uint32_t best = 0;
+        for (uint32_t i = 1; i < 64; ++i)
+            if (r.energies[i] < r.energies[best]) best = i;
+        info->best_replica = best;
+        info->best_energy = r.energies[best];
+        info->accepted = r.accepted;
+        info->latency_ms = ms_between(r.t_submit, r.t_done);
+        info->queue_ms = ms_between(r.t_submit, r.t_admit);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uint32_t waves,
+                          ige_server** out) {
+    return guarded([&] {
+        require(ctx && out, "NULL argument");
+        require(hbm_bytes >= (1u << 20), "arena must be at least 1 MiB");
+        require(round_ms > 0.0, "round_ms must be > 0");
+        require(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8 || waves == 16,
+                "waves must be 0, 1, 2, 4, 8 or 16");
+        auto s = std::make_unique<ige_server>();
+        s->device = ige::abi::context_device(ctx);
+        check(hipSetDevice(s->device), "hipSetDevice");
+        check(hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount,
+                                    s->device),
+              "device attribute");
+        check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream");
+        check(hipMalloc(&s->d_arena, hbm_bytes), "arena");
+        s->arena = ArenaAllocator(hbm_bytes);
+        s->waves = waves ? waves : 8;
+        s->round_ms = round_ms;
+        s->d_next.alloc(1);
+        s->d_desc.alloc(256);
+        s->d_queue.alloc(256);
+        s->d_list.alloc(256);
+        s->d_orders.alloc(256);
+        ige_server* raw = s.get();
+        s->loop = std::thread(serve_loop, raw);
+        *out = s.release();
+    });
+}
+
+int32_t ige_server_destroy(ige_server* s) {
+    delete s;
+    return IGE_OK;
+}
+
+int32_t ige_server_submit(ige_server* s, const ige_instance_desc* d, uint64_t* id) {
+    return guarded([&] {
+        require(s && d && id, "NULL argument");
+        require(d->n >= 1 && d->n <= kMaxVars, "num_vars must be in [1, 50000]");
+        require(d->sweeps >= 1 && d->sweeps <= kMaxSweeps, "sweeps must be in [1, 10000]");
+        auto r = std::make_unique<Request>();
+        r->t_submit = Clock::now();
+        r->L = ige::lower_instance(*d, "request");
+        const Layout lay(r->L);
+        std::lock_guard<std::mutex> g(s->mu);
+        if (!s->fatal.empty()) throw Failure(IGE_ERR_HIP, "server stopped: " + s->fatal);
+        if (lay.total > s->arena.capacity())
+            throw Failure(IGE_ERR_NOMEM, "request footprint exceeds the server's HBM arena");
+        r->id = s->next_id++;
+        *id = r->id;
+        s->pending.push_back(r->id);
+        s->reqs[r->id] = std::move(r);
+        s->cv_work.notify_one();
+    });
+}
+
+int32_t ige_server_poll(ige_server* s, uint64_t id, int32_t release, ige_request_info* info,
+                        uint64_t* words, int64_t* energies) {
+    return guarded([&] {
+        require(s && info, "NULL argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->reqs.find(id);
+        require(it != s->reqs.end(), "unknown request id");
+        const Request& r = *it->second;
+        fill_info(r, info);
+        if (r.status == IGE_REQ_DONE) {
+            if (words) std::memcpy(words, r.words.data(), r.words.size() * 8);
+            if (energies) std::memcpy(energies, r.energies.data(), 64 * 8);
+            if (release) s->reqs.erase(it);
+        }
+    });
+}
+
+int32_t ige_server_wait(ige_server* s, uint64_t id, double timeout_ms, ige_request_info* info) {
+    return guarded([&] {
+        require(s && info, "NULL argument");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto done = [&] {
+            auto it = s->reqs.find(id);
+            return it == s->reqs.end() || it->second->status == IGE_REQ_DONE ||
+                   it->second->status == IGE_REQ_FAILED;
+        };
+        s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done);
+        auto it = s->reqs.find(id);
+        require(it != s->reqs.end(), "unknown request id");
+        fill_info(*it->second, info);
+    });
+}
+
+int32_t ige_server_stats(ige_server* s, ige_server_counters* st) {
+    return guarded([&] {
+        require(s && st, "NULL argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        std::memset(st, 0, sizeof(*st));
+        st->queued = s->pending.size();
+        st->live = s->live.size();
+        st->completed = s->completed;
+        st->rounds = s->rounds;
+        st->attempts = s->attempts_done;
+        st->arena_used = s->arena.used();
+        st->arena_capacity = s->arena.capacity();
+        st->max_live = s->max_live_seen;
+        st->busy_ms = s->busy_ms;
+    });
+}
+
+}  // extern "C"

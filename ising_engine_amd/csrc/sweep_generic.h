@@ -1,13 +1,14 @@
-// sweep_generic.h -- the lane-per-replica Metropolis update of a run of
-// positions, shared by the single-state sweep kernel and the batched
-// persistent kernel.
+// sweep_generic.h -- the lane-per-replica Metropolis update of one aligned
+// group of (up to) four positions, shared by the single-state sweep kernel
+// and the persistent multi-instance kernel.
 //
-// One wavefront, lane r = replica r.  Every address is wave-uniform, so CSR
-// rows, couplings and neighbour words arrive through scalar loads, and a
-// neighbour's spin enters the field as v_cndmask(word-as-lane-mask).
+// One wavefront, lane r = replica r.  The wave first gathers the whole
+// group's CSR entries in one coalesced pass (lane e holds entry e: its
+// coupling and its neighbour's u64 word), so every neighbour word of the four
+// positions is in flight at once; it then walks the entries with readlane,
+// applying each neighbour word as a lane mask (v_cndmask on an SGPR pair).
 // Uniforms (word view): Philox4x32-10 with counter (p >> 2, slot, sweep,
-// 1 << 24 | lane), word p & 3, one call per lane per aligned group of four
-// positions.
+// 1 << 24 | lane), word p & 3: one call per lane per group.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,45 +23,82 @@ __device__ __forceinline__ bool lane_bit(uint64_t word) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// Updates positions [p0, p1) (all in one colour class) of one slot.
-// S/W: the slot's words for reading / writing (same memory).  S is not
-// restrict-qualified here: a caller whose kernel argument is restrict and
-// never written within the launch gets scalar loads; a persistent caller,
-// which reads words that other waves wrote after a barrier, must not (the
-// scalar cache is not coherent with vector stores) and passes a plain
-// pointer, which makes the loads vector loads.  thr(de)
-// returns metropolis_threshold(beta of this lane, de) for de >= 1.
-// Adds the accepted dE to eacc (per lane) and the accepted count to nacc.
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// Updates positions [p0, p1) of one slot; p0..p1-1 lie in one colour class
+// and in one aligned group of four (p0 >> 2 == (p1 - 1) >> 2).
+// S/W: the slot's words for reading / writing (same memory); read with
+// vector loads, so words written by other waves before a barrier are seen.
+// thr(de) returns metropolis_threshold(beta of this lane, de) for de >= 1.
+// Adds the accepted dE to eacc (per lane) and the accepted count to nacc
+// (wave-uniform).
 template <class Thr>
-__device__ __forceinline__ void generic_positions(
+__device__ __forceinline__ void generic_group(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
-    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
-    const uint64_t* S, uint64_t* W, uint32_t p0, uint32_t p1, uint32_t slot,
-    uint32_t sweep, uint32_t key0, uint32_t key1, uint32_t lane, Thr&& thr, int64_t& eacc,
-    uint32_t& nacc) {
-    U32x4 u{};
-    uint32_t group = 0xFFFFFFFFu;
-    for (uint32_t p = p0; p < p1; ++p) {
-        if ((p >> 2) != group) {
-            group = p >> 2;
-            u = philox4x32_10(group, slot, sweep, (kTagWord << 24) | lane, key0, key1);
+    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin, const uint64_t* S,
+    uint64_t* W, uint32_t p0, uint32_t p1, uint32_t slot, uint32_t sweep, uint32_t key0,
+    uint32_t key1, uint32_t lane, Thr&& thr, int64_t& eacc, uint32_t& nacc) {
+    const U32x4 u = philox4x32_10(p0 >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
+    const uint32_t cnt = p1 - p0;  // 1..4
+    // row bounds of the group's positions (uniform)
+    uint32_t rb[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) rb[q] = rp[p0 + min((uint32_t)q, cnt)];
+    const uint64_t xw = lane < cnt ? S[p0 + lane] : 0ull;
+    int32_t h[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) h[q] = (uint32_t)q < cnt ? lin[p0 + q] : 0;
+    for (uint32_t base = rb[0]; base < rb[4]; base += 64) {
+        const uint32_t e = base + lane;
+        uint64_t nw = 0;
+        int32_t nwt = 0;
+        if (e < rb[4]) {
+            nwt = wt[e];
+            nw = S[ci[e]];
         }
-        const uint32_t q = p & 3u;
-        const uint32_t uu = q == 0 ? u.v[0] : q == 1 ? u.v[1] : q == 2 ? u.v[2] : u.v[3];
-        const uint64_t x = S[p];
-        int32_t h = lin[p];
-        const uint32_t e1 = rp[p + 1];
-        for (uint32_t e = rp[p]; e < e1; ++e) {
-            const int32_t w = wt[e];
-            h += lane_bit(S[ci[e]]) ? w : 0;
+        const uint32_t top = min(base + 64, rb[4]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t a = max(rb[q], base), b = min(rb[q + 1], top);
+            for (uint32_t k = a; k < b; ++k) {
+                const uint64_t word = readlane64(nw, k - base);
+                const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
+                h[q] += lane_bit(word) ? w : 0;
+            }
         }
-        const int32_t de = lane_bit(x) ? -h : h;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if ((uint32_t)q >= cnt) break;
+        const uint64_t x = readlane64(xw, q);
+        const int32_t de = lane_bit(x) ? -h[q] : h[q];
+        const uint32_t wsel = (p0 + q) & 3u;
+        const uint32_t uu = wsel == 0 ? u.v[0] : wsel == 1 ? u.v[1] : wsel == 2 ? u.v[2] : u.v[3];
         bool acc = de <= 0;
         if (!acc) acc = uu < thr(de);
         const uint64_t m = wave_ballot(acc);
-        if (lane == 0) W[p] = x ^ m;
+        if (lane == 0) W[p0 + q] = x ^ m;
         eacc += acc ? de : 0;
         nacc += (uint32_t)__builtin_popcountll(m);
+    }
+}
+
+// Positions [p0, p1) of one colour class, group by group.
+template <class Thr>
+__device__ __forceinline__ void generic_positions(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin, const uint64_t* S,
+    uint64_t* W, uint32_t p0, uint32_t p1, uint32_t slot, uint32_t sweep, uint32_t key0,
+    uint32_t key1, uint32_t lane, Thr&& thr, int64_t& eacc, uint32_t& nacc) {
+    for (uint32_t p = p0; p < p1;) {
+        const uint32_t q1 = min(p1, (p & ~3u) + 4u);
+        generic_group(rp, ci, wt, lin, S, W, p, q1, slot, sweep, key0, key1, lane, thr, eacc,
+                      nacc);
+        p = q1;
     }
 }
 

@@ -171,3 +171,24 @@ def workload_3d_pt(L: int = 64, temps: int = 32, sweeps: int = 10000, seed: int 
     qb = hypercubic_pm_j(L, 3, seed)
     return dict(qubo=qb, cells=temps, sweeps=sweeps, exchange=True,
                 ladder=geometric_betas(2.0, 0.5, temps), name=f"3d_pmJ_L{L}_pt{temps}")
+
+
+def serving_request(rng, n_lo: int = 128, n_hi: int = 16384, w_max: int = 1000,
+                    sweeps_lo: int = 500, sweeps_hi: int = 5000) -> dict:
+    """One solve request: 50/50 Ising or QUBO, N log-uniform in [n_lo, n_hi], mean degree
+    uniform in 4..16, integer weights uniform in [-w_max, w_max] (nonzero), sweep budget
+    uniform in [sweeps_lo, sweeps_hi], anneal between the suggested temperatures."""
+    n = int(np.rint(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+    deg = int(rng.integers(4, 17))
+    pi, pj = _er_pairs(n, deg, rng)
+    w = rng.integers(-w_max, w_max + 1, size=pi.size).astype(np.int64)
+    w[w == 0] = 1
+    if rng.random() < 0.5:
+        qb = ising_to_qubo(n, pi, pj, w, name=f"ising_n{n}_d{deg}")
+    else:
+        lin = rng.integers(-w_max, w_max + 1, size=n).astype(np.int64)
+        qb = Qubo(n=n, offset=0, linear=lin, pi=pi, pj=pj, q=w, name=f"qubo_n{n}_d{deg}")
+    t_max, t_min = suggested_temp_range(qb)
+    return dict(qubo=qb, seed=int(rng.integers(1, 1 << 62)),
+                sweeps=int(rng.integers(sweeps_lo, sweeps_hi + 1)),
+                beta_lo=1.0 / t_max, beta_hi=1.0 / t_min)
