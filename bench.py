@@ -83,6 +83,62 @@ def cpu_baseline(qb, cells, ladder, budget_s: float):
                        f"{3 * max(1, steps // 3)} steps, median of 3 repetitions")
 
 
+def bench_batch(_lib, instances, ctx, count: int) -> dict:
+    """Independent sparse QUBO instances (N log-uniform 64..8192, degree 8, int weights),
+    2000-sweep anneals: one persistent launch vs plain per-instance launches."""
+    wl = instances.workload_qubo_batch(count=count)
+    b = _lib.Batch(ctx, wl["qubos"], wl["seeds"], wl["sweeps"], wl["beta_lo"], wl["beta_hi"])
+    waves, probe_ms = b.autotune(probe_sweeps=10)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(_lib.BATCH_PERSISTENT, waves=0)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    persistent = b.total_attempts / dt
+    # plain launches: the first 3 sweeps of every instance (the per-sweep cost of the plain
+    # path does not depend on the sweep index), scaled to attempts/s
+    b.reset()
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(_lib.BATCH_PLAIN, k0=0, k1=3)
+    b.synchronize()
+    dp = time.perf_counter() - t0
+    plain = b.total_attempts * 3 / wl["sweeps"][0] / dp
+    log(f"[bench] batch {count}: persistent {persistent:.4e} attempts/s ({dt:.2f} s, waves={waves}),"
+        f" plain {plain:.4e} attempts/s -> {persistent / plain:.1f}x")
+    b.close()
+    return {"workload": wl["name"], "instances": count, "sweeps": wl["sweeps"][0],
+            "value": persistent, "unit": "spin-flip attempts/s", "seconds": dt,
+            "plain_value": plain, "plain_sample": "first 3 sweeps of every instance",
+            "ratio_vs_plain": persistent / plain, "waves_per_instance": waves}
+
+
+def bench_serving(_lib, instances, ctx, rate: float, count: int) -> dict:
+    """Poisson stream of mixed Ising/QUBO solve requests (N log-uniform 128..16384, degree
+    4..16, weights in [-1000, 1000], 500..5000 sweeps) into the continuous-batching loop."""
+    rng = np.random.default_rng(11)
+    reqs = [instances.serving_request(rng) for _ in range(count)]
+    srv = _lib.Server(ctx, hbm_bytes=16 << 30, round_ms=10.0)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    t0 = time.perf_counter()
+    ids = []
+    for a, r in zip(arrivals, reqs):
+        while time.perf_counter() - t0 < a:
+            time.sleep(0.0005)
+        ids.append(srv.submit(r["qubo"], r["seed"], r["sweeps"], r["beta_lo"], r["beta_hi"]))
+    lat = np.array([srv.wait(rid, 600000.0).latency_ms for rid in ids])
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    attempts = sum(r["qubo"].n * 64 * r["sweeps"] for r in reqs)
+    srv.close()
+    out = {"rate_req_s": rate, "requests": count, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+           "attempts_per_s": attempts / wall, "wall_s": wall, "rounds": int(st.rounds),
+           "max_live": int(st.max_live)}
+    log(f"[bench] serving {rate}/s x {count}: p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +150,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -147,6 +208,13 @@ def main():
                     avg_launch_us=round(avg_launch_s * 1e6, 2),
                     bytes_per_attempt=round(bpa, 4))
 
+    batched = None
+    if not args.no_batch:
+        batched = bench_batch(_lib, instances, ctx, args.batch_count)
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(_lib, instances, ctx, args.serving_rate, args.serving_requests)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -154,7 +222,7 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']:.4e} attempts/s on {cpu['cores']} threads")
 
     out = {
-        "metric": "spin-flip attempts/s (sweep + DEO replica exchange), 1 MI355X",
+        "metric": "spin-flip attempts/s, batched vs plain on 1 MI355X; p50/p99 solve latency",
         "value": value,
         "unit": "spin-flip attempts/s",
         "n_gpus": 1,
@@ -171,6 +239,8 @@ def main():
                    "kernel": kname, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -158,12 +158,12 @@ int32_t ige_batch_destroy(ige_batch *batch);
 /* Back to the seeded initial spins, exact energies, zero counters. */
 int32_t ige_batch_reset(ige_batch *batch);
 /* Sweeps [k0, k1) of every instance's schedule (clamped to its length).
- * waves: workgroup size in wavefronts per instance (1, 2, 4, 8), 0 = tuned. */
+ * waves: workgroup size in wavefronts per instance (1, 2, 4, 8, 16), 0 = tuned. */
 int32_t ige_batch_run(ige_batch *batch, uint32_t mode, uint32_t waves, uint32_t k0, uint32_t k1);
-/* Times the persistent kernel at 1, 2, 4, 8 waves per instance on
+/* Times the persistent kernel at 1, 2, 4, 8, 16 waves per instance on
  * `probe_sweeps` sweeps, keeps the fastest, then resets the batch. */
 int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *waves_out,
-                           double *ms_out /* 4 entries, or NULL */);
+                           double *ms_out /* 5 entries, or NULL */);
 int32_t ige_batch_synchronize(ige_batch *batch);
 int32_t ige_batch_info(const ige_batch *batch, uint64_t *total_sites, uint64_t *total_attempts);
 int32_t ige_batch_read(ige_batch *batch, uint32_t index, uint64_t *words /* n, site order */,

@@ -447,7 +447,7 @@ class Batch:
 
     def autotune(self, probe_sweeps=20):
         w = c_uint32()
-        ms = (c_double * 4)()
+        ms = (c_double * 5)()
         check(self._lib.ige_batch_autotune(self.handle, probe_sweeps, ctypes.byref(w), ms))
         return w.value, list(ms)
 
