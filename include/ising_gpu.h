@@ -164,6 +164,12 @@ int32_t ige_batch_run(ige_batch *batch, uint32_t mode, uint32_t waves, uint32_t 
  * `probe_sweeps` sweeps, keeps the fastest, then resets the batch. */
 int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *waves_out,
                            double *ms_out /* 5 entries, or NULL */);
+/* Large instances run on a gang of workgroups (instance sites / gang_sites of
+ * them, at most max_gang <= 32) that meet at a device-wide barrier after every
+ * colour class.  spin_limit bounds each barrier wait; an expired wait makes
+ * the next synchronize/read fail with IGE_ERR_HIP.  Defaults: 16, 2048, 2^24. */
+int32_t ige_batch_configure(ige_batch *batch, uint32_t max_gang, uint32_t gang_sites,
+                            uint32_t spin_limit);
 int32_t ige_batch_synchronize(ige_batch *batch);
 int32_t ige_batch_info(const ige_batch *batch, uint64_t *total_sites, uint64_t *total_attempts);
 int32_t ige_batch_read(ige_batch *batch, uint32_t index, uint64_t *words /* n, site order */,
@@ -202,6 +208,9 @@ typedef struct {
 int32_t ige_server_create(ige_ctx *ctx, uint64_t hbm_bytes, double round_ms, uint32_t waves,
                           ige_server **out);
 int32_t ige_server_destroy(ige_server *server);
+/* Gang settings as ige_batch_configure; only before the first submit. */
+int32_t ige_server_configure(ige_server *server, uint32_t max_gang, uint32_t gang_sites,
+                             uint32_t spin_limit);
 /* Queues one request (validated like POST /api/v1/solve, server_api.rs:70-99);
  * thread-safe.  The schedule is that of ige_anneal_schedule. */
 int32_t ige_server_submit(ige_server *server, const ige_instance_desc *request, uint64_t *id);

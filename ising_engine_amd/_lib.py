@@ -37,6 +37,7 @@ EXPORTED = (
     "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
     "ige_batch_info", "ige_batch_read", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
+    "ige_batch_configure", "ige_server_configure",
 )
 
 REQ_QUEUED = 0
@@ -158,6 +159,8 @@ def load() -> ctypes.CDLL:
                                       POINTER(c_uint64), POINTER(c_int64)]),
         "ige_server_wait": (c_int32, [vp, c_uint64, c_double, POINTER(RequestInfo)]),
         "ige_server_stats": (c_int32, [vp, POINTER(ServerCounters)]),
+        "ige_batch_configure": (c_int32, [vp, c_uint32, c_uint32, c_uint32]),
+        "ige_server_configure": (c_int32, [vp, c_uint32, c_uint32, c_uint32]),
         "ige_profile_read": (c_int32, [vp, POINTER(c_double), POINTER(c_uint64),
                                        POINTER(c_double), POINTER(c_uint64)]),
     }
@@ -377,6 +380,9 @@ class Server:
                                           ctypes.byref(self.handle)))
         self.sizes = {}
 
+    def configure(self, max_gang=16, gang_sites=2048, spin_limit=1 << 24):
+        check(self._lib.ige_server_configure(self.handle, max_gang, gang_sites, spin_limit))
+
     def submit(self, qb, seed, sweeps, beta_lo, beta_hi) -> int:
         d = InstanceDesc()
         keep = []
@@ -441,6 +447,9 @@ class Batch:
 
     def reset(self):
         check(self._lib.ige_batch_reset(self.handle))
+
+    def configure(self, max_gang=16, gang_sites=2048, spin_limit=1 << 24):
+        check(self._lib.ige_batch_configure(self.handle, max_gang, gang_sites, spin_limit))
 
     def run(self, mode=BATCH_PERSISTENT, waves=0, k0=0, k1=0xFFFFFFFF):
         check(self._lib.ige_batch_run(self.handle, mode, waves, k0, k1))

@@ -44,10 +44,13 @@ struct ige_batch {
     uint64_t total_sites = 0, total_attempts = 0;
     uint32_t max_dtab = 0;
     uint32_t tuned_waves = 0;
+    uint32_t max_gang = 16, gang_sites = ige::kGangSites, spin_limit = 1u << 24;
+    std::vector<uint32_t> gang;
+    std::vector<uint32_t> tickets;
     int num_cus = 0;
     DevBuf<ige::InstDesc> d_desc;
     DevBuf<uint32_t> d_queue, d_rp, d_ci, d_cs, d_order, d_progress, d_zero_temp, d_thr,
-        d_scratch, d_list;
+        d_scratch, d_list, d_bars, d_err;
     DevBuf<const uint32_t*> d_orders;
     DevBuf<int32_t> d_wt, d_lin;
     DevBuf<uint64_t> d_spins;
@@ -72,6 +75,66 @@ void batch_reset(ige_batch* b) {
                        b->d_desc.p, b->d_list.p);
     check(hipGetLastError(), "batch energy");
     check(hipStreamSynchronize(b->stream), "batch reset");
+}
+
+ige::PersistentLaunch launch_args(ige_batch* b, uint32_t round) {
+    ige::PersistentLaunch L;
+    L.insts = b->d_desc.p;
+    L.queue = b->d_queue.p;
+    L.tickets = static_cast<uint32_t>(b->tickets.size());
+    L.next = b->d_next.p;
+    L.bars = b->d_bars.p;
+    L.nbars = b->count;
+    L.err = b->d_err.p;
+    L.round = round;
+    L.spin_limit = b->spin_limit;
+    L.num_cus = b->num_cus;
+    return L;
+}
+
+// Gang sizes, threshold modes, scratch tables, descriptors and tickets.
+void plan(ige_batch* b) {
+    b->gang.resize(b->count);
+    uint64_t scratch = 0;
+    for (uint32_t i = 0; i < b->count; ++i) {
+        const auto& L = b->inst[i];
+        b->gang[i] = ige::choose_gang(L.n, b->max_gang, b->gang_sites);
+        if (ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal)
+            scratch += static_cast<uint64_t>(b->gang[i]) * (L.dtab + 1);
+    }
+    b->d_scratch.alloc(std::max<uint64_t>(scratch, 1));
+    uint64_t scr = 0;
+    for (uint32_t i = 0; i < b->count; ++i) {
+        const auto& L = b->inst[i];
+        ige::InstDesc& D = b->desc[i];
+        D.gang = b->gang[i];
+        D.bar = b->d_bars.p + i;
+        D.thr_mode = ige::choose_thr_mode(L.dtab, L.n);
+        D.thr_scratch = nullptr;
+        if (D.thr_mode == ige::kThrGlobal) {
+            D.thr_scratch = b->d_scratch.p + scr;
+            scr += static_cast<uint64_t>(D.gang) * (L.dtab + 1);
+        }
+    }
+    // largest work per workgroup first; a gang's tickets stay consecutive
+    std::stable_sort(b->queue.begin(), b->queue.end(), [&](uint32_t a, uint32_t c) {
+        return b->inst[a].work() / b->gang[a] > b->inst[c].work() / b->gang[c];
+    });
+    ige::build_tickets(b->queue, b->gang, b->tickets);
+    b->d_desc.upload(b->desc.data(), b->count, b->stream);
+    b->d_queue.alloc(b->tickets.size());
+    b->d_queue.upload(b->tickets.data(), b->tickets.size(), b->stream);
+    check(hipStreamSynchronize(b->stream), "plan");
+}
+
+void check_error_flag(ige_batch* b) {
+    uint32_t e = 0;
+    check(hipMemcpy(&e, b->d_err.p, sizeof(e), hipMemcpyDeviceToHost), "error flag");
+    if (e) {
+        const uint32_t z = 0;
+        check(hipMemcpy(b->d_err.p, &z, sizeof(z), hipMemcpyHostToDevice), "error flag");
+        throw Failure(IGE_ERR_HIP, "a gang barrier wait exceeded its spin limit; results invalid");
+    }
 }
 
 void set_progress(ige_batch* b, uint32_t k0) {
@@ -138,14 +201,13 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
         b->count = count;
         b->inst.reserve(count);
-        uint64_t n_tot = 0, e_tot = 0, c_tot = 0, scratch_tot = 0;
+        uint64_t n_tot = 0, e_tot = 0, c_tot = 0;
         for (uint32_t i = 0; i < count; ++i) {
             b->inst.push_back(ige::lower_instance(desc[i], "instance " + std::to_string(i)));
             const auto& L = b->inst.back();
             n_tot += L.n;
             e_tot += L.ci.size();
             c_tot += L.cs.size();
-            if (ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal) scratch_tot += L.dtab + 1;
             b->max_dtab = std::max(b->max_dtab, L.dtab);
             b->total_sites += L.n;
             b->total_attempts += static_cast<uint64_t>(L.n) * 64ull * L.sweeps;
@@ -162,7 +224,8 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         b->d_energy.alloc(static_cast<size_t>(count) * 64);
         b->d_accepted.alloc(count);
         b->d_progress.alloc(count);
-        b->d_scratch.alloc(std::max<uint64_t>(scratch_tot, 1));
+        b->d_bars.alloc(count);
+        b->d_err.alloc(1);
         b->d_thr.alloc(static_cast<size_t>(b->max_dtab) + 1);
         std::vector<uint32_t> rp, ci, cs, order;
         std::vector<int32_t> wt, lin;
@@ -170,7 +233,6 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         rp.reserve(n_tot + count);
         ci.reserve(e_tot);
         wt.reserve(e_tot);
-        uint64_t scr = 0;
         b->desc.resize(count);
         for (uint32_t i = 0; i < count; ++i) {
             const auto& L = b->inst[i];
@@ -185,12 +247,6 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
             D.energy = b->d_energy.p + static_cast<size_t>(i) * 64;
             D.accepted = b->d_accepted.p + i;
             D.progress = b->d_progress.p + i;
-            D.thr_scratch = nullptr;
-            D.thr_mode = ige::choose_thr_mode(L.dtab, L.n);
-            if (D.thr_mode == ige::kThrGlobal) {
-                D.thr_scratch = b->d_scratch.p + scr;
-                scr += L.dtab + 1;
-            }
             orders[i] = b->d_order.p + lin.size();
             D.n = L.n;
             D.ncol = L.ncol;
@@ -211,9 +267,6 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         }
         b->queue.resize(count);
         std::iota(b->queue.begin(), b->queue.end(), 0u);
-        std::stable_sort(b->queue.begin(), b->queue.end(), [&](uint32_t a, uint32_t c) {
-            return b->inst[a].work() > b->inst[c].work();
-        });
         std::vector<uint32_t> list(count);
         std::iota(list.begin(), list.end(), 0u);
         b->d_rp.upload(rp.data(), rp.size(), b->stream);
@@ -223,9 +276,6 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         b->d_cs.upload(cs.data(), cs.size(), b->stream);
         b->d_order.upload(order.data(), order.size(), b->stream);
         b->d_desc.alloc(count);
-        b->d_desc.upload(b->desc.data(), count, b->stream);
-        b->d_queue.alloc(count);
-        b->d_queue.upload(b->queue.data(), count, b->stream);
         b->d_list.alloc(count);
         b->d_list.upload(list.data(), count, b->stream);
         b->d_orders.alloc(count);
@@ -234,7 +284,10 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         std::vector<uint32_t> zt(64, 0);
         b->d_zero_temp.alloc(64);
         b->d_zero_temp.upload(zt.data(), 64, b->stream);
+        const uint32_t zero = 0;
+        b->d_err.upload(&zero, 1, b->stream);
         check(hipStreamSynchronize(b->stream), "batch upload");
+        plan(b.get());
         batch_reset(b.get());
         *out = b.release();
     });
@@ -266,8 +319,7 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
             require(mode == IGE_BATCH_PERSISTENT, "unknown batch mode");
             const uint32_t w = waves ? waves : (b->tuned_waves ? b->tuned_waves : 8);
             set_progress(b, k0);
-            ige::launch_persistent(w, b->d_desc.p, b->d_queue.p, b->count, b->d_next.p, k1 - k0,
-                                   b->num_cus, b->stream);
+            ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
         }
     });
 }
@@ -285,12 +337,10 @@ int32_t ige_batch_autotune(ige_batch* b, uint32_t probe_sweeps, uint32_t* waves_
         uint32_t best_w = 8;
         for (int i = 0; i < 5; ++i) {
             set_progress(b, 0);
-            ige::launch_persistent(cand[i], b->d_desc.p, b->d_queue.p, b->count, b->d_next.p,
-                                   probe_sweeps, b->num_cus, b->stream);  // warm
+            ige::launch_persistent(cand[i], launch_args(b, probe_sweeps), b->stream);  // warm
             set_progress(b, 0);
             check(hipEventRecord(e0, b->stream), "event");
-            ige::launch_persistent(cand[i], b->d_desc.p, b->d_queue.p, b->count, b->d_next.p,
-                                   probe_sweeps, b->num_cus, b->stream);
+            ige::launch_persistent(cand[i], launch_args(b, probe_sweeps), b->stream);
             check(hipEventRecord(e1, b->stream), "event");
             check(hipEventSynchronize(e1), "event");
             float ms = 0.f;
@@ -313,6 +363,21 @@ int32_t ige_batch_synchronize(ige_batch* b) {
     return guarded([&] {
         require(b != nullptr, "NULL batch");
         check(hipStreamSynchronize(b->stream), "batch synchronize");
+        check_error_flag(b);
+    });
+}
+
+int32_t ige_batch_configure(ige_batch* b, uint32_t max_gang, uint32_t gang_sites,
+                            uint32_t spin_limit) {
+    return guarded([&] {
+        require(b != nullptr, "NULL batch");
+        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 32]");
+        check(hipSetDevice(b->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->stream), "configure");
+        b->max_gang = max_gang;
+        b->gang_sites = gang_sites ? gang_sites : ige::kGangSites;
+        b->spin_limit = spin_limit;
+        plan(b);
     });
 }
 
@@ -332,6 +397,7 @@ int32_t ige_batch_read(ige_batch* b, uint32_t index, uint64_t* words, int64_t* e
         const ige::LoweredInstance& L = b->inst[index];
         const ige::InstDesc& D = b->desc[index];
         check(hipStreamSynchronize(b->stream), "batch read");
+        check_error_flag(b);
         if (words) {
             std::vector<uint64_t> w(L.n);
             check(hipMemcpy(w.data(), D.spins, L.n * sizeof(uint64_t), hipMemcpyDeviceToHost),

@@ -94,32 +94,56 @@ inline LoweredInstance lower_instance(const ige_instance_desc& d, const std::str
     return L;
 }
 
+// Everything a persistent launch needs besides the descriptors.
+struct PersistentLaunch {
+    const InstDesc* insts = nullptr;
+    const uint32_t* queue = nullptr;  // tickets: instance << 5 | member
+    uint32_t tickets = 0;
+    unsigned int* next = nullptr;
+    uint32_t* bars = nullptr;         // one counter per instance
+    uint32_t nbars = 0;
+    uint32_t* err = nullptr;
+    uint32_t round = 1;
+    uint32_t spin_limit = 1u << 24;
+    int num_cus = 1;
+};
+
 template <int WAVES>
-void launch_persistent_waves(const InstDesc* insts, const uint32_t* queue, uint32_t count,
-                             unsigned int* next, uint32_t round, int num_cus, hipStream_t st) {
+void launch_persistent_waves(const PersistentLaunch& L, hipStream_t st) {
     int per_cu = 0;
     check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<WAVES>,
                                                        WAVES * 64, 0),
           "occupancy");
-    const uint32_t blocks =
-        std::max(1u, std::min<uint32_t>(count, static_cast<uint32_t>(per_cu * num_cus)));
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, next, 0u, 1u);
+    if (per_cu < 1) throw Failure(IGE_ERR_HIP, "persistent kernel does not fit on a CU");
+    // Every workgroup of the launch must be resident (gang waits rely on it).
+    const uint32_t blocks = std::max(
+        1u, std::min<uint32_t>(L.tickets, static_cast<uint32_t>(per_cu * L.num_cus)));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, L.next, 0u, 1u);
+    if (L.nbars)
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((L.nbars + 255) / 256), dim3(256), 0, st, L.bars,
+                           0u, L.nbars);
     hipLaunchKernelGGL(persistent_anneal_kernel<WAVES>, dim3(blocks), dim3(WAVES * 64), 0, st,
-                       insts, queue, count, next, round);
+                       L.insts, L.queue, L.tickets, L.next, L.round, L.spin_limit, L.err);
     check(hipGetLastError(), "persistent launch");
 }
 
-inline void launch_persistent(uint32_t waves, const InstDesc* insts, const uint32_t* queue,
-                              uint32_t count, unsigned int* next, uint32_t round, int num_cus,
-                              hipStream_t st) {
+inline void launch_persistent(uint32_t waves, const PersistentLaunch& L, hipStream_t st) {
     switch (waves) {
-        case 1: launch_persistent_waves<1>(insts, queue, count, next, round, num_cus, st); break;
-        case 2: launch_persistent_waves<2>(insts, queue, count, next, round, num_cus, st); break;
-        case 4: launch_persistent_waves<4>(insts, queue, count, next, round, num_cus, st); break;
-        case 8: launch_persistent_waves<8>(insts, queue, count, next, round, num_cus, st); break;
-        case 16: launch_persistent_waves<16>(insts, queue, count, next, round, num_cus, st); break;
+        case 1: launch_persistent_waves<1>(L, st); break;
+        case 2: launch_persistent_waves<2>(L, st); break;
+        case 4: launch_persistent_waves<4>(L, st); break;
+        case 8: launch_persistent_waves<8>(L, st); break;
+        case 16: launch_persistent_waves<16>(L, st); break;
         default: throw Failure(IGE_ERR_INVALID, "waves per instance must be 1, 2, 4, 8 or 16");
     }
+}
+
+// Tickets for instances in the given order: gang[i] consecutive tickets each.
+inline void build_tickets(const std::vector<uint32_t>& order, const std::vector<uint32_t>& gang,
+                          std::vector<uint32_t>& tickets) {
+    tickets.clear();
+    for (uint32_t id : order)
+        for (uint32_t m = 0; m < gang[id]; ++m) tickets.push_back((id << 5) | m);
 }
 
 }  // namespace ige

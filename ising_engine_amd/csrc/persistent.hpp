@@ -35,12 +35,16 @@ struct InstDesc {
     uint32_t n, ncol, sweeps, dtab, key0, key1;
     uint32_t round;        // sweeps to advance in this launch (0: the launch's default)
     uint32_t thr_mode;     // kThrLds, kThrGlobal or kThrDirect
+    uint32_t gang;         // workgroups cooperating on this instance (1..kMaxGang)
+    uint32_t* bar;         // gang barrier counter, zero at launch start (gang > 1)
     double beta_hi;
     double c;  // beta_k = beta_hi * det_exp(-(sweeps - 1 - k) * c)
     int64_t offset;
 };
 
 constexpr uint32_t kLdsThrEntries = 4096;
+constexpr uint32_t kMaxGang = 32;        // queue codes carry the member index in 5 bits
+constexpr uint32_t kGangSites = 2048;    // sites per workgroup before a gang is formed
 // Where a workgroup keeps the per-sweep thresholds thr[d], d <= dtab.
 constexpr uint32_t kThrLds = 0;     // dtab < kLdsThrEntries: table in LDS
 constexpr uint32_t kThrGlobal = 1;  // large range, enough sites to amortise a table
@@ -64,10 +68,48 @@ __host__ __device__ __forceinline__ double anneal_beta(double beta_hi, double c,
     return sweeps > 1 ? beta_hi * det_exp(-(double)(sweeps - 1 - k) * c) : beta_hi;
 }
 
+__host__ __device__ __forceinline__ uint32_t choose_gang(uint32_t n, uint32_t max_gang,
+                                                        uint32_t gang_sites) {
+    const uint32_t g = n / (gang_sites ? gang_sites : kGangSites);
+    const uint32_t cap = max_gang < kMaxGang ? max_gang : kMaxGang;
+    return g < 2 ? 1u : (g < cap ? g : cap);
+}
+
+// Barrier across the `gang` workgroups of one instance: every wave releases
+// its stores at device scope, one thread per workgroup arrives on the
+// counter and waits (bounded by spin_limit polls; on expiry the launch's
+// error flag is raised and every later wait is skipped), then every wave
+// acquires at device scope so that neighbour words written by the other
+// workgroups are read from L2, not from a stale L1 line.
+__device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
+                                             uint32_t spin_limit, uint32_t* err) {
+    __threadfence();
+    __syncthreads();
+    target += gang;
+    if (threadIdx.x == 0) {
+        atomicAdd(bar, 1u);
+        uint32_t polls = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+            if (++polls > spin_limit) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+// queue[t] = instance << 5 | member.  A gang's tickets are consecutive, so at
+// most one gang is incomplete at any time and it is completed by the next
+// workgroups to finish their work: with every workgroup of the launch
+// resident and gang <= grid size, every wait ends.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
     const InstDesc* __restrict__ insts, const uint32_t* __restrict__ queue, uint32_t count,
-    unsigned int* next, uint32_t round) {
+    unsigned int* next, uint32_t round, uint32_t spin_limit, uint32_t* err) {
     __shared__ uint32_t thr_lds[kLdsThrEntries];
     __shared__ int64_t part[WAVES][64];
     __shared__ uint32_t s_item;
@@ -79,9 +121,13 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= count) break;
-        const InstDesc I = insts[queue[item]];
+        const uint32_t code = queue[item];
+        const InstDesc I = insts[code >> 5];
+        const uint32_t member = code & 31u;
+        const uint32_t gang = I.gang;
+        uint32_t bar_target = 0;
         const uint32_t mode = I.thr_mode;
-        uint32_t* gthr = I.thr_scratch;
+        uint32_t* gthr = I.thr_scratch + (mode == kThrGlobal ? member * (I.dtab + 1) : 0u);
         const uint32_t k0 = *I.progress;
         const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
         int64_t eacc = 0;
@@ -102,16 +148,21 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
                                        : mode == kThrGlobal ? gthr[de]
                                                             : metropolis_threshold(beta, (int64_t)de);
             };
+            const uint32_t stride = gang * WAVES;
+            const uint32_t lane_of_gang = member * WAVES + wave;
             for (uint32_t c = 0; c < I.ncol; ++c) {
                 const uint32_t cb = I.cs[c], ce = I.cs[c + 1];
                 if (ce > cb) {
-                    for (uint32_t g = (cb >> 2) + wave; g <= ((ce - 1) >> 2); g += WAVES) {
+                    for (uint32_t g = (cb >> 2) + lane_of_gang; g <= ((ce - 1) >> 2); g += stride) {
                         const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
-                        generic_positions(I.rp, I.ci, I.wt, I.lin, I.spins, I.spins, p0, p1, 0u,
-                                          k, I.key0, I.key1, lane, thr, eacc, nacc);
+                        generic_group(I.rp, I.ci, I.wt, I.lin, I.spins, I.spins, p0, p1, 0u, k,
+                                      I.key0, I.key1, lane, thr, eacc, nacc);
                     }
                 }
-                __syncthreads();
+                if (gang > 1)
+                    gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+                else
+                    __syncthreads();
             }
         }
         part[wave][lane] = eacc;
@@ -122,9 +173,14 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
             int64_t tot = 0;
 #pragma unroll
             for (int w = 0; w < WAVES; ++w) tot += part[w][lane];
-            I.energy[lane] += tot;
-            if (lane == 0) *I.progress = k1;
+            if (gang > 1)
+                atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + lane),
+                          (unsigned long long)tot);
+            else
+                I.energy[lane] += tot;
         }
+        if (gang > 1) gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+        if (member == 0 && threadIdx.x == 0) *I.progress = k1;
         __syncthreads();
     }
 }

@@ -95,7 +95,7 @@ class ArenaAllocator {
 // Byte layout of one request inside the arena; each region 256-B aligned.
 struct Layout {
     uint64_t rp, ci, wt, lin, cs, order, spins, energy, accepted, progress, thr, total;
-    explicit Layout(const ige::LoweredInstance& L) {
+    Layout(const ige::LoweredInstance& L, uint32_t gang) {
         uint64_t o = 0;
         auto take = [&](uint64_t bytes) {
             const uint64_t at = o;
@@ -112,7 +112,9 @@ struct Layout {
         energy = take(64 * 8);
         accepted = take(8);
         progress = take(4);
-        thr = ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal ? take((L.dtab + 1ull) * 4) : o;
+        thr = ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal
+                  ? take(static_cast<uint64_t>(gang) * (L.dtab + 1ull) * 4)
+                  : o;
         total = o;
     }
 };
@@ -124,6 +126,7 @@ struct Request {
     int32_t status = IGE_REQ_QUEUED;
     uint64_t off = 0, bytes = 0;
     ige::InstDesc D{};
+    uint32_t gang = 1;
     uint32_t progress = 0;
     std::vector<uint8_t> staging;  // host image of the model regions (kept until uploaded)
     std::vector<uint64_t> words;   // result, site order
@@ -140,10 +143,11 @@ struct ige_server {
     uint32_t waves = 8;
     double round_ms = 10.0;
     uint32_t max_live = 4096;
+    uint32_t max_gang = 16, gang_sites = ige::kGangSites, spin_limit = 1u << 24;
     ArenaAllocator arena{0};
     uint8_t* d_arena = nullptr;
     DevBuf<ige::InstDesc> d_desc;
-    DevBuf<uint32_t> d_queue, d_list;
+    DevBuf<uint32_t> d_queue, d_list, d_bars, d_err;
     DevBuf<const uint32_t*> d_orders;
     DevBuf<unsigned int> d_next;
 
@@ -181,7 +185,7 @@ namespace {
 
 void upload_request(ige_server* s, Request& r) {
     const ige::LoweredInstance& L = r.L;
-    const Layout lay(L);
+    const Layout lay(L, r.gang);
     r.staging.assign(lay.spins, 0);  // model regions only: rp .. order
     auto put = [&](uint64_t at, const void* src, uint64_t bytes) {
         if (bytes) std::memcpy(r.staging.data() + at, src, bytes);
@@ -216,6 +220,8 @@ void upload_request(ige_server* s, Request& r) {
     D.key0 = static_cast<uint32_t>(L.seed);
     D.key1 = static_cast<uint32_t>(L.seed >> 32);
     D.round = 1;
+    D.gang = r.gang;
+    D.bar = nullptr;  // set per round
     D.beta_hi = L.beta_hi;
     D.c = L.c;
     D.offset = L.offset;
@@ -257,7 +263,7 @@ void serve_loop(ige_server* s) {
                 // 1. admission: FIFO while the head fits (no overtaking, no starvation)
                 while (!s->pending.empty() && s->live.size() < s->max_live) {
                     Request& r = *s->reqs[s->pending.front()];
-                    const Layout lay(r.L);
+                    const Layout lay(r.L, r.gang);
                     uint64_t off = 0;
                     if (!s->arena.alloc(lay.total, &off)) break;
                     r.off = off;
@@ -280,27 +286,34 @@ void serve_loop(ige_server* s) {
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
                 const uint64_t per_sweep = static_cast<uint64_t>(r.L.n) * 64;
-                uint32_t k = static_cast<uint32_t>(std::max(1.0, per_block / per_sweep));
+                uint32_t k = static_cast<uint32_t>(std::max(1.0, per_block * r.gang / per_sweep));
                 k = std::min(k, r.L.sweeps - r.progress);
                 r.D.round = k;
                 descs[i] = r.D;
                 round_attempts += per_sweep * k;
             }
             std::vector<std::pair<uint64_t, uint32_t>> order_by_work(L);
-            for (size_t i = 0; i < L; ++i)
-                order_by_work[i] = {static_cast<uint64_t>(s->live[i]->L.n) * descs[i].round,
+            std::vector<uint32_t> gangs(L), ord(L);
+            for (size_t i = 0; i < L; ++i) {
+                gangs[i] = s->live[i]->gang;
+                order_by_work[i] = {static_cast<uint64_t>(s->live[i]->L.n) * descs[i].round /
+                                        gangs[i],
                                     static_cast<uint32_t>(i)};
+            }
             std::sort(order_by_work.begin(), order_by_work.end(),
                       [](const auto& a, const auto& b) { return a.first > b.first; });
-            for (size_t i = 0; i < L; ++i) queue[i] = order_by_work[i].second;
+            for (size_t i = 0; i < L; ++i) ord[i] = order_by_work[i].second;
+            ige::build_tickets(ord, gangs, queue);
             if (s->d_desc.count < L) {
                 s->d_desc.alloc(std::max<size_t>(L, 2 * s->d_desc.count));
-                s->d_queue.alloc(s->d_desc.count);
                 s->d_list.alloc(s->d_desc.count);
                 s->d_orders.alloc(s->d_desc.count);
+                s->d_bars.alloc(s->d_desc.count);
             }
+            if (s->d_queue.count < queue.size()) s->d_queue.alloc(std::max<size_t>(queue.size(), 2 * s->d_queue.count));
+            for (size_t i = 0; i < L; ++i) descs[i].bar = s->d_bars.p + i;
             s->d_desc.upload(descs.data(), L, s->stream);
-            s->d_queue.upload(queue.data(), L, s->stream);
+            s->d_queue.upload(queue.data(), queue.size(), s->stream);
             if (!admitted.empty()) {
                 list.clear();
                 orders.assign(L, nullptr);
@@ -308,7 +321,7 @@ void serve_loop(ige_server* s) {
                     if (s->live[i]->status == IGE_REQ_RUNNING && s->live[i]->progress == 0 &&
                         std::find(admitted.begin(), admitted.end(), s->live[i]) != admitted.end()) {
                         list.push_back(static_cast<uint32_t>(i));
-                        const Layout lay(s->live[i]->L);
+                        const Layout lay(s->live[i]->L, s->live[i]->gang);
                         orders[i] = reinterpret_cast<const uint32_t*>(s->d_arena + s->live[i]->off +
                                                                       lay.order);
                     }
@@ -321,10 +334,25 @@ void serve_loop(ige_server* s) {
                 check(hipGetLastError(), "request init");
             }
             check(hipEventRecord(e0, s->stream), "event");
-            ige::launch_persistent(s->waves, s->d_desc.p, s->d_queue.p, static_cast<uint32_t>(L),
-                                   s->d_next.p, 1, s->num_cus, s->stream);
+            ige::PersistentLaunch pl;
+            pl.insts = s->d_desc.p;
+            pl.queue = s->d_queue.p;
+            pl.tickets = static_cast<uint32_t>(queue.size());
+            pl.next = s->d_next.p;
+            pl.bars = s->d_bars.p;
+            pl.nbars = static_cast<uint32_t>(L);
+            pl.err = s->d_err.p;
+            pl.round = 1;
+            pl.spin_limit = s->spin_limit;
+            pl.num_cus = s->num_cus;
+            ige::launch_persistent(s->waves, pl, s->stream);
             check(hipEventRecord(e1, s->stream), "event");
             check(hipEventSynchronize(e1), "round");
+            {
+                uint32_t e = 0;
+                check(hipMemcpy(&e, s->d_err.p, sizeof(e), hipMemcpyDeviceToHost), "error flag");
+                if (e) throw Failure(IGE_ERR_HIP, "a gang barrier wait exceeded its spin limit");
+            }
             float ms = 0.f;
             check(hipEventElapsedTime(&ms, e0, e1), "event");
             // refine the per-workgroup rate used to size the next round
@@ -332,8 +360,8 @@ void serve_loop(ige_server* s) {
                 int per_cu = 0;
                 (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
                     &per_cu, ige::persistent_anneal_kernel<8>, 512, 0);
-                const double blocks =
-                    std::max(1.0, std::min<double>(L, std::max(1, per_cu) * s->num_cus));
+                const double blocks = std::max(
+                    1.0, std::min<double>(queue.size(), std::max(1, per_cu) * s->num_cus));
                 if (ms > 0.05)
                     s->attempts_per_ms_per_block =
                         0.5 * s->attempts_per_ms_per_block + 0.5 * (round_attempts / ms / blocks);
@@ -425,9 +453,27 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->d_queue.alloc(256);
         s->d_list.alloc(256);
         s->d_orders.alloc(256);
+        s->d_bars.alloc(256);
+        s->d_err.alloc(1);
+        const uint32_t zero = 0;
+        s->d_err.upload(&zero, 1, s->stream);
+        check(hipStreamSynchronize(s->stream), "server init");
         ige_server* raw = s.get();
         s->loop = std::thread(serve_loop, raw);
         *out = s.release();
+    });
+}
+
+int32_t ige_server_configure(ige_server* s, uint32_t max_gang, uint32_t gang_sites,
+                             uint32_t spin_limit) {
+    return guarded([&] {
+        require(s != nullptr, "NULL server");
+        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 32]");
+        std::lock_guard<std::mutex> g(s->mu);
+        require(s->reqs.empty(), "configure before submitting requests");
+        s->max_gang = max_gang;
+        s->gang_sites = gang_sites ? gang_sites : ige::kGangSites;
+        s->spin_limit = spin_limit;
     });
 }
 
@@ -444,7 +490,8 @@ int32_t ige_server_submit(ige_server* s, const ige_instance_desc* d, uint64_t* i
         auto r = std::make_unique<Request>();
         r->t_submit = Clock::now();
         r->L = ige::lower_instance(*d, "request");
-        const Layout lay(r->L);
+        r->gang = ige::choose_gang(r->L.n, s->max_gang, s->gang_sites);
+        const Layout lay(r->L, r->gang);
         std::lock_guard<std::mutex> g(s->mu);
         if (!s->fatal.empty()) throw Failure(IGE_ERR_HIP, "server stopped: " + s->fatal);
         if (lay.total > s->arena.capacity())

@@ -109,3 +109,50 @@ def test_single_state_with_the_schedule_reproduces_an_instance(ctx):
         words, e, _ = batch.read(i)
         np.testing.assert_array_equal(st.spins()[0], words)
         np.testing.assert_array_equal(st.energies()[0], e)
+
+
+def test_gangs_give_identical_results(ctx):
+    """Large instances split over a gang of workgroups meeting at device-wide barriers
+    produce the same words and energies as one workgroup per instance."""
+    qubos, seeds, sw, b_lo, b_hi = small_batch(count=14, sweeps=16, seed=11)
+    batch = _lib.Batch(ctx, qubos, seeds, sw, b_lo, b_hi)
+    batch.configure(max_gang=1)
+    batch.run(0, waves=4)
+    ref = [batch.read(i) for i in range(14)]
+    for max_gang, sites, waves in [(4, 64, 4), (8, 32, 2), (3, 100, 8)]:
+        batch.configure(max_gang=max_gang, gang_sites=sites)
+        batch.reset()
+        batch.run(0, waves=waves)
+        batch.synchronize()
+        for i in range(14):
+            got = batch.read(i)
+            np.testing.assert_array_equal(got[0], ref[i][0], err_msg=f"gang {max_gang} inst {i}")
+            np.testing.assert_array_equal(got[1], ref[i][1], err_msg=f"gang {max_gang} inst {i}")
+            assert got[2] == ref[i][2]
+
+
+def test_expired_gang_wait_is_reported_not_hung(ctx):
+    """A zero spin limit makes a waiting workgroup give up at once: the launch still ends
+    and the failure surfaces as an error; the batch works again once reconfigured."""
+    qubos, seeds, sw, b_lo, b_hi = small_batch(count=6, sweeps=12, seed=13)
+    big = [instances.random_qubo(600, 6, 77 + i) for i in range(2)]
+    qubos = qubos + big
+    seeds = list(seeds) + [5, 6]
+    sw = sw + [12, 12]
+    for qb in big:
+        tmax, tmin = instances.suggested_temp_range(qb)
+        b_lo.append(1.0 / tmax)
+        b_hi.append(1.0 / tmin)
+    batch = _lib.Batch(ctx, qubos, seeds, sw, b_lo, b_hi)
+    batch.configure(max_gang=8, gang_sites=50, spin_limit=0)
+    batch.run(0, waves=1)
+    with pytest.raises(_lib.IgeError) as ei:
+        batch.synchronize()
+    assert ei.value.code == _lib.IGE_ERR_HIP
+    batch.configure(max_gang=1)
+    batch.reset()
+    batch.run(0, waves=1)
+    batch.synchronize()
+    words, e, _ = batch.read(7)
+    csr = ob.CSR.from_qubo(big[1])
+    np.testing.assert_array_equal(csr.energies(words[None, :])[0], e)

@@ -93,3 +93,19 @@ def test_request_validation_mirrors_the_reference(ctx):
     with pytest.raises(_lib.IgeError):
         srv.submit(qb, 1, 10, 0.1, 1.0)                               # num_vars > 50000
     srv.close()
+
+
+def test_gangs_in_the_serving_loop(ctx):
+    reqs = requests(10, 6, n_lo=500, n_hi=3000, sweeps_lo=20, sweeps_hi=60)
+    plain = _lib.Server(ctx, hbm_bytes=256 << 20, round_ms=2.0)
+    plain.configure(max_gang=1)
+    ref = [result(plain, submit(plain, r)) for r in reqs]
+    plain.close()
+    gang = _lib.Server(ctx, hbm_bytes=256 << 20, round_ms=2.0)
+    gang.configure(max_gang=8, gang_sites=256)
+    ids = [submit(gang, r) for r in reqs]
+    for (i, rid) in enumerate(ids):
+        got = result(gang, rid)
+        np.testing.assert_array_equal(got[1], ref[i][1])
+        np.testing.assert_array_equal(got[2], ref[i][2])
+    gang.close()
