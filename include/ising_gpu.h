@@ -167,7 +167,8 @@ int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *wa
 /* Large instances run on a gang of workgroups (instance sites / gang_sites of
  * them, at most max_gang <= 32) that meet at a device-wide barrier after every
  * colour class.  spin_limit bounds each barrier wait; an expired wait makes
- * the next synchronize/read fail with IGE_ERR_HIP.  Defaults: 16, 2048, 2^24. */
+ * the next synchronize/read fail with IGE_ERR_HIP.  Batch defaults: 1 (no gangs:
+ * throughput), 2048, 2^24; server defaults: 16, 2048, 2^24 (latency). */
 int32_t ige_batch_configure(ige_batch *batch, uint32_t max_gang, uint32_t gang_sites,
                             uint32_t spin_limit);
 int32_t ige_batch_synchronize(ige_batch *batch);

@@ -44,7 +44,7 @@ struct ige_batch {
     uint64_t total_sites = 0, total_attempts = 0;
     uint32_t max_dtab = 0;
     uint32_t tuned_waves = 0;
-    uint32_t max_gang = 16, gang_sites = ige::kGangSites, spin_limit = 1u << 24;
+    uint32_t max_gang = 1, gang_sites = ige::kGangSites, spin_limit = 1u << 24;
     std::vector<uint32_t> gang;
     std::vector<uint32_t> tickets;
     int num_cus = 0;
