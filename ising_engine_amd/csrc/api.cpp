@@ -40,6 +40,7 @@ struct ige_model {
     ige::MultispinShape ms;
     int64_t max_local = 0;
     uint32_t dmax = 0;
+    uint32_t regular_degree = 0;
     DevBuf<uint32_t> rp, ci, order, sgn;
     DevBuf<int32_t> wt, lin;
     ige::DevModel view() const {
@@ -53,6 +54,7 @@ struct ige_model {
         m.sgn = sgn.p;
         m.w0 = ms.w0;
         m.dmax = dmax;
+        m.regular_degree = regular_degree;
         m.offset = ir.offset;
         return m;
     }
@@ -149,6 +151,10 @@ void finish_model(ige_model* m) {
     m->colors = m->ir.greedy_coloring();
     m->co = ige::color_order(m->colors);
     m->ms = ige::multispin_shape(m->ir);
+    m->regular_degree = m->dmax;
+    for (uint32_t i = 0; i < m->ir.n; ++i)
+        if (m->ir.row_ptr[i + 1] - m->ir.row_ptr[i] != m->dmax) m->regular_degree = 0;
+    if (m->regular_degree != 4 && m->regular_degree != 6) m->regular_degree = 0;
     upload_model(m);
 }
 
@@ -274,7 +280,7 @@ int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, u
             s->dtab = static_cast<uint32_t>(model->max_local);
             s->thr.alloc(static_cast<size_t>(cells) * (s->dtab + 1));
         } else {
-            s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * 33);
+            s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * 39);
         }
         std::vector<unsigned long long> zeros(s->counters.count, 0);
         s->counters.upload(zeros.data(), zeros.size(), s->stream);

@@ -125,17 +125,27 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(
 // |q_ij| = w0 for every bond and lin_i = -1/2 sum_j q_ij, so that
 //   dE_i(replica r) = w0 (k_r - d_i / 2),
 //   k_r = #{ j : bit r of (x_i ^ x_j ^ [q_ij < 0]) }.
-// A lane counts k for its 64 replicas in B bit planes, accepts dE <= 0
-// outright, and resolves u_r < thr_r for the rest by comparing the uniform's
-// bit planes (Philox, plane view) against the threshold's bit planes from
-// most to least significant, stopping once every replica is decided.
-// Per-replica energy changes are summed in bit-sliced counters and turned
-// into per-replica integers by one 64x64 bit transpose per counter plane.
+// A lane counts k for its 64 replicas in B bit planes and accepts dE <= 0
+// outright.  For the rest it compares the uniform u_r with the threshold
+// thr_r six bits at a time: six Philox bit planes against the threshold's
+// six leading bit planes, every lane in lockstep, no early exit.  A replica
+// whose leading bits tie (probability 2^-6) goes to a per-wave queue in LDS;
+// the queue is drained by all 64 lanes at once, each comparing the low 26
+// bits of its entry's tail word with the threshold's low bits, and accepted
+// entries flip their word with an atomic XOR.  Per-replica energy changes of
+// the lockstep flips are summed in bit-sliced counters and turned into
+// integers by one 64x64 bit transpose per counter plane; queued flips add
+// theirs through an LDS accumulator.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kQStride = 33;  // 32 threshold bit planes + nonzero mask
+constexpr uint32_t kEagerPlanes = 6;
+// Per (slot, level m): planes 0..5 of the thresholds, the nonzero mask, and
+// the 64 thresholds themselves (two per u64).
+constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
+constexpr uint32_t kQueueCap = 512;  // deferred entries per wave
+constexpr uint32_t kTagTail = 5u;
 
 // 64x64 bit-matrix transpose across the wave: lane l holds row l on entry,
-// column l on exit (bit k of the result = bit l of row k).
+// column l on exit (bit k of the result = bit l of row k).  Branch-free.
 __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
     const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
                                0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
@@ -145,7 +155,10 @@ __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
         const uint32_t j = 32u >> s;
         const uint64_t m = masks[s];
         const uint64_t y = __shfl_xor(x, (int)j, 64);
-        x = (lane & j) ? ((x & ~m) | ((y & ~m) >> j)) : ((x & m) | ((y & m) << j));
+        const uint64_t hi = (lane & j) ? ~0ull : 0ull;  // 1s in lanes with bit j set
+        const uint64_t keep = (x & m & ~hi) | (x & ~m & hi);
+        const uint64_t take = (((y & m) << j) & ~hi) | (((y >> j) & m) & hi);
+        x = keep | take;
     }
     return x;
 }
@@ -161,28 +174,52 @@ __device__ __forceinline__ int64_t wave_column_sum(const uint64_t (&acc)[W], uin
     return tot;
 }
 
-template <int B, int W>
+// Resolves one queued replica: u_r's six leading bits tie with thr_r's, so
+// u_r < thr_r iff tail >> 6 < thr_r's low 26 bits.
+__device__ __forceinline__ bool resolve_tail(uint32_t p, uint32_t slot, uint32_t sweep,
+                                             uint32_t key0, uint32_t key1, uint32_t r,
+                                             uint32_t thr) {
+    const U32x4 o = philox4x32_10(p, slot, sweep, (kTagTail << 24) | (r >> 2), key0, key1);
+    const uint32_t q = r & 3u;
+    const uint32_t w = q == 0 ? o.v[0] : q == 1 ? o.v[1] : q == 2 ? o.v[2] : o.v[3];
+    return (w >> kEagerPlanes) < (thr & ((1u << (32 - kEagerPlanes)) - 1u));
+}
+
+// B: bit planes of the neighbour count (degree < 2^B).  W: width of the
+// energy counters.  DEG > 0: every site has exactly DEG neighbours and row p
+// starts at p * DEG (regular lattices); DEG == 0: any degree < 2^B.
+template <int B, int W, int DEG>
 __global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const uint32_t* __restrict__ sgn, const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr,
     uint32_t n, uint32_t dmax, int32_t w0, const unsigned long long* __restrict__ qplanes,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
     uint32_t per_lane, uint32_t sweep, uint32_t key0, uint32_t key1) {
-    constexpr int LMAX = 1 << (B - 1);  // positive levels k in (d/2, d], d < 2^B
+    constexpr int LMAX = DEG > 0 ? DEG - DEG / 2 : (1 << (B - 1));
     extern __shared__ unsigned long long lds_q[];
+    __shared__ uint32_t queue[kWavesPerBlock][kQueueCap];
+    __shared__ uint32_t qcount[kWavesPerBlock];
+    __shared__ unsigned long long edef[kWavesPerBlock][kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wv = uniform(threadIdx.x >> 6);
     const uint32_t slot = blockIdx.y;
     {
         const uint32_t total = (dmax + 1) * kQStride;
         const unsigned long long* src = qplanes + (uint64_t)slot * total;
         for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_q[i] = src[i];
+        edef[wv][lane] = 0;
+        if (lane == 0) qcount[wv] = 0;
         __syncthreads();
     }
-    const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + wv);
     const uint32_t base = c_begin + wave * kWave * per_lane;
     if (base >= c_end) return;
     const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
     uint64_t* Wr = spins_wr + (uint64_t)slot * n;
+    auto thr_of = [&](uint32_t m, uint32_t r) -> uint32_t {
+        const unsigned long long w = lds_q[m * kQStride + kEagerPlanes + 1 + (r >> 1)];
+        return (r & 1u) ? (uint32_t)(w >> 32) : (uint32_t)w;
+    };
 
     uint64_t kacc[W], dacc[W];
 #pragma unroll
@@ -193,12 +230,13 @@ __global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
         const uint32_t p = base + it * kWave + lane;
         if (p >= c_end) break;
         const uint64_t x = S[p];
-        const uint32_t e0 = rp[p], d = rp[p + 1] - e0;
+        const uint32_t e0 = DEG > 0 ? p * (uint32_t)DEG : rp[p];
+        const uint32_t d = DEG > 0 ? (uint32_t)DEG : rp[p + 1] - e0;
         const uint32_t sg = sgn[p];
         uint64_t c[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) c[b] = 0;
-        for (uint32_t e = 0; e < d; ++e) {
+        auto add_neighbour = [&](uint32_t e) {
             uint64_t carry = x ^ S[ci[e0 + e]] ^ (((sg >> e) & 1u) ? ~0ull : 0ull);
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -206,8 +244,14 @@ __global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
                 carry &= c[b];
                 c[b] = s;
             }
+        };
+        if (DEG > 0) {
+#pragma unroll
+            for (int e = 0; e < (DEG > 0 ? DEG : 1); ++e) add_neighbour((uint32_t)e);
+        } else {
+            for (uint32_t e = 0; e < d; ++e) add_neighbour(e);
         }
-        // Level masks for k = floor(d/2) + 1 + j, j < LMAX, k <= d; m = 2k - d.
+        // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
         uint64_t lev[LMAX];
         uint32_t mrow[LMAX];
         uint64_t pos = 0, und = 0;
@@ -218,25 +262,26 @@ __global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
 #pragma unroll
             for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
             lev[j] = eq;
-            const uint32_t mm = k <= d ? 2 * k - d : 0;
-            mrow[j] = mm * kQStride;
+            mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
             pos |= eq;
-            und |= eq & lds_q[mrow[j] + 32];
+            und |= eq & lds_q[mrow[j] + kEagerPlanes];
         }
-        uint64_t lt = 0, P0 = 0, P1 = 0;
-        for (uint32_t b = 0; b < 32 && und != 0; ++b) {
-            if ((b & 1u) == 0) {
-                const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | (b >> 1), key0,
-                                              key1);
-                P0 = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
-                P1 = (uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32);
-            }
-            const uint64_t P = (b & 1u) ? P1 : P0;
-            uint64_t T = 0;
+        // Six lockstep bit planes.
+        uint64_t lt = 0;
 #pragma unroll
-            for (int j = 0; j < LMAX; ++j) T |= lev[j] & lds_q[mrow[j] + b];
-            lt |= und & T & ~P;
-            und &= ~(P ^ T);
+        for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
+            const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t b = 2 * j + h;
+                const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
+                                          : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
+                uint64_t T = 0;
+#pragma unroll
+                for (int l = 0; l < LMAX; ++l) T |= lev[l] & lds_q[mrow[l] + b];
+                lt |= und & T & ~P;
+                und &= ~(P ^ T);
+            }
         }
         const uint64_t flip = ~pos | lt;
         Wr[p] = x ^ flip;
@@ -254,13 +299,47 @@ __global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
             cd = (dacc[b] & ad) | (cd & (dacc[b] ^ ad));
             dacc[b] = sd;
         }
+        // Queue the replicas whose six leading bits tie.
+        while (und) {
+            const uint32_t r = (uint32_t)__builtin_ctzll(und);
+            und &= und - 1;
+            uint32_t m = 0;
+#pragma unroll
+            for (int l = 0; l < LMAX; ++l)
+                if ((lev[l] >> r) & 1u) m = mrow[l] / kQStride;
+            const uint32_t at = atomicAdd(&qcount[wv], 1u);
+            if (at < kQueueCap) {
+                queue[wv][at] = (it << 17) | (lane << 11) | (r << 5) | m;
+            } else if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
+                // queue full: resolve in place
+                atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+                atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
+                nacc += 1;
+            }
+        }
     }
+    // Drain the queue with every lane of the wave.
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t queued = min(qcount[wv], kQueueCap);
+    for (uint32_t q = lane; q < queued; q += kWave) {
+        const uint32_t code = queue[wv][q];
+        const uint32_t it = code >> 17, sl = (code >> 11) & 63u, r = (code >> 5) & 63u,
+                       m = code & 31u;
+        const uint32_t p = base + it * kWave + sl;
+        if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
+            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+            atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
+            nacc += 1;
+        }
+    }
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
     const int64_t ksum = wave_column_sum<W>(kacc, lane);
     const int64_t dsum = wave_column_sum<W>(dacc, lane);
-    const int64_t de = ((int64_t)w0 * (2 * ksum - dsum)) / 2;
+    const int64_t de = ((int64_t)w0 * (2 * ksum - dsum)) / 2 + (int64_t)edef[wv][lane];
     atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
               (unsigned long long)de);
-    // Wave total of accepted flips.
     for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
     if (lane == 0) atomicAdd(accepted, (unsigned long long)nacc);
 }
@@ -279,13 +358,14 @@ __global__ __launch_bounds__(kWave) void qplanes_kernel(const double* __restrict
     unsigned long long* out = qplanes + ((uint64_t)slot * (dmax + 1) + m) * kQStride;
     uint64_t mine = 0;
 #pragma unroll
-    for (uint32_t b = 0; b < 32; ++b) {
+    for (uint32_t b = 0; b < kEagerPlanes; ++b) {
         const uint64_t plane = ballot((th >> (31u - b)) & 1u);
         if (r == b) mine = plane;
     }
     const uint64_t nz = ballot(th != 0);
-    if (r < 32) out[r] = mine;
-    if (r == 32) out[32] = nz;
+    if (r < kEagerPlanes) out[r] = mine;
+    if (r == kEagerPlanes) out[kEagerPlanes] = nz;
+    reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[r] = th;
 }
 
 __global__ void identity_perm_kernel(uint32_t* temp_of, uint32_t* slot_of, uint32_t cells) {
@@ -396,7 +476,7 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
     // Sites per lane: enough waves to fill the chip (~8K), at most 32 so the
-    // bit-sliced energy counters stay narrow.
+    // bit-sliced energy counters stay narrow (32 * 31 < 2^12).
     uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 8192 - 1) /
                         ((uint64_t)kWave * 8192);
     if (per_lane < 1) per_lane = 1;
@@ -405,16 +485,20 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     const dim3 grid(ceil_div(waves, kWavesPerBlock), s.cells);
     const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
-#define IGE_MS_LAUNCH(B, W)                                                                      \
-    hipLaunchKernelGGL((sweep_multispin_kernel<B, W>), grid, dim3(kBlock), lds, stream, m.rp,   \
-                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,    \
+#define IGE_MS_LAUNCH(B, W, DEG)                                                                   \
+    hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp, \
+                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,       \
                        s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1)
-    if (m.dmax < 8) {
-        IGE_MS_LAUNCH(3, 8);
+    if (m.regular_degree == 6) {
+        IGE_MS_LAUNCH(3, 8, 6);
+    } else if (m.regular_degree == 4) {
+        IGE_MS_LAUNCH(3, 8, 4);
+    } else if (m.dmax < 8) {
+        IGE_MS_LAUNCH(3, 8, 0);
     } else if (m.dmax < 16) {
-        IGE_MS_LAUNCH(4, 12);
+        IGE_MS_LAUNCH(4, 12, 0);
     } else if (m.dmax < 32) {
-        IGE_MS_LAUNCH(5, 12);
+        IGE_MS_LAUNCH(5, 12, 0);
     } else {
         return hipErrorInvalidValue;
     }

@@ -19,6 +19,7 @@ struct DevModel {
     const uint32_t* sgn = nullptr;  // n: bit e set iff entry e of the row has q < 0
     int32_t w0 = 0;                 // common |q| for the multispin kernel
     uint32_t dmax = 0;              // max degree
+    uint32_t regular_degree = 0;    // d when every site has degree d and row p starts at p*d
     int64_t offset = 0;
 };
 
@@ -47,9 +48,10 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
                                   uint32_t c_end, uint32_t sweep, uint64_t seed,
                                   const unsigned long long* qplanes, hipStream_t stream);
 
-// qplanes[(slot*(dmax+1) + m)*33 + b], b < 32: bit r = bit (31-b) of
-// metropolis_threshold(beta[temp_of[slot][r]], m*w0/2); b == 32: bit r set
-// iff that threshold is nonzero.
+// Per (slot, m), m <= dmax, a record of 39 u64 at qplanes[(slot*(dmax+1)+m)*39]:
+// words 0..5: bit r = bit (31-b) of thr_r = metropolis_threshold(beta of the
+// temperature (slot, r) holds, m*w0/2); word 6: bit r set iff thr_r != 0;
+// words 7..38: the 64 thr_r as u32.
 hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
                           unsigned long long* qplanes, hipStream_t stream);
 

@@ -55,6 +55,9 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 #define TAG_PLANE 2u
 #define TAG_EXCH 3u
 #define TAG_INIT 4u
+#define TAG_TAIL 5u
+/* Bit planes drawn for every replica before its tail word is consulted. */
+#define EAGER_PLANES 6u
 
 static void philox_seeded(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                           uint32_t out[4]) {
@@ -270,18 +273,25 @@ static void uniforms_word(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t s
     }
 }
 
+/* Plane view: the top EAGER_PLANES bits of u_r are bit r of planes
+ * P_0..P_5 (P_{2j}, P_{2j+1} = words 0|1 and 2|3 of Philox(pos, slot, sweep,
+ * TAG_PLANE<<24 | j)), most significant first; the remaining 26 bits are the
+ * top bits of the replica's tail word Philox(pos, slot, sweep,
+ * TAG_TAIL<<24 | r>>2)[r & 3]. */
 static void uniforms_plane(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t sweep,
                            uint32_t u[64]) {
-    uint64_t plane[32];
+    uint64_t plane[EAGER_PLANES];
     uint32_t o[4];
-    for (uint32_t j = 0; j < 16; ++j) {
+    for (uint32_t j = 0; j < EAGER_PLANES / 2; ++j) {
         philox_seeded(seed, pos, slot, sweep, (TAG_PLANE << 24) | j, o);
         plane[2 * j] = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
         plane[2 * j + 1] = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
     }
     for (uint32_t r = 0; r < 64; ++r) {
         uint32_t v = 0;
-        for (uint32_t b = 0; b < 32; ++b) v |= (uint32_t)bit(plane[b], r) << (31u - b);
+        for (uint32_t b = 0; b < EAGER_PLANES; ++b) v |= (uint32_t)bit(plane[b], r) << (31u - b);
+        philox_seeded(seed, pos, slot, sweep, (TAG_TAIL << 24) | (r >> 2), o);
+        v |= o[r & 3u] >> EAGER_PLANES;
         u[r] = v;
     }
 }
