@@ -189,7 +189,7 @@ __device__ __forceinline__ bool resolve_tail(uint32_t p, uint32_t slot, uint32_t
 // energy counters.  DEG > 0: every site has exactly DEG neighbours and row p
 // starts at p * DEG (regular lattices); DEG == 0: any degree < 2^B.
 template <int B, int W, int DEG>
-__global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
+__global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const uint32_t* __restrict__ sgn, const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr,
     uint32_t n, uint32_t dmax, int32_t w0, const unsigned long long* __restrict__ qplanes,
@@ -245,9 +245,30 @@ __global__ __launch_bounds__(kBlock) void sweep_multispin_kernel(
                 c[b] = s;
             }
         };
-        if (DEG > 0) {
-#pragma unroll
-            for (int e = 0; e < (DEG > 0 ? DEG : 1); ++e) add_neighbour((uint32_t)e);
+        auto bond = [&](uint32_t e) {
+            return x ^ S[ci[e0 + e]] ^ (((sg >> e) & 1u) ? ~0ull : 0ull);
+        };
+        if (DEG == 6 || DEG == 4) {
+            // carry-save adder tree: c = popcount of the bond words, bit-sliced
+            auto fa = [](uint64_t a, uint64_t b, uint64_t cin, uint64_t& sum, uint64_t& cout) {
+                const uint64_t t = a ^ b;
+                sum = t ^ cin;
+                cout = (a & b) | (cin & t);
+            };
+            uint64_t s1, c1;
+            fa(bond(0), bond(1), bond(2), s1, c1);
+            if (DEG == 6) {
+                uint64_t s2, c2;
+                fa(bond(3), bond(4), bond(5), s2, c2);
+                c[0] = s1 ^ s2;
+                fa(c1, c2, s1 & s2, c[1], c[2]);
+            } else {
+                const uint64_t k3 = bond(3);
+                c[0] = s1 ^ k3;
+                const uint64_t t = s1 & k3;
+                c[1] = c1 ^ t;
+                c[2] = c1 & t;
+            }
         } else {
             for (uint32_t e = 0; e < d; ++e) add_neighbour(e);
         }
