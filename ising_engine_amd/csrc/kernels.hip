@@ -496,10 +496,11 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
                                   const unsigned long long* qplanes, hipStream_t stream) {
     if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
-    // Sites per lane: enough waves to fill the chip (~8K), at most 32 so the
-    // bit-sliced energy counters stay narrow (32 * 31 < 2^12).
-    uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 8192 - 1) /
-                        ((uint64_t)kWave * 8192);
+    // Sites per lane: about 4096 waves (four per SIMD, all resident at once,
+    // which measured fastest on 3D L=64 x 32 slots: 16 sites per lane) and at
+    // most 32 so the bit-sliced energy counters stay narrow (32 * 31 < 2^12).
+    uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 4096 - 1) /
+                        ((uint64_t)kWave * 4096);
     if (per_lane < 1) per_lane = 1;
     if (per_lane > 32) per_lane = 32;
     const uint32_t waves = ceil_div(len, kWave * per_lane);
