@@ -142,6 +142,7 @@ constexpr uint32_t kEagerPlanes = 6;
 // the 64 thresholds themselves (two per u64).
 constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
 constexpr uint32_t kQueueCap = 512;  // deferred entries per wave
+constexpr uint32_t kMaxPerLane = 16; // sites per lane
 constexpr uint32_t kTagTail = 5u;
 
 // 64x64 bit-matrix transpose across the wave: lane l holds row l on entry,
@@ -200,6 +201,8 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     __shared__ uint32_t queue[kWavesPerBlock][kQueueCap];
     __shared__ uint32_t qcount[kWavesPerBlock];
     __shared__ unsigned long long edef[kWavesPerBlock][kWave];
+    __shared__ unsigned long long blk_e[kWave];
+    __shared__ uint32_t blk_done, blk_acc;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wv = uniform(threadIdx.x >> 6);
     const uint32_t slot = blockIdx.y;
@@ -209,11 +212,17 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_q[i] = src[i];
         edef[wv][lane] = 0;
         if (lane == 0) qcount[wv] = 0;
+        if (wv == 0) blk_e[lane] = 0;
+        if (threadIdx.x == 0) blk_done = blk_acc = 0;
         __syncthreads();
     }
     const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + wv);
     const uint32_t base = c_begin + wave * kWave * per_lane;
     if (base >= c_end) return;
+    // waves of this block that have sites (the rest returned above)
+    const uint32_t first_wave = blockIdx.x * kWavesPerBlock;
+    const uint32_t span = kWave * per_lane;
+    const uint32_t active = min(kWavesPerBlock, (c_end - c_begin + span - 1) / span - first_wave);
     const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
     uint64_t* Wr = spins_wr + (uint64_t)slot * n;
     auto thr_of = [&](uint32_t m, uint32_t r) -> uint32_t {
@@ -333,6 +342,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
                 queue[wv][at] = (it << 17) | (lane << 11) | (r << 5) | m;
             } else if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
                 // queue full: resolve in place
+                __threadfence_block();
                 atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
                 atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
                 nacc += 1;
@@ -359,10 +369,22 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const int64_t ksum = wave_column_sum<W>(kacc, lane);
     const int64_t dsum = wave_column_sum<W>(dacc, lane);
     const int64_t de = ((int64_t)w0 * (2 * ksum - dsum)) / 2 + (int64_t)edef[wv][lane];
-    atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
-              (unsigned long long)de);
     for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
-    if (lane == 0) atomicAdd(accepted, (unsigned long long)nacc);
+    // Combine the block's waves in LDS; the last wave to finish publishes.
+    atomicAdd(&blk_e[lane], (unsigned long long)de);
+    if (lane == 0) atomicAdd(&blk_acc, nacc);
+    __threadfence_block();
+    uint32_t done = 0;
+    if (lane == 0) done = atomicAdd(&blk_done, 1u);
+    done = __shfl(done, 0, 64);
+    if (done + 1 == active) {
+        __threadfence_block();
+        atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
+                  __hip_atomic_load(&blk_e[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (lane == 0)
+            atomicAdd(accepted, (unsigned long long)__hip_atomic_load(
+                                    &blk_acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    }
 }
 
 // One wave per (slot, m), m in [0, dmax]: lane r holds the threshold of the
@@ -498,11 +520,11 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     const uint32_t len = c_end - c_begin;
     // Sites per lane: about 4096 waves (four per SIMD, all resident at once,
     // which measured fastest on 3D L=64 x 32 slots: 16 sites per lane) and at
-    // most 32 so the bit-sliced energy counters stay narrow (32 * 31 < 2^12).
+    // most kMaxPerLane so the bit-sliced energy counters stay narrow.
     uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 4096 - 1) /
                         ((uint64_t)kWave * 4096);
     if (per_lane < 1) per_lane = 1;
-    if (per_lane > 32) per_lane = 32;
+    if (per_lane > kMaxPerLane) per_lane = kMaxPerLane;
     const uint32_t waves = ceil_div(len, kWave * per_lane);
     const dim3 grid(ceil_div(waves, kWavesPerBlock), s.cells);
     const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
