@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(*a):
@@ -34,6 +35,40 @@ def bytes_per_attempt(cells: int, deg: float, w_b: int = 1, w_l: int = 1) -> flo
     plus the model stream once per site shared by all cells (row pointer 4 B, deg x
     (4 B column + w_b coupling), w_l linear)."""
     return (16.0 * cells + 4 + deg * (4 + w_b) + w_l) / (64.0 * cells)
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy), or
+    None when no summary matches."""
+    for name in sorted(os.listdir(PROFILES)) if os.path.isdir(PROFILES) else []:
+        if not (name.startswith("traffic_") and name.endswith(".json")):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            t = json.load(f)
+        if t.get("workload") == workload and t.get("kernel", "").startswith(f"{kernel}<"):
+            return int(t["traffic_bytes_per_launch"])
+    return None
+
+
+def counter_probe(_lib, instances, L: int, temps: int, sweeps: int):
+    """The program the HBM counter passes run: `sweeps` PT steps of the workload, then one
+    1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the
+    counters' units."""
+    import torch
+    wl = instances.workload_3d_pt(L=L, temps=temps)
+    qb = wl["qubo"]
+    ctx = _lib.Context(0)
+    m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+    st = _lib.State(m, wl["cells"], 0x5EED)
+    st.set_ladder(wl["ladder"])
+    st.run(sweeps, exchange=True)
+    st.synchronize()
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] counter probe: {sweeps} PT steps of {wl['name']} and a 1 GiB copy done")
 
 
 def cpu_baseline(qb, cells, ladder, budget_s: float):
@@ -155,6 +190,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--counter-probe", type=int, default=0, metavar="SWEEPS",
+                    help="only run SWEEPS PT steps and a 1 GiB copy (for rocprofv3 --pmc passes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -163,6 +200,9 @@ def main():
 
     from ising_engine_amd import _lib, instances
 
+    if args.counter_probe:
+        counter_probe(_lib, instances, args.L, args.temps, args.counter_probe)
+        return
     kernel = {"auto": _lib.KERNEL_AUTO, "generic": _lib.KERNEL_GENERIC,
               "multispin": _lib.KERNEL_MULTISPIN}[args.kernel]
     wl = instances.workload_3d_pt(L=args.L, temps=args.temps)
@@ -202,8 +242,10 @@ def main():
     achieved = attempts_per_launch * bpa / avg_launch_s / 1e9
     log(f"[bench] sweep launch avg {avg_launch_s * 1e6:.1f} us over {launches} launches; "
         f"exchange avg {ms_x / max(n_x, 1) * 1e3:.1f} us; {bpa:.4f} B/attempt")
+    traffic = measured_traffic(f"sweep_{kname}_kernel", wl["name"])
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
+                    algorithmic_bytes_per_launch=round(attempts_per_launch * bpa),
                     kernel=f"sweep_{kname}_kernel",
                     avg_launch_us=round(avg_launch_s * 1e6, 2),
                     bytes_per_attempt=round(bpa, 4))
