@@ -218,28 +218,54 @@ def test_infinite_temperature_accepts_everything():
     assert st.stats[0] == st.stats[1]
 
 
+BOLTZ_ARGS = (5, 0, [1, -2, 1, 0, 2], [0, 1, 2, 0, 1], [1, 2, 3, 4, 3], [-2, 3, -1, 2, -2])
+
+
+def boltzmann_pvalue(csr_target, beta_target, csr_run, beta_run, view, seed=12):
+    """Chi-square p-value of the 64 oracle chains run on (csr_run, beta_run), sampled
+    300 times after burn-in, against the exact distribution exp(-beta E)/Z of
+    (csr_target, beta_target) by enumeration."""
+    n = csr_target.n
+    states = list(itertools.product([0, 1], repeat=n))
+    es = np.array([csr_target.energy(list(s)) for s in states], dtype=np.float64)
+    p = np.exp(-beta_target * (es - es.min()))
+    p /= p.sum()
+    st = ob.OracleState(csr_run, cells=1, seed=seed, view=view)
+    st.run(20, exchange=False, ladder=[beta_run])
+    counts = np.zeros(len(states))
+    for _ in range(300):
+        st.run(1, exchange=False, ladder=[beta_run])
+        w = st.words[0]
+        for r in range(64):
+            idx = sum(((int(w[i]) >> r) & 1) << (n - 1 - i) for i in range(n))
+            counts[idx] += 1
+    from scipy.stats import chisquare
+    return chisquare(counts, p * counts.sum())[1]
+
+
 @pytest.mark.parametrize("view", [ob.VIEW_WORD, ob.VIEW_PLANE])
 def test_metropolis_samples_exact_boltzmann(view):
     # Exact enumeration on a 5-spin model: the empirical distribution of the
     # 64 chains after burn-in matches exp(-beta E)/Z (chi-square, p > 1e-4).
-    csr = ob.CSR(5, 0, [1, -2, 1, 0, 2], [0, 1, 2, 0, 1], [1, 2, 3, 4, 3], [-2, 3, -1, 2, -2])
+    csr = ob.CSR(*BOLTZ_ARGS)
+    assert boltzmann_pvalue(csr, 0.6, csr, 0.6, view) > 1e-4
+
+
+@pytest.mark.parametrize("view", [ob.VIEW_WORD, ob.VIEW_PLANE])
+@pytest.mark.parametrize("mistake", ["beta_doubled", "linear_sign", "coupling_dropped"])
+def test_boltzmann_check_catches_planted_mistakes(view, mistake):
+    # The same check fails when the chains run a plausibly wrong sweep: a factor
+    # of two in beta, a flipped sign of the linear term, one coupling lost.
+    n, off, lin, pi, pj, q = BOLTZ_ARGS
+    target = ob.CSR(*BOLTZ_ARGS)
     beta = 0.6
-    states = list(itertools.product([0, 1], repeat=5))
-    es = np.array([csr.energy(list(s)) for s in states], dtype=np.float64)
-    p = np.exp(-beta * (es - es.min()))
-    p /= p.sum()
-    st = ob.OracleState(csr, cells=1, seed=12, view=view)
-    st.run(20, exchange=False, ladder=[beta])
-    counts = np.zeros(len(states))
-    for _ in range(300):
-        st.run(1, exchange=False, ladder=[beta])
-        w = st.words[0]
-        for r in range(64):
-            idx = sum(((int(w[i]) >> r) & 1) << (4 - i) for i in range(5))
-            counts[idx] += 1
-    from scipy.stats import chisquare
-    _, pval = chisquare(counts, p * counts.sum())
-    assert pval > 1e-4
+    if mistake == "beta_doubled":
+        run, beta_run = target, 2 * beta
+    elif mistake == "linear_sign":
+        run, beta_run = ob.CSR(n, off, [-v for v in lin], pi, pj, q), beta
+    else:
+        run, beta_run = ob.CSR(n, off, lin, pi[:-1], pj[:-1], q[:-1]), beta
+    assert boltzmann_pvalue(target, beta, run, beta_run, view) < 1e-6
 
 
 def test_views_differ_but_agree_in_law():
