@@ -27,7 +27,9 @@
  *   ige_exchange            ReplicaExchange::apply     src/engine_v2/operators/replica_exchange.rs:61-109
  *                           (DEO schedule of engine.rs:713-818)
  *   ige_read_energies       SpinState::energies_into   src/engine_v2/backends/sparse_bitslice.rs:273-278
- *   ige_read_fields         SparseBitSlice fields ledger
+ *   ige_read_fields         SparseBitSlice fields ledger; on a dense state the fields
+ *                           are recomputed on int8 matrix cores (the dense SK / QUBO
+ *                           instances of src/bin/sk_parisi_benchmark.rs:56-110)
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:26-27
  *   ige_recompute_energies  SpinState::audit / energy_of
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:114-137,317-346
@@ -57,9 +59,12 @@ extern "C" {
 #define IGE_ERR_UNSUPPORTED (-4) /* model/kernel combination not supported  */
 #define IGE_ERR_RANGE (-5)       /* value outside the exact integer range   */
 
-#define IGE_KERNEL_AUTO 0      /* multispin when eligible, else generic     */
+#define IGE_KERNEL_AUTO 0      /* multispin when eligible, else dense when the
+                                  model is dense and eligible, else generic */
 #define IGE_KERNEL_GENERIC 1   /* lane = replica, any integer CSR model     */
 #define IGE_KERNEL_MULTISPIN 2 /* lane = site, bit-sliced +-J arithmetic    */
+#define IGE_KERNEL_DENSE 3     /* dense q = scale*int8: positions in order,
+                                  fields recomputed on int8 matrix cores    */
 
 #define IGE_RUN_EXCHANGE 1u /* DEO replica exchange after every sweep       */
 
@@ -74,7 +79,7 @@ typedef struct {
     uint32_t max_degree;
     int32_t multispin_eligible; /* 1 when every |q_ij| == w0 and the Ising field is 0 */
     int32_t coupling_magnitude; /* w0 when eligible, else 0                        */
-    int32_t reserved;
+    int32_t dense_scale;     /* s when every q_ij = s * int8 and n <= 8192, else 0 */
     int64_t max_local_field; /* max_i |lin_i| + sum_j |q_ij|: bound on |dE|      */
 } ige_model_info;
 
@@ -84,7 +89,7 @@ typedef struct {
     uint64_t accepted;  /* accepted single-spin proposals                */
     uint64_t proposed; /* proposals (= sweeps * n * 64 * cells)         */
     uint32_t phase;    /* DEO phase of the next exchange (= exchanges & 1) */
-    uint32_t kernel;   /* IGE_KERNEL_GENERIC or IGE_KERNEL_MULTISPIN    */
+    uint32_t kernel;   /* IGE_KERNEL_GENERIC, _MULTISPIN or _DENSE      */
 } ige_stats;
 
 /* ---- context ---- */

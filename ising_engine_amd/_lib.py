@@ -24,6 +24,7 @@ IGE_ERR_RANGE = -5
 KERNEL_AUTO = 0
 KERNEL_GENERIC = 1
 KERNEL_MULTISPIN = 2
+KERNEL_DENSE = 3
 RUN_EXCHANGE = 1
 
 EXPORTED = (
@@ -53,7 +54,7 @@ class ModelInfo(ctypes.Structure):
     _fields_ = [
         ("n", c_uint32), ("num_colors", c_uint32), ("nnz", c_uint64),
         ("max_degree", c_uint32), ("multispin_eligible", c_int32),
-        ("coupling_magnitude", c_int32), ("reserved", c_int32),
+        ("coupling_magnitude", c_int32), ("dense_scale", c_int32),
         ("max_local_field", c_int64),
     ]
 

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -38,11 +39,42 @@ struct ige_model {
     std::vector<uint32_t> colors;
     ige::ColorOrder co;
     ige::MultispinShape ms;
+    ige::DenseShape ds;
     int64_t max_local = 0;
     uint32_t dmax = 0;
     uint32_t regular_degree = 0;
     DevBuf<uint32_t> rp, ci, order, sgn;
     DevBuf<int32_t> wt, lin;
+    // Dense int8 matrix, built on the first dense state of this model.
+    mutable std::mutex dense_mu;
+    mutable DevBuf<int8_t> dense_a;
+    mutable DevBuf<int32_t> dense_lin;
+    mutable uint32_t n_pad = 0;
+    void ensure_dense() const {
+        std::lock_guard<std::mutex> lock(dense_mu);
+        if (n_pad) return;
+        const uint32_t np = (ir.n + 63) / 64 * 64;
+        const std::vector<int8_t> a = ige::dense_matrix(ir, co, np, ds.scale);
+        std::vector<int32_t> l(np, 0);
+        for (uint32_t p = 0; p < ir.n; ++p) l[p] = ir.linear[co.order[p]];
+        dense_a.alloc(a.size());
+        dense_lin.alloc(np);
+        dense_a.upload(a.data(), a.size(), nullptr);
+        dense_lin.upload(l.data(), np, nullptr);
+        check(hipStreamSynchronize(nullptr), "dense upload");
+        n_pad = np;
+    }
+    ige::DevDense dense_view() const {
+        ige::DevDense d;
+        d.n = ir.n;
+        d.n_pad = n_pad;
+        d.a = dense_a.p;
+        d.lin = dense_lin.p;
+        d.order = order.p;
+        d.scale = ds.scale;
+        d.dtab = static_cast<uint32_t>(max_local);
+        return d;
+    }
     ige::DevModel view() const {
         ige::DevModel m;
         m.n = ir.n;
@@ -151,6 +183,7 @@ void finish_model(ige_model* m) {
     m->colors = m->ir.greedy_coloring();
     m->co = ige::color_order(m->colors);
     m->ms = ige::multispin_shape(m->ir);
+    m->ds = ige::dense_shape(m->ir);
     m->regular_degree = m->dmax;
     for (uint32_t i = 0; i < m->ir.n; ++i)
         if (m->ir.row_ptr[i + 1] - m->ir.row_ptr[i] != m->dmax) m->regular_degree = 0;
@@ -163,8 +196,9 @@ void refresh_thresholds(ige_state* s, const double* dev_beta) {
     auto st = s->view();
     if (s->kernel == IGE_KERNEL_GENERIC)
         check(ige::launch_thr_table(dev_beta, s->thr.p, s->cells, s->dtab, s->stream), "thr");
-    else
+    else if (s->kernel == IGE_KERNEL_MULTISPIN)
         check(ige::launch_qplanes(m, st, dev_beta, s->qplanes.p, s->stream), "qplanes");
+    // the dense kernel evaluates its thresholds from beta directly
 }
 
 void recompute_energy_into_state(ige_state* s) {
@@ -228,6 +262,7 @@ int32_t ige_model_get_info(const ige_model* m, ige_model_info* info) {
         info->max_degree = m->dmax;
         info->multispin_eligible = m->ms.eligible ? 1 : 0;
         info->coupling_magnitude = m->ms.w0;
+        info->dense_scale = m->ds.eligible ? m->ds.scale : 0;
         info->max_local_field = m->max_local;
     });
 }
@@ -253,19 +288,24 @@ int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, u
     return guarded([&] {
         require(ctx && model && out, "NULL argument");
         require(cells >= 1 && cells <= 65536, "cells must be in [1, 65536]");
-        require(kernel <= IGE_KERNEL_MULTISPIN, "unknown kernel id");
+        require(kernel <= IGE_KERNEL_DENSE, "unknown kernel id");
         if (kernel == IGE_KERNEL_MULTISPIN && !model->ms.eligible)
             throw Failure(IGE_ERR_UNSUPPORTED,
                           "multispin kernel needs |q_ij| == w0, zero Ising field, degree <= 31");
+        if (kernel == IGE_KERNEL_DENSE && !model->ds.eligible)
+            throw Failure(IGE_ERR_UNSUPPORTED,
+                          "dense kernel needs q_ij = scale * int8 and n <= 8192");
         set_device(ctx);
         auto s = std::make_unique<ige_state>();
         s->ctx = ctx;
         s->model = model;
         s->cells = cells;
         s->seed = seed;
-        s->kernel = kernel == IGE_KERNEL_AUTO
-                        ? (model->ms.eligible ? IGE_KERNEL_MULTISPIN : IGE_KERNEL_GENERIC)
-                        : kernel;
+        s->kernel = kernel;
+        if (kernel == IGE_KERNEL_AUTO)
+            s->kernel = model->ms.eligible                        ? IGE_KERNEL_MULTISPIN
+                        : (model->ds.eligible && model->ds.dense) ? IGE_KERNEL_DENSE
+                                                                  : IGE_KERNEL_GENERIC;
         check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&s->ev0), "hipEventCreate");
         check(hipEventCreate(&s->ev1), "hipEventCreate");
@@ -279,6 +319,8 @@ int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, u
         if (s->kernel == IGE_KERNEL_GENERIC) {
             s->dtab = static_cast<uint32_t>(model->max_local);
             s->thr.alloc(static_cast<size_t>(cells) * (s->dtab + 1));
+        } else if (s->kernel == IGE_KERNEL_DENSE) {
+            model->ensure_dense();
         } else {
             s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * 39);
         }
@@ -352,7 +394,14 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             if (sched || (s->kernel == IGE_KERNEL_MULTISPIN && exch && k > 0))
                 refresh_thresholds(s, beta);
             if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
-            for (uint32_t c = 0; c < s->model->co.num_colors; ++c) {
+            const uint32_t launches =
+                s->kernel == IGE_KERNEL_DENSE ? 1u : s->model->co.num_colors;
+            if (s->kernel == IGE_KERNEL_DENSE)
+                check(ige::launch_sweep_dense(s->model->dense_view(), st, s->sweep, s->seed, beta,
+                                              s->stream),
+                      "sweep_dense");
+            for (uint32_t c = 0; s->kernel != IGE_KERNEL_DENSE && c < s->model->co.num_colors;
+                 ++c) {
                 if (s->kernel == IGE_KERNEL_GENERIC)
                     check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
                                                     s->thr.p, s->dtab, s->stream),
@@ -368,7 +417,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                 float ms = 0.f;
                 check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
                 s->prof_ms += ms;
-                s->prof_launches += s->model->co.num_colors;
+                s->prof_launches += launches;
             }
             if (exch) {
                 if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
@@ -466,7 +515,11 @@ int32_t ige_read_fields(ige_state* s, int32_t* out) {
         const size_t total = static_cast<size_t>(s->cells) * s->model->ir.n * 64;
         DevBuf<int32_t> f;
         f.alloc(total);
-        check(ige::launch_fields(s->model->view(), s->view(), f.p, s->stream), "fields");
+        if (s->kernel == IGE_KERNEL_DENSE)
+            check(ige::launch_dense_fields(s->model->dense_view(), s->view(), f.p, s->stream),
+                  "dense fields");
+        else
+            check(ige::launch_fields(s->model->view(), s->view(), f.p, s->stream), "fields");
         f.download(out, total, s->stream);
         check(hipStreamSynchronize(s->stream), "read fields");
     });

@@ -84,6 +84,15 @@ IGE_HD uint32_t metropolis_threshold(double beta, int64_t de) {
     return prob_to_u32(det_exp(-(beta * (double)de)));
 }
 
+// Largest dE whose threshold can be nonzero at beta: thr(beta, d) = 0 once
+// beta * d > 40 (det_exp returns 0 there); two units of slack keep the bound
+// safe against rounding of 40 / beta.
+IGE_HD uint32_t threshold_limit(double beta, uint32_t dtab) {
+    if (!(beta > 0.0)) return dtab;
+    const double lim = 40.0 / beta;
+    return lim >= (double)dtab ? dtab : ((uint32_t)lim + 2u < dtab ? (uint32_t)lim + 2u : dtab);
+}
+
 // Replica-exchange threshold for delta = (b_t - b_{t+1}) (E_t - E_{t+1}) < 0.
 IGE_HD uint32_t exchange_threshold(double delta) { return prob_to_u32(det_exp(delta)); }
 

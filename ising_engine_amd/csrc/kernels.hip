@@ -145,25 +145,6 @@ constexpr uint32_t kQueueCap = 512;  // deferred entries per wave
 constexpr uint32_t kMaxPerLane = 16; // sites per lane
 constexpr uint32_t kTagTail = 5u;
 
-// 64x64 bit-matrix transpose across the wave: lane l holds row l on entry,
-// column l on exit (bit k of the result = bit l of row k).  Branch-free.
-__device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
-                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
-                               0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const uint32_t j = 32u >> s;
-        const uint64_t m = masks[s];
-        const uint64_t y = __shfl_xor(x, (int)j, 64);
-        const uint64_t hi = (lane & j) ? ~0ull : 0ull;  // 1s in lanes with bit j set
-        const uint64_t keep = (x & m & ~hi) | (x & ~m & hi);
-        const uint64_t take = (((y & m) << j) & ~hi) | (((y >> j) & m) & hi);
-        x = keep | take;
-    }
-    return x;
-}
-
 // Sum over the wave's lanes of the per-lane bit-sliced counter acc[0..W-1],
 // returned in lane r for replica r.
 template <int W>

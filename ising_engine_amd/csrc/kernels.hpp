@@ -23,6 +23,16 @@ struct DevModel {
     int64_t offset = 0;
 };
 
+// Dense couplings for the matrix-core path: q = scale * a, a int8.
+struct DevDense {
+    uint32_t n = 0, n_pad = 0;      // n_pad: n rounded up to a multiple of 64
+    const int8_t* a = nullptr;      // [n_pad][n_pad] position space, zero padded
+    const int32_t* lin = nullptr;   // n_pad linear terms, zero padded
+    const uint32_t* order = nullptr;  // n: position -> site
+    int32_t scale = 1;
+    uint32_t dtab = 0;              // bound on |dE| (max local field)
+};
+
 struct DevState {
     uint32_t cells = 0;
     uint64_t* spins = nullptr;      // [cells][n] MSC words, position order
@@ -58,6 +68,16 @@ hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* be
 // DEO exchange number `round`: pairs (t, t+1) with t = round mod 2.
 hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round, uint64_t seed,
                            hipStream_t stream);
+
+// Dense sweep: all positions in order, one workgroup per slot, fields of each
+// block of 64 positions recomputed on int8 matrix cores.
+hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t sweep, uint64_t seed,
+                              const double* beta, hipStream_t stream);
+// Local fields of every (slot, site, lane) on the matrix cores, layout as
+// launch_fields.
+hipError_t launch_dense_fields(const DevDense& d, const DevState& s, int32_t* out,
+                               hipStream_t stream);
+size_t dense_lds_bytes_host(uint32_t n_pad);
 
 hipError_t launch_init_spins(const DevModel& m, const DevState& s, uint64_t seed,
                              hipStream_t stream);

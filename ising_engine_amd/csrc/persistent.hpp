@@ -54,15 +54,6 @@ __host__ __device__ __forceinline__ uint32_t choose_thr_mode(uint32_t dtab, uint
     return dtab < kLdsThrEntries ? kThrLds : (dtab <= 32u * n ? kThrGlobal : kThrDirect);
 }
 
-// Largest dE whose threshold can be nonzero at beta: thr(beta, d) = 0 once
-// beta * d > 40 (det_exp returns 0 there); two units of slack keep the bound
-// safe against rounding of 40 / beta.
-__host__ __device__ __forceinline__ uint32_t threshold_limit(double beta, uint32_t dtab) {
-    if (!(beta > 0.0)) return dtab;
-    const double lim = 40.0 / beta;
-    return lim >= (double)dtab ? dtab : min(dtab, (uint32_t)lim + 2u);
-}
-
 __host__ __device__ __forceinline__ double anneal_beta(double beta_hi, double c, uint32_t sweeps,
                                                        uint32_t k) {
     return sweeps > 1 ? beta_hi * det_exp(-(double)(sweeps - 1 - k) * c) : beta_hi;

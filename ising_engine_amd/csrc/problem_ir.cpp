@@ -2,6 +2,7 @@
 #include "problem_ir.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <cstdlib>
 #include <limits>
 
@@ -136,6 +137,32 @@ MultispinShape multispin_shape(const ProblemIR& ir) {
     s.eligible = true;
     s.w0 = static_cast<int32_t>(w0);
     return s;
+}
+
+DenseShape dense_shape(const ProblemIR& ir) {
+    DenseShape d;
+    if (ir.n == 0 || ir.n > kDenseMaxN) return d;
+    int64_t g = 0;
+    for (int32_t w : ir.weights) g = std::gcd(g, static_cast<int64_t>(std::llabs(w)));
+    if (g == 0) g = 1;  // no couplings: any scale works
+    for (int32_t w : ir.weights)
+        if (std::llabs(static_cast<long long>(w)) / g > 127) return d;
+    d.eligible = true;
+    d.scale = static_cast<int32_t>(g);
+    const uint64_t full = static_cast<uint64_t>(ir.n) * (ir.n - 1);
+    d.dense = ir.n >= 64 && 4 * ir.nnz() >= full;
+    return d;
+}
+
+std::vector<int8_t> dense_matrix(const ProblemIR& ir, const ColorOrder& co, uint32_t n_pad,
+                                 int32_t scale) {
+    std::vector<int8_t> a(static_cast<size_t>(n_pad) * n_pad, 0);
+    for (uint32_t i = 0; i < ir.n; ++i) {
+        const size_t row = static_cast<size_t>(co.pos_of[i]) * n_pad;
+        for (uint32_t e = ir.row_ptr[i]; e < ir.row_ptr[i + 1]; ++e)
+            a[row + co.pos_of[ir.col_idx[e]]] = static_cast<int8_t>(ir.weights[e] / scale);
+    }
+    return a;
 }
 
 }  // namespace ige

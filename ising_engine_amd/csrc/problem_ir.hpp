@@ -71,4 +71,23 @@ struct MultispinShape {
 };
 MultispinShape multispin_shape(const ProblemIR& ir);
 
+// Dense-coupling structure for the int8 matrix-core field recompute: every
+// q_ij = scale * a_ij with a_ij an int8 (|a_ij| <= 127), scale = gcd of |q_ij|,
+// and n small enough for the replica-major spin bits of one slot to sit in
+// LDS.  `dense` marks models whose both-triangle CSR fills at least a quarter
+// of the n x n matrix (fully connected SK, dense QUBO), where the automatic
+// kernel choice takes this path.
+constexpr uint32_t kDenseMaxN = 8192;
+struct DenseShape {
+    bool eligible = false;
+    bool dense = false;
+    int32_t scale = 0;
+};
+DenseShape dense_shape(const ProblemIR& ir);
+
+// a[p * n_pad + p'] = q(order[p], order[p']) / scale in position space,
+// zero on the diagonal and in the padding up to n_pad (a multiple of 64).
+std::vector<int8_t> dense_matrix(const ProblemIR& ir, const ColorOrder& co, uint32_t n_pad,
+                                 int32_t scale);
+
 }  // namespace ige

@@ -23,6 +23,25 @@ __device__ __forceinline__ bool lane_bit(uint64_t word) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// 64x64 bit-matrix transpose across the wave: lane l holds row l on entry,
+// column l on exit (bit k of the result = bit l of row k).  Branch-free.
+__device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                               0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const uint32_t j = 32u >> s;
+        const uint64_t m = masks[s];
+        const uint64_t y = __shfl_xor(x, (int)j, 64);
+        const uint64_t hi = (lane & j) ? ~0ull : 0ull;  // 1s in lanes with bit j set
+        const uint64_t keep = (x & m & ~hi) | (x & ~m & hi);
+        const uint64_t take = (((y & m) << j) & ~hi) | (((y >> j) & m) & hi);
+        x = keep | take;
+    }
+    return x;
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
     const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
     const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);

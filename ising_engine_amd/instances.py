@@ -92,6 +92,31 @@ def erdos_renyi_pm_j(n: int, mean_degree: float, seed: int) -> Qubo:
     return ising_to_qubo(n, bi, bj, J, name=f"ER_n{n}_d{mean_degree:g}")
 
 
+def sk_int8(n: int, seed: int, gain: float = 32.0) -> Qubo:
+    """Sherrington-Kirkpatrick: J_ij ~ N(0, 1/n) for every pair, quantised to int8 by
+    J_q = clamp(round(J sqrt(n) gain), -127, 127) (zeros dropped by the CSR build), lowered
+    to QUBO with sigma = 1 - 2x (src/bin/sk_parisi_benchmark.rs:56-110)."""
+    rng = np.random.default_rng(seed)
+    iu, ju = np.triu_indices(n, 1)
+    J = np.clip(np.rint(rng.standard_normal(iu.size) * gain), -127, 127).astype(np.int64)
+    keep = J != 0
+    return ising_to_qubo(n, iu[keep], ju[keep], J[keep], name=f"SK_n{n}_int8")
+
+
+def dense_qubo(n: int, seed: int, scale: int = 3, density: float = 0.6) -> Qubo:
+    r"""Dense QUBO whose couplings are scale * int8: q_ij = scale * a_ij, a_ij uniform in
+    [-127, 127] \ {0} on a fraction `density` of all pairs; linear terms uniform in
+    [-2000, 2000]."""
+    rng = np.random.default_rng(seed)
+    iu, ju = np.triu_indices(n, 1)
+    keep = rng.random(iu.size) < density
+    a = rng.integers(-127, 128, size=int(keep.sum())).astype(np.int64)
+    a[a == 0] = 1
+    lin = rng.integers(-2000, 2001, size=n).astype(np.int64)
+    return Qubo(n=n, offset=0, linear=lin, pi=iu[keep].astype(np.uint32),
+                pj=ju[keep].astype(np.uint32), q=scale * a, name=f"denseQUBO_n{n}_s{scale}")
+
+
 def random_qubo(n: int, mean_degree: float, seed: int, offdiag: int = 100, diag: int = 200) -> Qubo:
     """Sparse QUBO: ER graph, Q_ij int uniform in [-offdiag, offdiag] (zeros redrawn to +1),
     Q_ii uniform in [-diag, diag] as the linear term."""
@@ -171,6 +196,25 @@ def workload_3d_pt(L: int = 64, temps: int = 32, sweeps: int = 10000, seed: int 
     qb = hypercubic_pm_j(L, 3, seed)
     return dict(qubo=qb, cells=temps, sweeps=sweeps, exchange=True,
                 ladder=geometric_betas(2.0, 0.5, temps), name=f"3d_pmJ_L{L}_pt{temps}")
+
+
+def workload_er_anneal(n: int = 20000, degree: float = 6, sweeps: int = 5000,
+                       seed: int = 3) -> dict:
+    """Max-Cut-shaped Ising: Erdos-Renyi G(n, degree/(n-1)) with J = +-1, 64 replicas, a
+    geometric anneal over the suggested temperature range."""
+    qb = erdos_renyi_pm_j(n, degree, seed)
+    t_max, t_min = suggested_temp_range(qb)
+    return dict(qubo=qb, cells=1, sweeps=sweeps, beta_lo=1.0 / t_max, beta_hi=1.0 / t_min,
+                name=f"er_pmJ_n{n}_d{degree:g}_anneal")
+
+
+def workload_sk_anneal(n: int = 4096, sweeps: int = 5000, seed: int = 5) -> dict:
+    """Dense SK with int8 couplings (gain 32), 64 replicas, a geometric anneal over the
+    suggested temperature range."""
+    qb = sk_int8(n, seed)
+    t_max, t_min = suggested_temp_range(qb)
+    return dict(qubo=qb, cells=1, sweeps=sweeps, beta_lo=1.0 / t_max, beta_hi=1.0 / t_min,
+                name=f"sk_int8_n{n}_anneal")
 
 
 def serving_request(rng, n_lo: int = 128, n_hi: int = 16384, w_max: int = 1000,
