@@ -118,6 +118,137 @@ def cpu_baseline(qb, cells, ladder, budget_s: float):
                        f"{3 * max(1, steps // 3)} steps, median of 3 repetitions")
 
 
+def cpu_sample(qb, cells, temps, budget_s: float) -> float:
+    """engine.rs::step port on one thread per cell: attempts/s over about budget_s
+    (None when budget_s is 0)."""
+    if budget_s <= 0:
+        return None
+    import ctypes
+    from ctypes import POINTER, c_double, c_int, c_int8, c_uint32, c_uint64
+    from oracle import binding as ob
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "libcpu_baseline.so"))
+    lib.cb_run.argtypes = [c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double),
+                           POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, c_uint64,
+                           c_int, POINTER(c_int8), POINTER(c_double), POINTER(c_int)]
+    lib.cb_run.restype = c_int
+    csr = ob.CSR.from_qubo(qb)
+    w = csr.w.astype(np.float64)
+    lin = csr.linear.astype(np.float64)
+    t = np.ascontiguousarray(temps, dtype=np.float64)
+    spins = np.random.default_rng(5).integers(0, 2, size=(cells, qb.n, 64), dtype=np.int8)
+    e = np.zeros((cells, 64), dtype=np.float64)
+    phase = c_int(0)
+    P = lambda a, ty: a.ctypes.data_as(POINTER(ty))
+    steps, dt = 1, 0.0
+    while True:
+        t0 = time.perf_counter()
+        lib.cb_run(qb.n, P(csr.row_ptr, c_uint32), P(csr.col_idx, c_uint32), P(w, c_double),
+                   P(lin, c_double), cells, P(t, c_double), steps, 9, min(cells, 16),
+                   P(spins, c_int8), P(e, c_double), ctypes.byref(phase))
+        dt = time.perf_counter() - t0
+        if dt >= budget_s / 4 or steps >= 1 << 20:
+            break
+        steps *= 4
+    return steps * qb.n * 64 * cells / dt
+
+
+def anneal_once(_lib, model, seed, sched, kernel=0):
+    """One 64-replica anneal over `sched` (one beta per sweep), timed on the device."""
+    st = _lib.State(model, 1, seed, kernel=kernel)
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.run(len(sched), beta_schedule=sched[:, None])
+    st.synchronize()
+    dt = time.perf_counter() - t0
+    if not np.array_equal(st.energies(), st.recompute_energies()):
+        raise SystemExit("tracked energies drifted from recompute")
+    return dt, st
+
+
+def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: float) -> dict:
+    """The single-cell anneal workloads: 2D +-J L=32 (1000 sweeps), Erdos-Renyi N=20000
+    d=6 +-J (5000 sweeps, with time-to-target), dense SK N=4096 int8 (matrix-core fields)."""
+    kname = {1: "generic", 2: "multispin", 3: "dense"}
+    out = {}
+    # 2D +-J L=32, beta 0.1 -> 3.0 over 1000 sweeps
+    wl = instances.workload_2d_ej()
+    qb = wl["qubo"]
+    m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+    sched = _lib.anneal_schedule(0.1, 3.0, 1000)
+    anneal_once(_lib, m, 1, sched)
+    dt, st = anneal_once(_lib, m, 2, sched)
+    out[wl["name"]] = {"value": 1000 * qb.n * 64 / dt, "unit": "spin-flip attempts/s",
+                       "seconds": dt, "kernel": kname[st.stats()[0].kernel],
+                       "best_energy_per_spin": float(st.energies().min()) / qb.n,
+                       "cpu_1thread": cpu_sample(qb, 1, [1.0], cpu_s)}
+    log(f"[bench] {wl['name']}: {out[wl['name']]['value']:.3e} attempts/s ({dt * 1e3:.1f} ms)")
+    # Erdos-Renyi N=20000, mean degree 6, +-J: anneal throughput and TTS(0.99)
+    wl = instances.workload_er_anneal()
+    qb = wl["qubo"]
+    m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+    sched = _lib.anneal_schedule(wl["beta_lo"], wl["beta_hi"], wl["sweeps"])
+    anneal_once(_lib, m, 1, sched[:50])
+    times, bests = [], []
+    for r in range(tts_runs):
+        dt, st = anneal_once(_lib, m, 1000 + r, sched)
+        times.append(dt)
+        bests.append(int(st.energies().min()))
+    t_run = float(np.median(times))
+    e_best = min(bests)
+    target = e_best + int(0.005 * abs(e_best))
+    p_s = sum(b <= target for b in bests) / len(bests)
+    p_best = sum(b <= e_best for b in bests) / len(bests)
+    tts = lambda p: (t_run if p >= 1.0 else (float("inf") if p <= 0.0 else
+                                              t_run * np.log(0.01) / np.log(1.0 - p)))
+    out[wl["name"]] = {"value": wl["sweeps"] * qb.n * 64 / t_run, "unit": "spin-flip attempts/s",
+                       "seconds_per_anneal": t_run, "kernel": kname[st.stats()[0].kernel],
+                       "runs": tts_runs, "best_energy": e_best,
+                       "target_energy": target, "target": "within 0.5% of the best of all runs",
+                       "p_success": p_s, "tts99_s": tts(p_s),
+                       "p_success_best": p_best, "tts99_best_s": tts(p_best),
+                       "temperature_range": [1.0 / wl["beta_lo"], 1.0 / wl["beta_hi"]],
+                       "cpu_1thread": cpu_sample(qb, 1, [1.0], cpu_s)}
+    log(f"[bench] {wl['name']}: {out[wl['name']]['value']:.3e} attempts/s, TTS99 "
+        f"{tts(p_s):.3f} s (p_s {p_s:.2f})")
+    # Dense SK N=4096, int8 couplings: the matrix-core field recompute
+    wl = instances.workload_sk_anneal()
+    qb = wl["qubo"]
+    m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+    sched = _lib.anneal_schedule(wl["beta_lo"], wl["beta_hi"], wl["sweeps"])
+    st = _lib.State(m, 1, 3)
+    st.run(10, beta_schedule=sched[:10, None])
+    st.synchronize()
+    k0 = wl["sweeps"] // 2
+    win = sched[k0:k0 + sk_sweeps, None]
+    st.profile(True)
+    t0 = time.perf_counter()
+    st.run(len(win), beta_schedule=win)
+    st.synchronize()
+    dt = time.perf_counter() - t0
+    ms, launches, _, _ = st.profile_read()
+    st.profile(False)
+    if not np.array_equal(st.energies(), st.recompute_energies()):
+        raise SystemExit("tracked energies drifted from recompute")
+    per_sweep = dt / len(win)
+    ops = 2.0 * qb.n * qb.n * 64  # int8 multiply-adds of one full field recompute, x2
+    mfma_tops = ops / (ms / 1e3 / launches) / 1e12
+    out[wl["name"]] = {"value": qb.n * 64 / per_sweep, "unit": "spin-flip attempts/s",
+                       "ms_per_sweep": per_sweep * 1e3,
+                       "projected_anneal_s": per_sweep * wl["sweeps"],
+                       "sample": f"{len(win)} sweeps from the middle of the "
+                                 f"{wl['sweeps']}-sweep schedule",
+                       "kernel": kname[st.stats()[0].kernel],
+                       "roofline": {"bound": "mfma", "achieved": round(mfma_tops, 3),
+                                    "peak": 5000.0, "unit": "TOP/s (int8)",
+                                    "frac": mfma_tops / 5000.0,
+                                    "frac_of_one_cu": mfma_tops / (5000.0 / 256),
+                                    "ops_per_sweep": ops},
+                       "cpu_1thread": cpu_sample(qb, 1, [1.0], cpu_s)}
+    log(f"[bench] {wl['name']}: {per_sweep * 1e3:.2f} ms/sweep, {mfma_tops:.2f} TOP/s int8")
+    return out
+
+
 def bench_batch(_lib, instances, ctx, count: int) -> dict:
     """Independent sparse QUBO instances (N log-uniform 64..8192, degree 8, int weights),
     2000-sweep anneals: one persistent launch vs plain per-instance launches."""
@@ -187,6 +318,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true")
+    ap.add_argument("--tts-runs", type=int, default=12)
+    ap.add_argument("--sk-sweeps", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--serving-requests", type=int, default=300)
@@ -253,6 +387,10 @@ def main():
     batched = None
     if not args.no_batch:
         batched = bench_batch(_lib, instances, ctx, args.batch_count)
+    workloads = None
+    if not args.no_workloads:
+        workloads = bench_workloads(_lib, instances, ctx, args.tts_runs, args.sk_sweeps,
+                                    cpu_s=0.0 if args.no_cpu_baseline else 2.0)
     serving = None
     if not args.no_serving:
         serving = bench_serving(_lib, instances, ctx, args.serving_rate, args.serving_requests)
@@ -283,6 +421,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
 

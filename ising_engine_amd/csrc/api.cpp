@@ -48,19 +48,27 @@ struct ige_model {
     // Dense int8 matrix, built on the first dense state of this model.
     mutable std::mutex dense_mu;
     mutable DevBuf<int8_t> dense_a;
-    mutable DevBuf<int32_t> dense_lin;
+    mutable DevBuf<int32_t> dense_lin, dense_wdiag;
     mutable uint32_t n_pad = 0;
     void ensure_dense() const {
         std::lock_guard<std::mutex> lock(dense_mu);
         if (n_pad) return;
-        const uint32_t np = (ir.n + 63) / 64 * 64;
+        const uint32_t np = (ir.n + 511) / 512 * 512;
         const std::vector<int8_t> a = ige::dense_matrix(ir, co, np, ds.scale);
         std::vector<int32_t> l(np, 0);
         for (uint32_t p = 0; p < ir.n; ++p) l[p] = ir.linear[co.order[p]];
+        std::vector<int32_t> wd(static_cast<size_t>(np / 64) * 4096);
+        for (uint32_t b = 0; b < np / 64; ++b)
+            for (uint32_t p = 0; p < 64; ++p)
+                for (uint32_t k = 0; k < 64; ++k)
+                    wd[(static_cast<size_t>(b) * 64 + p) * 64 + k] =
+                        ds.scale * a[static_cast<size_t>(b * 64 + p) * np + b * 64 + k];
         dense_a.alloc(a.size());
         dense_lin.alloc(np);
+        dense_wdiag.alloc(wd.size());
         dense_a.upload(a.data(), a.size(), nullptr);
         dense_lin.upload(l.data(), np, nullptr);
+        dense_wdiag.upload(wd.data(), wd.size(), nullptr);
         check(hipStreamSynchronize(nullptr), "dense upload");
         n_pad = np;
     }
@@ -69,6 +77,7 @@ struct ige_model {
         d.n = ir.n;
         d.n_pad = n_pad;
         d.a = dense_a.p;
+        d.wdiag = dense_wdiag.p;
         d.lin = dense_lin.p;
         d.order = order.p;
         d.scale = ds.scale;

@@ -41,26 +41,19 @@ typedef int32_t v16i __attribute__((ext_vector_type(16)));
 constexpr uint32_t kDenseThreads = 256;  // 4 waves
 constexpr uint32_t kHStride = 65;        // padded row of the block-field tile
 
-struct DenseLds {
-    uint64_t* xt;      // [64][xs] replica-major spin bits, xs = n_pad / 64 + 1
-    int32_t* hblk;     // [64][kHStride] fields of the block, [position][replica]
-    uint32_t* ublk;    // [64][64] uniforms, [position][lane]
-    int32_t* wblk;     // [64][64] scale * a[p0 + p][p0 + k], [p][k]
+constexpr int kPrefetch = 8;             // k chunks of the coupling rows in flight
+
+// LDS: the replica-major spin bits xt[64][xs] (xs = n_pad / 64 + 1) are
+// dynamic; the per-block tiles are static so that the unrolled sequential
+// loop addresses them with immediate offsets.
+struct BlockTiles {
+    int32_t h[64 * kHStride];  // fields of the block, [position][replica]
+    uint32_t u[64 * 64];       // uniforms, [position][lane]
+    int4 w[64 * 16];           // scale * a[p0 + p][p0 + k], row p = 16 int4
 };
 
 __host__ __device__ inline size_t dense_lds_bytes(uint32_t n_pad) {
-    const size_t xs = n_pad / 64 + 1;
-    return 64 * xs * sizeof(uint64_t) + 64 * kHStride * sizeof(int32_t) +
-           2 * 64 * 64 * sizeof(uint32_t);
-}
-
-__device__ __forceinline__ DenseLds carve(unsigned char* base, uint32_t xs) {
-    DenseLds l;
-    l.xt = reinterpret_cast<uint64_t*>(base);
-    l.hblk = reinterpret_cast<int32_t*>(l.xt + 64 * xs);
-    l.ublk = reinterpret_cast<uint32_t*>(l.hblk + 64 * kHStride);
-    l.wblk = reinterpret_cast<int32_t*>(l.ublk + 64 * 64);
-    return l;
+    return 64 * (size_t)(n_pad / 64 + 1) * sizeof(uint64_t);
 }
 
 // 16 bits -> 16 bytes, byte i = bit i (0 or 1).  n * 0x00204081 places the
@@ -101,14 +94,30 @@ __device__ __forceinline__ void block_fields(const DevDense& d, const uint64_t* 
     const int8_t* arow = d.a + (size_t)(p0 + rbase + (lane & 31)) * d.n_pad + kh;
     const uint64_t* xr = xt + (size_t)(jbase + (lane & 31)) * xs;
     v16i acc = {};
-    for (uint32_t kc = 0; kc < nchunk; ++kc) {
-        const uint64_t xb = xr[kc];
-        const v4i a0 = *reinterpret_cast<const v4i*>(arow + (size_t)kc * 64);
-        const v4i a1 = *reinterpret_cast<const v4i*>(arow + (size_t)kc * 64 + 32);
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, spread16((uint32_t)(xb >> kh)), acc, 0,
-                                                    0, 0);
-        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-            a1, spread16((uint32_t)(xb >> (32 + kh))), acc, 0, 0, 0);
+    // Coupling rows stream from L2 / HBM with kPrefetch chunks in flight.
+    // n_pad is a multiple of 64 * kPrefetch, so the loop needs no guards; the
+    // last kPrefetch iterations re-load the final chunk instead of running
+    // past the row.
+    v4i pa0[kPrefetch], pa1[kPrefetch];
+#pragma unroll
+    for (int i = 0; i < kPrefetch; ++i) {
+        pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 64);
+        pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 64 + 32);
+    }
+    for (uint32_t kb = 0; kb < nchunk; kb += kPrefetch) {
+#pragma unroll
+        for (int i = 0; i < kPrefetch; ++i) {
+            const uint32_t kc = kb + i;
+            const v4i a0 = pa0[i], a1 = pa1[i];
+            const uint32_t kn = min(kc + (uint32_t)kPrefetch, nchunk - 1);
+            pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 64);
+            pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 64 + 32);
+            const uint64_t xb = xr[kc];
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, spread16((uint32_t)(xb >> kh)), acc,
+                                                        0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                a1, spread16((uint32_t)(xb >> (32 + kh))), acc, 0, 0, 0);
+        }
     }
     const uint32_t j = jbase + (lane & 31);
 #pragma unroll
@@ -118,95 +127,145 @@ __device__ __forceinline__ void block_fields(const DevDense& d, const uint64_t* 
     }
 }
 
-// u < metropolis_threshold(beta, de) for 1 <= de <= dlim.  a estimates
-// T = 2^32 exp(-beta de) within a relative 1e-5 (single-precision product
-// and v_exp_f32); the deterministic threshold lies within T (1 +- 1e-14) + 1.
-// Outside the band [a (1 - 1e-4) - 4, a (1 + 1e-4) + 4] the answer is
-// therefore decided by a; inside it the threshold is evaluated.
+// Acceptance u < metropolis_threshold(beta, de).  The threshold is first
+// bracketed: a = 2^32 2^x with x = -beta de log2(e) in double precision
+// (clamped to [-200, 0]), 2^frac(x) on v_exp_f32 and 2^floor(x) exact, so
+// |a / T - 1| <= 3e-7 for T = 2^32 exp(-beta de), while the deterministic
+// threshold lies within T (1 +- 1e-14) + 1.  Outside the band
+// [a (1 - 2e-6) - 4, a (1 + 2e-6) + 4] the comparison is decided by a; a
+// proposal inside it (probability below 4e-6) is "ambiguous" and needs the
+// threshold itself.
 __device__ __attribute__((noinline)) bool exact_accept(uint32_t u, int32_t de, double beta) {
     return u < metropolis_threshold(beta, (int64_t)de);
 }
 
-__device__ __forceinline__ bool dense_accept(uint32_t u, int32_t de, float nb, double beta) {
-    const float a = __builtin_amdgcn_exp2f((float)de * nb) * 4294967296.0f;
-    const float lo = a * 0.9999f - 4.0f;
-    const float hi = a * 1.0001f + 4.0f;
-    if (lo > 0.0f && u < (uint32_t)lo) return true;
-    if (hi < 4294967040.0f && u > (uint32_t)hi) return false;
-    return exact_accept(u, de, beta);
+__device__ __forceinline__ void threshold_band(int32_t de, double nbd, uint32_t& lo_u,
+                                               uint32_t& hi_u) {
+    const double x = fmin(fmax(nbd * (double)de, -200.0), 0.0);
+    const double xi = floor(x);
+    const float a = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(x - xi)), (int)xi + 32);
+    const float lo = a * 0.999998f - 4.0f;
+    const float hi = a * 1.000002f + 4.0f;
+    lo_u = (uint32_t)fmaxf(lo, 0.0f);
+    hi_u = hi < 4294967040.0f ? (uint32_t)hi : 0xFFFFFFFFu;
+}
+
+// The 64 positions of one block in order (lane = replica).  hb: the block's
+// fields at its start; wv: lane k holds the word of position p0 + k; wt: the
+// block's couplings scale * a[p0 + p][p0 + k] in LDS (read as broadcasts).
+// kFast: straight-line code for a full block; ambiguous proposals are only
+// reported (the returned mask is nonzero) and the caller runs the block again
+// with kFast = false, which evaluates their thresholds and honours cnt < 64.
+template <bool kFast>
+__device__ __forceinline__ uint64_t block_chain(int32_t (&hb)[64], uint64_t& wv,
+                                                const uint32_t* ut, const int4* wt, uint32_t cnt,
+                                                uint32_t lane, uint32_t dlim, double nbd,
+                                                double beta, int64_t& eacc, uint32_t& nacc) {
+    uint64_t amb_any = 0;
+    // Row p of the block couplings (entries k > p) is read one position ahead.
+    int4 wn[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) wn[j] = wt[j];
+#pragma unroll
+    for (int p = 0; p < 64; ++p) {
+        int4 wc[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) wc[j] = wn[j];
+        if (p + 1 < 64) {
+#pragma unroll
+            for (int j = (p + 2) / 4; j < 16; ++j) wn[j] = wt[(p + 1) * 16 + j];
+        }
+        if (!kFast && (uint32_t)p >= cnt) continue;  // past n in the last block
+        const uint64_t x = readlane64(wv, p);
+        const bool xb = lane_bit(x);
+        const int32_t de = xb ? -hb[p] : hb[p];
+        const uint32_t u = ut[p * 64 + lane];
+        uint32_t lo_u, hi_u;
+        threshold_band(de, nbd, lo_u, hi_u);
+        const bool pos = de > 0 && (uint32_t)de <= dlim;
+        bool acc = de <= 0 || (pos && u < lo_u);
+        const bool amb = pos && u >= lo_u && u <= hi_u;
+        if (kFast)
+            amb_any |= wave_ballot(amb);
+        else if (amb)
+            acc = exact_accept(u, de, beta);
+        const uint64_t m = wave_ballot(acc);
+        const int32_t dx = acc ? (xb ? -1 : 1) : 0;
+#pragma unroll
+        for (int k = p + 1; k < 64; ++k) {
+            const int4 c = wc[k >> 2];
+            const int32_t w = (k & 3) == 0 ? c.x : (k & 3) == 1 ? c.y : (k & 3) == 2 ? c.z : c.w;
+            hb[k] += __mul24(dx, w);
+        }
+        eacc += acc ? de : 0;
+        nacc += (uint32_t)__builtin_popcountll(m);
+        wv = lane == (uint32_t)p ? (x ^ m) : wv;
+    }
+    return amb_any;
 }
 
 __global__ __launch_bounds__(kDenseThreads) void sweep_dense_kernel(DevDense d, DevState s,
                                                                     const double* __restrict__ beta,
                                                                     uint32_t sweep, uint32_t key0,
                                                                     uint32_t key1) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const uint32_t nchunk = d.n_pad / 64, xs = nchunk + 1;
-    const DenseLds L = carve(lds_raw, xs);
+    extern __shared__ __align__(16) uint64_t xt[];
+    __shared__ __align__(16) BlockTiles T;
+    const uint32_t nchunk = d.n_pad / 64, xs = nchunk + 1, nblk = (d.n + 63) / 64;
     const uint32_t slot = blockIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t n = d.n;
     uint64_t* words = s.spins + (size_t)slot * n;
-    load_bit_rows(words, n, nchunk, xs, L.xt, wave, lane);
+    load_bit_rows(words, n, nchunk, xs, xt, wave, lane);
     const double b = beta[s.temp_of[slot * 64 + lane]];
     const uint32_t dlim = threshold_limit(b, d.dtab);
-    const float nb = (float)(-b * 1.4426950408889634);
+    const double nbd = -b * 1.4426950408889634;
     int64_t eacc = 0;
     uint32_t nacc = 0;
     __syncthreads();
-    for (uint32_t blk = 0; blk < nchunk; ++blk) {
+    for (uint32_t blk = 0; blk < nblk; ++blk) {
         const uint32_t p0 = blk * 64;
         const uint32_t cnt = min(64u, n - p0);
-        // 1. uniforms of positions p0 + 16 wave .. +15, and the block couplings
+        // 1. uniforms of positions p0 + 16 wave .. +15
 #pragma unroll
         for (uint32_t g = 0; g < 4; ++g) {
             const uint32_t q = 16 * wave + 4 * g;
             const U32x4 u =
                 philox4x32_10((p0 + q) >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
 #pragma unroll
-            for (uint32_t t = 0; t < 4; ++t) L.ublk[(q + t) * 64 + lane] = u.v[t];
+            for (uint32_t t = 0; t < 4; ++t) T.u[(q + t) * 64 + lane] = u.v[t];
         }
         {
-            const uint32_t k = threadIdx.x >> 2, c0 = (threadIdx.x & 3u) * 16;
-            const v4i a = *reinterpret_cast<const v4i*>(d.a + (size_t)(p0 + k) * d.n_pad + p0 + c0);
+            const int4* src = reinterpret_cast<const int4*>(d.wdiag + (size_t)blk * 4096);
 #pragma unroll
-            for (uint32_t t = 0; t < 16; ++t) {
-                const int32_t av = (int32_t)(int8_t)((uint32_t)a[t >> 2] >> (8 * (t & 3)));
-                L.wblk[k * 64 + c0 + t] = d.scale * av;
-            }
+            for (uint32_t t = 0; t < 4; ++t) T.w[t * 256 + threadIdx.x] = src[t * 256 + threadIdx.x];
         }
         // 2. fields of the block on the matrix cores
-        block_fields(d, L.xt, xs, p0, L.hblk, wave, lane);
+        block_fields(d, xt, xs, p0, T.h, wave, lane);
         __syncthreads();
         // 3. the block's positions in order, lane = replica
         if (wave == 0) {
             int32_t hb[64];
 #pragma unroll
-            for (int k = 0; k < 64; ++k) hb[k] = L.hblk[k * kHStride + lane];
-            uint64_t wv = lane < cnt ? words[p0 + lane] : 0ull;
+            for (int k = 0; k < 64; ++k) hb[k] = T.h[k * kHStride + lane];
+            const uint64_t w0 = lane < cnt ? words[p0 + lane] : 0ull;
+            const int64_t e0 = eacc;
+            const uint32_t n0 = nacc;
+            uint64_t wv = w0;
+            bool redo = cnt < 64;
+            if (!redo)
+                redo = block_chain<true>(hb, wv, T.u, T.w, 64, lane, dlim, nbd, b, eacc, nacc) != 0;
+            if (redo) {
 #pragma unroll
-            for (int p = 0; p < 64; ++p) {
-                if ((uint32_t)p >= cnt) continue;  // past n in the last block
-                const uint64_t x = readlane64(wv, p);
-                const bool xb = lane_bit(x);
-                const int32_t de = xb ? -hb[p] : hb[p];
-                bool acc = de <= 0;
-                if (!acc && (uint32_t)de <= dlim)
-                    acc = dense_accept(L.ublk[p * 64 + lane], de, nb, b);
-                const uint64_t m = wave_ballot(acc);
-                if (m) {
-                    const int32_t dx = acc ? (xb ? -1 : 1) : 0;
-#pragma unroll
-                    for (int k = p + 1; k < 64; ++k) hb[k] += dx * L.wblk[p * 64 + k];
-                    eacc += acc ? de : 0;
-                    nacc += (uint32_t)__builtin_popcountll(m);
-                    wv = lane == (uint32_t)p ? (x ^ m) : wv;
-                }
+                for (int k = 0; k < 64; ++k) hb[k] = T.h[k * kHStride + lane];
+                wv = w0;
+                eacc = e0;
+                nacc = n0;
+                block_chain<false>(hb, wv, T.u, T.w, cnt, lane, dlim, nbd, b, eacc, nacc);
             }
             // 4. new words to HBM and to the bit rows
             if (lane < cnt) words[p0 + lane] = wv;
-            L.xt[lane * xs + blk] = wave_transpose(wv, lane);
+            xt[lane * xs + blk] = wave_transpose(wv, lane);
         }
         __syncthreads();
     }
@@ -220,20 +279,20 @@ __global__ __launch_bounds__(kDenseThreads) void sweep_dense_kernel(DevDense d, 
 // (slot, block of 64 positions); out[(slot * n + site) * 64 + lane].
 __global__ __launch_bounds__(kDenseThreads) void dense_fields_kernel(DevDense d, DevState s,
                                                                      int32_t* __restrict__ out) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+    extern __shared__ __align__(16) uint64_t xt[];
+    __shared__ __align__(16) int32_t h[64 * kHStride];
     const uint32_t nchunk = d.n_pad / 64, xs = nchunk + 1;
-    const DenseLds L = carve(lds_raw, xs);
     const uint32_t slot = blockIdx.x, blk = blockIdx.y;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t n = d.n, p0 = blk * 64;
-    load_bit_rows(s.spins + (size_t)slot * n, n, nchunk, xs, L.xt, wave, lane);
+    load_bit_rows(s.spins + (size_t)slot * n, n, nchunk, xs, xt, wave, lane);
     __syncthreads();
-    block_fields(d, L.xt, xs, p0, L.hblk, wave, lane);
+    block_fields(d, xt, xs, p0, h, wave, lane);
     __syncthreads();
     const uint32_t cnt = min(64u, n - p0);
     for (uint32_t i = wave; i < cnt; i += 4)
-        out[((size_t)slot * n + d.order[p0 + i]) * 64 + lane] = L.hblk[i * kHStride + lane];
+        out[((size_t)slot * n + d.order[p0 + i]) * 64 + lane] = h[i * kHStride + lane];
 }
 
 bool set_lds_limit(const void* fn, size_t bytes) {
@@ -260,7 +319,8 @@ hipError_t launch_dense_fields(const DevDense& d, const DevState& s, int32_t* ou
     const size_t lds = dense_lds_bytes(d.n_pad);
     if (!set_lds_limit(reinterpret_cast<const void*>(&dense_fields_kernel), lds))
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(dense_fields_kernel, dim3(s.cells, d.n_pad / 64), dim3(kDenseThreads), lds,
+    // one workgroup per block that holds positions (n_pad may add empty blocks)
+    hipLaunchKernelGGL(dense_fields_kernel, dim3(s.cells, (d.n + 63) / 64), dim3(kDenseThreads), lds,
                        stream, d, s, out);
     return hipGetLastError();
 }

@@ -25,8 +25,9 @@ struct DevModel {
 
 // Dense couplings for the matrix-core path: q = scale * a, a int8.
 struct DevDense {
-    uint32_t n = 0, n_pad = 0;      // n_pad: n rounded up to a multiple of 64
+    uint32_t n = 0, n_pad = 0;      // n_pad: n rounded up to a multiple of 512
     const int8_t* a = nullptr;      // [n_pad][n_pad] position space, zero padded
+    const int32_t* wdiag = nullptr; // [n_pad / 64][64][64] scale * a inside each diagonal block
     const int32_t* lin = nullptr;   // n_pad linear terms, zero padded
     const uint32_t* order = nullptr;  // n: position -> site
     int32_t scale = 1;
