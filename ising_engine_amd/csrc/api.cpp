@@ -63,10 +63,20 @@ struct ige_model {
                 for (uint32_t k = 0; k < 64; ++k)
                     wd[(static_cast<size_t>(b) * 64 + p) * 64 + k] =
                         ds.scale * a[static_cast<size_t>(b * 64 + p) * np + b * 64 + k];
-        dense_a.alloc(a.size());
+        // matrix-core operand order: tile (g, c) = rows 32g.., columns 32c.., byte
+        // 16 l + b = row 32g + l % 32, column 32c + 16 (l / 32) + b
+        std::vector<int8_t> at(a.size());
+        const uint32_t nt = np / 32;
+        for (uint32_t g = 0; g < nt; ++g)
+            for (uint32_t c = 0; c < nt; ++c)
+                for (uint32_t l = 0; l < 64; ++l)
+                    for (uint32_t bb = 0; bb < 16; ++bb)
+                        at[(static_cast<size_t>(g) * nt + c) * 1024 + 16 * l + bb] =
+                            a[static_cast<size_t>(32 * g + l % 32) * np + 32 * c + 16 * (l / 32) + bb];
+        dense_a.alloc(at.size());
         dense_lin.alloc(np);
         dense_wdiag.alloc(wd.size());
-        dense_a.upload(a.data(), a.size(), nullptr);
+        dense_a.upload(at.data(), at.size(), nullptr);
         dense_lin.upload(l.data(), np, nullptr);
         dense_wdiag.upload(wd.data(), wd.size(), nullptr);
         check(hipStreamSynchronize(nullptr), "dense upload");

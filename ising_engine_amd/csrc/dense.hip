@@ -47,9 +47,10 @@ constexpr int kPrefetch = 8;             // k chunks of the coupling rows in fli
 // dynamic; the per-block tiles are static so that the unrolled sequential
 // loop addresses them with immediate offsets.
 struct BlockTiles {
-    int32_t h[64 * kHStride];  // fields of the block, [position][replica]
-    uint32_t u[64 * 64];       // uniforms, [position][lane]
-    int4 w[64 * 16];           // scale * a[p0 + p][p0 + k], row p = 16 int4
+    int32_t h[2][64 * kHStride];  // fields of this block and the next, [position][replica]
+    uint32_t u[64 * 64];          // uniforms, [position][lane]
+    int4 w[64 * 16];              // scale * a[p0 + p][p0 + k], row p = 16 int4
+    uint64_t xnew[64];            // the block's new words as replica-major bit rows
 };
 
 __host__ __device__ inline size_t dense_lds_bytes(uint32_t n_pad) {
@@ -85,13 +86,18 @@ __device__ __forceinline__ void load_bit_rows(const uint64_t* words, uint32_t n,
 // each 32-wide k step; A and B use the same k assignment, so the product
 // does not depend on how the instruction orders k internally.  Result
 // element v of lane l is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+// The coupling matrix is stored in exactly this operand order: tile (g, c)
+// holds rows 32g.. and columns 32c.. as 64 lanes x 16 bytes, so every load
+// of a wave reads 1 KB of contiguous memory (tiles of one row group follow
+// each other along k).
 __device__ __forceinline__ void block_fields(const DevDense& d, const uint64_t* xt, uint32_t xs,
                                              uint32_t p0, int32_t* hblk, uint32_t wave,
                                              uint32_t lane) {
     const uint32_t nchunk = d.n_pad / 64;
     const uint32_t rbase = 32 * (wave >> 1), jbase = 32 * (wave & 1);
     const uint32_t kh = 16 * (lane >> 5);
-    const int8_t* arow = d.a + (size_t)(p0 + rbase + (lane & 31)) * d.n_pad + kh;
+    const uint32_t ntile = d.n_pad / 32;  // tiles per row group
+    const int8_t* arow = d.a + ((size_t)((p0 + rbase) / 32) * ntile) * 1024 + 16 * lane;
     const uint64_t* xr = xt + (size_t)(jbase + (lane & 31)) * xs;
     v16i acc = {};
     // Coupling rows stream from L2 / HBM with kPrefetch chunks in flight.
@@ -101,8 +107,8 @@ __device__ __forceinline__ void block_fields(const DevDense& d, const uint64_t* 
     v4i pa0[kPrefetch], pa1[kPrefetch];
 #pragma unroll
     for (int i = 0; i < kPrefetch; ++i) {
-        pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 64);
-        pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 64 + 32);
+        pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 2048);
+        pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 2048 + 1024);
     }
     for (uint32_t kb = 0; kb < nchunk; kb += kPrefetch) {
 #pragma unroll
@@ -110,8 +116,8 @@ __device__ __forceinline__ void block_fields(const DevDense& d, const uint64_t* 
             const uint32_t kc = kb + i;
             const v4i a0 = pa0[i], a1 = pa1[i];
             const uint32_t kn = min(kc + (uint32_t)kPrefetch, nchunk - 1);
-            pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 64);
-            pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 64 + 32);
+            pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 2048);
+            pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 2048 + 1024);
             const uint64_t xb = xr[kc];
             acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, spread16((uint32_t)(xb >> kh)), acc,
                                                         0, 0, 0);
@@ -125,6 +131,100 @@ __device__ __forceinline__ void block_fields(const DevDense& d, const uint64_t* 
         const uint32_t i = rbase + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         hblk[i * kHStride + j] = d.lin[p0 + i] + d.scale * acc[v];
     }
+}
+
+// Rows 32 rh .. +31 of the fields of positions [p0, p0 + 64), all 64 replicas
+// (two 32 x 32 tiles sharing each coupling load), into hblk; one wave.
+__device__ __forceinline__ void half_block_fields(const DevDense& d, const uint64_t* xt,
+                                                  uint32_t xs, uint32_t p0, int32_t* hblk,
+                                                  uint32_t rh, uint32_t lane) {
+    const uint32_t nchunk = d.n_pad / 64;
+    const uint32_t kh = 16 * (lane >> 5);
+    const uint32_t ntile = d.n_pad / 32;
+    const int8_t* arow = d.a + ((size_t)((p0 + 32 * rh) / 32) * ntile) * 1024 + 16 * lane;
+    const uint64_t* xr0 = xt + (size_t)(lane & 31) * xs;
+    const uint64_t* xr1 = xt + (size_t)(32 + (lane & 31)) * xs;
+    v16i acc0 = {}, acc1 = {};
+    v4i pa0[kPrefetch], pa1[kPrefetch];
+#pragma unroll
+    for (int i = 0; i < kPrefetch; ++i) {
+        pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 2048);
+        pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)i * 2048 + 1024);
+    }
+    for (uint32_t kb = 0; kb < nchunk; kb += kPrefetch) {
+#pragma unroll
+        for (int i = 0; i < kPrefetch; ++i) {
+            const uint32_t kc = kb + i;
+            const v4i a0 = pa0[i], a1 = pa1[i];
+            const uint32_t kn = min(kc + (uint32_t)kPrefetch, nchunk - 1);
+            pa0[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 2048);
+            pa1[i] = *reinterpret_cast<const v4i*>(arow + (size_t)kn * 2048 + 1024);
+            const uint64_t x0 = xr0[kc], x1 = xr1[kc];
+            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, spread16((uint32_t)(x0 >> kh)), acc0,
+                                                         0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, spread16((uint32_t)(x1 >> kh)), acc1,
+                                                         0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                a1, spread16((uint32_t)(x0 >> (32 + kh))), acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                a1, spread16((uint32_t)(x1 >> (32 + kh))), acc1, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t i = 32 * rh + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        hblk[i * kHStride + (lane & 31)] = d.lin[p0 + i] + d.scale * acc0[v];
+        hblk[i * kHStride + 32 + (lane & 31)] = d.lin[p0 + i] + d.scale * acc1[v];
+    }
+}
+
+// Adds to the fields of block blk + 1 (hnext) the change made by block blk's
+// flips: scale * a[next rows][block blk columns] . (new - old), the spin
+// changes as -1 / 0 / +1 bytes; wave w owns one 32 x 32 tile.
+__device__ __forceinline__ void next_block_correction(const DevDense& d, const uint64_t* xt,
+                                                      uint32_t xs, uint32_t blk,
+                                                      const uint64_t* xnew, int32_t* hnext,
+                                                      uint32_t wave, uint32_t lane) {
+    const uint32_t ntile = d.n_pad / 32;
+    const uint32_t rh = wave >> 1, jh = wave & 1;
+    const uint32_t kh = 16 * (lane >> 5);
+    const uint32_t j = 32 * jh + (lane & 31);
+    const int8_t* t = d.a + ((size_t)(2 * (blk + 1) + rh) * ntile + 2 * blk) * 1024 + 16 * lane;
+    const uint64_t o = xt[(size_t)j * xs + blk], nw = xnew[j];
+    const uint64_t up = nw & ~o, dn = o & ~nw;
+    v16i acc = {};
+#pragma unroll
+    for (uint32_t c = 0; c < 2; ++c) {
+        const v4i a = *reinterpret_cast<const v4i*>(t + c * 1024);
+        const uint32_t sh = 32 * c + kh;
+        const v4i bp = spread16((uint32_t)(up >> sh));
+        const v4i bn = spread16((uint32_t)(dn >> sh));
+        const v4i bd = bp | (bn * 0xFF);  // bytes 0x01 (+1), 0xFF (-1) or 0
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bd, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t i = 32 * rh + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        hnext[i * kHStride + j] += d.scale * acc[v];
+    }
+}
+
+// Uniforms of block blk (wave w: positions 16 w .. +15) and its couplings.
+__device__ __forceinline__ void stage_block(const DevDense& d, BlockTiles& T, uint32_t blk,
+                                            uint32_t slot, uint32_t sweep, uint32_t key0,
+                                            uint32_t key1, uint32_t wave, uint32_t lane) {
+    const uint32_t p0 = blk * 64;
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) {
+        const uint32_t q = 16 * wave + 4 * g;
+        const U32x4 u =
+            philox4x32_10((p0 + q) >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) T.u[(q + t) * 64 + lane] = u.v[t];
+    }
+    const int4* src = reinterpret_cast<const int4*>(d.wdiag + (size_t)blk * 4096);
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) T.w[t * 256 + threadIdx.x] = src[t * 256 + threadIdx.x];
 }
 
 // Acceptance u < metropolis_threshold(beta, de).  The threshold is first
@@ -223,31 +323,24 @@ __global__ __launch_bounds__(kDenseThreads) void sweep_dense_kernel(DevDense d, 
     int64_t eacc = 0;
     uint32_t nacc = 0;
     __syncthreads();
+    // Block 0: uniforms, couplings and fields.
+    stage_block(d, T, 0, slot, sweep, key0, key1, wave, lane);
+    if (wave == 1 || wave == 2) half_block_fields(d, xt, xs, 0, T.h[0], wave - 1, lane);
+    __syncthreads();
+    // Per block: wave 0 runs the block while waves 1 and 2 compute the next
+    // block's fields from the spins before this block; after a barrier all
+    // waves add this block's flips to those fields (matrix cores) and stage
+    // the next block's uniforms and couplings.
     for (uint32_t blk = 0; blk < nblk; ++blk) {
         const uint32_t p0 = blk * 64;
         const uint32_t cnt = min(64u, n - p0);
-        // 1. uniforms of positions p0 + 16 wave .. +15
-#pragma unroll
-        for (uint32_t g = 0; g < 4; ++g) {
-            const uint32_t q = 16 * wave + 4 * g;
-            const U32x4 u =
-                philox4x32_10((p0 + q) >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
-#pragma unroll
-            for (uint32_t t = 0; t < 4; ++t) T.u[(q + t) * 64 + lane] = u.v[t];
-        }
-        {
-            const int4* src = reinterpret_cast<const int4*>(d.wdiag + (size_t)blk * 4096);
-#pragma unroll
-            for (uint32_t t = 0; t < 4; ++t) T.w[t * 256 + threadIdx.x] = src[t * 256 + threadIdx.x];
-        }
-        // 2. fields of the block on the matrix cores
-        block_fields(d, xt, xs, p0, T.h, wave, lane);
-        __syncthreads();
-        // 3. the block's positions in order, lane = replica
+        const uint32_t cur = blk & 1u;
+        const bool more = blk + 1 < nblk;
         if (wave == 0) {
+            const int32_t* hc = T.h[cur];
             int32_t hb[64];
 #pragma unroll
-            for (int k = 0; k < 64; ++k) hb[k] = T.h[k * kHStride + lane];
+            for (int k = 0; k < 64; ++k) hb[k] = hc[k * kHStride + lane];
             const uint64_t w0 = lane < cnt ? words[p0 + lane] : 0ull;
             const int64_t e0 = eacc;
             const uint32_t n0 = nacc;
@@ -257,16 +350,24 @@ __global__ __launch_bounds__(kDenseThreads) void sweep_dense_kernel(DevDense d, 
                 redo = block_chain<true>(hb, wv, T.u, T.w, 64, lane, dlim, nbd, b, eacc, nacc) != 0;
             if (redo) {
 #pragma unroll
-                for (int k = 0; k < 64; ++k) hb[k] = T.h[k * kHStride + lane];
+                for (int k = 0; k < 64; ++k) hb[k] = hc[k * kHStride + lane];
                 wv = w0;
                 eacc = e0;
                 nacc = n0;
                 block_chain<false>(hb, wv, T.u, T.w, cnt, lane, dlim, nbd, b, eacc, nacc);
             }
-            // 4. new words to HBM and to the bit rows
             if (lane < cnt) words[p0 + lane] = wv;
-            xt[lane * xs + blk] = wave_transpose(wv, lane);
+            T.xnew[lane] = wave_transpose(wv, lane);
+        } else if (more && (wave == 1 || wave == 2)) {
+            half_block_fields(d, xt, xs, p0 + 64, T.h[cur ^ 1u], wave - 1, lane);
         }
+        __syncthreads();
+        if (more) {
+            next_block_correction(d, xt, xs, blk, T.xnew, T.h[cur ^ 1u], wave, lane);
+            stage_block(d, T, blk + 1, slot, sweep, key0, key1, wave, lane);
+        }
+        __syncthreads();
+        if (wave == 0) xt[lane * xs + blk] = T.xnew[lane];
         __syncthreads();
     }
     if (wave == 0) {

@@ -26,7 +26,8 @@ struct DevModel {
 // Dense couplings for the matrix-core path: q = scale * a, a int8.
 struct DevDense {
     uint32_t n = 0, n_pad = 0;      // n_pad: n rounded up to a multiple of 512
-    const int8_t* a = nullptr;      // [n_pad][n_pad] position space, zero padded
+    const int8_t* a = nullptr;      // n_pad^2 position space, zero padded, in 32x32
+                                    // operand tiles (see block_fields)
     const int32_t* wdiag = nullptr; // [n_pad / 64][64][64] scale * a inside each diagonal block
     const int32_t* lin = nullptr;   // n_pad linear terms, zero padded
     const uint32_t* order = nullptr;  // n: position -> site

@@ -45,6 +45,9 @@ CASES = [
     ("sk_n1000_pt", lambda: instances.sk_int8(1000, 4), 4, True, 4, False),
     ("dense_qubo_n200_scale3_pt", lambda: instances.dense_qubo(200, 5, scale=3), 3, True, 10,
      False),
+    # scale 7: block corrections exceed int16, the int32 update path runs
+    ("dense_qubo_n130_scale7_pt", lambda: instances.dense_qubo(130, 8, scale=7), 2, True, 10,
+     False),
     ("sparse_qubo_n150_forced_dense", lambda: instances.random_qubo(150, 6, 6), 2, True, 10,
      False),
     ("pmJ_2d_L8_forced_dense", lambda: instances.hypercubic_pm_j(8, 2, 7), 2, True, 10, False),
