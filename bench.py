@@ -51,24 +51,33 @@ def measured_traffic(kernel: str, workload: str):
     return None
 
 
-def counter_probe(_lib, instances, L: int, temps: int, sweeps: int):
-    """The program the HBM counter passes run: `sweeps` PT steps of the workload, then one
-    1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the
-    counters' units."""
+def counter_probe(_lib, instances, L: int, temps: int, sweeps: int, workload: str = "pt"):
+    """The program the counter passes run: `sweeps` PT steps of the workload (or, for
+    workload "sk", sweeps of the dense SK N=4096 anneal), then one 1 GiB device-to-device
+    copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters' units."""
     import torch
-    wl = instances.workload_3d_pt(L=L, temps=temps)
-    qb = wl["qubo"]
     ctx = _lib.Context(0)
-    m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
-    st = _lib.State(m, wl["cells"], 0x5EED)
-    st.set_ladder(wl["ladder"])
-    st.run(sweeps, exchange=True)
+    if workload == "sk":
+        wl = instances.workload_sk_anneal()
+        qb = wl["qubo"]
+        m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+        st = _lib.State(m, 1, 3)
+        sched = _lib.anneal_schedule(wl["beta_lo"], wl["beta_hi"], wl["sweeps"])
+        k0 = wl["sweeps"] // 2
+        st.run(sweeps, beta_schedule=sched[k0:k0 + sweeps, None])
+    else:
+        wl = instances.workload_3d_pt(L=L, temps=temps)
+        qb = wl["qubo"]
+        m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+        st = _lib.State(m, wl["cells"], 0x5EED)
+        st.set_ladder(wl["ladder"])
+        st.run(sweeps, exchange=True)
     st.synchronize()
     a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     b.copy_(a)
     torch.cuda.synchronize()
-    log(f"[bench] counter probe: {sweeps} PT steps of {wl['name']} and a 1 GiB copy done")
+    log(f"[bench] counter probe: {sweeps} steps of {wl['name']} and a 1 GiB copy done")
 
 
 def cpu_baseline(qb, cells, ladder, budget_s: float):
@@ -325,7 +334,9 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--counter-probe", type=int, default=0, metavar="SWEEPS",
-                    help="only run SWEEPS PT steps and a 1 GiB copy (for rocprofv3 --pmc passes)")
+                    help="only run SWEEPS steps of --probe-workload and a 1 GiB copy "
+                         "(for rocprofv3 --pmc passes)")
+    ap.add_argument("--probe-workload", choices=["pt", "sk"], default="pt")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -335,7 +346,7 @@ def main():
     from ising_engine_amd import _lib, instances
 
     if args.counter_probe:
-        counter_probe(_lib, instances, args.L, args.temps, args.counter_probe)
+        counter_probe(_lib, instances, args.L, args.temps, args.counter_probe, args.probe_workload)
         return
     kernel = {"auto": _lib.KERNEL_AUTO, "generic": _lib.KERNEL_GENERIC,
               "multispin": _lib.KERNEL_MULTISPIN}[args.kernel]
