@@ -31,13 +31,16 @@ __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
                                0x3333333333333333ull, 0x5555555555555555ull};
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
+        // Stage j: lanes l and l ^ j swap the off-diagonal j x j blocks of
+        // each 2j x 2j block.  The lane with bit j clear keeps the low j bits
+        // of each 2j-bit group and takes its partner's, shifted up; the other
+        // keeps the high bits and takes its partner's, shifted down.
         const uint32_t j = 32u >> s;
-        const uint64_t m = masks[s];
+        const bool h = (lane & j) != 0;
         const uint64_t y = __shfl_xor(x, (int)j, 64);
-        const uint64_t hi = (lane & j) ? ~0ull : 0ull;  // 1s in lanes with bit j set
-        const uint64_t keep = (x & m & ~hi) | (x & ~m & hi);
-        const uint64_t take = (((y & m) << j) & ~hi) | (((y >> j) & m) & hi);
-        x = keep | take;
+        const uint64_t ys = h ? (y >> j) : (y << j);
+        const uint64_t keep = h ? ~masks[s] : masks[s];
+        x = (x & keep) | (ys & ~keep);
     }
     return x;
 }
