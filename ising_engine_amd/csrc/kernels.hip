@@ -148,7 +148,7 @@ constexpr uint32_t kTagTail = 5u;
 // Sum over the wave's lanes of the per-lane bit-sliced counter acc[0..W-1],
 // returned in lane r for replica r.
 template <int W>
-__device__ __forceinline__ int64_t wave_column_sum(const uint64_t (&acc)[W], uint32_t lane) {
+__device__ __forceinline__ int64_t wave_column_sum(const uint64_t* acc, uint32_t lane) {
     int64_t tot = 0;
 #pragma unroll
     for (int b = 0; b < W; ++b)
@@ -177,6 +177,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     uint32_t n, uint32_t dmax, int32_t w0, const unsigned long long* __restrict__ qplanes,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
     uint32_t per_lane, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    static_assert(kMaxPerLane <= 16 && W >= B + 4, "energy counters too narrow");
     constexpr int LMAX = DEG > 0 ? DEG - DEG / 2 : (1 << (B - 1));
     extern __shared__ unsigned long long lds_q[];
     __shared__ uint32_t queue[kWavesPerBlock][kQueueCap];
@@ -211,6 +212,10 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         return (r & 1u) ? (uint32_t)(w >> 32) : (uint32_t)w;
     };
 
+    // Bit-sliced per-replica sums over this lane's sites: kacc = sum k * flip,
+    // dacc = sum d * flip (for a regular lattice d is fixed, and dacc counts
+    // flips in its low kCnt planes instead).
+    constexpr int kCnt = 5;  // flips per lane <= kMaxPerLane = 16
     uint64_t kacc[W], dacc[W];
 #pragma unroll
     for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
@@ -298,17 +303,25 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         Wr[p] = x ^ flip;
         nacc += (uint32_t)__builtin_popcountll(flip);
         // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
-        uint64_t ck = 0, cd = 0;
+        uint64_t ck = 0, cd = DEG > 0 ? flip : 0ull;
 #pragma unroll
         for (int b = 0; b < W; ++b) {
             const uint64_t ak = b < B ? (c[b] & flip) : 0ull;
-            const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
             const uint64_t sk = kacc[b] ^ ak ^ ck;
             ck = (kacc[b] & ak) | (ck & (kacc[b] ^ ak));
             kacc[b] = sk;
-            const uint64_t sd = dacc[b] ^ ad ^ cd;
-            cd = (dacc[b] & ad) | (cd & (dacc[b] ^ ad));
-            dacc[b] = sd;
+            if (DEG > 0) {
+                if (b < kCnt) {  // dacc += flip (incrementer)
+                    const uint64_t sd = dacc[b] ^ cd;
+                    cd &= dacc[b];
+                    dacc[b] = sd;
+                }
+            } else {
+                const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
+                const uint64_t sd = dacc[b] ^ ad ^ cd;
+                cd = (dacc[b] & ad) | (cd & (dacc[b] ^ ad));
+                dacc[b] = sd;
+            }
         }
         // Queue the replicas whose six leading bits tie.
         while (und) {
@@ -348,7 +361,8 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
     const int64_t ksum = wave_column_sum<W>(kacc, lane);
-    const int64_t dsum = wave_column_sum<W>(dacc, lane);
+    const int64_t dsum = DEG > 0 ? (int64_t)DEG * wave_column_sum<kCnt>(dacc, lane)
+                                 : wave_column_sum<W>(dacc, lane);
     const int64_t de = ((int64_t)w0 * (2 * ksum - dsum)) / 2 + (int64_t)edef[wv][lane];
     for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
     // Combine the block's waves in LDS; the last wave to finish publishes.
@@ -514,16 +528,18 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp, \
                        m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,       \
                        s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1)
+    // Counter width W = B + 4: at most kMaxPerLane = 16 sites per lane, each
+    // adding k * flip <= 2^B - 1, so every sum stays below 2^(B+4).
     if (m.regular_degree == 6) {
-        IGE_MS_LAUNCH(3, 8, 6);
+        IGE_MS_LAUNCH(3, 7, 6);
     } else if (m.regular_degree == 4) {
-        IGE_MS_LAUNCH(3, 8, 4);
+        IGE_MS_LAUNCH(3, 7, 4);
     } else if (m.dmax < 8) {
-        IGE_MS_LAUNCH(3, 8, 0);
+        IGE_MS_LAUNCH(3, 7, 0);
     } else if (m.dmax < 16) {
-        IGE_MS_LAUNCH(4, 12, 0);
+        IGE_MS_LAUNCH(4, 8, 0);
     } else if (m.dmax < 32) {
-        IGE_MS_LAUNCH(5, 12, 0);
+        IGE_MS_LAUNCH(5, 9, 0);
     } else {
         return hipErrorInvalidValue;
     }
