@@ -141,7 +141,7 @@ constexpr uint32_t kEagerPlanes = 6;
 // Per (slot, level m): planes 0..5 of the thresholds, the nonzero mask, and
 // the 64 thresholds themselves (two per u64).
 constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
-constexpr uint32_t kQueueCap = 512;  // deferred entries per wave
+constexpr uint32_t kQueueCap = 1024; // deferred entries per wave
 constexpr uint32_t kMaxPerLane = 16; // sites per lane
 constexpr uint32_t kTagTail = 5u;
 
