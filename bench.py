@@ -37,18 +37,18 @@ def bytes_per_attempt(cells: int, deg: float, w_b: int = 1, w_l: int = 1) -> flo
     return (16.0 * cells + 4 + deg * (4 + w_b) + w_l) / (64.0 * cells)
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy), or
-    None when no summary matches."""
+def measured_counters(kernel: str, workload: str) -> dict:
+    """The committed counter summary of `kernel` on `workload` (HBM bytes per launch from
+    FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy; VALU
+    busy), or {} when none matches."""
     for name in sorted(os.listdir(PROFILES)) if os.path.isdir(PROFILES) else []:
         if not (name.startswith("traffic_") and name.endswith(".json")):
             continue
         with open(os.path.join(PROFILES, name)) as f:
             t = json.load(f)
         if t.get("workload") == workload and t.get("kernel", "").startswith(f"{kernel}<"):
-            return int(t["traffic_bytes_per_launch"])
-    return None
+            return t
+    return {}
 
 
 def counter_probe(_lib, instances, L: int, temps: int, sweeps: int, workload: str = "pt"):
@@ -387,9 +387,11 @@ def main():
     achieved = attempts_per_launch * bpa / avg_launch_s / 1e9
     log(f"[bench] sweep launch avg {avg_launch_s * 1e6:.1f} us over {launches} launches; "
         f"exchange avg {ms_x / max(n_x, 1) * 1e3:.1f} us; {bpa:.4f} B/attempt")
-    traffic = measured_traffic(f"sweep_{kname}_kernel", wl["name"])
+    counters = measured_counters(f"sweep_{kname}_kernel", wl["name"])
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
+                    frac=round(achieved / HBM_PEAK_GBS, 5),
+                    traffic=counters.get("traffic_bytes_per_launch"),
+                    valu_busy=counters.get("valu_busy"),
                     algorithmic_bytes_per_launch=round(attempts_per_launch * bpa),
                     kernel=f"sweep_{kname}_kernel",
                     avg_launch_us=round(avg_launch_s * 1e6, 2),
