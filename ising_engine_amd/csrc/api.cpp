@@ -341,7 +341,7 @@ int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, u
         } else if (s->kernel == IGE_KERNEL_DENSE) {
             model->ensure_dense();
         } else {
-            s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * 39);
+            s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * ige::kQStride);
         }
         std::vector<unsigned long long> zeros(s->counters.count, 0);
         s->counters.upload(zeros.data(), zeros.size(), s->stream);

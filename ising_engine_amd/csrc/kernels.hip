@@ -127,20 +127,17 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(
 //   k_r = #{ j : bit r of (x_i ^ x_j ^ [q_ij < 0]) }.
 // A lane counts k for its 64 replicas in B bit planes and accepts dE <= 0
 // outright.  For the rest it compares the uniform u_r with the threshold
-// thr_r six bits at a time: six Philox bit planes against the threshold's
-// six leading bit planes, every lane in lockstep, no early exit.  A replica
-// whose leading bits tie (probability 2^-6) goes to a per-wave queue in LDS;
-// the queue is drained by all 64 lanes at once, each comparing the low 26
-// bits of its entry's tail word with the threshold's low bits, and accepted
+// thr_r on its kEagerPlanes (eight) leading bits: eight Philox bit planes
+// against the threshold's eight leading bit planes, every lane in lockstep, no
+// early exit.  A replica whose leading bits tie (probability 2^-8) goes to a
+// per-wave queue in LDS; the queue is drained by all 64 lanes at once, each
+// comparing the low 24 bits of its entry's tail word with the threshold's low
+// bits, and accepted
 // entries flip their word with an atomic XOR.  Per-replica energy changes of
 // the lockstep flips are summed in bit-sliced counters and turned into
 // integers by one 64x64 bit transpose per counter plane; queued flips add
 // theirs through an LDS accumulator.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kEagerPlanes = 6;
-// Per (slot, level m): planes 0..5 of the thresholds, the nonzero mask, and
-// the 64 thresholds themselves (two per u64).
-constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
 constexpr uint32_t kQueueCap = 1024; // deferred entries per wave
 constexpr uint32_t kMaxPerLane = 16; // sites per lane
 constexpr uint32_t kTagTail = 5u;
@@ -156,8 +153,8 @@ __device__ __forceinline__ int64_t wave_column_sum(const uint64_t* acc, uint32_t
     return tot;
 }
 
-// Resolves one queued replica: u_r's six leading bits tie with thr_r's, so
-// u_r < thr_r iff tail >> 6 < thr_r's low 26 bits.
+// Resolves one queued replica: u_r's kEagerPlanes leading bits tie with
+// thr_r's, so u_r < thr_r iff tail >> kEagerPlanes < thr_r's low bits.
 __device__ __forceinline__ bool resolve_tail(uint32_t p, uint32_t slot, uint32_t sweep,
                                              uint32_t key0, uint32_t key1, uint32_t r,
                                              uint32_t thr) {
@@ -282,7 +279,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
             pos |= eq;
             und |= eq & lds_q[mrow[j] + kEagerPlanes];
         }
-        // Six lockstep bit planes.
+        // kEagerPlanes lockstep bit planes, two per Philox call.
         uint64_t lt = 0;
 #pragma unroll
         for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
@@ -323,7 +320,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
                 dacc[b] = sd;
             }
         }
-        // Queue the replicas whose six leading bits tie.
+        // Queue the replicas whose leading bits tie.
         while (und) {
             const uint32_t r = (uint32_t)__builtin_ctzll(und);
             und &= und - 1;

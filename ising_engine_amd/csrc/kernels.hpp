@@ -60,10 +60,17 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
                                   uint32_t c_end, uint32_t sweep, uint64_t seed,
                                   const unsigned long long* qplanes, hipStream_t stream);
 
-// Per (slot, m), m <= dmax, a record of 39 u64 at qplanes[(slot*(dmax+1)+m)*39]:
-// words 0..5: bit r = bit (31-b) of thr_r = metropolis_threshold(beta of the
-// temperature (slot, r) holds, m*w0/2); word 6: bit r set iff thr_r != 0;
-// words 7..38: the 64 thr_r as u32.
+// Uniforms of the multispin kernel: kEagerPlanes leading bits as bit planes
+// (two per Philox call), the rest from a tail word drawn only on a tie.
+constexpr uint32_t kEagerPlanes = 8;
+// Per (slot, level m): the threshold's kEagerPlanes leading bit planes, the
+// nonzero mask and the 64 thresholds themselves (two per u64).
+constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
+
+// Per (slot, m), m <= dmax, a record of kQStride u64 at qplanes[(slot*(dmax+1)+m)*kQStride]:
+// words 0..kEagerPlanes-1: bit r of word b = bit (31-b) of thr_r =
+// metropolis_threshold(beta of the temperature (slot, r) holds, m*w0/2);
+// word kEagerPlanes: bit r set iff thr_r != 0; then the 64 thr_r as u32.
 hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
                           unsigned long long* qplanes, hipStream_t stream);
 

@@ -57,7 +57,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 #define TAG_INIT 4u
 #define TAG_TAIL 5u
 /* Bit planes drawn for every replica before its tail word is consulted. */
-#define EAGER_PLANES 6u
+#define EAGER_PLANES 8u
 
 static void philox_seeded(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                           uint32_t out[4]) {
@@ -274,8 +274,8 @@ static void uniforms_word(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t s
 }
 
 /* Plane view: the top EAGER_PLANES bits of u_r are bit r of planes
- * P_0..P_5 (P_{2j}, P_{2j+1} = words 0|1 and 2|3 of Philox(pos, slot, sweep,
- * TAG_PLANE<<24 | j)), most significant first; the remaining 26 bits are the
+ * P_0..P_7 (P_{2j}, P_{2j+1} = words 0|1 and 2|3 of Philox(pos, slot, sweep,
+ * TAG_PLANE<<24 | j)), most significant first; the remaining 24 bits are the
  * top bits of the replica's tail word Philox(pos, slot, sweep,
  * TAG_TAIL<<24 | r>>2)[r & 3]. */
 static void uniforms_plane(uint64_t seed, uint32_t pos, uint32_t slot, uint32_t sweep,

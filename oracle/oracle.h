@@ -26,7 +26,7 @@
  * cannot be built here (rand_chacha 0.3.1, rand 0.8.5):
  *   - randoms come from Philox4x32-10 (Salmon et al., SC'11) keyed by
  *     (seed) with counters (position, slot, sweep, tag|index), read either
- *     as one 32-bit word per replica (word view) or as six bit planes over
+ *     as one 32-bit word per replica (word view) or as eight bit planes over
  *     the 64 replicas followed by a per-replica tail word (plane view);
  *   - acceptance compares a 32-bit uniform u with
  *     thr = min(floor(exp(-beta*dE) * 2^32), 2^32-1), exp evaluated by
