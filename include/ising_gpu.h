@@ -173,7 +173,10 @@ int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *wa
  * them, at most max_gang <= 32) that meet at a device-wide barrier after every
  * colour class.  spin_limit bounds each barrier wait; an expired wait makes
  * the next synchronize/read fail with IGE_ERR_HIP.  Batch defaults: 1 (no gangs:
- * throughput), 2048, 2^24; server defaults: 16, 2048, 2^24 (latency). */
+ * throughput), 2048, 2^24.  The server sizes gangs per round from its load:
+ * n / max(gang_sites, live sites / resident workgroups) members, at most
+ * max_gang and at most n / gang_sites; server defaults: 16, 2048, 2^24 (a gang
+ * barrier per colour class costs more than 256 sites of work on this GPU). */
 int32_t ige_batch_configure(ige_batch *batch, uint32_t max_gang, uint32_t gang_sites,
                             uint32_t spin_limit);
 int32_t ige_batch_synchronize(ige_batch *batch);
@@ -214,7 +217,8 @@ typedef struct {
 int32_t ige_server_create(ige_ctx *ctx, uint64_t hbm_bytes, double round_ms, uint32_t waves,
                           ige_server **out);
 int32_t ige_server_destroy(ige_server *server);
-/* Gang settings as ige_batch_configure; only before the first submit. */
+/* Gang settings as ige_batch_configure (per-round sizing described there);
+ * only before the first submit. */
 int32_t ige_server_configure(ige_server *server, uint32_t max_gang, uint32_t gang_sites,
                              uint32_t spin_limit);
 /* Queues one request (validated like POST /api/v1/solve, server_api.rs:70-99);

@@ -50,7 +50,7 @@ struct ige_batch {
     int num_cus = 0;
     DevBuf<ige::InstDesc> d_desc;
     DevBuf<uint32_t> d_queue, d_rp, d_ci, d_cs, d_order, d_progress, d_zero_temp, d_thr,
-        d_scratch, d_list, d_bars, d_err;
+        d_list, d_bars, d_err;
     DevBuf<const uint32_t*> d_orders;
     DevBuf<int32_t> d_wt, d_lin;
     DevBuf<uint64_t> d_spins;
@@ -95,26 +95,16 @@ ige::PersistentLaunch launch_args(ige_batch* b, uint32_t round) {
 // Gang sizes, threshold modes, scratch tables, descriptors and tickets.
 void plan(ige_batch* b) {
     b->gang.resize(b->count);
-    uint64_t scratch = 0;
     for (uint32_t i = 0; i < b->count; ++i) {
         const auto& L = b->inst[i];
         b->gang[i] = ige::choose_gang(L.n, b->max_gang, b->gang_sites);
-        if (ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal)
-            scratch += static_cast<uint64_t>(b->gang[i]) * (L.dtab + 1);
     }
-    b->d_scratch.alloc(std::max<uint64_t>(scratch, 1));
-    uint64_t scr = 0;
     for (uint32_t i = 0; i < b->count; ++i) {
         const auto& L = b->inst[i];
         ige::InstDesc& D = b->desc[i];
         D.gang = b->gang[i];
         D.bar = b->d_bars.p + i;
-        D.thr_mode = ige::choose_thr_mode(L.dtab, L.n);
-        D.thr_scratch = nullptr;
-        if (D.thr_mode == ige::kThrGlobal) {
-            D.thr_scratch = b->d_scratch.p + scr;
-            scr += static_cast<uint64_t>(D.gang) * (L.dtab + 1);
-        }
+        D.thr_mode = ige::choose_thr_mode(L.dtab);
     }
     // largest work per workgroup first; a gang's tickets stay consecutive
     std::stable_sort(b->queue.begin(), b->queue.end(), [&](uint32_t a, uint32_t c) {

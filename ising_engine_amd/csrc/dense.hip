@@ -227,28 +227,14 @@ __device__ __forceinline__ void stage_block(const DevDense& d, BlockTiles& T, ui
     for (uint32_t t = 0; t < 4; ++t) T.w[t * 256 + threadIdx.x] = src[t * 256 + threadIdx.x];
 }
 
-// Acceptance u < metropolis_threshold(beta, de).  The threshold is first
-// bracketed: a = 2^32 2^x with x = -beta de log2(e) in double precision
-// (clamped to [-200, 0]), 2^frac(x) on v_exp_f32 and 2^floor(x) exact, so
-// |a / T - 1| <= 3e-7 for T = 2^32 exp(-beta de), while the deterministic
-// threshold lies within T (1 +- 1e-14) + 1.  Outside the band
-// [a (1 - 2e-6) - 4, a (1 + 2e-6) + 4] the comparison is decided by a; a
-// proposal inside it (probability below 4e-6) is "ambiguous" and needs the
-// threshold itself.
+// Acceptance u < metropolis_threshold(beta, de) with threshold_band: a
+// proposal inside the band is "ambiguous" and needs the threshold itself,
+// evaluated out of line to keep the unrolled block code small.
 __device__ __attribute__((noinline)) bool exact_accept(uint32_t u, int32_t de, double beta) {
     return u < metropolis_threshold(beta, (int64_t)de);
 }
 
-__device__ __forceinline__ void threshold_band(int32_t de, double nbd, uint32_t& lo_u,
-                                               uint32_t& hi_u) {
-    const double x = fmin(fmax(nbd * (double)de, -200.0), 0.0);
-    const double xi = floor(x);
-    const float a = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(x - xi)), (int)xi + 32);
-    const float lo = a * 0.999998f - 4.0f;
-    const float hi = a * 1.000002f + 4.0f;
-    lo_u = (uint32_t)fmaxf(lo, 0.0f);
-    hi_u = hi < 4294967040.0f ? (uint32_t)hi : 0xFFFFFFFFu;
-}
+
 
 // The 64 positions of one block in order (lane = replica).  hb: the block's
 // fields at its start; wv: lane k holds the word of position p0 + k; wt: the

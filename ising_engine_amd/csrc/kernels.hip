@@ -78,8 +78,9 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
     int64_t eacc = 0;
     uint32_t nacc = 0;
     generic_positions(rp, ci, wt, lin, S, W, p0, p1, slot, sweep, key0, key1, lane,
-                      [&](int32_t de) {
-                          return kLdsThr ? lds_thr[trow + (uint32_t)de] : thr[trow + (uint32_t)de];
+                      [&](uint32_t uu, int32_t de) {
+                          return uu < (kLdsThr ? lds_thr[trow + (uint32_t)de]
+                                               : thr[trow + (uint32_t)de]);
                       },
                       eacc, nacc);
     atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),

@@ -31,10 +31,9 @@ struct InstDesc {
     int64_t* energy;       // 64 tracked energies
     unsigned long long* accepted;
     uint32_t* progress;    // sweeps done so far
-    uint32_t* thr_scratch; // dtab+1 entries, used in kThrGlobal mode
     uint32_t n, ncol, sweeps, dtab, key0, key1;
     uint32_t round;        // sweeps to advance in this launch (0: the launch's default)
-    uint32_t thr_mode;     // kThrLds, kThrGlobal or kThrDirect
+    uint32_t thr_mode;     // kThrLds or kThrBand
     uint32_t gang;         // workgroups cooperating on this instance (1..kMaxGang)
     uint32_t* bar;         // gang barrier counter, zero at launch start (gang > 1)
     double beta_hi;
@@ -45,13 +44,12 @@ struct InstDesc {
 constexpr uint32_t kLdsThrEntries = 4096;
 constexpr uint32_t kMaxGang = 32;        // queue codes carry the member index in 5 bits
 constexpr uint32_t kGangSites = 2048;    // sites per workgroup before a gang is formed
-// Where a workgroup keeps the per-sweep thresholds thr[d], d <= dtab.
-constexpr uint32_t kThrLds = 0;     // dtab < kLdsThrEntries: table in LDS
-constexpr uint32_t kThrGlobal = 1;  // large range, enough sites to amortise a table
-constexpr uint32_t kThrDirect = 2;  // large range, few sites: evaluate per proposal
+// How a workgroup decides u < thr(beta, dE) for dE <= dtab.
+constexpr uint32_t kThrLds = 0;   // dtab < kLdsThrEntries: per-sweep table in LDS
+constexpr uint32_t kThrBand = 1;  // larger ranges: threshold_band, no table
 
-__host__ __device__ __forceinline__ uint32_t choose_thr_mode(uint32_t dtab, uint32_t n) {
-    return dtab < kLdsThrEntries ? kThrLds : (dtab <= 32u * n ? kThrGlobal : kThrDirect);
+__host__ __device__ __forceinline__ uint32_t choose_thr_mode(uint32_t dtab) {
+    return dtab < kLdsThrEntries ? kThrLds : kThrBand;
 }
 
 __host__ __device__ __forceinline__ double anneal_beta(double beta_hi, double c, uint32_t sweeps,
@@ -118,7 +116,6 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         const uint32_t gang = I.gang;
         uint32_t bar_target = 0;
         const uint32_t mode = I.thr_mode;
-        uint32_t* gthr = I.thr_scratch + (mode == kThrGlobal ? member * (I.dtab + 1) : 0u);
         const uint32_t k0 = *I.progress;
         const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
         int64_t eacc = 0;
@@ -126,18 +123,15 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         for (uint32_t k = k0; k < k1; ++k) {
             const double beta = anneal_beta(I.beta_hi, I.c, I.sweeps, k);
             const uint32_t dlim = threshold_limit(beta, I.dtab);
-            if (mode != kThrDirect) {
-                uint32_t* tab = mode == kThrLds ? thr_lds : gthr;
+            if (mode == kThrLds) {
                 for (uint32_t d = threadIdx.x; d <= dlim; d += WAVES * 64)
-                    tab[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
-                if (mode == kThrGlobal) __threadfence_block();
+                    thr_lds[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
                 __syncthreads();
             }
-            auto thr = [&](int32_t de) -> uint32_t {
-                if ((uint32_t)de > dlim) return 0u;
-                return mode == kThrLds ? thr_lds[de]
-                                       : mode == kThrGlobal ? gthr[de]
-                                                            : metropolis_threshold(beta, (int64_t)de);
+            const double nbd = -beta * 1.4426950408889634;
+            auto accept = [&](uint32_t uu, int32_t de) -> bool {
+                if ((uint32_t)de > dlim) return false;
+                return mode == kThrLds ? uu < thr_lds[de] : band_accept(uu, de, nbd, beta);
             };
             const uint32_t stride = gang * WAVES;
             const uint32_t lane_of_gang = member * WAVES + wave;
@@ -147,7 +141,7 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
                     for (uint32_t g = (cb >> 2) + lane_of_gang; g <= ((ce - 1) >> 2); g += stride) {
                         const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
                         generic_group(I.rp, I.ci, I.wt, I.lin, I.spins, I.spins, p0, p1, 0u, k,
-                                      I.key0, I.key1, lane, thr, eacc, nacc);
+                                      I.key0, I.key1, lane, accept, eacc, nacc);
                     }
                 }
                 if (gang > 1)

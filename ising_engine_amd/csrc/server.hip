@@ -44,6 +44,7 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kAlign = 256;
 constexpr uint32_t kMaxVars = 50000;     // server_api.rs:15
 constexpr uint32_t kMaxSweeps = 10000;   // server_api.rs:16
+constexpr uint32_t kServerGangSites = 2048;  // fewest sites a gang member gets
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -94,8 +95,8 @@ class ArenaAllocator {
 
 // Byte layout of one request inside the arena; each region 256-B aligned.
 struct Layout {
-    uint64_t rp, ci, wt, lin, cs, order, spins, energy, accepted, progress, thr, total;
-    Layout(const ige::LoweredInstance& L, uint32_t gang) {
+    uint64_t rp, ci, wt, lin, cs, order, spins, energy, accepted, progress, total;
+    explicit Layout(const ige::LoweredInstance& L) {
         uint64_t o = 0;
         auto take = [&](uint64_t bytes) {
             const uint64_t at = o;
@@ -112,9 +113,6 @@ struct Layout {
         energy = take(64 * 8);
         accepted = take(8);
         progress = take(4);
-        thr = ige::choose_thr_mode(L.dtab, L.n) == ige::kThrGlobal
-                  ? take(static_cast<uint64_t>(gang) * (L.dtab + 1ull) * 4)
-                  : o;
         total = o;
     }
 };
@@ -126,7 +124,8 @@ struct Request {
     int32_t status = IGE_REQ_QUEUED;
     uint64_t off = 0, bytes = 0;
     ige::InstDesc D{};
-    uint32_t gang = 1;
+    uint32_t gang = 1;      // workgroups in the current round
+    uint32_t gang_cap = 1;  // most workgroups it may get
     uint32_t progress = 0;
     std::vector<uint8_t> staging;  // host image of the model regions (kept until uploaded)
     std::vector<uint64_t> words;   // result, site order
@@ -143,7 +142,11 @@ struct ige_server {
     uint32_t waves = 8;
     double round_ms = 10.0;
     uint32_t max_live = 4096;
-    uint32_t max_gang = 16, gang_sites = ige::kGangSites, spin_limit = 1u << 24;
+    // Gangs: a request of n sites gets about n / s_eff workgroups per round, where
+    // s_eff >= gang_sites grows with the load so the live requests together ask
+    // for about the resident workgroups (see plan_gangs).
+    uint32_t max_gang = 16, gang_sites = kServerGangSites, spin_limit = 1u << 24;
+    uint32_t resident_blocks = 0;  // workgroups of the persistent kernel resident at once
     ArenaAllocator arena{0};
     uint8_t* d_arena = nullptr;
     DevBuf<ige::InstDesc> d_desc;
@@ -185,7 +188,7 @@ namespace {
 
 void upload_request(ige_server* s, Request& r) {
     const ige::LoweredInstance& L = r.L;
-    const Layout lay(L, r.gang);
+    const Layout lay(L);
     r.staging.assign(lay.spins, 0);  // model regions only: rp .. order
     auto put = [&](uint64_t at, const void* src, uint64_t bytes) {
         if (bytes) std::memcpy(r.staging.data() + at, src, bytes);
@@ -210,9 +213,7 @@ void upload_request(ige_server* s, Request& r) {
     D.energy = reinterpret_cast<int64_t*>(base + lay.energy);
     D.accepted = reinterpret_cast<unsigned long long*>(base + lay.accepted);
     D.progress = reinterpret_cast<uint32_t*>(base + lay.progress);
-    D.thr_mode = ige::choose_thr_mode(L.dtab, L.n);
-    D.thr_scratch =
-        D.thr_mode == ige::kThrGlobal ? reinterpret_cast<uint32_t*>(base + lay.thr) : nullptr;
+    D.thr_mode = ige::choose_thr_mode(L.dtab);
     D.n = L.n;
     D.ncol = L.ncol;
     D.sweeps = L.sweeps;
@@ -225,6 +226,23 @@ void upload_request(ige_server* s, Request& r) {
     D.beta_hi = L.beta_hi;
     D.c = L.c;
     D.offset = L.offset;
+}
+
+// Workgroups per live request for the next round: n / s_eff, at most the
+// request's gang_cap, with s_eff = max(gang_sites, live sites / resident
+// workgroups).  Lightly loaded, requests spread over many CUs (latency);
+// heavily loaded, each keeps one workgroup and no barrier is paid (throughput).
+// A request's trajectory does not depend on its gang.
+void plan_gangs(ige_server* s) {
+    uint64_t sites = 0;
+    for (Request* r : s->live) sites += r->L.n;
+    const double per_wg = std::max<double>(
+        s->gang_sites, static_cast<double>(sites) / std::max<uint32_t>(1, s->resident_blocks));
+    for (Request* r : s->live) {
+        const uint32_t g = static_cast<uint32_t>(std::max(1.0, std::floor(r->L.n / per_wg)));
+        r->gang = std::min(g, r->gang_cap);
+        r->D.gang = r->gang;
+    }
 }
 
 void retire(ige_server* s, Request& r) {
@@ -263,7 +281,7 @@ void serve_loop(ige_server* s) {
                 // 1. admission: FIFO while the head fits (no overtaking, no starvation)
                 while (!s->pending.empty() && s->live.size() < s->max_live) {
                     Request& r = *s->reqs[s->pending.front()];
-                    const Layout lay(r.L, r.gang);
+                    const Layout lay(r.L);
                     uint64_t off = 0;
                     if (!s->arena.alloc(lay.total, &off)) break;
                     r.off = off;
@@ -278,6 +296,7 @@ void serve_loop(ige_server* s) {
             }
             for (Request* r : admitted) upload_request(s, *r);
             // 2. one round over every live request
+            plan_gangs(s);
             const size_t L = s->live.size();
             descs.resize(L);
             queue.resize(L);
@@ -321,7 +340,7 @@ void serve_loop(ige_server* s) {
                     if (s->live[i]->status == IGE_REQ_RUNNING && s->live[i]->progress == 0 &&
                         std::find(admitted.begin(), admitted.end(), s->live[i]) != admitted.end()) {
                         list.push_back(static_cast<uint32_t>(i));
-                        const Layout lay(s->live[i]->L, s->live[i]->gang);
+                        const Layout lay(s->live[i]->L);
                         orders[i] = reinterpret_cast<const uint32_t*>(s->d_arena + s->live[i]->off +
                                                                       lay.order);
                     }
@@ -448,6 +467,7 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->arena = ArenaAllocator(hbm_bytes);
         s->waves = waves ? waves : 8;
         s->round_ms = round_ms;
+        s->resident_blocks = ige::resident_blocks(s->waves, s->num_cus);
         s->d_next.alloc(1);
         s->d_desc.alloc(256);
         s->d_queue.alloc(256);
@@ -472,7 +492,7 @@ int32_t ige_server_configure(ige_server* s, uint32_t max_gang, uint32_t gang_sit
         std::lock_guard<std::mutex> g(s->mu);
         require(s->reqs.empty(), "configure before submitting requests");
         s->max_gang = max_gang;
-        s->gang_sites = gang_sites ? gang_sites : ige::kGangSites;
+        s->gang_sites = gang_sites ? gang_sites : kServerGangSites;
         s->spin_limit = spin_limit;
     });
 }
@@ -490,8 +510,9 @@ int32_t ige_server_submit(ige_server* s, const ige_instance_desc* d, uint64_t* i
         auto r = std::make_unique<Request>();
         r->t_submit = Clock::now();
         r->L = ige::lower_instance(*d, "request");
-        r->gang = ige::choose_gang(r->L.n, s->max_gang, s->gang_sites);
-        const Layout lay(r->L, r->gang);
+        r->gang_cap = ige::choose_gang(r->L.n, s->max_gang, s->gang_sites);
+        r->gang = 1;
+        const Layout lay(r->L);
         std::lock_guard<std::mutex> g(s->mu);
         if (!s->fatal.empty()) throw Failure(IGE_ERR_HIP, "server stopped: " + s->fatal);
         if (lay.total > s->arena.capacity())

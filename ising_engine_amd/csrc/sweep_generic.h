@@ -51,19 +51,47 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
     return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
+// Acceptance u < metropolis_threshold(beta, de) without a table.  The
+// threshold is first bracketed: a = 2^32 2^x with x = -beta de log2(e) in
+// double precision (clamped to [-200, 0]), 2^frac(x) on v_exp_f32 and
+// 2^floor(x) exact, so |a / T - 1| <= 3e-7 for T = 2^32 exp(-beta de), while
+// the deterministic threshold lies within T (1 +- 1e-14) + 1.  Outside the
+// band [a (1 - 2e-6) - 4, a (1 + 2e-6) + 4] the comparison is decided by a;
+// inside it (probability below 4e-6) the threshold itself is evaluated.
+// nbd = -beta log2(e).
+__device__ __forceinline__ void threshold_band(int32_t de, double nbd, uint32_t& lo_u,
+                                               uint32_t& hi_u) {
+    const double x = fmin(fmax(nbd * (double)de, -200.0), 0.0);
+    const double xi = floor(x);
+    const float a = __builtin_ldexpf(__builtin_amdgcn_exp2f((float)(x - xi)), (int)xi + 32);
+    const float lo = a * 0.999998f - 4.0f;
+    const float hi = a * 1.000002f + 4.0f;
+    lo_u = (uint32_t)fmaxf(lo, 0.0f);
+    hi_u = hi < 4294967040.0f ? (uint32_t)hi : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ bool band_accept(uint32_t u, int32_t de, double nbd, double beta) {
+    uint32_t lo_u, hi_u;
+    threshold_band(de, nbd, lo_u, hi_u);
+    if (u < lo_u) return true;
+    if (u > hi_u) return false;
+    return u < metropolis_threshold(beta, (int64_t)de);
+}
+
 // Updates positions [p0, p1) of one slot; p0..p1-1 lie in one colour class
 // and in one aligned group of four (p0 >> 2 == (p1 - 1) >> 2).
 // S/W: the slot's words for reading / writing (same memory); read with
 // vector loads, so words written by other waves before a barrier are seen.
-// thr(de) returns metropolis_threshold(beta of this lane, de) for de >= 1.
+// accept(u, de) returns u < metropolis_threshold(beta of this lane, de) for
+// de >= 1.
 // Adds the accepted dE to eacc (per lane) and the accepted count to nacc
 // (wave-uniform).
-template <class Thr>
+template <class Acc>
 __device__ __forceinline__ void generic_group(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin, const uint64_t* S,
     uint64_t* W, uint32_t p0, uint32_t p1, uint32_t slot, uint32_t sweep, uint32_t key0,
-    uint32_t key1, uint32_t lane, Thr&& thr, int64_t& eacc, uint32_t& nacc) {
+    uint32_t key1, uint32_t lane, Acc&& accept, int64_t& eacc, uint32_t& nacc) {
     const U32x4 u = philox4x32_10(p0 >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
     const uint32_t cnt = p1 - p0;  // 1..4
     // row bounds of the group's positions (uniform)
@@ -101,7 +129,7 @@ __device__ __forceinline__ void generic_group(
         const uint32_t wsel = (p0 + q) & 3u;
         const uint32_t uu = wsel == 0 ? u.v[0] : wsel == 1 ? u.v[1] : wsel == 2 ? u.v[2] : u.v[3];
         bool acc = de <= 0;
-        if (!acc) acc = uu < thr(de);
+        if (!acc) acc = accept(uu, de);
         const uint64_t m = wave_ballot(acc);
         if (lane == 0) W[p0 + q] = x ^ m;
         eacc += acc ? de : 0;
@@ -110,15 +138,15 @@ __device__ __forceinline__ void generic_group(
 }
 
 // Positions [p0, p1) of one colour class, group by group.
-template <class Thr>
+template <class Acc>
 __device__ __forceinline__ void generic_positions(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin, const uint64_t* S,
     uint64_t* W, uint32_t p0, uint32_t p1, uint32_t slot, uint32_t sweep, uint32_t key0,
-    uint32_t key1, uint32_t lane, Thr&& thr, int64_t& eacc, uint32_t& nacc) {
+    uint32_t key1, uint32_t lane, Acc&& accept, int64_t& eacc, uint32_t& nacc) {
     for (uint32_t p = p0; p < p1;) {
         const uint32_t q1 = min(p1, (p & ~3u) + 4u);
-        generic_group(rp, ci, wt, lin, S, W, p, q1, slot, sweep, key0, key1, lane, thr, eacc,
+        generic_group(rp, ci, wt, lin, S, W, p, q1, slot, sweep, key0, key1, lane, accept, eacc,
                       nacc);
         p = q1;
     }

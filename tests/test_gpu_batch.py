@@ -29,6 +29,9 @@ def small_batch(count=14, sweeps=24, seed=3):
         n = int(rng.choice([1, 5, 37, 64, 100, 257, 400, 613]))
         if i % 5 == 4:
             qubos.append(instances.hypercubic_pm_j(4, 2, 100 + i))  # a +-J model in the mix
+        elif i % 7 == 3:
+            # |dE| range beyond the LDS threshold table: thresholds from the band test
+            qubos.append(instances.random_qubo(n, 8, 1000 + i, offdiag=2000, diag=3000))
         else:
             qubos.append(instances.random_qubo(n, 6, 1000 + i))
     seeds = rng.integers(1, 1 << 62, size=count)
