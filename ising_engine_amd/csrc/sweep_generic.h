@@ -23,25 +23,59 @@ __device__ __forceinline__ bool lane_bit(uint64_t word) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Value of v in lane (lane ^ J), with VALU lane exchanges where gfx950 has
+// them: v_permlane32_swap / v_permlane16_swap (swap one half of the wave with
+// the other), DPP row_ror:8 and quad_perm within rows, ds_swizzle for xor 4.
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, uint32_t lane) {
+    if constexpr (J == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32u) ? p[0] : p[1];
+    } else if constexpr (J == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16u) ? p[0] : p[1];
+    } else if constexpr (J == 8) {
+        return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    } else if constexpr (J == 4) {
+        return __builtin_amdgcn_ds_swizzle(v, (4 << 10) | 0x1F);  // and 31, xor 4
+    } else if constexpr (J == 2) {
+        return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);  // quad_perm 2301
+    } else {
+        static_assert(J == 1, "lane distance must be a power of two <= 32");
+        return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);  // quad_perm 1032
+    }
+}
+
+// One stage of the 64x64 transpose for lane distance J < 32: lanes l and
+// l ^ J swap the off-diagonal J x J blocks of each 2J x 2J block.  The lane
+// with bit J clear keeps the low J bits of each 2J-bit group and takes its
+// partner's, shifted up; the other keeps the high bits and takes its
+// partner's, shifted down.
+template <int J>
+__device__ __forceinline__ uint64_t transpose_stage(uint64_t x, uint64_t m, uint32_t lane) {
+    const uint64_t y = (uint64_t)xor_lane<J>((uint32_t)x, lane) |
+                       ((uint64_t)xor_lane<J>((uint32_t)(x >> 32), lane) << 32);
+    const bool h = (lane & (uint32_t)J) != 0;
+    const uint64_t ys = h ? (y >> J) : (y << J);
+    const uint64_t keep = h ? ~m : m;
+    return (x & keep) | (ys & ~keep);
+}
+
 // 64x64 bit-matrix transpose across the wave: lane l holds row l on entry,
 // column l on exit (bit k of the result = bit l of row k).  Branch-free.
 __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
-                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
-                               0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        // Stage j: lanes l and l ^ j swap the off-diagonal j x j blocks of
-        // each 2j x 2j block.  The lane with bit j clear keeps the low j bits
-        // of each 2j-bit group and takes its partner's, shifted up; the other
-        // keeps the high bits and takes its partner's, shifted down.
-        const uint32_t j = 32u >> s;
-        const bool h = (lane & j) != 0;
-        const uint64_t y = __shfl_xor(x, (int)j, 64);
-        const uint64_t ys = h ? (y >> j) : (y << j);
-        const uint64_t keep = h ? ~masks[s] : masks[s];
-        x = (x & keep) | (ys & ~keep);
+    // Distance 32: the lower half of the wave trades its high words for the
+    // upper half's low words, one v_permlane32_swap on the two 32-bit halves.
+    {
+        const auto p = __builtin_amdgcn_permlane32_swap((uint32_t)x, (uint32_t)(x >> 32), false,
+                                                        false);
+        x = (uint64_t)p[0] | ((uint64_t)p[1] << 32);
     }
+    x = transpose_stage<16>(x, 0x0000FFFF0000FFFFull, lane);
+    x = transpose_stage<8>(x, 0x00FF00FF00FF00FFull, lane);
+    x = transpose_stage<4>(x, 0x0F0F0F0F0F0F0F0Full, lane);
+    x = transpose_stage<2>(x, 0x3333333333333333ull, lane);
+    x = transpose_stage<1>(x, 0x5555555555555555ull, lane);
     return x;
 }
 
