@@ -104,6 +104,7 @@ void plan(ige_batch* b) {
         ige::InstDesc& D = b->desc[i];
         D.gang = b->gang[i];
         D.bar = b->d_bars.p + i;
+        D.ticks = nullptr;
         D.thr_mode = ige::choose_thr_mode(L.dtab);
     }
     // largest work per workgroup first; a gang's tickets stay consecutive

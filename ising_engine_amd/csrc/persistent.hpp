@@ -36,6 +36,7 @@ struct InstDesc {
     uint32_t thr_mode;     // kThrLds or kThrBand
     uint32_t gang;         // workgroups cooperating on this instance (1..kMaxGang)
     uint32_t* bar;         // gang barrier counter, zero at launch start (gang > 1)
+    uint64_t* ticks;       // if set: wall-clock ticks member 0 spent on this launch's sweeps
     double beta_hi;
     double c;  // beta_k = beta_hi * det_exp(-(sweeps - 1 - k) * c)
     int64_t offset;
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         const uint32_t gang = I.gang;
         uint32_t bar_target = 0;
         const uint32_t mode = I.thr_mode;
+        const uint64_t t_start = wall_clock64();
         const uint32_t k0 = *I.progress;
         const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
         int64_t eacc = 0;
@@ -165,7 +167,10 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
                 I.energy[lane] += tot;
         }
         if (gang > 1) gang_barrier(I.bar, bar_target, gang, spin_limit, err);
-        if (member == 0 && threadIdx.x == 0) *I.progress = k1;
+        if (member == 0 && threadIdx.x == 0) {
+            *I.progress = k1;
+            if (I.ticks) *I.ticks = wall_clock64() - t_start;
+        }
         __syncthreads();
     }
 }

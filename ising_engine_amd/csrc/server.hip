@@ -127,6 +127,7 @@ struct Request {
     uint32_t gang = 1;      // workgroups in the current round
     uint32_t gang_cap = 1;  // most workgroups it may get
     uint32_t progress = 0;
+    double attempts_per_tick = 0.0;  // measured on the device, 0 until its first round
     std::vector<uint8_t> staging;  // host image of the model regions (kept until uploaded)
     std::vector<uint64_t> words;   // result, site order
     std::vector<int64_t> energies;
@@ -151,6 +152,9 @@ struct ige_server {
     uint8_t* d_arena = nullptr;
     DevBuf<ige::InstDesc> d_desc;
     DevBuf<uint32_t> d_queue, d_list, d_bars, d_err;
+    DevBuf<uint64_t> d_ticks;
+    std::vector<uint64_t> h_ticks;
+    double ticks_per_ms = 1.0e5;  // device wall clock (hipDeviceAttributeWallClockRate)
     DevBuf<const uint32_t*> d_orders;
     DevBuf<unsigned int> d_next;
 
@@ -222,7 +226,8 @@ void upload_request(ige_server* s, Request& r) {
     D.key1 = static_cast<uint32_t>(L.seed >> 32);
     D.round = 1;
     D.gang = r.gang;
-    D.bar = nullptr;  // set per round
+    D.bar = nullptr;    // set per round
+    D.ticks = nullptr;  // set per round
     D.beta_hi = L.beta_hi;
     D.c = L.c;
     D.offset = L.offset;
@@ -301,11 +306,17 @@ void serve_loop(ige_server* s) {
             descs.resize(L);
             queue.resize(L);
             uint64_t round_attempts = 0;
+            // Sweeps per request: about round_ms of its own measured rate (its
+            // member 0's device time), or of the per-workgroup estimate before
+            // its first round, so that no request stretches the round.
             const double per_block = std::max(1.0, s->attempts_per_ms_per_block) * s->round_ms;
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
                 const uint64_t per_sweep = static_cast<uint64_t>(r.L.n) * 64;
-                uint32_t k = static_cast<uint32_t>(std::max(1.0, per_block * r.gang / per_sweep));
+                const double budget = r.attempts_per_tick > 0.0
+                                          ? r.attempts_per_tick * s->ticks_per_ms * s->round_ms
+                                          : per_block * r.gang;
+                uint32_t k = static_cast<uint32_t>(std::max(1.0, budget / per_sweep));
                 k = std::min(k, r.L.sweeps - r.progress);
                 r.D.round = k;
                 descs[i] = r.D;
@@ -328,9 +339,13 @@ void serve_loop(ige_server* s) {
                 s->d_list.alloc(s->d_desc.count);
                 s->d_orders.alloc(s->d_desc.count);
                 s->d_bars.alloc(s->d_desc.count);
+                s->d_ticks.alloc(s->d_desc.count);
             }
             if (s->d_queue.count < queue.size()) s->d_queue.alloc(std::max<size_t>(queue.size(), 2 * s->d_queue.count));
-            for (size_t i = 0; i < L; ++i) descs[i].bar = s->d_bars.p + i;
+            for (size_t i = 0; i < L; ++i) {
+                descs[i].bar = s->d_bars.p + i;
+                descs[i].ticks = s->d_ticks.p + i;
+            }
             s->d_desc.upload(descs.data(), L, s->stream);
             s->d_queue.upload(queue.data(), queue.size(), s->stream);
             if (!admitted.empty()) {
@@ -374,6 +389,21 @@ void serve_loop(ige_server* s) {
             }
             float ms = 0.f;
             check(hipEventElapsedTime(&ms, e0, e1), "event");
+            // per-request device time of this round -> its rate for the next one
+            s->h_ticks.resize(L);
+            check(hipMemcpy(s->h_ticks.data(), s->d_ticks.p, L * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost),
+                  "round ticks");
+            for (size_t i = 0; i < L; ++i) {
+                Request& r = *s->live[i];
+                const double done = static_cast<double>(r.L.n) * 64.0 * r.D.round;
+                if (s->h_ticks[i] > 0 && r.D.round > 0) {
+                    const double rate = done / static_cast<double>(s->h_ticks[i]);
+                    r.attempts_per_tick = r.attempts_per_tick > 0.0
+                                              ? 0.5 * r.attempts_per_tick + 0.5 * rate
+                                              : rate;
+                }
+            }
             // refine the per-workgroup rate used to size the next round
             {
                 int per_cu = 0;
@@ -474,6 +504,14 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->d_list.alloc(256);
         s->d_orders.alloc(256);
         s->d_bars.alloc(256);
+        s->d_ticks.alloc(256);
+        {
+            int khz = 0;
+            if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) ==
+                    hipSuccess &&
+                khz > 0)
+                s->ticks_per_ms = khz;
+        }
         s->d_err.alloc(1);
         const uint32_t zero = 0;
         s->d_err.upload(&zero, 1, s->stream);
