@@ -174,9 +174,9 @@ int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *wa
  * colour class.  spin_limit bounds each barrier wait; an expired wait makes
  * the next synchronize/read fail with IGE_ERR_HIP.  Batch defaults: 1 (no gangs:
  * throughput), 2048, 2^24.  The server sizes gangs per round from its load:
- * n / max(gang_sites, live sites / resident workgroups) members, at most
- * max_gang and at most n / gang_sites; server defaults: 16, 2048, 2^24 (a gang
- * barrier per colour class costs more than 256 sites of work on this GPU). */
+ * n / max(gang_sites, live sites / CUs) members, at most max_gang and at most
+ * n / gang_sites; server defaults: 16, 1024, 2^24 (a gang barrier per colour
+ * class costs more than a few hundred sites of work on this GPU). */
 int32_t ige_batch_configure(ige_batch *batch, uint32_t max_gang, uint32_t gang_sites,
                             uint32_t spin_limit);
 int32_t ige_batch_synchronize(ige_batch *batch);

@@ -138,27 +138,6 @@ inline void launch_persistent(uint32_t waves, const PersistentLaunch& L, hipStre
     }
 }
 
-// Workgroups of the persistent kernel (waves per workgroup) resident at once.
-template <int WAVES>
-inline uint32_t resident_blocks_waves(int num_cus) {
-    int per_cu = 0;
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<WAVES>,
-                                                       WAVES * 64, 0),
-          "occupancy");
-    return static_cast<uint32_t>(std::max(per_cu, 1) * std::max(num_cus, 1));
-}
-
-inline uint32_t resident_blocks(uint32_t waves, int num_cus) {
-    switch (waves) {
-        case 1: return resident_blocks_waves<1>(num_cus);
-        case 2: return resident_blocks_waves<2>(num_cus);
-        case 4: return resident_blocks_waves<4>(num_cus);
-        case 8: return resident_blocks_waves<8>(num_cus);
-        case 16: return resident_blocks_waves<16>(num_cus);
-        default: throw Failure(IGE_ERR_INVALID, "waves per instance must be 1, 2, 4, 8 or 16");
-    }
-}
-
 // Tickets for instances in the given order: gang[i] consecutive tickets each.
 inline void build_tickets(const std::vector<uint32_t>& order, const std::vector<uint32_t>& gang,
                           std::vector<uint32_t>& tickets) {

@@ -44,7 +44,7 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kAlign = 256;
 constexpr uint32_t kMaxVars = 50000;     // server_api.rs:15
 constexpr uint32_t kMaxSweeps = 10000;   // server_api.rs:16
-constexpr uint32_t kServerGangSites = 2048;  // fewest sites a gang member gets
+constexpr uint32_t kServerGangSites = 1024;  // fewest sites a gang member gets
 
 uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -144,10 +144,8 @@ struct ige_server {
     double round_ms = 10.0;
     uint32_t max_live = 4096;
     // Gangs: a request of n sites gets about n / s_eff workgroups per round, where
-    // s_eff >= gang_sites grows with the load so the live requests together ask
-    // for about the resident workgroups (see plan_gangs).
+    // s_eff >= gang_sites grows with the load (see plan_gangs).
     uint32_t max_gang = 16, gang_sites = kServerGangSites, spin_limit = 1u << 24;
-    uint32_t resident_blocks = 0;  // workgroups of the persistent kernel resident at once
     ArenaAllocator arena{0};
     uint8_t* d_arena = nullptr;
     DevBuf<ige::InstDesc> d_desc;
@@ -234,15 +232,16 @@ void upload_request(ige_server* s, Request& r) {
 }
 
 // Workgroups per live request for the next round: n / s_eff, at most the
-// request's gang_cap, with s_eff = max(gang_sites, live sites / resident
-// workgroups).  Lightly loaded, requests spread over many CUs (latency);
-// heavily loaded, each keeps one workgroup and no barrier is paid (throughput).
-// A request's trajectory does not depend on its gang.
+// request's gang_cap, with s_eff = max(gang_sites, live sites / CUs): the
+// gangs together ask for about one workgroup per CU.  Lightly loaded,
+// requests spread over many CUs (latency); loaded, each keeps one workgroup
+// and no barrier is paid (throughput).  A request's trajectory does not depend
+// on its gang.
 void plan_gangs(ige_server* s) {
     uint64_t sites = 0;
     for (Request* r : s->live) sites += r->L.n;
     const double per_wg = std::max<double>(
-        s->gang_sites, static_cast<double>(sites) / std::max<uint32_t>(1, s->resident_blocks));
+        s->gang_sites, static_cast<double>(sites) / std::max(1, s->num_cus));
     for (Request* r : s->live) {
         const uint32_t g = static_cast<uint32_t>(std::max(1.0, std::floor(r->L.n / per_wg)));
         r->gang = std::min(g, r->gang_cap);
@@ -497,7 +496,6 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->arena = ArenaAllocator(hbm_bytes);
         s->waves = waves ? waves : 8;
         s->round_ms = round_ms;
-        s->resident_blocks = ige::resident_blocks(s->waves, s->num_cus);
         s->d_next.alloc(1);
         s->d_desc.alloc(256);
         s->d_queue.alloc(256);
