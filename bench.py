@@ -343,7 +343,8 @@ def main():
     ap.add_argument("--tts-runs", type=int, default=32)
     ap.add_argument("--sk-sweeps", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-rates", type=str, default="100,250",
+                    help="comma-separated Poisson request rates (req/s)")
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--counter-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS steps of --probe-workload and a 1 GiB copy "
@@ -418,7 +419,8 @@ def main():
                                     cpu_s=0.0 if args.no_cpu_baseline else 2.0)
     serving = None
     if not args.no_serving:
-        serving = bench_serving(_lib, instances, ctx, args.serving_rate, args.serving_requests)
+        serving = [bench_serving(_lib, instances, ctx, float(r), args.serving_requests)
+                   for r in args.serving_rates.split(",") if r]
 
     cpu = None
     if not args.no_cpu_baseline:
