@@ -42,6 +42,11 @@
  *   ige_read_stats          Report{accepted,proposed} + StepScratch swap statistics
  *                                                      src/engine_v2/operator.rs:34-52,
  *                                                      src/solver/engine.rs:454-557
+ *   ige_steepest_descent    SteepestDescent::apply     src/engine_v2/operators/steepest_descent.rs:80-124
+ *   ige_model_polish        steepest_descent_1opt      src/solver/local_search.rs:19-60
+ *   IGE_RUN_TRACK_BEST,     UltimateSolver::update_incumbent and the anytime
+ *   ige_update_best,        best-state return          src/solver/ultimate.rs:299-336,
+ *   ige_read_best                                      src/solver/ultimate.rs:369-373,397-431
  */
 #ifndef ISING_GPU_H
 #define ISING_GPU_H
@@ -66,7 +71,11 @@ extern "C" {
 #define IGE_KERNEL_DENSE 3     /* dense q = scale*int8: positions in order,
                                   fields recomputed on int8 matrix cores    */
 
-#define IGE_RUN_EXCHANGE 1u /* DEO replica exchange after every sweep       */
+#define IGE_RUN_EXCHANGE 1u   /* DEO replica exchange after every sweep     */
+#define IGE_RUN_TRACK_BEST 2u /* incumbent update after every sweep (and its
+                                 exchange), see ige_update_best             */
+
+#define IGE_STEPS_UNLIMITED 0xFFFFFFFFu /* descend until no replica can improve */
 
 typedef struct ige_ctx ige_ctx;
 typedef struct ige_model ige_model;
@@ -131,6 +140,40 @@ int32_t ige_read_fields(ige_state *state, int32_t *fields /* [cells][n][64] */);
 int32_t ige_recompute_energies(ige_state *state, int64_t *energies /* [cells][64] */);
 int32_t ige_read_stats(ige_state *state, ige_stats *stats, uint64_t *pair_attempts,
                        uint64_t *pair_accepts /* cells-1 each, or NULL */);
+
+/* ---- deterministic descent and the incumbent ---- */
+typedef struct {
+    uint32_t steps;     /* steps in which at least one replica flipped          */
+    uint32_t converged; /* 1 if a step found no improving flip (local minimum)  */
+    uint64_t flips;     /* flips over all replicas                              */
+    uint64_t proposed;  /* (steps + converged) * n * 64 * cells, as the
+                           reference counts one scan of every (site, replica) */
+} ige_descent_report;
+
+/* Steepest descent of every replica: per step each replica flips the site
+ * with the most negative dE (lowest site on a tie); at most max_steps steps
+ * (IGE_STEPS_UNLIMITED: until a step finds nothing).  clamped: NULL or n bytes
+ * in site order, nonzero = never flipped.  Tracked energies stay exact;
+ * sweep/exchange counters and Metropolis statistics are not touched.
+ * Synchronous. */
+int32_t ige_steepest_descent(ige_state *state, uint32_t max_steps, const uint8_t *clamped,
+                             ige_descent_report *report);
+
+/* Incumbent: the replica with the lowest tracked energy, ties to the lowest
+ * (temperature index, lane), replaces the best-seen state when strictly lower
+ * (or when there is none yet).  ige_update_best does one such update now;
+ * ige_run with IGE_RUN_TRACK_BEST does one after every sweep.  Asynchronous. */
+int32_t ige_update_best(ige_state *state);
+/* x: n bytes in site order (or NULL); energy: its exact energy; sweep: the
+ * sweeps completed when it was found (or NULL).  IGE_ERR_INVALID before the
+ * first update. */
+int32_t ige_read_best(ige_state *state, uint8_t *x, int64_t *energy, uint32_t *sweep);
+
+/* Polish `count` states of n bytes each ([count][n], site order, in/out) to
+ * 1-opt local minima, all at once (64 states per cell).  clamped: NULL or n
+ * bytes.  energy: [count] exact energies after, flips: [count] (or NULL). */
+int32_t ige_model_polish(const ige_model *model, uint32_t count, uint8_t *x,
+                         const uint8_t *clamped, int64_t *energy, uint32_t *flips);
 
 /* ---- batches: thousands of independent instances in one persistent launch ---- */
 typedef struct ige_batch ige_batch;

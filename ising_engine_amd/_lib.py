@@ -26,6 +26,8 @@ KERNEL_GENERIC = 1
 KERNEL_MULTISPIN = 2
 KERNEL_DENSE = 3
 RUN_EXCHANGE = 1
+RUN_TRACK_BEST = 2
+STEPS_UNLIMITED = 0xFFFFFFFF
 
 EXPORTED = (
     "ige_context_create", "ige_context_destroy", "ige_last_error", "ige_version",
@@ -38,7 +40,8 @@ EXPORTED = (
     "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
     "ige_batch_info", "ige_batch_read", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
-    "ige_batch_configure", "ige_server_configure",
+    "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
+    "ige_read_best", "ige_model_polish",
 )
 
 REQ_QUEUED = 0
@@ -90,6 +93,11 @@ class ServerCounters(ctypes.Structure):
         ("rounds", c_uint64), ("attempts", c_uint64), ("arena_used", c_uint64),
         ("arena_capacity", c_uint64), ("max_live", c_uint64), ("busy_ms", c_double),
     ]
+
+
+class DescentReport(ctypes.Structure):
+    _fields_ = [("steps", c_uint32), ("converged", c_uint32), ("flips", c_uint64),
+                ("proposed", c_uint64)]
 
 
 class IgeError(RuntimeError):
@@ -164,6 +172,14 @@ def load() -> ctypes.CDLL:
         "ige_server_configure": (c_int32, [vp, c_uint32, c_uint32, c_uint32]),
         "ige_profile_read": (c_int32, [vp, POINTER(c_double), POINTER(c_uint64),
                                        POINTER(c_double), POINTER(c_uint64)]),
+        "ige_steepest_descent": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
+                                           POINTER(DescentReport)]),
+        "ige_update_best": (c_int32, [vp]),
+        "ige_read_best": (c_int32, [vp, POINTER(ctypes.c_uint8), POINTER(c_int64),
+                                    POINTER(c_uint32)]),
+        "ige_model_polish": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
+                                       POINTER(ctypes.c_uint8), POINTER(c_int64),
+                                       POINTER(c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -234,6 +250,26 @@ class Model:
                                       _ptr(w, c_int32)))
         return rp, ci, w
 
+    def polish(self, x, clamped=None):
+        """steepest_descent_1opt on every row of x ([count][n] bytes, site order), in place
+        on a copy.  Returns (x, energies, flips)."""
+        x = np.array(x, dtype=np.uint8, order="C", copy=True)
+        if x.ndim == 1:
+            x = x[None, :]
+        if x.shape[1] != self.n:
+            raise ValueError("x must be [count][n]")
+        cl = None
+        if clamped is not None:
+            cl = np.ascontiguousarray(clamped, dtype=np.uint8)
+            if cl.shape != (self.n,):
+                raise ValueError("clamped must have length n")
+        e = np.empty(x.shape[0], dtype=np.int64)
+        f = np.empty(x.shape[0], dtype=np.uint32)
+        check(self._lib.ige_model_polish(self.handle, x.shape[0], _ptr(x, ctypes.c_uint8),
+                                         _ptr(cl, ctypes.c_uint8), _ptr(e, c_int64),
+                                         _ptr(f, c_uint32)))
+        return x, e, f
+
     def close(self):
         if self.handle:
             self._lib.ige_model_destroy(self.handle)
@@ -269,15 +305,37 @@ class State:
             raise ValueError("ladder needs one beta per cell")
         check(self._lib.ige_set_ladder(self.handle, _ptr(b, c_double)))
 
-    def run(self, sweeps: int, exchange: bool = False, beta_schedule=None) -> None:
+    def run(self, sweeps: int, exchange: bool = False, beta_schedule=None,
+            track_best: bool = False) -> None:
         sched = None
         if beta_schedule is not None:
             sched = np.ascontiguousarray(beta_schedule, dtype=np.float64)
             if sched.shape != (sweeps, self.cells):
                 raise ValueError("beta_schedule must be [sweeps][cells]")
         self._sched_keepalive = sched
-        check(self._lib.ige_run(self.handle, int(sweeps), RUN_EXCHANGE if exchange else 0,
-                                _ptr(sched, c_double)))
+        flags = (RUN_EXCHANGE if exchange else 0) | (RUN_TRACK_BEST if track_best else 0)
+        check(self._lib.ige_run(self.handle, int(sweeps), flags, _ptr(sched, c_double)))
+
+    def steepest_descent(self, max_steps: int = STEPS_UNLIMITED, clamped=None) -> DescentReport:
+        cl = None if clamped is None else np.ascontiguousarray(clamped, dtype=np.uint8)
+        if cl is not None and cl.shape != (self.n,):
+            raise ValueError("clamped must have length n")
+        rep = DescentReport()
+        check(self._lib.ige_steepest_descent(self.handle, int(max_steps),
+                                             _ptr(cl, ctypes.c_uint8), ctypes.byref(rep)))
+        return rep
+
+    def update_best(self) -> None:
+        check(self._lib.ige_update_best(self.handle))
+
+    def best(self):
+        """(x bytes in site order, energy, sweeps completed when found)."""
+        x = np.empty(self.n, dtype=np.uint8)
+        e = c_int64()
+        sw = c_uint32()
+        check(self._lib.ige_read_best(self.handle, _ptr(x, ctypes.c_uint8), ctypes.byref(e),
+                                      ctypes.byref(sw)))
+        return x, e.value, sw.value
 
     def exchange(self, rounds: int) -> None:
         check(self._lib.ige_exchange(self.handle, int(rounds)))

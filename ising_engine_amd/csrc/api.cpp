@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -43,7 +44,7 @@ struct ige_model {
     int64_t max_local = 0;
     uint32_t dmax = 0;
     uint32_t regular_degree = 0;
-    DevBuf<uint32_t> rp, ci, order, sgn;
+    DevBuf<uint32_t> rp, ci, order, pos_of, sgn;
     DevBuf<int32_t> wt, lin;
     // Dense int8 matrix, built on the first dense state of this model.
     mutable std::mutex dense_mu;
@@ -129,6 +130,8 @@ struct ige_state {
     DevBuf<unsigned long long> counters, qplanes;  // counters: accepted, then pair stats
     DevBuf<double> beta;                             // current ladder (+ schedule rows)
     DevBuf<int32_t> fields;
+    DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
+    DevBuf<uint8_t> best_x;        // n bytes, site order
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -183,12 +186,14 @@ void upload_model(ige_model* m) {
     m->wt.alloc(wt.size());
     m->lin.alloc(lin.size());
     m->order.alloc(n);
+    m->pos_of.alloc(n);
     m->sgn.alloc(n);
     m->rp.upload(rp.data(), rp.size(), nullptr);
     m->ci.upload(ci.data(), ci.size(), nullptr);
     m->wt.upload(wt.data(), wt.size(), nullptr);
     m->lin.upload(lin.data(), lin.size(), nullptr);
     m->order.upload(co.order.data(), n, nullptr);
+    m->pos_of.upload(co.pos_of.data(), n, nullptr);
     m->sgn.upload(sgn.data(), n, nullptr);
     check(hipStreamSynchronize(nullptr), "model upload");
 }
@@ -218,6 +223,77 @@ void refresh_thresholds(ige_state* s, const double* dev_beta) {
     else if (s->kernel == IGE_KERNEL_MULTISPIN)
         check(ige::launch_qplanes(m, st, dev_beta, s->qplanes.p, s->stream), "qplanes");
     // the dense kernel evaluates its thresholds from beta directly
+}
+
+void ensure_best(ige_state* s) {
+    if (s->best.p) return;
+    s->best.alloc(1);
+    s->best_x.alloc(s->model->ir.n);
+    ige::BestRecord zero{};
+    s->best.upload(&zero, 1, s->stream);
+    check(hipStreamSynchronize(s->stream), "best init");
+}
+
+void track_best(ige_state* s) {
+    check(ige::launch_track_best(s->model->view(), s->view(), s->sweep, s->best.p, s->best_x.p,
+                                 s->stream),
+          "track best");
+}
+
+struct DescentResult {
+    uint32_t steps = 0;
+    uint32_t converged = 0;
+    uint64_t flips = 0;
+};
+
+// Steepest descent of every replica of `st` (spins and tracked energies in
+// place); flips[cells*64] receives one per flip.  dE comes from the
+// matrix-core fields on a dense model and from the CSR rows otherwise.
+// Steps are launched in growing batches; the host reads the control words
+// between batches and stops once a step has flipped nothing.
+DescentResult descend(const ige_model* model, const ige::DevState& st, const uint8_t* clamped,
+                      uint32_t max_steps, uint32_t* flips, hipStream_t stream) {
+    const uint32_t n = model->ir.n;
+    DevBuf<unsigned long long> keys, ctl;
+    keys.alloc(static_cast<size_t>(st.cells) * 64);
+    ctl.alloc(ige::kDescentCtlWords);
+    DevBuf<uint8_t> cl;
+    if (clamped) {
+        cl.alloc(n);
+        cl.upload(clamped, n, stream);
+    }
+    DevBuf<int32_t> fields;
+    const bool dense = model->ds.eligible && model->ds.dense &&
+                       static_cast<uint64_t>(st.cells) * n * 64 * sizeof(int32_t) <= (1ull << 32);
+    if (dense) {
+        model->ensure_dense();
+        fields.alloc(static_cast<size_t>(st.cells) * n * 64);
+    }
+    check(ige::launch_descent_reset(keys.p, st.cells, ctl.p, stream), "descent reset");
+    unsigned long long c[ige::kDescentCtlWords] = {};
+    uint64_t launched = 0;
+    uint32_t batch = 1;
+    while (launched < max_steps) {
+        const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(batch, max_steps - launched));
+        for (uint32_t i = 0; i < k; ++i) {
+            if (dense)
+                check(ige::launch_dense_fields(model->dense_view(), st, fields.p, stream),
+                      "descent fields");
+            check(ige::launch_descent_step(model->view(), st, dense ? fields.p : nullptr,
+                                           model->pos_of.p, cl.p, keys.p, flips, ctl.p, stream),
+                  "descent step");
+        }
+        launched += k;
+        ctl.download(c, ige::kDescentCtlWords, stream);
+        check(hipStreamSynchronize(stream), "descent");
+        if (c[ige::kDescentDone]) break;
+        batch = std::min<uint32_t>(batch * 2, 64);
+    }
+    DescentResult r;
+    r.steps = static_cast<uint32_t>(c[ige::kDescentSteps]);
+    r.converged = c[ige::kDescentDone] ? 1u : 0u;
+    r.flips = c[ige::kDescentFlips];
+    return r;
 }
 
 void recompute_energy_into_state(ige_state* s) {
@@ -388,12 +464,14 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
     return guarded([&] {
         require(s != nullptr, "NULL state");
         require(sched != nullptr || s->ladder_set, "no ladder set and no beta schedule given");
-        require((flags & ~IGE_RUN_EXCHANGE) == 0, "unknown flag");
+        require((flags & ~(IGE_RUN_EXCHANGE | IGE_RUN_TRACK_BEST)) == 0, "unknown flag");
         require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull &&
                     static_cast<uint64_t>(s->xround) + sweeps <= 0xFFFFFFFFull,
                 "sweep or exchange counter would exceed 2^32-1");
         set_device(s->ctx);
         const bool exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
+        const bool track = (flags & IGE_RUN_TRACK_BEST) != 0;
+        if (track) ensure_best(s);
         DevBuf<double> sched_dev;
         if (sched) {
             const size_t total = static_cast<size_t>(sweeps) * s->cells;
@@ -452,6 +530,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                 s->xround += 1;
             }
             s->sweep += 1;
+            if (track) track_best(s);
         }
         if (sched) {
             // Leave the ladder at the schedule's last row.
@@ -573,6 +652,110 @@ int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
         for (uint32_t t = 0; t + 1 < s->cells; ++t) {
             if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];
             if (pair_accepts) pair_accepts[t] = c[2 + 2 * t];
+        }
+    });
+}
+
+int32_t ige_steepest_descent(ige_state* s, uint32_t max_steps, const uint8_t* clamped,
+                             ige_descent_report* report) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        set_device(s->ctx);
+        DevBuf<uint32_t> flips;
+        flips.alloc(static_cast<size_t>(s->cells) * 64);
+        std::vector<uint32_t> zeros(flips.count, 0);
+        flips.upload(zeros.data(), zeros.size(), s->stream);
+        const DescentResult r = descend(s->model, s->view(), clamped, max_steps, flips.p, s->stream);
+        if (report) {
+            report->steps = r.steps;
+            report->converged = r.converged;
+            report->flips = r.flips;
+            report->proposed = (static_cast<uint64_t>(r.steps) + r.converged) * s->model->ir.n *
+                               64ull * s->cells;
+        }
+    });
+}
+
+int32_t ige_update_best(ige_state* s) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        set_device(s->ctx);
+        ensure_best(s);
+        track_best(s);
+    });
+}
+
+int32_t ige_read_best(ige_state* s, uint8_t* x, int64_t* energy, uint32_t* sweep) {
+    return guarded([&] {
+        require(s && energy, "NULL argument");
+        require(s->best.p != nullptr, "no incumbent yet: call ige_update_best or run with "
+                                      "IGE_RUN_TRACK_BEST");
+        set_device(s->ctx);
+        ige::BestRecord b{};
+        s->best.download(&b, 1, s->stream);
+        if (x) s->best_x.download(x, s->model->ir.n, s->stream);
+        check(hipStreamSynchronize(s->stream), "read best");
+        require(b.valid != 0, "no incumbent yet");
+        *energy = b.energy;
+        if (sweep) *sweep = b.sweep;
+    });
+}
+
+int32_t ige_model_polish(const ige_model* model, uint32_t count, uint8_t* x,
+                         const uint8_t* clamped, int64_t* energy, uint32_t* flips) {
+    return guarded([&] {
+        require(model && x && energy, "NULL argument");
+        require(count >= 1 && count <= 65536u * 64u, "count must be in [1, 2^22]");
+        set_device(model->ctx);
+        const uint32_t n = model->ir.n;
+        const uint32_t cells = (count + 63) / 64;
+        const auto& order = model->co.order;
+        std::vector<uint64_t> w(static_cast<size_t>(cells) * n, 0);
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint8_t* xi = x + static_cast<size_t>(i) * n;
+            uint64_t* wc = w.data() + static_cast<size_t>(i / 64) * n;
+            const uint64_t bit = 1ull << (i % 64);
+            for (uint32_t p = 0; p < n; ++p) {
+                require(xi[order[p]] <= 1, "x must hold 0/1 bytes");
+                if (xi[order[p]]) wc[p] |= bit;
+            }
+        }
+        hipStream_t stream = nullptr;
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+        } guard{stream};
+        DevBuf<uint64_t> spins;
+        DevBuf<int64_t> e;
+        DevBuf<uint32_t> fl;
+        spins.alloc(w.size());
+        e.alloc(static_cast<size_t>(cells) * 64);
+        fl.alloc(static_cast<size_t>(cells) * 64);
+        spins.upload(w.data(), w.size(), stream);
+        std::vector<uint32_t> zeros(fl.count, 0);
+        fl.upload(zeros.data(), zeros.size(), stream);
+        ige::DevState st;
+        st.cells = cells;
+        st.spins = spins.p;
+        st.energy = e.p;
+        check(ige::launch_energy(model->view(), st, e.p, stream), "energy");
+        (void)descend(model, st, clamped, IGE_STEPS_UNLIMITED, fl.p, stream);
+        std::vector<int64_t> eh(e.count);
+        std::vector<uint32_t> fh(fl.count);
+        spins.download(w.data(), w.size(), stream);
+        e.download(eh.data(), eh.size(), stream);
+        fl.download(fh.data(), fh.size(), stream);
+        check(hipStreamSynchronize(stream), "polish");
+        for (uint32_t i = 0; i < count; ++i) {
+            uint8_t* xi = x + static_cast<size_t>(i) * n;
+            const uint64_t* wc = w.data() + static_cast<size_t>(i / 64) * n;
+            for (uint32_t p = 0; p < n; ++p) xi[order[p]] = (wc[p] >> (i % 64)) & 1u;
+            energy[i] = eh[i];
+            if (flips) flips[i] = fh[i];
         }
     });
 }

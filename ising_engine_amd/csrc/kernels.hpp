@@ -96,4 +96,37 @@ hipError_t launch_energy(const DevModel& m, const DevState& s, int64_t* out, hip
 hipError_t launch_fields(const DevModel& m, const DevState& s, int32_t* out, hipStream_t stream);
 hipError_t launch_identity_perm(const DevState& s, hipStream_t stream);
 
+// Steepest descent (every replica flips its most negative-dE site per step).
+// ctl: kDescentCtlWords counters, cleared by launch_descent_reset with the
+// keys ([cells][64]); after a step without a flip ctl[kDescentDone] is 1 and
+// further steps do nothing.
+enum : uint32_t {
+    kDescentDone = 0,       // 1 once a step flipped nothing
+    kDescentStepFlips = 1,  // flips of the step in progress
+    kDescentSteps = 2,      // steps that flipped at least one replica
+    kDescentFlips = 3,      // flips of all finished steps
+    kDescentCtlWords = 4
+};
+hipError_t launch_descent_reset(unsigned long long* keys, uint32_t cells, unsigned long long* ctl,
+                                hipStream_t stream);
+// One step.  dense_fields: NULL (dE from the CSR rows) or [cells][n][64]
+// fields of the current spins in site order (launch_dense_fields).
+// pos_of: n, site -> position.  clamped: NULL or n bytes in site order.
+// flips: [cells][64], one added per flip.
+hipError_t launch_descent_step(const DevModel& m, const DevState& s, const int32_t* dense_fields,
+                               const uint32_t* pos_of, const uint8_t* clamped,
+                               unsigned long long* keys, uint32_t* flips, unsigned long long* ctl,
+                               hipStream_t stream);
+
+// Best-seen replica: energy, where it was found, and (separately) its bits.
+struct BestRecord {
+    int64_t energy;
+    uint32_t valid;
+    uint32_t sweep;  // sweeps completed when it was found
+    uint32_t temp;   // temperature index it held
+    uint32_t lane;
+};
+hipError_t launch_track_best(const DevModel& m, const DevState& s, uint32_t sweep,
+                             BestRecord* best, uint8_t* best_x, hipStream_t stream);
+
 }  // namespace ige

@@ -65,6 +65,14 @@ def load():
     lib.or_geometric_ladder.restype = None
     lib.or_suggested_range.argtypes = [c_uint32, u32p, i64p, i64p, dp, dp]
     lib.or_suggested_range.restype = None
+    u8p, i32p = POINTER(ctypes.c_uint8), POINTER(ctypes.c_int32)
+    lib.or_steepest_descent.argtypes = [c_uint32, u32p, u32p, i64p, i64p, u8p, c_uint32,
+                                        c_uint32, u64p, i64p, u32p]
+    lib.or_steepest_descent.restype = c_uint32
+    lib.or_polish_1opt.argtypes = [c_uint32, u32p, u32p, i64p, i64p, u8p, u8p, u64p]
+    lib.or_polish_1opt.restype = c_int64
+    lib.or_update_best.argtypes = [c_uint32, c_uint32, u64p, u32p, i64p, i64p, i32p, u8p]
+    lib.or_update_best.restype = c_int
     _lib = lib
     return lib
 
@@ -161,6 +169,51 @@ class CSR:
         load().or_energies(n, rp, ci, w, _p(self.linear, c_int64), self.offset, cells,
                            _p(words, c_uint64), _p(out, c_int64))
         return out
+
+
+    def steepest_descent(self, words, energies, max_steps, clamped=None, flips=None):
+        """SteepestDescent::apply on every replica, in place on words / energies.
+        Returns (steps that flipped, per-replica flip counts [cells][64])."""
+        cells = words.shape[0]
+        if flips is None:
+            flips = np.zeros((cells, 64), dtype=np.uint32)
+        cl = None if clamped is None else np.ascontiguousarray(clamped, dtype=np.uint8)
+        n, rp, ci, w = self._args()
+        steps = load().or_steepest_descent(n, rp, ci, w, _p(self.linear, c_int64),
+                                           _p(cl, ctypes.c_uint8), cells, max_steps,
+                                           _p(words, c_uint64), _p(energies, c_int64),
+                                           _p(flips, c_uint32))
+        return int(steps), flips
+
+    def polish_1opt(self, x, clamped=None):
+        """steepest_descent_1opt on one byte state, in place. Returns (improvement, flips)."""
+        assert x.dtype == np.uint8 and x.flags.c_contiguous and x.shape == (self.n,)
+        cl = None if clamped is None else np.ascontiguousarray(clamped, dtype=np.uint8)
+        flips = c_uint64()
+        n, rp, ci, w = self._args()
+        imp = load().or_polish_1opt(n, rp, ci, w, _p(self.linear, c_int64),
+                                    _p(cl, ctypes.c_uint8), _p(x, ctypes.c_uint8),
+                                    ctypes.byref(flips))
+        return int(imp), int(flips.value)
+
+
+class Incumbent:
+    """update_incumbent restated: best (energy, bytes) seen over tracked energies."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.energy = ctypes.c_int64(0)
+        self.valid = ctypes.c_int32(0)
+        self.x = np.zeros(n, dtype=np.uint8)
+
+    def update(self, words, temp_of, energies) -> bool:
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        temp_of = np.ascontiguousarray(temp_of, dtype=np.uint32)
+        energies = np.ascontiguousarray(energies, dtype=np.int64)
+        return bool(load().or_update_best(self.n, words.shape[0], _p(words, c_uint64),
+                                          _p(temp_of, c_uint32), _p(energies, c_int64),
+                                          ctypes.byref(self.energy), ctypes.byref(self.valid),
+                                          _p(self.x, ctypes.c_uint8)))
 
 
 def init_words(n: int, cells: int, seed: int) -> np.ndarray:

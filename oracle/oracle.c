@@ -442,3 +442,133 @@ void or_suggested_range(uint32_t n, const uint32_t *row_ptr, const int64_t *w,
     *t_max = tmax;
     *t_min = tmin;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Deterministic descent and the incumbent                                    */
+/* ------------------------------------------------------------------------- */
+
+/* dE of flipping site i of replica `lane`: (1 - 2 x_i) * field_i. */
+static int64_t flip_gain(const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+                         const int64_t *linear, const uint64_t *slot_words, uint32_t site,
+                         uint32_t lane) {
+    const int64_t h = field_of(row_ptr, col_idx, w, linear, slot_words, site, lane);
+    return bit(slot_words[site], lane) ? -h : h;
+}
+
+/* steepest_descent.rs:80-124 (SteepestDescent::apply) over every replica of
+ * every slot: per step each replica flips the site of its most negative dE
+ * (strict <, so the lowest site wins a tie and dE = 0 never moves); the loop
+ * ends after `max_steps` steps or after a step in which no replica moved.
+ * Clamped sites (local_search.rs is_clamped) are never chosen. */
+uint32_t or_steepest_descent(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                             const int64_t *w, const int64_t *linear, const uint8_t *clamped,
+                             uint32_t cells, uint32_t max_steps, uint64_t *words,
+                             int64_t *energies, uint32_t *flips) {
+    const uint64_t reps = (uint64_t)cells * 64;
+    int64_t *best_de = (int64_t *)malloc(reps * sizeof(int64_t));
+    uint32_t *best_site = (uint32_t *)malloc(reps * sizeof(uint32_t));
+    uint32_t steps = 0;
+    for (uint32_t k = 0; k < max_steps; ++k) {
+        for (uint64_t q = 0; q < reps; ++q) {
+            best_de[q] = 0;
+            best_site[q] = UINT32_MAX;
+        }
+        for (uint32_t s = 0; s < cells; ++s) {
+            const uint64_t *sw = words + (uint64_t)s * n;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (clamped && clamped[i]) continue;
+                for (uint32_t r = 0; r < 64; ++r) {
+                    const int64_t de = flip_gain(row_ptr, col_idx, w, linear, sw, i, r);
+                    if (de < best_de[s * 64 + r]) {
+                        best_de[s * 64 + r] = de;
+                        best_site[s * 64 + r] = i;
+                    }
+                }
+            }
+        }
+        int any = 0;
+        for (uint32_t s = 0; s < cells; ++s)
+            for (uint32_t r = 0; r < 64; ++r) {
+                const uint64_t q = (uint64_t)s * 64 + r;
+                if (best_site[q] == UINT32_MAX) continue;
+                words[(uint64_t)s * n + best_site[q]] ^= 1ull << r;
+                energies[q] += best_de[q];
+                flips[q] += 1;
+                any = 1;
+            }
+        if (!any) break;
+        ++steps;
+    }
+    free(best_de);
+    free(best_site);
+    return steps;
+}
+
+/* local_search.rs:19-60 (steepest_descent_1opt) on one state of bytes:
+ * gains g_i = (1 - 2x_i) field_i kept incrementally, the most negative gain
+ * flipped (lowest index on ties) until none is negative.  Returns the total
+ * improvement (>= 0); *flips receives the number of flips. */
+int64_t or_polish_1opt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                       const int64_t *w, const int64_t *linear, const uint8_t *clamped,
+                       uint8_t *x, uint64_t *flips) {
+    int64_t *g = (int64_t *)malloc(((size_t)n + 1) * sizeof(int64_t));
+    for (uint32_t i = 0; i < n; ++i) {
+        int64_t h = linear[i];
+        for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+            if (x[col_idx[e]]) h += w[e];
+        g[i] = x[i] ? -h : h;
+    }
+    int64_t improvement = 0;
+    uint64_t count = 0;
+    for (;;) {
+        uint32_t best = UINT32_MAX;
+        int64_t best_g = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            if (!(clamped && clamped[i]) && g[i] < best_g) {
+                best = i;
+                best_g = g[i];
+            }
+        if (best == UINT32_MAX) break;
+        improvement -= best_g;
+        ++count;
+        const int64_t dir = x[best] ? -1 : 1; /* +1 for 0 -> 1 */
+        x[best] ^= 1u;
+        g[best] = -best_g;
+        for (uint32_t e = row_ptr[best]; e < row_ptr[best + 1]; ++e) {
+            const uint32_t j = col_idx[e];
+            g[j] += (x[j] ? -1 : 1) * w[e] * dir;
+        }
+    }
+    free(g);
+    if (flips) *flips = count;
+    return improvement;
+}
+
+/* ultimate.rs:397-431 (update_incumbent): the replica with the lowest
+ * tracked energy, scanned in (temperature, lane) order with strict <, replaces
+ * the incumbent when its energy is strictly lower (or no incumbent exists).
+ * words are [cells][n] in site order; best_x receives n bytes.  Returns 1 if
+ * the incumbent changed. */
+int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint32_t *temp_of,
+                   const int64_t *energies, int64_t *best_e, int32_t *valid, uint8_t *best_x) {
+    uint64_t arg = UINT64_MAX;
+    int64_t e_min = 0;
+    uint64_t key_min = UINT64_MAX; /* temperature * 64 + lane of the current minimum */
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t r = 0; r < 64; ++r) {
+            const uint64_t q = (uint64_t)s * 64 + r;
+            const uint64_t key = (uint64_t)temp_of[q] * 64 + r;
+            if (arg == UINT64_MAX || energies[q] < e_min ||
+                (energies[q] == e_min && key < key_min)) {
+                arg = q;
+                e_min = energies[q];
+                key_min = key;
+            }
+        }
+    if (arg == UINT64_MAX || (*valid && e_min >= *best_e)) return 0;
+    const uint32_t s = (uint32_t)(arg / 64), r = (uint32_t)(arg % 64);
+    for (uint32_t i = 0; i < n; ++i) best_x[i] = (uint8_t)bit(words[(uint64_t)s * n + i], r);
+    *best_e = e_min;
+    *valid = 1;
+    return 1;
+}

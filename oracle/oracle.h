@@ -20,6 +20,9 @@
  *                        lane r of temperature t against lane r of t+1)
  *   or_geometric_ladder  src/engine_v2/decision.rs:133-142
  *   or_suggested_range   src/solver/schedule.rs:21-43
+ *   or_steepest_descent  src/engine_v2/operators/steepest_descent.rs:80-124
+ *   or_polish_1opt       src/solver/local_search.rs:19-60
+ *   or_update_best       src/solver/ultimate.rs:397-431
  *
  * Parity mode departs from the reference in exactly two declared places,
  * because the reference's ChaCha8 streams are a third-party dependency that
@@ -106,6 +109,23 @@ void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *be
 void or_geometric_ladder(uint32_t k, double t_hi, double t_lo, double *temps);
 void or_suggested_range(uint32_t n, const uint32_t *row_ptr, const int64_t *w,
                         const int64_t *linear, double *t_max, double *t_min);
+
+/* ---- deterministic descent and the incumbent ---- */
+/* Every replica of every slot flips its most negative-dE site (lowest site on
+ * ties) once per step; stops after max_steps or a step without a flip.
+ * clamped: NULL or [n] bytes (nonzero = never flipped).  words / energies in
+ * and out; flips[cells*64] added to.  Returns the steps that flipped. */
+uint32_t or_steepest_descent(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                             const int64_t *w, const int64_t *linear, const uint8_t *clamped,
+                             uint32_t cells, uint32_t max_steps, uint64_t *words,
+                             int64_t *energies, uint32_t *flips);
+/* One state of n bytes to a 1-opt local minimum; returns the improvement. */
+int64_t or_polish_1opt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                       const int64_t *w, const int64_t *linear, const uint8_t *clamped,
+                       uint8_t *x, uint64_t *flips);
+/* Incumbent update from tracked energies; words [cells][n] site order. */
+int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint32_t *temp_of,
+                   const int64_t *energies, int64_t *best_e, int32_t *valid, uint8_t *best_x);
 
 #ifdef __cplusplus
 }
