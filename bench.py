@@ -252,6 +252,15 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
     if not np.array_equal(st.energies(), st.recompute_energies()):
         raise SystemExit("tracked energies drifted from recompute")
     per_sweep = dt / len(win)
+    # end-of-run polish (1-opt) of every replica of the window's last state, 64 states at once
+    x64 = np.zeros((64, qb.n), dtype=np.uint8)
+    words = st.spins()[0]
+    for r in range(64):
+        x64[r] = (words >> np.uint64(r)) & np.uint64(1)
+    m.polish(x64[:1])
+    t0 = time.perf_counter()
+    _, e_pol, f_pol = m.polish(x64)
+    dt_pol = time.perf_counter() - t0
     ops = 2.0 * qb.n * qb.n * 64  # int8 multiply-adds of one full field recompute, x2
     mfma_tops = ops / (ms / 1e3 / launches) / 1e12
     out[wl["name"]] = {"value": qb.n * 64 / per_sweep, "unit": "spin-flip attempts/s",
@@ -260,6 +269,10 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
                        "sample": f"{len(win)} sweeps from the middle of the "
                                  f"{wl['sweeps']}-sweep schedule",
                        "kernel": kname[st.stats()[0].kernel],
+                       "polish_64_states_s": dt_pol,
+                       "polish_flips_max": int(f_pol.max()),
+                       "polish_best_energy": int(e_pol.min()),
+                       "best_energy_before_polish": int(st.energies().min()),
                        "roofline": {"bound": "mfma", "achieved": round(mfma_tops, 3),
                                     "peak": 5000.0, "unit": "TOP/s (int8)",
                                     "frac": mfma_tops / 5000.0,
@@ -268,6 +281,45 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
                        "cpu_1thread": cpu_sample(qb, 1, [1.0], cpu_s)}
     log(f"[bench] {wl['name']}: {per_sweep * 1e3:.2f} ms/sweep, {mfma_tops:.2f} TOP/s int8")
     return out
+
+
+def bench_descent(_lib, st, qb, cells: int, sweeps: int, steps: int = 200) -> dict:
+    """SteepestDescent on the timed parallel-tempering state: `steps` steps, each one scan of
+    every (slot, site, replica) and one flip per replica that can improve; and the cost per
+    sweep of the incumbent update (one argmin launch after each sweep)."""
+    n, nnz = qb.n, 2 * len(qb.q)
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.run(sweeps, exchange=True)
+    st.synchronize()
+    plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st.run(sweeps, exchange=True, track_best=True)
+    st.synchronize()
+    tracked = time.perf_counter() - t0
+    _, e_best, _ = st.best()
+    before = st.energies().copy()
+    st.steepest_descent(2)  # first-call allocations outside the timed region
+    t0 = time.perf_counter()
+    rep = st.steepest_descent(steps)
+    dt = time.perf_counter() - t0
+    after = st.energies()
+    if not np.array_equal(after, st.recompute_energies()):
+        raise SystemExit("tracked energies drifted from recompute after descent")
+    done = rep.steps + rep.converged
+    # one scan reads every slot's words once and the CSR rows once (rp, lin, order; ci, wt)
+    scan_bytes = cells * n * 8 + n * 12 + nnz * 8
+    log(f"[bench] descent: {rep.steps} steps, {rep.flips} flips in {dt * 1e3:.1f} ms "
+        f"({dt / done * 1e6:.1f} us/step); incumbent update "
+        f"{(tracked - plain) / sweeps * 1e6:.1f} us/sweep")
+    return {"value": rep.proposed / dt, "unit": "site-replica scans/s (Report.proposed)",
+            "steps": rep.steps, "converged": rep.converged, "flips": rep.flips,
+            "seconds": dt, "us_per_step": dt / done * 1e6,
+            "scan_bytes_per_step": scan_bytes,
+            "mean_improvement_per_replica": float((before - after).mean()),
+            "incumbent_energy": e_best,
+            "incumbent_update_us_per_sweep": (tracked - plain) / sweeps * 1e6,
+            "sweeps_timed_with_and_without_tracking": sweeps}
 
 
 def bench_batch(_lib, instances, ctx, count: int) -> dict:
@@ -340,6 +392,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-workloads", action="store_true")
+    ap.add_argument("--no-descent", action="store_true")
     ap.add_argument("--tts-runs", type=int, default=32)
     ap.add_argument("--sk-sweeps", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -410,6 +463,10 @@ def main():
                     avg_launch_us=round(avg_launch_s * 1e6, 2),
                     bytes_per_attempt=round(bpa, 4))
 
+    descent = None
+    if not args.no_descent:
+        descent = bench_descent(_lib, st, qb, cells, args.profile_steps)
+
     batched = None
     if not args.no_batch:
         batched = bench_batch(_lib, instances, ctx, args.batch_count)
@@ -449,6 +506,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "workloads": workloads,
+        "descent": descent,
     }
     print(json.dumps(out), flush=True)
 

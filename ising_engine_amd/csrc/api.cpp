@@ -132,6 +132,7 @@ struct ige_state {
     DevBuf<int32_t> fields;
     DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
     DevBuf<uint8_t> best_x;        // n bytes, site order
+    DevBuf<uint32_t> best_take;    // decision of the last update
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -229,14 +230,15 @@ void ensure_best(ige_state* s) {
     if (s->best.p) return;
     s->best.alloc(1);
     s->best_x.alloc(s->model->ir.n);
+    s->best_take.alloc(2);
     ige::BestRecord zero{};
     s->best.upload(&zero, 1, s->stream);
     check(hipStreamSynchronize(s->stream), "best init");
 }
 
 void track_best(ige_state* s) {
-    check(ige::launch_track_best(s->model->view(), s->view(), s->sweep, s->best.p, s->best_x.p,
-                                 s->stream),
+    check(ige::launch_track_best(s->model->view(), s->view(), s->sweep, s->best.p,
+                                 s->best_take.p, s->best_x.p, s->stream),
           "track best");
 }
 
@@ -257,10 +259,20 @@ DescentResult descend(const ige_model* model, const ige::DevState& st, const uin
     DevBuf<unsigned long long> keys, ctl;
     keys.alloc(static_cast<size_t>(st.cells) * 64);
     ctl.alloc(ige::kDescentCtlWords);
+    require(n < 0x80000000u, "descent needs n < 2^31");
     DevBuf<uint8_t> cl;
+    DevBuf<uint32_t> aux;  // position -> site | clamped << 31
     if (clamped) {
         cl.alloc(n);
         cl.upload(clamped, n, stream);
+        std::vector<uint32_t> a(n);
+        for (uint32_t p = 0; p < n; ++p) {
+            const uint32_t site = model->co.order[p];
+            a[p] = site | (clamped[site] ? 0x80000000u : 0u);
+        }
+        aux.alloc(n);
+        aux.upload(a.data(), n, stream);
+        check(hipStreamSynchronize(stream), "descent upload");
     }
     DevBuf<int32_t> fields;
     const bool dense = model->ds.eligible && model->ds.dense &&
@@ -280,7 +292,9 @@ DescentResult descend(const ige_model* model, const ige::DevState& st, const uin
                 check(ige::launch_dense_fields(model->dense_view(), st, fields.p, stream),
                       "descent fields");
             check(ige::launch_descent_step(model->view(), st, dense ? fields.p : nullptr,
-                                           model->pos_of.p, cl.p, keys.p, flips, ctl.p, stream),
+                                           model->pos_of.p, cl.p,
+                                           clamped ? aux.p : model->order.p, keys.p, flips,
+                                           ctl.p, stream),
                   "descent step");
         }
         launched += k;

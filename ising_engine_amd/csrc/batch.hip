@@ -28,6 +28,16 @@
 #include "problem_ir.hpp"
 #include "batch_internal.hpp"
 
+namespace ige {
+
+// thr[d] = metropolis_threshold(beta, d) for d in [1, dtab], thr[0] = 2^32 - 1.
+static __global__ void thr_value_kernel(double beta, uint32_t* thr, uint32_t dtab) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d <= dtab) thr[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
+}
+
+}  // namespace ige
+
 using ige::abi::check;
 using ige::abi::DevBuf;
 using ige::abi::Failure;

@@ -18,11 +18,6 @@
 
 namespace ige {
 
-static __global__ void thr_value_kernel(double beta, uint32_t* thr, uint32_t dtab) {
-    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d <= dtab) thr[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
-}
-
 static __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint32_t count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;

@@ -421,31 +421,53 @@ __global__ void init_spins_kernel(const uint32_t* __restrict__ order, uint64_t* 
     spins[i] = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
 }
 
-// Energy from scratch: one block per slot, lane = replica; accumulates
-// A = sum_i x_i (2 lin_i + sum_j q_ij x_j) so that E = offset + A/2 exactly.
-__global__ __launch_bounds__(kBlock) void energy_kernel(
+// Energy from scratch, lane = replica: A = sum_i x_i (lin_i + h_i) with
+// h_i = lin_i + sum_j q_ij x_j, so that E = offset + A / 2 exactly.
+// Workgroup b serves slot b / blocks_per_slot; its wave w the `chunk`
+// positions from ((b % blocks_per_slot) * 4 + w) * chunk, in groups of eight
+// whose rows are gathered together.  The workgroup's per-lane partial sums
+// are added into acc[slot][lane] (zeroed beforehand; two's-complement adds of
+// signed partials), then energy_finish turns A into E.
+constexpr int kEnergyGroup = 8;
+constexpr int kEnergyInterleave = 1;
+
+__global__ __launch_bounds__(kBlock) void energy_partial_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
-    const uint64_t* __restrict__ spins, uint32_t n, int64_t offset, int64_t* out) {
-    __shared__ int64_t part[kBlock];
+    const uint64_t* __restrict__ spins, uint32_t n, uint32_t chunk, uint32_t blocks_per_slot,
+    unsigned long long* __restrict__ acc) {
+    __shared__ int64_t part[kWavesPerBlock][kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wv = uniform(threadIdx.x >> 6);
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = blockIdx.x / blocks_per_slot;
+    const uint64_t p0_64 = ((uint64_t)(blockIdx.x % blocks_per_slot) * kWavesPerBlock + wv) * chunk;
+    const uint32_t p0 = (uint32_t)min<uint64_t>(p0_64, n);
+    const uint32_t p1 = min(n, p0 + chunk);
     const uint64_t* S = spins + (uint64_t)slot * n;
     int64_t a = 0;
-    for (uint32_t p = wv; p < n; p += kWavesPerBlock) {
-        const uint64_t x = S[p];
-        int64_t f = 2 * (int64_t)lin[p];
-        for (uint32_t e = rp[p]; e < rp[p + 1]; ++e) f += lane_bit(S[ci[e]]) ? wt[e] : 0;
-        a += lane_bit(x) ? f : 0;
-    }
-    part[threadIdx.x] = a;
+    fields_groups<kEnergyGroup, kEnergyInterleave>(
+        rp, ci, wt, lin, reinterpret_cast<const uint32_t*>(lin), S, p0, p1, lane,
+        [&](uint32_t, uint64_t x, int32_t h, uint32_t l) {
+            a += lane_bit(x) ? (int64_t)h + (int32_t)l : 0;
+        });
+    part[wv][lane] = a;
     __syncthreads();
-    if (threadIdx.x < kWave) {
-        int64_t tot = 0;
-        for (uint32_t w = 0; w < kWavesPerBlock; ++w) tot += part[w * kWave + lane];
-        out[slot * kWave + lane] = offset + tot / 2;
+    if (wv == 0) {
+        int64_t t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kWavesPerBlock; ++w) t += part[w][lane];
+        if (t) atomicAdd(acc + (uint64_t)slot * kWave + lane, (unsigned long long)t);
     }
+}
+
+__global__ void energy_zero_kernel(int64_t* out, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = 0;
+}
+
+__global__ void energy_finish_kernel(int64_t* out, uint32_t count, int64_t offset) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = offset + out[i] / 2;
 }
 
 __global__ __launch_bounds__(kBlock) void fields_kernel(
@@ -579,8 +601,21 @@ hipError_t launch_init_spins(const DevModel& m, const DevState& s, uint64_t seed
 }
 
 hipError_t launch_energy(const DevModel& m, const DevState& s, int64_t* out, hipStream_t stream) {
-    hipLaunchKernelGGL(energy_kernel, dim3(s.cells), dim3(kBlock), 0, stream, m.rp, m.ci, m.wt,
-                       m.lin, s.spins, m.n, m.offset, out);
+    const uint32_t count = s.cells * kWave;
+    hipLaunchKernelGGL(energy_zero_kernel, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, stream,
+                       out, count);
+    // About 16K waves, positions per wave a multiple of one interleaved gather.
+    const uint64_t unit = kEnergyGroup * kEnergyInterleave;
+    uint64_t chunk = ((uint64_t)m.n * s.cells + 16383) / 16384;
+    chunk = (chunk + unit - 1) / unit * unit;
+    if (chunk < unit) chunk = unit;
+    if (chunk > 4096) chunk = 4096;
+    const uint32_t blocks_per_slot = ceil_div(m.n, chunk * kWavesPerBlock);
+    hipLaunchKernelGGL(energy_partial_kernel, dim3((uint32_t)((uint64_t)blocks_per_slot * s.cells)),
+                       dim3(kBlock), 0, stream, m.rp, m.ci, m.wt, m.lin, s.spins, m.n,
+                       (uint32_t)chunk, blocks_per_slot, reinterpret_cast<unsigned long long*>(out));
+    hipLaunchKernelGGL(energy_finish_kernel, dim3(ceil_div(count, kBlock)), dim3(kBlock), 0, stream,
+                       out, count, m.offset);
     return hipGetLastError();
 }
 

@@ -111,12 +111,14 @@ hipError_t launch_descent_reset(unsigned long long* keys, uint32_t cells, unsign
                                 hipStream_t stream);
 // One step.  dense_fields: NULL (dE from the CSR rows) or [cells][n][64]
 // fields of the current spins in site order (launch_dense_fields).
-// pos_of: n, site -> position.  clamped: NULL or n bytes in site order.
-// flips: [cells][64], one added per flip.
+// pos_of: n, site -> position.  clamped: NULL or n bytes in site order (read
+// by the dense scan).  site_aux: n, position -> site with bit 31 set for a
+// clamped site (read by the sparse scan; the model's order when nothing is
+// clamped).  flips: [cells][64], one added per flip.
 hipError_t launch_descent_step(const DevModel& m, const DevState& s, const int32_t* dense_fields,
                                const uint32_t* pos_of, const uint8_t* clamped,
-                               unsigned long long* keys, uint32_t* flips, unsigned long long* ctl,
-                               hipStream_t stream);
+                               const uint32_t* site_aux, unsigned long long* keys, uint32_t* flips,
+                               unsigned long long* ctl, hipStream_t stream);
 
 // Best-seen replica: energy, where it was found, and (separately) its bits.
 struct BestRecord {
@@ -126,7 +128,9 @@ struct BestRecord {
     uint32_t temp;   // temperature index it held
     uint32_t lane;
 };
+// take: 2 words of scratch (decision and winning slot * 64 + lane).
 hipError_t launch_track_best(const DevModel& m, const DevState& s, uint32_t sweep,
-                             BestRecord* best, uint8_t* best_x, hipStream_t stream);
+                             BestRecord* best, uint32_t* take, uint8_t* best_x,
+                             hipStream_t stream);
 
 }  // namespace ige

@@ -171,6 +171,94 @@ __device__ __forceinline__ void generic_group(
     }
 }
 
+// Adds to h[q] the entries [max(rb[q], base), min(rb[q+1], top)) of row q,
+// whose couplings and neighbour words lane (k - base) holds.
+template <int G>
+__device__ __forceinline__ void walk_entries(const uint32_t (&rb)[G + 1], uint32_t base,
+                                             uint32_t top, uint64_t nw, int32_t nwt,
+                                             int32_t (&h)[G]) {
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        const uint32_t a = max(rb[q], base), b = min(rb[q + 1], top);
+        for (uint32_t k = a; k < b; ++k) {
+            const uint64_t word = readlane64(nw, k - base);
+            const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
+            h[q] += lane_bit(word) ? w : 0;
+        }
+    }
+}
+
+// Local fields of positions [p0, p1) of one slot, lane = replica, K groups of
+// G consecutive positions at a time.  The K groups' CSR entries are gathered
+// together (lane e: coupling and neighbour word of a group's entry e, the
+// first 64 entries of each group; longer groups fetch the rest in place), so
+// K independent load chains are in flight before any is walked.  For every
+// position p, visit(p, x, h, aux) receives its word x (wave-uniform), its
+// field h = lin_p + sum_j q_pj x_j (per lane) and aux[p] (0 if aux is NULL).
+template <int G, int K, class Visit>
+__device__ __forceinline__ void fields_groups(const uint32_t* __restrict__ rp,
+                                              const uint32_t* __restrict__ ci,
+                                              const int32_t* __restrict__ wt,
+                                              const int32_t* __restrict__ lin,
+                                              const uint32_t* __restrict__ aux,
+                                              const uint64_t* S, uint32_t p0, uint32_t p1,
+                                              uint32_t lane, Visit&& visit) {
+    for (uint32_t g0 = p0; g0 < p1; g0 += G * K) {
+        uint32_t rb[K][G + 1], cnt[K];
+        uint64_t xw[K], nw[K];
+        int32_t hl[K], nwt[K];
+        uint32_t ax[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t g = min(p1, g0 + k * G);
+            cnt[k] = min((uint32_t)G, p1 - g);
+#pragma unroll
+            for (int q = 0; q <= G; ++q) rb[k][q] = rp[g + min((uint32_t)q, cnt[k])];
+            xw[k] = 0;
+            hl[k] = 0;
+            ax[k] = 0;
+            if (lane < cnt[k]) {
+                xw[k] = S[g + lane];
+                hl[k] = lin[g + lane];
+                if (aux) ax[k] = aux[g + lane];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t e = rb[k][0] + lane;
+            nw[k] = 0;
+            nwt[k] = 0;
+            if (e < rb[k][G]) {
+                nwt[k] = wt[e];
+                nw[k] = S[ci[e]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            int32_t h[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) h[q] = __builtin_amdgcn_readlane(hl[k], q);
+            walk_entries<G>(rb[k], rb[k][0], min(rb[k][0] + 64, rb[k][G]), nw[k], nwt[k], h);
+            for (uint32_t base = rb[k][0] + 64; base < rb[k][G]; base += 64) {
+                const uint32_t e = base + lane;
+                uint64_t w2 = 0;
+                int32_t t2 = 0;
+                if (e < rb[k][G]) {
+                    t2 = wt[e];
+                    w2 = S[ci[e]];
+                }
+                walk_entries<G>(rb[k], base, min(base + 64, rb[k][G]), w2, t2, h);
+            }
+            const uint32_t g = g0 + k * G;
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                if ((uint32_t)q >= cnt[k]) continue;
+                visit(g + q, readlane64(xw[k], q), h[q], __builtin_amdgcn_readlane(ax[k], q));
+            }
+        }
+    }
+}
+
 // Positions [p0, p1) of one colour class, group by group.
 template <class Acc>
 __device__ __forceinline__ void generic_positions(
