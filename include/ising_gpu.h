@@ -36,7 +36,9 @@
  *   ige_read_spins          SpinState::spin / extract_into
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:280-289
  *   ige_server_*            POST /api/v1/solve request loop
- *                                                      src/bin/server_api.rs:111-213
+ *                                                      src/bin/server_api.rs:111-213;
+ *                           ige_server_solution: the solution UltimateSolver::solve returns
+ *                                                      src/solver/ultimate.rs:299-374
  *   ige_batch_*             no single counterpart: many UltimateSolver-style anneals
  *                           (src/solver/ultimate.rs:234-336) in one launch
  *   ige_read_stats          Report{accepted,proposed} + StepScratch swap statistics
@@ -190,6 +192,12 @@ typedef struct {
     uint32_t sweeps;       /* anneal length                        */
     double beta_lo;        /* schedule of ige_anneal_schedule      */
     double beta_hi;
+    /* Serving only (a batch ignores both): incumbent update after every
+     * track_every-th sweep and after the last (UltimateSolver's update after
+     * each exchange round, ultimate.rs:329-335); 0 = none.  polish != 0:
+     * the solution is polished to a 1-opt local minimum (ultimate.rs:374). */
+    uint32_t track_every;
+    uint32_t polish;
 } ige_instance_desc;
 
 #define IGE_BATCH_PERSISTENT 0u /* one launch, device work queue           */
@@ -246,6 +254,9 @@ typedef struct {
     uint64_t accepted;
     double latency_ms;     /* submit -> result ready                        */
     double queue_ms;       /* submit -> admitted to the device              */
+    int64_t solution_energy; /* energy of the solution (ige_server_solution) */
+    uint32_t solution_sweep; /* sweeps done when the solution's state was seen */
+    uint32_t polish_flips;   /* flips of the 1-opt polish                   */
 } ige_request_info;
 
 typedef struct {
@@ -273,6 +284,12 @@ int32_t ige_server_poll(ige_server *server, uint64_t id, int32_t release, ige_re
                         uint64_t *words /* n, or NULL */, int64_t *energies /* 64, or NULL */);
 int32_t ige_server_wait(ige_server *server, uint64_t id, double timeout_ms,
                         ige_request_info *info);
+/* The request's solution as UltimateSolver::solve returns it
+ * (ultimate.rs:336-374): the best state seen at the incumbent updates
+ * (track_every > 0) or else the lowest-energy final replica (lowest lane on
+ * a tie), then 1-opt polished when the request asked for it.  x: n bytes in
+ * site order. */
+int32_t ige_server_solution(ige_server *server, uint64_t id, uint8_t *x, int64_t *energy);
 int32_t ige_server_stats(ige_server *server, ige_server_counters *stats);
 
 /* Per-launch timing of the sweep kernels with HIP events on the state's

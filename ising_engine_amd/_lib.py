@@ -41,7 +41,7 @@ EXPORTED = (
     "ige_batch_info", "ige_batch_read", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
     "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
-    "ige_read_best", "ige_model_polish",
+    "ige_read_best", "ige_model_polish", "ige_server_solution",
 )
 
 REQ_QUEUED = 0
@@ -75,7 +75,8 @@ class InstanceDesc(ctypes.Structure):
         ("n", c_uint32), ("offset", c_int64), ("linear", POINTER(c_int32)),
         ("npairs", c_uint64), ("pi", POINTER(c_uint32)), ("pj", POINTER(c_uint32)),
         ("q", POINTER(c_int32)), ("seed", c_uint64), ("sweeps", c_uint32),
-        ("beta_lo", c_double), ("beta_hi", c_double),
+        ("beta_lo", c_double), ("beta_hi", c_double), ("track_every", c_uint32),
+        ("polish", c_uint32),
     ]
 
 
@@ -83,7 +84,8 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [
         ("id", c_uint64), ("status", c_int32), ("n", c_uint32), ("best_energy", c_int64),
         ("best_replica", c_uint32), ("reserved", c_uint32), ("accepted", c_uint64),
-        ("latency_ms", c_double), ("queue_ms", c_double),
+        ("latency_ms", c_double), ("queue_ms", c_double), ("solution_energy", c_int64),
+        ("solution_sweep", c_uint32), ("polish_flips", c_uint32),
     ]
 
 
@@ -175,6 +177,8 @@ def load() -> ctypes.CDLL:
         "ige_steepest_descent": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
                                            POINTER(DescentReport)]),
         "ige_update_best": (c_int32, [vp]),
+        "ige_server_solution": (c_int32, [vp, c_uint64, POINTER(ctypes.c_uint8),
+                                          POINTER(c_int64)]),
         "ige_read_best": (c_int32, [vp, POINTER(ctypes.c_uint8), POINTER(c_int64),
                                     POINTER(c_uint32)]),
         "ige_model_polish": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
@@ -442,10 +446,13 @@ class Server:
     def configure(self, max_gang=16, gang_sites=2048, spin_limit=1 << 24):
         check(self._lib.ige_server_configure(self.handle, max_gang, gang_sites, spin_limit))
 
-    def submit(self, qb, seed, sweeps, beta_lo, beta_hi) -> int:
+    def submit(self, qb, seed, sweeps, beta_lo, beta_hi, track_every: int = 0,
+               polish: bool = False) -> int:
         d = InstanceDesc()
         keep = []
         _fill_desc(d, qb, seed, sweeps, beta_lo, beta_hi, keep)
+        d.track_every = int(track_every)
+        d.polish = 1 if polish else 0
         rid = c_uint64()
         check(self._lib.ige_server_submit(self.handle, ctypes.byref(d), ctypes.byref(rid)))
         self.sizes[rid.value] = qb.n
@@ -461,6 +468,14 @@ class Server:
         if release and info.status == REQ_DONE:
             self.sizes.pop(rid, None)
         return info, words, e
+
+    def solution(self, rid: int):
+        """(x bytes in site order, energy) of a finished request."""
+        x = np.empty(self.sizes[rid], dtype=np.uint8)
+        e = c_int64()
+        check(self._lib.ige_server_solution(self.handle, rid, _ptr(x, ctypes.c_uint8),
+                                            ctypes.byref(e)))
+        return x, e.value
 
     def wait(self, rid: int, timeout_ms: float = 60000.0):
         info = RequestInfo()

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "abi_util.hpp"
+#include "descent_host.hpp"
 #include "ige_math.h"
 #include "kernels.hpp"
 #include "problem_ir.hpp"
@@ -242,72 +243,25 @@ void track_best(ige_state* s) {
           "track best");
 }
 
-struct DescentResult {
-    uint32_t steps = 0;
-    uint32_t converged = 0;
-    uint64_t flips = 0;
-};
+using ige::DescentResult;
 
-// Steepest descent of every replica of `st` (spins and tracked energies in
-// place); flips[cells*64] receives one per flip.  dE comes from the
-// matrix-core fields on a dense model and from the CSR rows otherwise.
-// Steps are launched in growing batches; the host reads the control words
-// between batches and stops once a step has flipped nothing.
+// Steepest descent of every replica of `st`: dE from the matrix-core fields
+// on a dense model, from the CSR rows otherwise.
 DescentResult descend(const ige_model* model, const ige::DevState& st, const uint8_t* clamped,
                       uint32_t max_steps, uint32_t* flips, hipStream_t stream) {
     const uint32_t n = model->ir.n;
-    DevBuf<unsigned long long> keys, ctl;
-    keys.alloc(static_cast<size_t>(st.cells) * 64);
-    ctl.alloc(ige::kDescentCtlWords);
-    require(n < 0x80000000u, "descent needs n < 2^31");
-    DevBuf<uint8_t> cl;
-    DevBuf<uint32_t> aux;  // position -> site | clamped << 31
-    if (clamped) {
-        cl.alloc(n);
-        cl.upload(clamped, n, stream);
-        std::vector<uint32_t> a(n);
-        for (uint32_t p = 0; p < n; ++p) {
-            const uint32_t site = model->co.order[p];
-            a[p] = site | (clamped[site] ? 0x80000000u : 0u);
-        }
-        aux.alloc(n);
-        aux.upload(a.data(), n, stream);
-        check(hipStreamSynchronize(stream), "descent upload");
-    }
     DevBuf<int32_t> fields;
     const bool dense = model->ds.eligible && model->ds.dense &&
                        static_cast<uint64_t>(st.cells) * n * 64 * sizeof(int32_t) <= (1ull << 32);
+    ige::DevDense dv;
     if (dense) {
         model->ensure_dense();
+        dv = model->dense_view();
         fields.alloc(static_cast<size_t>(st.cells) * n * 64);
     }
-    check(ige::launch_descent_reset(keys.p, st.cells, ctl.p, stream), "descent reset");
-    unsigned long long c[ige::kDescentCtlWords] = {};
-    uint64_t launched = 0;
-    uint32_t batch = 1;
-    while (launched < max_steps) {
-        const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(batch, max_steps - launched));
-        for (uint32_t i = 0; i < k; ++i) {
-            if (dense)
-                check(ige::launch_dense_fields(model->dense_view(), st, fields.p, stream),
-                      "descent fields");
-            check(ige::launch_descent_step(model->view(), st, dense ? fields.p : nullptr,
-                                           model->pos_of.p, cl.p,
-                                           clamped ? aux.p : model->order.p, keys.p, flips,
-                                           ctl.p, stream),
-                  "descent step");
-        }
-        launched += k;
-        ctl.download(c, ige::kDescentCtlWords, stream);
-        check(hipStreamSynchronize(stream), "descent");
-        if (c[ige::kDescentDone]) break;
-        batch = std::min<uint32_t>(batch * 2, 64);
-    }
-    DescentResult r;
-    r.steps = static_cast<uint32_t>(c[ige::kDescentSteps]);
-    r.converged = c[ige::kDescentDone] ? 1u : 0u;
-    r.flips = c[ige::kDescentFlips];
-    return r;
+    return ige::run_descent(model->view(), model->pos_of.p, model->co.order.data(),
+                            dense ? &dv : nullptr, fields.p, st, clamped, max_steps, flips,
+                            stream);
 }
 
 void recompute_energy_into_state(ige_state* s) {

@@ -35,6 +35,7 @@ namespace ige {
 // Host-side description of one instance once lowered to position space.
 struct LoweredInstance {
     uint32_t n = 0, ncol = 0, sweeps = 0, dtab = 0;
+    uint32_t track_every = 0, polish = 0;  // serving: incumbent cadence, 1-opt polish
     uint64_t seed = 0;
     double beta_hi = 0.0, c = 0.0;
     int64_t offset = 0;
@@ -44,6 +45,7 @@ struct LoweredInstance {
     std::vector<int32_t> lin;
     std::vector<uint32_t> cs;   // ncol+1
     std::vector<uint32_t> order;  // position -> site
+    std::vector<uint32_t> pos_of; // site -> position
     uint64_t work() const { return (static_cast<uint64_t>(n) + ci.size()) * sweeps; }
 };
 
@@ -86,6 +88,9 @@ inline LoweredInstance lower_instance(const ige_instance_desc& d, const std::str
     }
     L.cs = co.class_start;
     L.order = co.order;
+    L.pos_of = co.pos_of;
+    L.track_every = d.track_every;
+    L.polish = d.polish;
     return L;
 }
 

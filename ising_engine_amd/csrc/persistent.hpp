@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ige_math.h"
+#include "kernels.hpp"
 #include "sweep_generic.h"
 
 namespace ige {
@@ -37,6 +38,9 @@ struct InstDesc {
     uint32_t gang;         // workgroups cooperating on this instance (1..kMaxGang)
     uint32_t* bar;         // gang barrier counter, zero at launch start (gang > 1)
     uint64_t* ticks;       // if set: wall-clock ticks member 0 spent on this launch's sweeps
+    BestRecord* best;      // if set with track_every: the best-seen replica's record
+    uint8_t* best_x;       // n bytes, position order: its spins
+    uint32_t track_every;  // incumbent update after every track_every-th sweep and the last
     double beta_hi;
     double c;  // beta_k = beta_hi * det_exp(-(sweeps - 1 - k) * c)
     int64_t offset;
@@ -90,6 +94,74 @@ __device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, ui
     }
     __syncthreads();
     __threadfence();
+}
+
+// Incumbent update of UltimateSolver::update_incumbent (src/solver/ultimate.rs:397-431)
+// after `sweep` sweeps, for one member of the instance's gang.  The member's
+// energy changes are published first, so every member then reads the same 64
+// energies and takes the same decision: lowest energy, lowest lane on a tie,
+// replacing the record only when strictly lower.  A barrier separates every
+// member's read of the record from member 0's write.  Each member copies the
+// winning lane's bits of exactly the positions it updates in a sweep, so no
+// other member can change them first.
+template <int WAVES>
+__device__ __forceinline__ void track_point(const InstDesc& I, uint32_t sweep, uint32_t member,
+                                            uint32_t gang, uint32_t lane, uint32_t wave,
+                                            int64_t& eacc, int64_t (*part)[64],
+                                            uint32_t& bar_target, uint32_t spin_limit,
+                                            uint32_t* err) {
+    part[wave][lane] = eacc;
+    eacc = 0;
+    __syncthreads();
+    if (wave == 0) {
+        int64_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) tot += part[w][lane];
+        if (gang > 1)
+            atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + lane),
+                      (unsigned long long)tot);
+        else
+            I.energy[lane] += tot;
+    }
+    if (gang > 1)
+        gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+    else
+        __syncthreads();
+    int64_t e = I.energy[lane];
+    uint32_t l = lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t elo = __shfl_xor((uint32_t)e, off), ehi = __shfl_xor((uint32_t)(e >> 32), off);
+        const int64_t e2 = (int64_t)(((uint64_t)ehi << 32) | elo);
+        const uint32_t l2 = __shfl_xor(l, off);
+        if (e2 < e || (e2 == e && l2 < l)) {
+            e = e2;
+            l = l2;
+        }
+    }
+    const bool take = !I.best->valid || e < I.best->energy;
+    if (gang > 1)
+        gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+    else
+        __syncthreads();
+    if (!take) return;
+    const uint32_t stride = gang * WAVES;
+    const uint32_t lane_of_gang = member * WAVES + wave;
+    for (uint32_t c = 0; c < I.ncol; ++c) {
+        const uint32_t cb = I.cs[c], ce = I.cs[c + 1];
+        if (ce <= cb) continue;
+        for (uint32_t g = (cb >> 2) + lane_of_gang; g <= ((ce - 1) >> 2); g += stride) {
+            const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
+            if (lane < p1 - p0) I.best_x[p0 + lane] = (uint8_t)((I.spins[p0 + lane] >> l) & 1u);
+        }
+    }
+    if (member == 0 && threadIdx.x == 0) {
+        I.best->energy = e;
+        I.best->valid = 1;
+        I.best->sweep = sweep;
+        I.best->temp = 0;
+        I.best->lane = l;
+    }
 }
 
 // queue[t] = instance << 5 | member.  A gang's tickets are consecutive, so at
@@ -151,6 +223,9 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
                 else
                     __syncthreads();
             }
+            if (I.track_every && ((k + 1) % I.track_every == 0 || k + 1 == I.sweeps))
+                track_point<WAVES>(I, k + 1, member, gang, lane, wave, eacc, part, bar_target,
+                                   spin_limit, err);
         }
         part[wave][lane] = eacc;
         // nacc is already the wave's total (a popcount of the accept ballot).
@@ -189,6 +264,7 @@ static __global__ __launch_bounds__(256) void instance_init_kernel(const InstDes
     if (threadIdx.x == 0) {
         *I.progress = 0;
         *I.accepted = 0;
+        if (I.best) I.best->valid = 0;
     }
 }
 

@@ -8,8 +8,10 @@
 //   2. run one persistent launch in which every live request advances by its
 //      own number of sweeps, sized so that a round takes about the target
 //      round time whatever the request's size;
-//   3. retire finished requests: copy the best replica, the 64 energies and
-//      the counters back and free their arena space.
+//   3. retire finished requests: copy the final replicas, their energies and
+//      the counters back, form the solution (best seen at the incumbent
+//      updates, or the best final replica; 1-opt polished on request) and
+//      free their arena space.
 // Requests that arrive while others run are admitted at the next round, so
 // their work overlaps on the device with the requests already there.  The
 // uniforms are keyed by (request seed, position, sweep), so a request's
@@ -33,6 +35,7 @@
 #include <vector>
 
 #include "batch_internal.hpp"
+#include "descent_host.hpp"
 
 using ige::abi::DevBuf;
 using ige::abi::guarded;
@@ -94,8 +97,10 @@ class ArenaAllocator {
 };
 
 // Byte layout of one request inside the arena; each region 256-B aligned.
+// The model regions (rp .. pos_of) come first and are uploaded together.
 struct Layout {
-    uint64_t rp, ci, wt, lin, cs, order, spins, energy, accepted, progress, total;
+    uint64_t rp, ci, wt, lin, cs, order, pos_of, spins, energy, accepted, progress, best, best_x,
+        total;
     explicit Layout(const ige::LoweredInstance& L) {
         uint64_t o = 0;
         auto take = [&](uint64_t bytes) {
@@ -109,10 +114,13 @@ struct Layout {
         lin = take(L.n * 4ull);
         cs = take(L.cs.size() * 4ull);
         order = take(L.n * 4ull);
+        pos_of = take(L.n * 4ull);
         spins = take(L.n * 8ull);
         energy = take(64 * 8);
         accepted = take(8);
         progress = take(4);
+        best = take(sizeof(ige::BestRecord));
+        best_x = take(L.n);
         total = o;
     }
 };
@@ -132,6 +140,9 @@ struct Request {
     std::vector<uint64_t> words;   // result, site order
     std::vector<int64_t> energies;
     uint64_t accepted = 0;
+    std::vector<uint8_t> solution;  // UltimateSolver-style solution, site order
+    int64_t solution_energy = 0;
+    uint32_t solution_sweep = 0, polish_flips = 0;
 };
 
 }  // namespace
@@ -155,6 +166,10 @@ struct ige_server {
     double ticks_per_ms = 1.0e5;  // device wall clock (hipDeviceAttributeWallClockRate)
     DevBuf<const uint32_t*> d_orders;
     DevBuf<unsigned int> d_next;
+    // polish scratch: one cell of words, its 64 energies and flip counts
+    DevBuf<uint64_t> d_pol_words;
+    DevBuf<int64_t> d_pol_energy;
+    DevBuf<uint32_t> d_pol_flips;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -201,6 +216,7 @@ void upload_request(ige_server* s, Request& r) {
     put(lay.lin, L.lin.data(), L.lin.size() * 4);
     put(lay.cs, L.cs.data(), L.cs.size() * 4);
     put(lay.order, L.order.data(), L.order.size() * 4);
+    put(lay.pos_of, L.pos_of.data(), L.pos_of.size() * 4);
     uint8_t* base = s->d_arena + r.off;
     check(hipMemcpyAsync(base, r.staging.data(), r.staging.size(), hipMemcpyHostToDevice,
                          s->stream),
@@ -215,6 +231,9 @@ void upload_request(ige_server* s, Request& r) {
     D.energy = reinterpret_cast<int64_t*>(base + lay.energy);
     D.accepted = reinterpret_cast<unsigned long long*>(base + lay.accepted);
     D.progress = reinterpret_cast<uint32_t*>(base + lay.progress);
+    D.best = reinterpret_cast<ige::BestRecord*>(base + lay.best);
+    D.best_x = base + lay.best_x;
+    D.track_every = L.track_every;
     D.thr_mode = ige::choose_thr_mode(L.dtab);
     D.n = L.n;
     D.ncol = L.ncol;
@@ -249,6 +268,50 @@ void plan_gangs(ige_server* s) {
     }
 }
 
+// 1-opt polish of one state (bytes, position order) on the request's own CSR
+// in the arena.  Every lane of a scratch cell holds the state, so all lanes
+// follow the same deterministic path and the run ends when lane 0's does.
+void polish_state(ige_server* s, Request& r, std::vector<uint8_t>& x, int64_t& energy) {
+    const ige::LoweredInstance& L = r.L;
+    const Layout lay(L);
+    uint8_t* base = s->d_arena + r.off;
+    if (s->d_pol_words.count < L.n) s->d_pol_words.alloc(std::max<size_t>(L.n, 1024));
+    if (!s->d_pol_energy.p) {
+        s->d_pol_energy.alloc(64);
+        s->d_pol_flips.alloc(64);
+    }
+    std::vector<uint64_t> w(L.n);
+    for (uint32_t p = 0; p < L.n; ++p) w[p] = x[p] ? ~0ull : 0ull;
+    s->d_pol_words.upload(w.data(), L.n, s->stream);
+    std::vector<uint32_t> zeros(64, 0);
+    s->d_pol_flips.upload(zeros.data(), 64, s->stream);
+    ige::DevModel m;
+    m.n = L.n;
+    m.rp = r.D.rp;
+    m.ci = r.D.ci;
+    m.wt = r.D.wt;
+    m.lin = r.D.lin;
+    m.order = reinterpret_cast<const uint32_t*>(base + lay.order);
+    m.offset = L.offset;
+    ige::DevState st;
+    st.cells = 1;
+    st.spins = s->d_pol_words.p;
+    st.energy = s->d_pol_energy.p;
+    check(ige::launch_energy(m, st, st.energy, s->stream), "polish energy");
+    (void)ige::run_descent(
+        m, reinterpret_cast<const uint32_t*>(base + lay.pos_of), L.order.data(), nullptr, nullptr,
+        st, nullptr, IGE_STEPS_UNLIMITED, s->d_pol_flips.p, s->stream);
+    int64_t e0 = 0;
+    uint32_t f0 = 0;
+    s->d_pol_words.download(w.data(), L.n, s->stream);
+    check(hipMemcpyAsync(&e0, st.energy, 8, hipMemcpyDeviceToHost, s->stream), "polish energy");
+    check(hipMemcpyAsync(&f0, s->d_pol_flips.p, 4, hipMemcpyDeviceToHost, s->stream), "flips");
+    check(hipStreamSynchronize(s->stream), "polish");
+    for (uint32_t p = 0; p < L.n; ++p) x[p] = static_cast<uint8_t>(w[p] & 1u);
+    energy = e0;
+    r.polish_flips = f0;
+}
+
 void retire(ige_server* s, Request& r) {
     const ige::LoweredInstance& L = r.L;
     std::vector<uint64_t> w(L.n);
@@ -259,10 +322,36 @@ void retire(ige_server* s, Request& r) {
           "result energies");
     unsigned long long acc = 0;
     check(hipMemcpyAsync(&acc, r.D.accepted, 8, hipMemcpyDeviceToHost, s->stream), "accepted");
+    ige::BestRecord best{};
+    std::vector<uint8_t> x(L.n);
+    if (L.track_every) {
+        check(hipMemcpyAsync(&best, r.D.best, sizeof(best), hipMemcpyDeviceToHost, s->stream),
+              "best record");
+        check(hipMemcpyAsync(x.data(), r.D.best_x, L.n, hipMemcpyDeviceToHost, s->stream),
+              "best state");
+    }
     check(hipStreamSynchronize(s->stream), "retire");
     r.words.assign(L.n, 0);
     for (uint32_t p = 0; p < L.n; ++p) r.words[L.order[p]] = w[p];
     r.accepted = acc;
+    // the solution: best seen at the incumbent updates, else the best final replica
+    int64_t e = 0;
+    if (L.track_every) {
+        if (!best.valid) throw Failure(IGE_ERR_HIP, "incumbent record missing after the anneal");
+        e = best.energy;
+        r.solution_sweep = best.sweep;
+    } else {
+        uint32_t lane = 0;
+        for (uint32_t i = 1; i < 64; ++i)
+            if (r.energies[i] < r.energies[lane]) lane = i;
+        for (uint32_t p = 0; p < L.n; ++p) x[p] = static_cast<uint8_t>((w[p] >> lane) & 1u);
+        e = r.energies[lane];
+        r.solution_sweep = L.sweeps;
+    }
+    if (L.polish) polish_state(s, r, x, e);
+    r.solution.assign(L.n, 0);
+    for (uint32_t p = 0; p < L.n; ++p) r.solution[L.order[p]] = x[p];
+    r.solution_energy = e;
     r.staging.clear();
     r.staging.shrink_to_fit();
 }
@@ -470,6 +559,9 @@ void fill_info(const Request& r, ige_request_info* info) {
         info->accepted = r.accepted;
         info->latency_ms = ms_between(r.t_submit, r.t_done);
         info->queue_ms = ms_between(r.t_submit, r.t_admit);
+        info->solution_energy = r.solution_energy;
+        info->solution_sweep = r.solution_sweep;
+        info->polish_flips = r.polish_flips;
     }
 }
 
@@ -575,6 +667,19 @@ int32_t ige_server_poll(ige_server* s, uint64_t id, int32_t release, ige_request
             if (energies) std::memcpy(energies, r.energies.data(), 64 * 8);
             if (release) s->reqs.erase(it);
         }
+    });
+}
+
+int32_t ige_server_solution(ige_server* s, uint64_t id, uint8_t* x, int64_t* energy) {
+    return guarded([&] {
+        require(s && energy, "NULL argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->reqs.find(id);
+        require(it != s->reqs.end(), "unknown request id");
+        const Request& r = *it->second;
+        require(r.status == IGE_REQ_DONE, "request not done");
+        if (x) std::memcpy(x, r.solution.data(), r.solution.size());
+        *energy = r.solution_energy;
     });
 }
 

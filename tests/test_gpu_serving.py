@@ -109,3 +109,56 @@ def test_gangs_in_the_serving_loop(ctx):
         np.testing.assert_array_equal(got[1], ref[i][1])
         np.testing.assert_array_equal(got[2], ref[i][2])
     gang.close()
+
+
+def reference_solution(ctx, r, track_every, polish):
+    """UltimateSolver-style solution from a one-cell state: the anneal in chunks ending at
+    every multiple of track_every and at the last sweep, an incumbent update after each chunk
+    (or the best final replica), then the 1-opt polish of the model API."""
+    qb = r["qubo"]
+    m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+    st = _lib.State(m, 1, r["seed"], kernel=_lib.KERNEL_GENERIC)
+    sched = _lib.anneal_schedule(r["beta_lo"], r["beta_hi"], r["sweeps"])
+    S = r["sweeps"]
+    if track_every:
+        k = 0
+        while k < S:
+            step = min(track_every - k % track_every, S - k)
+            st.run(step, beta_schedule=sched[k:k + step, None])
+            k += step
+            st.update_best()
+        x, e, sw = st.best()
+    else:
+        st.run(S, beta_schedule=sched[:, None])
+        words, es = st.spins()[0], st.energies()[0]
+        lane = int(np.argmin(es))
+        x = ((words >> np.uint64(lane)) & np.uint64(1)).astype(np.uint8)
+        e, sw = int(es[lane]), S
+    flips = 0
+    if polish:
+        xp, ep, fp = m.polish(x)
+        x, e, flips = xp[0], int(ep[0]), int(fp[0])
+    return x, e, sw, flips
+
+
+@pytest.mark.parametrize("gangs", [False, True], ids=["one_workgroup", "gangs"])
+def test_solution_is_best_seen_then_polished(ctx, gangs):
+    reqs = requests(6, 7, n_lo=200, n_hi=3000, sweeps_lo=20, sweeps_hi=70)
+    modes = [(1, True), (7, True), (25, False), (0, True), (10, True), (0, False)]
+    srv = _lib.Server(ctx, hbm_bytes=256 << 20, round_ms=1.0)
+    if gangs:
+        srv.configure(max_gang=4, gang_sites=256)
+    print(f"[serving solution] gangs={gangs}", flush=True)
+    ids = [srv.submit(r["qubo"], r["seed"], r["sweeps"], r["beta_lo"], r["beta_hi"],
+                      track_every=t, polish=p) for r, (t, p) in zip(reqs, modes)]
+    for r, (t, p), rid in zip(reqs, modes, ids):
+        info = srv.wait(rid, 120000.0)
+        assert info.status == _lib.REQ_DONE, info.status
+        x, e = srv.solution(rid)
+        xr, er, swr, fr = reference_solution(ctx, r, t, p)
+        np.testing.assert_array_equal(x, xr)
+        assert e == er == info.solution_energy == int(r["qubo"].energy(x))
+        assert info.solution_sweep == swr and info.polish_flips == fr
+        if t:
+            assert e <= info.best_energy  # best seen is never worse than the final best
+    srv.close()
