@@ -352,9 +352,13 @@ def bench_batch(_lib, instances, ctx, count: int) -> dict:
             "ratio_vs_plain": persistent / plain, "waves_per_instance": waves}
 
 
-def bench_serving(_lib, instances, ctx, rate: float, count: int) -> dict:
+def bench_serving(_lib, instances, ctx, rate: float, count: int, track_every: int = 100,
+                  polish: bool = True) -> dict:
     """Poisson stream of mixed Ising/QUBO solve requests (N log-uniform 128..16384, degree
-    4..16, weights in [-1000, 1000], 500..5000 sweeps) into the continuous-batching loop."""
+    4..16, weights in [-1000, 1000], 500..5000 sweeps) into the continuous-batching loop.
+    Each request returns what UltimateSolver::solve returns: the best state seen at an
+    incumbent update every `track_every` sweeps (the reference's default sweeps per
+    exchange round), 1-opt polished."""
     rng = np.random.default_rng(11)
     reqs = [instances.serving_request(rng) for _ in range(count)]
     srv = _lib.Server(ctx, hbm_bytes=16 << 30, round_ms=10.0)
@@ -364,8 +368,17 @@ def bench_serving(_lib, instances, ctx, rate: float, count: int) -> dict:
     for a, r in zip(arrivals, reqs):
         while time.perf_counter() - t0 < a:
             time.sleep(0.0005)
-        ids.append(srv.submit(r["qubo"], r["seed"], r["sweeps"], r["beta_lo"], r["beta_hi"]))
-    lat = np.array([srv.wait(rid, 600000.0).latency_ms for rid in ids])
+        ids.append(srv.submit(r["qubo"], r["seed"], r["sweeps"], r["beta_lo"], r["beta_hi"],
+                              track_every=track_every, polish=polish))
+    infos = [srv.wait(rid, 600000.0) for rid in ids]
+    if any(i.status != _lib.REQ_DONE for i in infos):
+        raise SystemExit("a serving request did not complete")
+    lat = np.array([i.latency_ms for i in infos])
+    # the solution is never worse than the final best replica, and its energy is exact
+    for k in (0, len(ids) // 2, len(ids) - 1):
+        x, e = srv.solution(ids[k])
+        if e != int(reqs[k]["qubo"].energy(x)) or e > infos[k].best_energy:
+            raise SystemExit("serving solution energy check failed")
     wall = time.perf_counter() - t0
     st = srv.stats()
     attempts = sum(r["qubo"].n * 64 * r["sweeps"] for r in reqs)
@@ -373,7 +386,10 @@ def bench_serving(_lib, instances, ctx, rate: float, count: int) -> dict:
     out = {"rate_req_s": rate, "requests": count, "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
            "attempts_per_s": attempts / wall, "wall_s": wall, "rounds": int(st.rounds),
-           "max_live": int(st.max_live)}
+           "max_live": int(st.max_live), "track_every": track_every, "polish": polish,
+           "mean_polish_flips": float(np.mean([i.polish_flips for i in infos])),
+           "mean_solution_gain": float(np.mean([i.best_energy - i.solution_energy
+                                                for i in infos]))}
     log(f"[bench] serving {rate}/s x {count}: p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms")
     return out
 

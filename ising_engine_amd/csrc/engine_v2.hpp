@@ -413,6 +413,37 @@ class GpuMetropolisPt : public Operator {
     }
 };
 
+// steepest_descent.rs:80-124 on the device: per budget sweep, every replica
+// flips its most improving site (lowest site on a tie); stops early once no
+// replica can improve.  Monotone and deterministic; needs no temperature.
+class GpuSteepestDescent : public Operator {
+   public:
+    static constexpr const char* NAME = "gpu_steepest_descent";
+    OperatorDescriptor descriptor() const override {
+        OperatorDescriptor d;
+        d.name = NAME;
+        d.capabilities = CapabilitySet::empty().with(Capability::Exploitation);
+        d.constraints = Constraints{true, false, true, false, false, false, true};
+        d.observables = ObservableSet{}.with(Observable::Acceptance).with(Observable::EnergyDelta);
+        d.guarantees.monotone = true;
+        d.guarantees.deterministic = true;
+        d.complexity = Complexity::LinearEdges;
+        return d;
+    }
+    CostEstimate cost_model(InstanceShape s) const override {
+        return {static_cast<double>(s.n * std::max<size_t>(s.num_replicas, 1))};
+    }
+    Report apply(GpuBitSlice& state, const RuntimeView&, uint64_t&, Budget budget) override {
+        ige_descent_report rep{};
+        check(ige_steepest_descent(state.device(), budget.sweeps, nullptr, &rep));
+        Report r;
+        r.accepted = rep.flips;
+        r.proposed = rep.proposed;
+        r.work = static_cast<double>(budget.sweeps) * state.num_vars() * state.num_replicas();
+        return r;
+    }
+};
+
 // ---- registry.rs ---------------------------------------------------------
 class OperatorRegistry {
    public:
@@ -424,6 +455,7 @@ class OperatorRegistry {
         reg.register_factory([] { return std::make_unique<GpuMetropolisSweep>(); });
         reg.register_factory([] { return std::make_unique<GpuReplicaExchange>(); });
         reg.register_factory([] { return std::make_unique<GpuMetropolisPt>(); });
+        reg.register_factory([] { return std::make_unique<GpuSteepestDescent>(); });
         return reg;
     }
     // registry.rs:70-82: the descriptor is read once, up front.

@@ -36,8 +36,9 @@ struct Mock : Operator {
 
 static void registry_tests() {
     OperatorRegistry reg = OperatorRegistry::standard_gpu();
-    CHECK(reg.len() == 3);
+    CHECK(reg.len() == 4);
     CHECK(reg.contains("gpu_metropolis_sweep"));
+    CHECK(reg.contains("gpu_steepest_descent"));
     CHECK(reg.contains("gpu_replica_exchange"));
     CHECK(reg.contains("gpu_metropolis_pt"));
     CHECK(reg.lookup("nope") == nullptr);
@@ -45,7 +46,7 @@ static void registry_tests() {
     CHECK(op && op->name() == "gpu_metropolis_pt");
     CHECK(op->nature() == Nature::TrajectoryPreserving);
     reg.register_factory([] { return std::make_unique<Mock>(); });
-    CHECK(reg.len() == 4);
+    CHECK(reg.len() == 5);
     const OperatorDescriptor* md = reg.metadata("mock");
     CHECK(md && md->capabilities.contains(Capability::Warmstart));
 
@@ -57,7 +58,7 @@ static void registry_tests() {
     CapabilityQuery g;
     g.backend = Backend::GpuBitSlice;
     found = reg.find(g);
-    CHECK(found.size() == 3 && std::is_sorted(found.begin(), found.end()));
+    CHECK(found.size() == 4 && std::is_sorted(found.begin(), found.end()));
     CapabilityQuery cpu;
     cpu.backend = Backend::SparseBitSlice;
     CHECK((reg.find(cpu) == std::vector<std::string>{"mock"}));
@@ -69,11 +70,15 @@ static void registry_tests() {
     CapabilityQuery single;  // one replica: exchange needs several
     single.multi_replica = false;
     single.backend = Backend::GpuBitSlice;
-    CHECK((reg.find(single) == std::vector<std::string>{"gpu_metropolis_sweep"}));
+    CHECK((reg.find(single) ==
+           std::vector<std::string>{"gpu_metropolis_sweep", "gpu_steepest_descent"}));
+    const OperatorDescriptor* sd = reg.metadata("gpu_steepest_descent");
+    CHECK(sd && sd->guarantees.monotone && sd->guarantees.deterministic &&
+          !sd->constraints.needs_temperature);
 
     CapabilityQuery any;
     any.any_of = CapabilitySet::empty().with(Capability::Exploitation).with(Capability::Warmstart);
-    CHECK(reg.find(any).size() == 3);
+    CHECK(reg.find(any).size() == 4);
 }
 
 // 3D +-J L=4 periodic, Ising lowered to QUBO (pair -4J, linear 2 sum J).
@@ -159,6 +164,18 @@ static void gpu_tests() {
     CHECK(a.audit() == 0.0);
     ige_stats st = a.stats();
     CHECK(st.sweeps == 70 && st.exchanges == 46);
+
+    // Steepest descent (steepest_descent.rs:80-124): monotone per replica, exact,
+    // and once at a local minimum a step flips nothing and ends the call.
+    std::vector<double> e0(a.num_replicas()), e1(a.num_replicas());
+    a.energies_into(e0.data());
+    auto sd = reg.lookup("gpu_steepest_descent");
+    Report rd = sd->apply(a, view, rng, Budget{1000});
+    a.energies_into(e1.data());
+    for (size_t r = 0; r < e0.size(); ++r) CHECK(e1[r] <= e0[r]);
+    CHECK(rd.accepted > 0 && a.audit() == 0.0);
+    Report rz = sd->apply(a, view, rng, Budget{5});
+    CHECK(rz.accepted == 0 && rz.proposed == 64ull * 64 * 4);
 
     // A wrong-sized ladder is an error, not a silent fallback.
     RuntimeView bad;
