@@ -24,6 +24,8 @@
  *                                                      src/solver/ultimate.rs:271-275
  *   ige_run                 MetropolisSweep::apply     src/engine_v2/operators/metropolis_sweep.rs:95-146
  *                           engine::step (sweep + DEO) src/solver/engine.rs:573-819
+ *                           with IGE_RUN_HEATBATH: GibbsColorSweep::apply
+ *                                                      src/engine_v2/operators/gibbs_color_sweep.rs:119-174
  *   ige_exchange            ReplicaExchange::apply     src/engine_v2/operators/replica_exchange.rs:61-109
  *                           (DEO schedule of engine.rs:713-818)
  *   ige_read_energies       SpinState::energies_into   src/engine_v2/backends/sparse_bitslice.rs:273-278
@@ -76,6 +78,9 @@ extern "C" {
 #define IGE_RUN_EXCHANGE 1u   /* DEO replica exchange after every sweep     */
 #define IGE_RUN_TRACK_BEST 2u /* incumbent update after every sweep (and its
                                  exchange), see ige_update_best             */
+#define IGE_RUN_HEATBATH 4u   /* heat-bath (Gibbs) sweeps instead of Metropolis:
+                                 new spin = 1 with P = 1/(1+exp(beta h)),
+                                 lane-per-replica kernel on any state       */
 
 #define IGE_STEPS_UNLIMITED 0xFFFFFFFFu /* descend until no replica can improve */
 

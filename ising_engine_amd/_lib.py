@@ -27,6 +27,7 @@ KERNEL_MULTISPIN = 2
 KERNEL_DENSE = 3
 RUN_EXCHANGE = 1
 RUN_TRACK_BEST = 2
+RUN_HEATBATH = 4
 STEPS_UNLIMITED = 0xFFFFFFFF
 
 EXPORTED = (
@@ -310,14 +311,15 @@ class State:
         check(self._lib.ige_set_ladder(self.handle, _ptr(b, c_double)))
 
     def run(self, sweeps: int, exchange: bool = False, beta_schedule=None,
-            track_best: bool = False) -> None:
+            track_best: bool = False, heatbath: bool = False) -> None:
         sched = None
         if beta_schedule is not None:
             sched = np.ascontiguousarray(beta_schedule, dtype=np.float64)
             if sched.shape != (sweeps, self.cells):
                 raise ValueError("beta_schedule must be [sweeps][cells]")
         self._sched_keepalive = sched
-        flags = (RUN_EXCHANGE if exchange else 0) | (RUN_TRACK_BEST if track_best else 0)
+        flags = ((RUN_EXCHANGE if exchange else 0) | (RUN_TRACK_BEST if track_best else 0) |
+                 (RUN_HEATBATH if heatbath else 0))
         check(self._lib.ige_run(self.handle, int(sweeps), flags, _ptr(sched, c_double)))
 
     def steepest_descent(self, max_steps: int = STEPS_UNLIMITED, clamped=None) -> DescentReport:

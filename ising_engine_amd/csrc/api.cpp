@@ -130,6 +130,7 @@ struct ige_state {
     DevBuf<uint32_t> temp_of, slot_of, thr;
     DevBuf<unsigned long long> counters, qplanes;  // counters: accepted, then pair stats
     DevBuf<double> beta;                             // current ladder (+ schedule rows)
+    DevBuf<uint64_t> hb;                             // heat-bath thresholds, on first use
     DevBuf<int32_t> fields;
     DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
     DevBuf<uint8_t> best_x;        // n bytes, site order
@@ -432,7 +433,8 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
     return guarded([&] {
         require(s != nullptr, "NULL state");
         require(sched != nullptr || s->ladder_set, "no ladder set and no beta schedule given");
-        require((flags & ~(IGE_RUN_EXCHANGE | IGE_RUN_TRACK_BEST)) == 0, "unknown flag");
+        require((flags & ~(IGE_RUN_EXCHANGE | IGE_RUN_TRACK_BEST | IGE_RUN_HEATBATH)) == 0,
+                "unknown flag");
         require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull &&
                     static_cast<uint64_t>(s->xround) + sweeps <= 0xFFFFFFFFull,
                 "sweep or exchange counter would exceed 2^32-1");
@@ -440,6 +442,9 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         const bool exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
         const bool track = (flags & IGE_RUN_TRACK_BEST) != 0;
         if (track) ensure_best(s);
+        const bool heat = (flags & IGE_RUN_HEATBATH) != 0;
+        const uint32_t hb_dtab = static_cast<uint32_t>(s->model->max_local);
+        if (heat && !s->hb.p) s->hb.alloc(static_cast<size_t>(s->cells) * (2ull * hb_dtab + 1));
         DevBuf<double> sched_dev;
         if (sched) {
             const size_t total = static_cast<size_t>(sweeps) * s->cells;
@@ -456,17 +461,28 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             const double* beta = sched ? sched_dev.p + static_cast<size_t>(k) * s->cells : s->beta.p;
             // Thresholds depend on beta and, for the multispin planes, on
             // which temperature each (slot, lane) holds after the last swap.
-            if (sched || (s->kernel == IGE_KERNEL_MULTISPIN && exch && k > 0))
+            // Heat-bath tables are indexed by temperature: only beta matters.
+            if (heat) {
+                if (sched || k == 0)
+                    check(ige::launch_heatbath_table(beta, s->hb.p, s->cells, hb_dtab, s->stream),
+                          "heat-bath table");
+            } else if (sched || (s->kernel == IGE_KERNEL_MULTISPIN && exch && k > 0)) {
                 refresh_thresholds(s, beta);
+            }
             if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
             const uint32_t launches =
-                s->kernel == IGE_KERNEL_DENSE ? 1u : s->model->co.num_colors;
-            if (s->kernel == IGE_KERNEL_DENSE)
+                (s->kernel == IGE_KERNEL_DENSE && !heat) ? 1u : s->model->co.num_colors;
+            if (heat) {
+                for (uint32_t c = 0; c < s->model->co.num_colors; ++c)
+                    check(ige::launch_sweep_heatbath(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
+                                                     s->hb.p, hb_dtab, s->stream),
+                          "sweep_heatbath");
+            } else if (s->kernel == IGE_KERNEL_DENSE)
                 check(ige::launch_sweep_dense(s->model->dense_view(), st, s->sweep, s->seed, beta,
                                               s->stream),
                       "sweep_dense");
-            for (uint32_t c = 0; s->kernel != IGE_KERNEL_DENSE && c < s->model->co.num_colors;
-                 ++c) {
+            for (uint32_t c = 0;
+                 !heat && s->kernel != IGE_KERNEL_DENSE && c < s->model->co.num_colors; ++c) {
                 if (s->kernel == IGE_KERNEL_GENERIC)
                     check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
                                                     s->thr.p, s->dtab, s->stream),

@@ -102,6 +102,17 @@ IGE_HD uint32_t threshold_limit(double beta, uint32_t dtab) {
     return lim >= (double)dtab ? dtab : ((uint32_t)lim + 2u < dtab ? (uint32_t)lim + 2u : dtab);
 }
 
+// Heat-bath threshold: the new spin of a site with local field h is 1 iff
+// u < thr, thr = floor(2^32 P) in [0, 2^32] with P = 1 / (1 + exp(beta h)),
+// exp evaluated only at -beta |h| <= 0 (P = e / (1 + e) for h >= 0, 1 / (1 + e)
+// for h < 0, e = exp(-beta |h|)).  beta -> inf gives 0, 2^31 or 2^32.
+IGE_HD uint64_t heatbath_threshold(double beta, int64_t h) {
+    const double a = beta * (double)(h < 0 ? -h : h);
+    const double e = det_exp(-a);
+    const double p = h >= 0 ? e / (1.0 + e) : 1.0 / (1.0 + e);
+    return (uint64_t)(p * 4294967296.0);
+}
+
 // Replica-exchange threshold for delta = (b_t - b_{t+1}) (E_t - E_{t+1}) < 0.
 IGE_HD uint32_t exchange_threshold(double delta) { return prob_to_u32(det_exp(delta)); }
 

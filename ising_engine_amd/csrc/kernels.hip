@@ -89,6 +89,63 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Heat-bath sweep (gibbs_color_sweep.rs:119-174), lane = replica, same
+// mapping and uniforms as sweep_generic_kernel: the new spin is
+// u < hb[temperature][h + dtab] with h the local field, and the lane flips
+// when that differs from the current spin.  hb: [cells][2 dtab + 1] u64
+// thresholds in [0, 2^32] (launch_heatbath_table).
+// ---------------------------------------------------------------------------
+__global__ void heatbath_table_kernel(const double* __restrict__ beta, uint64_t* __restrict__ hb,
+                                      uint32_t cells, uint32_t dtab) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t row = 2ull * dtab + 1;
+    if (i >= row * cells) return;
+    const uint32_t t = (uint32_t)(i / row);
+    const int64_t h = (int64_t)(i % row) - (int64_t)dtab;
+    hb[i] = heatbath_threshold(beta[t], h);
+}
+
+template <bool kLdsThr>
+__global__ __launch_bounds__(kBlock) void sweep_heatbath_kernel(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
+    const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr, uint32_t n, uint32_t cells,
+    const uint32_t* __restrict__ temp_of, const uint64_t* __restrict__ hb, uint32_t dtab,
+    int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
+    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    extern __shared__ uint64_t lds_hb[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t row = 2 * dtab + 1;
+    if (kLdsThr) {
+        const uint32_t total = cells * row;
+        for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_hb[i] = hb[i];
+        __syncthreads();
+    }
+    const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    const uint32_t slot = wave / waves_per_slot;
+    const uint32_t p0 = c_begin + (wave % waves_per_slot) * chunk;
+    if (slot >= cells || p0 >= c_end) return;
+    const uint32_t p1 = min(p0 + chunk, c_end);
+
+    const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
+    uint64_t* W = spins_wr + (uint64_t)slot * n;
+    const uint32_t trow = temp_of[slot * kWave + lane] * row + dtab;
+
+    int64_t eacc = 0;
+    uint32_t nacc = 0;
+    generic_positions<true>(rp, ci, wt, lin, S, W, p0, p1, slot, sweep, key0, key1, lane,
+                            [&](uint32_t uu, int32_t de, bool x) {
+                                const int32_t h = x ? -de : de;
+                                const uint64_t t = kLdsThr ? lds_hb[trow + h] : hb[trow + h];
+                                return ((uint64_t)uu < t) != x;
+                            },
+                            eacc, nacc);
+    atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * kWave + lane),
+              (unsigned long long)eacc);
+    if (lane == 0) atomicAdd(accepted, (unsigned long long)nacc);
+}
+
+// ---------------------------------------------------------------------------
 // DEO replica exchange number `round`: one wave per adjacent pair (t, t+1)
 // with t = round mod 2; lane r decides for replica lane r and swaps
 // temperature indices.
@@ -522,6 +579,42 @@ hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c
     } else {
         hipLaunchKernelGGL(sweep_generic_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
                            m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, thr,
+                           dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
+                           waves_per_slot, sweep, key0, key1);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_heatbath_table(const double* beta, uint64_t* hb, uint32_t cells, uint32_t dtab,
+                                 hipStream_t stream) {
+    const uint64_t total = (2ull * dtab + 1) * cells;
+    hipLaunchKernelGGL(heatbath_table_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0,
+                       stream, beta, hb, cells, dtab);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                 uint32_t c_end, uint32_t sweep, uint64_t seed, const uint64_t* hb,
+                                 uint32_t dtab, hipStream_t stream) {
+    if (c_end <= c_begin) return hipSuccess;
+    const uint32_t len = c_end - c_begin;
+    // the generic kernel's launch shape
+    uint64_t chunk = ((uint64_t)len * s.cells + 16383) / 16384;
+    if (chunk < 16) chunk = 16;
+    if (chunk > 4096) chunk = 4096;
+    const uint32_t waves_per_slot = ceil_div(len, chunk);
+    const uint64_t waves = (uint64_t)waves_per_slot * s.cells;
+    const dim3 grid(ceil_div(waves, kWavesPerBlock));
+    const uint64_t lds_bytes = (uint64_t)s.cells * (2ull * dtab + 1) * sizeof(uint64_t);
+    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
+    if (lds_bytes <= 32768) {
+        hipLaunchKernelGGL(sweep_heatbath_kernel<true>, grid, dim3(kBlock), (uint32_t)lds_bytes,
+                           stream, m.rp, m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells,
+                           s.temp_of, hb, dtab, s.energy, s.accepted, c_begin, c_end,
+                           (uint32_t)chunk, waves_per_slot, sweep, key0, key1);
+    } else {
+        hipLaunchKernelGGL(sweep_heatbath_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
+                           m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, hb,
                            dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
                            waves_per_slot, sweep, key0, key1);
     }

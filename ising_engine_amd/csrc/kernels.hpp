@@ -54,6 +54,15 @@ hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c
                                 uint32_t c_end, uint32_t sweep, uint64_t seed,
                                 const uint32_t* thr, uint32_t dtab, hipStream_t stream);
 
+// Heat-bath thresholds hb[t*(2 dtab+1) + h + dtab] = heatbath_threshold(beta[t], h)
+// for h in [-dtab, dtab], and the heat-bath sweep of one colour class
+// (lane = replica, the generic kernel's uniforms and mapping).
+hipError_t launch_heatbath_table(const double* beta, uint64_t* hb, uint32_t cells, uint32_t dtab,
+                                 hipStream_t stream);
+hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                 uint32_t c_end, uint32_t sweep, uint64_t seed, const uint64_t* hb,
+                                 uint32_t dtab, hipStream_t stream);
+
 // Multispin sweep of one colour class (lane = site, bit-sliced +-J).
 // qplanes: [cells][dmax+1][33] threshold bit planes, see launch_qplanes.
 hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,

@@ -116,11 +116,13 @@ __device__ __forceinline__ bool band_accept(uint32_t u, int32_t de, double nbd, 
 // and in one aligned group of four (p0 >> 2 == (p1 - 1) >> 2).
 // S/W: the slot's words for reading / writing (same memory); read with
 // vector loads, so words written by other waves before a barrier are seen.
-// accept(u, de) returns u < metropolis_threshold(beta of this lane, de) for
-// de >= 1.
+// Metropolis (kHeatBath false): accept(u, de) returns
+// u < metropolis_threshold(beta of this lane, de) for de >= 1; de <= 0 is
+// accepted outright.  Heat-bath (kHeatBath true): accept(u, de, x) decides the
+// flip itself from the uniform, dE and the current spin x.
 // Adds the accepted dE to eacc (per lane) and the accepted count to nacc
 // (wave-uniform).
-template <class Acc>
+template <bool kHeatBath = false, class Acc>
 __device__ __forceinline__ void generic_group(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin, const uint64_t* S,
@@ -162,8 +164,13 @@ __device__ __forceinline__ void generic_group(
         const int32_t de = lane_bit(x) ? -h[q] : h[q];
         const uint32_t wsel = (p0 + q) & 3u;
         const uint32_t uu = wsel == 0 ? u.v[0] : wsel == 1 ? u.v[1] : wsel == 2 ? u.v[2] : u.v[3];
-        bool acc = de <= 0;
-        if (!acc) acc = accept(uu, de);
+        bool acc;
+        if constexpr (kHeatBath) {
+            acc = accept(uu, de, lane_bit(x));
+        } else {
+            acc = de <= 0;
+            if (!acc) acc = accept(uu, de);
+        }
         const uint64_t m = wave_ballot(acc);
         if (lane == 0) W[p0 + q] = x ^ m;
         eacc += acc ? de : 0;
@@ -260,7 +267,7 @@ __device__ __forceinline__ void fields_groups(const uint32_t* __restrict__ rp,
 }
 
 // Positions [p0, p1) of one colour class, group by group.
-template <class Acc>
+template <bool kHeatBath = false, class Acc>
 __device__ __forceinline__ void generic_positions(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin, const uint64_t* S,
@@ -268,8 +275,8 @@ __device__ __forceinline__ void generic_positions(
     uint32_t key1, uint32_t lane, Acc&& accept, int64_t& eacc, uint32_t& nacc) {
     for (uint32_t p = p0; p < p1;) {
         const uint32_t q1 = min(p1, (p & ~3u) + 4u);
-        generic_group(rp, ci, wt, lin, S, W, p, q1, slot, sweep, key0, key1, lane, accept, eacc,
-                      nacc);
+        generic_group<kHeatBath>(rp, ci, wt, lin, S, W, p, q1, slot, sweep, key0, key1, lane,
+                                 accept, eacc, nacc);
         p = q1;
     }
 }

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 
 VIEW_WORD = 0
 VIEW_PLANE = 1
+VIEW_HEATBATH = 2  # heat-bath rule, word-view uniforms
 
 _lib = None
 
@@ -38,6 +39,8 @@ def load():
     lib.or_det_exp.restype = c_double
     lib.or_threshold.argtypes = [c_double, c_int64]
     lib.or_threshold.restype = c_uint32
+    lib.or_heatbath_threshold.argtypes = [c_double, c_int64]
+    lib.or_heatbath_threshold.restype = c_uint64
     lib.or_exchange_threshold.argtypes = [c_double]
     lib.or_exchange_threshold.restype = c_uint32
     lib.or_ir_from_pairs.argtypes = [c_uint32, c_uint64, u32p, u32p, i64p, u32p, u32p, i64p]
@@ -92,6 +95,10 @@ def det_exp(x: float) -> float:
 
 def threshold(beta: float, de: int) -> int:
     return load().or_threshold(float(beta), int(de))
+
+
+def heatbath_threshold(beta: float, h: int) -> int:
+    return load().or_heatbath_threshold(float(beta), int(h))
 
 
 def exchange_threshold(delta: float) -> int:

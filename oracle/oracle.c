@@ -109,6 +109,17 @@ uint32_t or_threshold(double beta, int64_t de) {
 
 uint32_t or_exchange_threshold(double delta) { return prob_to_u32(or_det_exp(delta)); }
 
+/* gibbs_color_sweep.rs:131-158: P(x_i = 1) = 1 / (1 + exp(beta h)) as a threshold on a
+ * 32-bit uniform, floor(P 2^32) in [0, 2^32]; exp is evaluated only at -beta |h| <= 0:
+ * P = e / (1 + e) for h >= 0 and 1 / (1 + e) for h < 0, e = exp(-beta |h|).  beta -> inf
+ * (T = 0) gives 0, 1/2 or 1 as the reference's limit does. */
+uint64_t or_heatbath_threshold(double beta, int64_t h) {
+    const double a = beta * (double)(h < 0 ? -h : h);
+    const double e = or_det_exp(-a);
+    const double p = h >= 0 ? e / (1.0 + e) : 1.0 / (1.0 + e);
+    return (uint64_t)(p * 4294967296.0);
+}
+
 /* ------------------------------------------------------------------------- */
 /* ProblemIR (src/engine_v2/ir.rs)                                            */
 /* ------------------------------------------------------------------------- */
@@ -335,13 +346,15 @@ void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *be
 
 /* ------------------------------------------------------------------------- */
 /* The run: metropolis_sweep.rs:95-146 in colour order, then engine.rs:713-818 */
+/* With view OR_HEATBATH the sweep is gibbs_color_sweep.rs:119-174 instead:    */
+/* the new spin is u < or_heatbath_threshold(beta, h), word-view uniforms.     */
 /* ------------------------------------------------------------------------- */
 int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
            const int64_t *linear, uint32_t cells, uint64_t seed, uint32_t view, uint32_t exchange,
            uint32_t sweep0, uint32_t xround0, uint32_t sweeps, const double *betas,
            uint32_t betas_per_sweep, uint64_t *words, uint32_t *temp_of, int64_t *energies,
            uint64_t *stats) {
-    if (cells == 0 || view > 1) return -1;
+    if (cells == 0 || view > OR_HEATBATH) return -1;
     for (uint32_t i = 0; i < cells * 64u; ++i)
         if (temp_of[i] >= cells) return -1;
     uint32_t *colors = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
@@ -359,16 +372,21 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
                 uint32_t site = order[pos];
                 for (uint32_t s = 0; s < cells; ++s) {
                     uint64_t *sw = words + (uint64_t)s * n;
-                    if (view == OR_VIEW_WORD)
-                        uniforms_word(seed, pos, s, sweep, u);
-                    else
+                    if (view == OR_VIEW_PLANE)
                         uniforms_plane(seed, pos, s, sweep, u);
+                    else
+                        uniforms_word(seed, pos, s, sweep, u);
                     uint64_t flip = 0;
                     for (uint32_t r = 0; r < 64; ++r) {
                         int64_t h = field_of(row_ptr, col_idx, w, linear, sw, site, r);
-                        int64_t de = bit(sw[site], r) ? -h : h;
+                        int x = bit(sw[site], r);
+                        int64_t de = x ? -h : h;
                         double b = beta[temp_of[s * 64 + r]];
-                        int accept = (de <= 0) || (u[r] < or_threshold(b, de));
+                        int accept;
+                        if (view == OR_HEATBATH)
+                            accept = ((uint64_t)u[r] < or_heatbath_threshold(b, h)) != x;
+                        else
+                            accept = (de <= 0) || (u[r] < or_threshold(b, de));
                         if (accept) {
                             flip |= 1ull << r;
                             energies[s * 64 + r] += de;

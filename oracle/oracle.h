@@ -16,6 +16,7 @@
  *   or_energies          src/engine_v2/backends/sparse_bitslice.rs:114-137
  *   or_run (sweep part)  src/engine_v2/operators/metropolis_sweep.rs:95-146
  *                        with ΔE = (1-2x)·h from sparse_bitslice.rs:247-255
+ *   or_run (OR_HEATBATH) src/engine_v2/operators/gibbs_color_sweep.rs:119-174
  *   or_run (exchange)    src/solver/engine.rs:713-818 (DEO even/odd phase,
  *                        lane r of temperature t against lane r of t+1)
  *   or_geometric_ladder  src/engine_v2/decision.rs:133-142
@@ -52,6 +53,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 double or_det_exp(double x);
 uint32_t or_threshold(double beta, int64_t de);      /* de >= 1 */
 uint32_t or_exchange_threshold(double delta);        /* delta < 0 */
+uint64_t or_heatbath_threshold(double beta, int64_t h); /* in [0, 2^32] */
 
 /* ---- model ---- */
 /* Returns nnz (>= 0) or -1 on invalid pairs. Outputs sized n+1 / 2*npairs. */
@@ -80,7 +82,9 @@ void or_energies(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
 void or_init_words(uint32_t n, uint32_t cells, uint64_t seed, uint64_t *words);
 
 /* ---- the run ---- */
-enum { OR_VIEW_WORD = 0, OR_VIEW_PLANE = 1 };
+/* Metropolis with word- or plane-view uniforms, or the heat-bath rule with
+ * word-view uniforms. */
+enum { OR_VIEW_WORD = 0, OR_VIEW_PLANE = 1, OR_HEATBATH = 2 };
 
 /*
  * Run `sweeps` colour-ordered Metropolis sweeps over `cells` slots of 64
