@@ -46,6 +46,10 @@
  *   ige_read_stats          Report{accepted,proposed} + StepScratch swap statistics
  *                                                      src/engine_v2/operator.rs:34-52,
  *                                                      src/solver/engine.rs:454-557
+ *   ige_problem_parse       parse_rudy + maxcut_to_qubo src/engine_v2/frontend.rs:40-84;
+ *                           parse_rudy_maxcut / parse_orlib_bqp / parse_biqmac_sparse /
+ *                           parse_qplib                src/benchmark/instances.rs:103-344
+ *   ige_problem_get_info    ProblemIR::is_integral     src/engine_v2/ir.rs:143-148
  *   ige_steepest_descent    SteepestDescent::apply     src/engine_v2/operators/steepest_descent.rs:80-124
  *   ige_model_polish        steepest_descent_1opt      src/solver/local_search.rs:19-60
  *   IGE_RUN_TRACK_BEST,     UltimateSolver::update_incumbent and the anytime
@@ -123,6 +127,41 @@ int32_t ige_model_get_info(const ige_model *model, ige_model_info *info);
 int32_t ige_model_coloring(const ige_model *model, uint32_t *colors /* n */);
 int32_t ige_model_csr(const ige_model *model, uint32_t *row_ptr /* n+1 */,
                       uint32_t *col_idx /* nnz */, int32_t *w /* nnz */);
+
+/* ---- instance formats (host only, no device) ---- */
+typedef struct ige_problem ige_problem;
+
+#define IGE_FORMAT_RUDY 0u   /* rudy MaxCut as the engine_v2 frontend reads it:
+                                `u v [w]`, w = 1 when absent, bad lines skipped */
+#define IGE_FORMAT_GSET 1u   /* rudy MaxCut as the benchmark parser reads it
+                                (G-Set, Biq Mac MaxCut): `i j w`, '#'/'%'
+                                comments, out-of-range vertices an error     */
+#define IGE_FORMAT_ORLIB 2u  /* OR-Library BQP, maximise x'Qx; index = problem */
+#define IGE_FORMAT_BIQMAC 3u /* Biq Mac sparse BQP, minimise x'Qx            */
+#define IGE_FORMAT_QPLIB 4u  /* QPLIB, binary unconstrained (QBN) only        */
+
+typedef struct {
+    uint32_t n;
+    uint32_t problems_in_file; /* OR-Library files hold several             */
+    uint64_t npairs;           /* distinct pairs i < j with a nonzero sum     */
+    int32_t maximize;          /* native objective = -energy                  */
+    int32_t integral;          /* every coefficient an integer (is_integral)  */
+    double offset;
+} ige_problem_info;
+
+/* Parses `len` bytes of text into the minimisation form E = offset + lin.x +
+ * sum q_ij x_i x_j (f64, the reference's conventions per format).  A format
+ * error is IGE_ERR_INVALID with the parser's message. */
+int32_t ige_problem_parse(const char *text, uint64_t len, uint32_t format, uint32_t index,
+                          ige_problem **out);
+int32_t ige_problem_destroy(ige_problem *problem);
+int32_t ige_problem_get_info(const ige_problem *problem, ige_problem_info *info);
+/* The integer arrays ige_model_from_pairs takes (pairs sorted, i < j):
+ * IGE_ERR_RANGE when a coefficient is not an integer or exceeds int32. */
+int32_t ige_problem_pairs(const ige_problem *problem, int64_t *offset, int32_t *linear /* n */,
+                          uint32_t *pi, uint32_t *pj, int32_t *q /* npairs each */);
+/* f64 energy of x (n bytes 0/1). */
+int32_t ige_problem_energy(const ige_problem *problem, const uint8_t *x, double *energy);
 
 /* ---- replica state ---- */
 /* init_words: [cells][n] in site order, or NULL for the seeded default. */

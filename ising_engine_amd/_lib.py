@@ -42,8 +42,15 @@ EXPORTED = (
     "ige_batch_info", "ige_batch_read", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
     "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
-    "ige_read_best", "ige_model_polish", "ige_server_solution",
+    "ige_read_best", "ige_model_polish", "ige_server_solution", "ige_problem_parse",
+    "ige_problem_destroy", "ige_problem_get_info", "ige_problem_pairs", "ige_problem_energy",
 )
+
+FORMAT_RUDY = 0
+FORMAT_GSET = 1
+FORMAT_ORLIB = 2
+FORMAT_BIQMAC = 3
+FORMAT_QPLIB = 4
 
 REQ_QUEUED = 0
 REQ_RUNNING = 1
@@ -101,6 +108,11 @@ class ServerCounters(ctypes.Structure):
 class DescentReport(ctypes.Structure):
     _fields_ = [("steps", c_uint32), ("converged", c_uint32), ("flips", c_uint64),
                 ("proposed", c_uint64)]
+
+
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [("n", c_uint32), ("problems_in_file", c_uint32), ("npairs", c_uint64),
+                ("maximize", c_int32), ("integral", c_int32), ("offset", c_double)]
 
 
 class IgeError(RuntimeError):
@@ -178,6 +190,12 @@ def load() -> ctypes.CDLL:
         "ige_steepest_descent": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
                                            POINTER(DescentReport)]),
         "ige_update_best": (c_int32, [vp]),
+        "ige_problem_parse": (c_int32, [c_char_p, c_uint64, c_uint32, c_uint32, POINTER(vp)]),
+        "ige_problem_destroy": (c_int32, [vp]),
+        "ige_problem_get_info": (c_int32, [vp, POINTER(ProblemInfo)]),
+        "ige_problem_pairs": (c_int32, [vp, POINTER(c_int64), POINTER(c_int32),
+                                        POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]),
+        "ige_problem_energy": (c_int32, [vp, POINTER(ctypes.c_uint8), POINTER(c_double)]),
         "ige_server_solution": (c_int32, [vp, c_uint64, POINTER(ctypes.c_uint8),
                                           POINTER(c_int64)]),
         "ige_read_best": (c_int32, [vp, POINTER(ctypes.c_uint8), POINTER(c_int64),
@@ -278,6 +296,55 @@ class Model:
     def close(self):
         if self.handle:
             self._lib.ige_model_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Problem:
+    """A parsed instance file in minimisation form (host only; no device needed)."""
+
+    def __init__(self, text, fmt: int, index: int = 0):
+        self._lib = load()
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        self.handle = c_void_p()
+        check(self._lib.ige_problem_parse(data, len(data), fmt, index, ctypes.byref(self.handle)))
+        self.info = ProblemInfo()
+        check(self._lib.ige_problem_get_info(self.handle, ctypes.byref(self.info)))
+        self.n = self.info.n
+
+    def pairs(self):
+        """(offset, linear int32[n], pi, pj, q int32[npairs]); raises IGE_ERR_RANGE when the
+        instance is not integral."""
+        off = c_int64()
+        lin = np.empty(self.n, dtype=np.int32)
+        m = self.info.npairs
+        pi = np.empty(max(m, 1), dtype=np.uint32)
+        pj = np.empty(max(m, 1), dtype=np.uint32)
+        q = np.empty(max(m, 1), dtype=np.int32)
+        check(self._lib.ige_problem_pairs(self.handle, ctypes.byref(off), _ptr(lin, c_int32),
+                                          _ptr(pi, c_uint32), _ptr(pj, c_uint32), _ptr(q, c_int32)))
+        return off.value, lin, pi[:m], pj[:m], q[:m]
+
+    def energy(self, x) -> float:
+        x = np.ascontiguousarray(x, dtype=np.uint8)
+        if x.shape != (self.n,):
+            raise ValueError("x must have length n")
+        e = c_double()
+        check(self._lib.ige_problem_energy(self.handle, _ptr(x, ctypes.c_uint8), ctypes.byref(e)))
+        return e.value
+
+    def model(self, ctx: "Context") -> "Model":
+        off, lin, pi, pj, q = self.pairs()
+        return Model(ctx, self.n, off, lin, pi, pj, q)
+
+    def close(self):
+        if self.handle:
+            self._lib.ige_problem_destroy(self.handle)
             self.handle = c_void_p()
 
     def __del__(self):
