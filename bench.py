@@ -322,6 +322,44 @@ def bench_descent(_lib, st, qb, cells: int, sweeps: int, steps: int = 200) -> di
             "sweeps_timed_with_and_without_tracking": sweeps}
 
 
+def bench_gset(_lib, ctx, cpu_s: float) -> dict:
+    """solve_v2 (rudy file -> MaxCut QUBO -> heat-bath anneal of 256 all-zero replicas over
+    200 temperatures, best state tracked every sweep) on the four G-Set instances the
+    reference ships; wall time of the whole solve (parse, upload, sweeps, result)."""
+    from ising_engine_amd import solve_v2
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "gset")
+    out = {}
+    for name in ("G1", "G22", "G39", "G55"):
+        text = open(os.path.join(gdir, f"{name}.txt")).read()
+        solve_v2.solve(ctx, text, 1, 5, 256)
+        t0 = time.perf_counter()
+        x, e, done = solve_v2.solve(ctx, text, 1, 200, 256)
+        dt = time.perf_counter() - t0
+        prob = _lib.Problem(text, _lib.FORMAT_RUDY)
+        if e != prob.energy(x):
+            raise SystemExit(f"{name}: solve_v2 energy does not match its state")
+        out[name] = {"n": prob.n, "edges": int(prob.info.npairs), "seconds": dt, "cut": -e,
+                     "sweeps": done, "replicas": 256,
+                     "attempts_per_s": done * prob.n * 256 / dt}
+        log(f"[bench] solve_v2 {name}: cut {-e} in {dt * 1e3:.1f} ms")
+    if cpu_s > 0:
+        from oracle import binding as ob
+        prob = _lib.Problem(open(os.path.join(gdir, "G1.txt")).read(), _lib.FORMAT_RUDY)
+        off, lin, pi, pj, q = prob.pairs()
+        csr = ob.CSR(prob.n, off, lin, pi, pj, q)
+        st = ob.OracleState(csr, 4, 1, words=np.zeros((4, prob.n), np.uint64),
+                            view=ob.VIEW_HEATBATH)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_s:
+            st.run(1, exchange=False, schedule=np.full((1, 4), 0.5))
+            k += 1
+        dt = time.perf_counter() - t0
+        out["cpu_1thread_G1"] = {"value": k * prob.n * 256 / dt, "unit": "spin-flip attempts/s",
+                                 "kind": "port", "cores": 1,
+                                 "sample": f"{k} heat-bath sweeps of G1, 256 replicas"}
+    return out
+
+
 def bench_batch(_lib, instances, ctx, count: int) -> dict:
     """Independent sparse QUBO instances (N log-uniform 64..8192, degree 8, int weights),
     2000-sweep anneals: one persistent launch vs plain per-instance launches."""
@@ -409,6 +447,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-workloads", action="store_true")
     ap.add_argument("--no-descent", action="store_true")
+    ap.add_argument("--no-gset", action="store_true")
     ap.add_argument("--tts-runs", type=int, default=32)
     ap.add_argument("--sk-sweeps", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -490,6 +529,9 @@ def main():
     if not args.no_workloads:
         workloads = bench_workloads(_lib, instances, ctx, args.tts_runs, args.sk_sweeps,
                                     cpu_s=0.0 if args.no_cpu_baseline else 2.0)
+    gset = None
+    if not args.no_gset:
+        gset = bench_gset(_lib, ctx, cpu_s=0.0 if args.no_cpu_baseline else 2.0)
     serving = None
     if not args.no_serving:
         serving = [bench_serving(_lib, instances, ctx, float(r), args.serving_requests)
@@ -523,6 +565,7 @@ def main():
         "serving": serving,
         "workloads": workloads,
         "descent": descent,
+        "gset_solve_v2": gset,
     }
     print(json.dumps(out), flush=True)
 

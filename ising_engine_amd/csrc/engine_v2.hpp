@@ -413,6 +413,40 @@ class GpuMetropolisPt : public Operator {
     }
 };
 
+// gibbs_color_sweep.rs:119-174 on the device: heat-bath colour sweeps, new
+// spin 1 with probability 1 / (1 + exp(h / T)), one temperature per cell.
+class GpuGibbsColorSweep : public Operator {
+   public:
+    static constexpr const char* NAME = "gpu_gibbs_color_sweep";
+    OperatorDescriptor descriptor() const override {
+        OperatorDescriptor d;
+        d.name = NAME;
+        d.capabilities = CapabilitySet::empty()
+                             .with(Capability::Exploration)
+                             .with(Capability::Exploitation);
+        d.constraints = Constraints{true, false, true, false, false, true, true};
+        d.observables = ObservableSet{}.with(Observable::Acceptance).with(Observable::EnergyDelta);
+        d.guarantees.equilibrates = true;
+        d.guarantees.deterministic = true;
+        d.complexity = Complexity::LinearEdges;
+        return d;
+    }
+    CostEstimate cost_model(InstanceShape s) const override {
+        return {static_cast<double>((s.n + 2 * s.num_pairs) * std::max<size_t>(s.num_replicas, 1))};
+    }
+    Report apply(GpuBitSlice& state, const RuntimeView& view, uint64_t&, Budget budget) override {
+        state.ensure_ladder(view.temperatures);
+        const ige_stats before = state.stats();
+        check(ige_run(state.device(), budget.sweeps, IGE_RUN_HEATBATH, nullptr));
+        const ige_stats after = state.stats();
+        Report r;
+        r.accepted = after.accepted - before.accepted;
+        r.proposed = after.proposed - before.proposed;
+        r.work = static_cast<double>(state.num_vars()) * state.num_replicas() * budget.sweeps;
+        return r;
+    }
+};
+
 // steepest_descent.rs:80-124 on the device: per budget sweep, every replica
 // flips its most improving site (lowest site on a tie); stops early once no
 // replica can improve.  Monotone and deterministic; needs no temperature.
@@ -456,6 +490,7 @@ class OperatorRegistry {
         reg.register_factory([] { return std::make_unique<GpuReplicaExchange>(); });
         reg.register_factory([] { return std::make_unique<GpuMetropolisPt>(); });
         reg.register_factory([] { return std::make_unique<GpuSteepestDescent>(); });
+        reg.register_factory([] { return std::make_unique<GpuGibbsColorSweep>(); });
         return reg;
     }
     // registry.rs:70-82: the descriptor is read once, up front.

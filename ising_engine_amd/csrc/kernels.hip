@@ -33,6 +33,22 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return wave_ballot(p); }
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
+// Positions per wave of the lane-per-replica sweeps over a colour class of
+// `len` positions in each of `cells` slots: about 16K waves per launch
+// (256 CUs x 4 SIMDs x 16) and at least one aligned group of 4 positions (one
+// Philox call per lane) per wave; a class too small to give 128 waves at that
+// is split further, down to one position per wave, since there the launch's
+// duration is one wave's walk along its rows.
+uint64_t lane_per_replica_chunk(uint32_t len, uint32_t cells) {
+    uint64_t chunk = ((uint64_t)len * cells + 16383) / 16384;
+    if (chunk < 4) {
+        chunk = 4;
+        while (chunk > 1 && (uint64_t)ceil_div(len, chunk) * cells < 128) chunk >>= 1;
+    }
+    if (chunk > 4096) chunk = 4096;
+    return chunk;
+}
+
 // ---------------------------------------------------------------------------
 // Threshold table: one thread per (t, d).
 // ---------------------------------------------------------------------------
@@ -561,11 +577,7 @@ hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c
                                 const uint32_t* thr, uint32_t dtab, hipStream_t stream) {
     if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
-    // Aim for >= 16K waves per launch (256 CUs x 4 SIMDs x 16), at least 16
-    // positions per wave so the Philox groups amortise.
-    uint64_t chunk = ((uint64_t)len * s.cells + 16383) / 16384;
-    if (chunk < 16) chunk = 16;
-    if (chunk > 4096) chunk = 4096;
+    const uint64_t chunk = lane_per_replica_chunk(len, s.cells);
     const uint32_t waves_per_slot = ceil_div(len, chunk);
     const uint64_t waves = (uint64_t)waves_per_slot * s.cells;
     const dim3 grid(ceil_div(waves, kWavesPerBlock));
@@ -598,10 +610,7 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
                                  uint32_t dtab, hipStream_t stream) {
     if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
-    // the generic kernel's launch shape
-    uint64_t chunk = ((uint64_t)len * s.cells + 16383) / 16384;
-    if (chunk < 16) chunk = 16;
-    if (chunk > 4096) chunk = 4096;
+    const uint64_t chunk = lane_per_replica_chunk(len, s.cells);
     const uint32_t waves_per_slot = ceil_div(len, chunk);
     const uint64_t waves = (uint64_t)waves_per_slot * s.cells;
     const dim3 grid(ceil_div(waves, kWavesPerBlock));

@@ -36,8 +36,9 @@ struct Mock : Operator {
 
 static void registry_tests() {
     OperatorRegistry reg = OperatorRegistry::standard_gpu();
-    CHECK(reg.len() == 4);
+    CHECK(reg.len() == 5);
     CHECK(reg.contains("gpu_metropolis_sweep"));
+    CHECK(reg.contains("gpu_gibbs_color_sweep"));
     CHECK(reg.contains("gpu_steepest_descent"));
     CHECK(reg.contains("gpu_replica_exchange"));
     CHECK(reg.contains("gpu_metropolis_pt"));
@@ -46,7 +47,7 @@ static void registry_tests() {
     CHECK(op && op->name() == "gpu_metropolis_pt");
     CHECK(op->nature() == Nature::TrajectoryPreserving);
     reg.register_factory([] { return std::make_unique<Mock>(); });
-    CHECK(reg.len() == 5);
+    CHECK(reg.len() == 6);
     const OperatorDescriptor* md = reg.metadata("mock");
     CHECK(md && md->capabilities.contains(Capability::Warmstart));
 
@@ -58,7 +59,7 @@ static void registry_tests() {
     CapabilityQuery g;
     g.backend = Backend::GpuBitSlice;
     found = reg.find(g);
-    CHECK(found.size() == 4 && std::is_sorted(found.begin(), found.end()));
+    CHECK(found.size() == 5 && std::is_sorted(found.begin(), found.end()));
     CapabilityQuery cpu;
     cpu.backend = Backend::SparseBitSlice;
     CHECK((reg.find(cpu) == std::vector<std::string>{"mock"}));
@@ -70,15 +71,16 @@ static void registry_tests() {
     CapabilityQuery single;  // one replica: exchange needs several
     single.multi_replica = false;
     single.backend = Backend::GpuBitSlice;
-    CHECK((reg.find(single) ==
-           std::vector<std::string>{"gpu_metropolis_sweep", "gpu_steepest_descent"}));
+    CHECK((reg.find(single) == std::vector<std::string>{"gpu_gibbs_color_sweep",
+                                                        "gpu_metropolis_sweep",
+                                                        "gpu_steepest_descent"}));
     const OperatorDescriptor* sd = reg.metadata("gpu_steepest_descent");
     CHECK(sd && sd->guarantees.monotone && sd->guarantees.deterministic &&
           !sd->constraints.needs_temperature);
 
     CapabilityQuery any;
     any.any_of = CapabilitySet::empty().with(Capability::Exploitation).with(Capability::Warmstart);
-    CHECK(reg.find(any).size() == 4);
+    CHECK(reg.find(any).size() == 5);
 }
 
 // 3D +-J L=4 periodic, Ising lowered to QUBO (pair -4J, linear 2 sum J).
@@ -176,6 +178,14 @@ static void gpu_tests() {
     CHECK(rd.accepted > 0 && a.audit() == 0.0);
     Report rz = sd->apply(a, view, rng, Budget{5});
     CHECK(rz.accepted == 0 && rz.proposed == 64ull * 64 * 4);
+
+    // Heat-bath sweeps (gibbs_color_sweep.rs:119-174): exact bookkeeping, one proposal
+    // per (site, replica) and sweep, and the sweep counter shared with Metropolis.
+    auto gibbs = reg.lookup("gpu_gibbs_color_sweep");
+    const ige_stats g0 = a.stats();
+    Report rg = gibbs->apply(a, view, rng, Budget{12});
+    CHECK(rg.proposed == 12ull * 64 * 64 * 4 && rg.accepted > 0 && rg.accepted < rg.proposed);
+    CHECK(a.stats().sweeps == g0.sweeps + 12 && a.audit() == 0.0);
 
     // A wrong-sized ladder is an error, not a silent fallback.
     RuntimeView bad;
