@@ -50,6 +50,8 @@
  *                           parse_rudy_maxcut / parse_orlib_bqp / parse_biqmac_sparse /
  *                           parse_qplib                src/benchmark/instances.rs:103-344
  *   ige_problem_get_info    ProblemIR::is_integral     src/engine_v2/ir.rs:143-148
+ *   ige_icm, IGE_RUN_ICM    icm_sweep / icm_pair_move  src/solver/icm.rs:91-185
+ *                           (called per exchange round, src/solver/ultimate.rs:326-328)
  *   ige_steepest_descent    SteepestDescent::apply     src/engine_v2/operators/steepest_descent.rs:80-124
  *   ige_model_polish        steepest_descent_1opt      src/solver/local_search.rs:19-60
  *   IGE_RUN_TRACK_BEST,     UltimateSolver::update_incumbent and the anytime
@@ -82,6 +84,8 @@ extern "C" {
 #define IGE_RUN_EXCHANGE 1u   /* DEO replica exchange after every sweep     */
 #define IGE_RUN_TRACK_BEST 2u /* incumbent update after every sweep (and its
                                  exchange), see ige_update_best             */
+#define IGE_RUN_ICM 8u        /* one ICM sweep after every sweep (and its
+                                 exchange), before the incumbent update      */
 #define IGE_RUN_HEATBATH 4u   /* heat-bath (Gibbs) sweeps instead of Metropolis:
                                  new spin = 1 with P = 1/(1+exp(beta h)),
                                  lane-per-replica kernel on any state       */
@@ -204,6 +208,13 @@ typedef struct {
  * Synchronous. */
 int32_t ige_steepest_descent(ige_state *state, uint32_t max_steps, const uint8_t *clamped,
                              ige_descent_report *report);
+
+/* `rounds` ICM sweeps (Houdayer moves, icm.rs:163-185): for every temperature t
+ * and r < 32 the replicas (t, r) and (t, r + 32) flip, both, the connected
+ * cluster of sites where they disagree that contains a random seed site;
+ * tracked energies move by +-dE exactly.  Move number k of a pair is keyed on
+ * the state's ICM counter.  flipped: NULL or the total cluster size added up. */
+int32_t ige_icm(ige_state *state, uint32_t rounds, uint64_t *flipped);
 
 /* Incumbent: the replica with the lowest tracked energy, ties to the lowest
  * (temperature index, lane), replaces the best-seen state when strictly lower

@@ -28,6 +28,7 @@ KERNEL_DENSE = 3
 RUN_EXCHANGE = 1
 RUN_TRACK_BEST = 2
 RUN_HEATBATH = 4
+RUN_ICM = 8
 STEPS_UNLIMITED = 0xFFFFFFFF
 
 EXPORTED = (
@@ -44,6 +45,7 @@ EXPORTED = (
     "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
     "ige_read_best", "ige_model_polish", "ige_server_solution", "ige_problem_parse",
     "ige_problem_destroy", "ige_problem_get_info", "ige_problem_pairs", "ige_problem_energy",
+    "ige_icm",
 )
 
 FORMAT_RUDY = 0
@@ -190,6 +192,7 @@ def load() -> ctypes.CDLL:
         "ige_steepest_descent": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
                                            POINTER(DescentReport)]),
         "ige_update_best": (c_int32, [vp]),
+        "ige_icm": (c_int32, [vp, c_uint32, POINTER(c_uint64)]),
         "ige_problem_parse": (c_int32, [c_char_p, c_uint64, c_uint32, c_uint32, POINTER(vp)]),
         "ige_problem_destroy": (c_int32, [vp]),
         "ige_problem_get_info": (c_int32, [vp, POINTER(ProblemInfo)]),
@@ -378,7 +381,7 @@ class State:
         check(self._lib.ige_set_ladder(self.handle, _ptr(b, c_double)))
 
     def run(self, sweeps: int, exchange: bool = False, beta_schedule=None,
-            track_best: bool = False, heatbath: bool = False) -> None:
+            track_best: bool = False, heatbath: bool = False, icm: bool = False) -> None:
         sched = None
         if beta_schedule is not None:
             sched = np.ascontiguousarray(beta_schedule, dtype=np.float64)
@@ -386,7 +389,7 @@ class State:
                 raise ValueError("beta_schedule must be [sweeps][cells]")
         self._sched_keepalive = sched
         flags = ((RUN_EXCHANGE if exchange else 0) | (RUN_TRACK_BEST if track_best else 0) |
-                 (RUN_HEATBATH if heatbath else 0))
+                 (RUN_HEATBATH if heatbath else 0) | (RUN_ICM if icm else 0))
         check(self._lib.ige_run(self.handle, int(sweeps), flags, _ptr(sched, c_double)))
 
     def steepest_descent(self, max_steps: int = STEPS_UNLIMITED, clamped=None) -> DescentReport:
@@ -397,6 +400,12 @@ class State:
         check(self._lib.ige_steepest_descent(self.handle, int(max_steps),
                                              _ptr(cl, ctypes.c_uint8), ctypes.byref(rep)))
         return rep
+
+    def icm(self, rounds: int = 1) -> int:
+        """ICM sweeps; returns the total size of the clusters flipped."""
+        f = c_uint64()
+        check(self._lib.ige_icm(self.handle, int(rounds), ctypes.byref(f)))
+        return f.value
 
     def update_best(self) -> None:
         check(self._lib.ige_update_best(self.handle))

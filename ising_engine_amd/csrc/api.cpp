@@ -122,6 +122,7 @@ struct ige_state {
     hipStream_t stream = nullptr;
     uint32_t sweep = 0;   // sweeps run so far: the counter sweep uniforms are keyed on
     uint32_t xround = 0;  // exchanges run so far: keys exchange uniforms, phase = xround & 1
+    uint32_t icm_round = 0;  // ICM sweeps run so far: keys the cluster seeds
     uint32_t dtab = 0;
     bool ladder_set = false;
     std::vector<double> ladder;
@@ -133,6 +134,7 @@ struct ige_state {
     DevBuf<uint64_t> hb;                             // heat-bath thresholds, on first use
     DevBuf<int32_t> fields;
     DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
+    DevBuf<uint32_t> icm_scratch;  // ICM bitmaps and queues, on first use
     DevBuf<uint8_t> best_x;        // n bytes, site order
     DevBuf<uint32_t> best_take;    // decision of the last update
     bool profile = false;
@@ -263,6 +265,26 @@ DescentResult descend(const ige_model* model, const ige::DevState& st, const uin
     return ige::run_descent(model->view(), model->pos_of.p, model->co.order.data(),
                             dense ? &dv : nullptr, fields.p, st, clamped, max_steps, flips,
                             stream);
+}
+
+// ICM sweeps (asynchronous).  The scratch, a visited bitmap and a queue per
+// move, is kept on the state and capped at 256 MiB; the moves of one sweep run
+// in as many launches as that needs.
+void icm_rounds(ige_state* s, uint32_t rounds, unsigned long long* flipped) {
+    const uint32_t n = s->model->ir.n;
+    const uint32_t pairs = s->cells * 32;
+    const uint64_t per = ige::icm_scratch_words(n);
+    const uint64_t fit = std::max<uint64_t>(1, (256ull << 20) / 4 / per);
+    const uint32_t batch = static_cast<uint32_t>(std::min<uint64_t>(pairs, fit));
+    if (!s->icm_scratch.p) s->icm_scratch.alloc(static_cast<size_t>(batch) * per);
+    for (uint32_t k = 0; k < rounds; ++k) {
+        for (uint32_t p0 = 0; p0 < pairs; p0 += batch)
+            check(ige::launch_icm(s->model->view(), s->view(), s->model->pos_of.p, p0,
+                                  std::min(batch, pairs - p0), s->icm_round, s->seed,
+                                  s->icm_scratch.p, flipped, s->stream),
+                  "icm");
+        s->icm_round += 1;
+    }
 }
 
 void recompute_energy_into_state(ige_state* s) {
@@ -433,8 +455,12 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
     return guarded([&] {
         require(s != nullptr, "NULL state");
         require(sched != nullptr || s->ladder_set, "no ladder set and no beta schedule given");
-        require((flags & ~(IGE_RUN_EXCHANGE | IGE_RUN_TRACK_BEST | IGE_RUN_HEATBATH)) == 0,
+        require((flags & ~(IGE_RUN_EXCHANGE | IGE_RUN_TRACK_BEST | IGE_RUN_HEATBATH |
+                           IGE_RUN_ICM)) == 0,
                 "unknown flag");
+        require(static_cast<uint64_t>(s->icm_round) + sweeps <= 0xFFFFFFFFull,
+                "ICM counter would exceed 2^32-1");
+        const bool icm = (flags & IGE_RUN_ICM) != 0;
         require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull &&
                     static_cast<uint64_t>(s->xround) + sweeps <= 0xFFFFFFFFull,
                 "sweep or exchange counter would exceed 2^32-1");
@@ -514,6 +540,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                 s->xround += 1;
             }
             s->sweep += 1;
+            if (icm) icm_rounds(s, 1, nullptr);
             if (track) track_best(s);
         }
         if (sched) {
@@ -656,6 +683,28 @@ int32_t ige_steepest_descent(ige_state* s, uint32_t max_steps, const uint8_t* cl
             report->flips = r.flips;
             report->proposed = (static_cast<uint64_t>(r.steps) + r.converged) * s->model->ir.n *
                                64ull * s->cells;
+        }
+    });
+}
+
+int32_t ige_icm(ige_state* s, uint32_t rounds, uint64_t* flipped) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        require(static_cast<uint64_t>(s->icm_round) + rounds <= 0xFFFFFFFFull,
+                "ICM counter would exceed 2^32-1");
+        set_device(s->ctx);
+        DevBuf<unsigned long long> count;
+        if (flipped) {
+            count.alloc(1);
+            const unsigned long long zero = 0;
+            count.upload(&zero, 1, s->stream);
+        }
+        icm_rounds(s, rounds, flipped ? count.p : nullptr);
+        if (flipped) {
+            unsigned long long c = 0;
+            count.download(&c, 1, s->stream);
+            check(hipStreamSynchronize(s->stream), "icm count");
+            *flipped = c;
         }
     });
 }

@@ -129,6 +129,15 @@ hipError_t launch_descent_step(const DevModel& m, const DevState& s, const int32
                                const uint32_t* site_aux, unsigned long long* keys, uint32_t* flips,
                                unsigned long long* ctl, hipStream_t stream);
 
+// Isoenergetic cluster moves for pairs [pair0, pair0 + pairs) of the
+// cells * 32 pairs (temperature t = pair / 32, lanes r = pair % 32 and r + 32),
+// ICM sweep number `round`.  scratch: pairs * icm_scratch_words(n) u32.
+// flipped: NULL or a counter the cluster sizes are added to.
+uint64_t icm_scratch_words(uint32_t n);
+hipError_t launch_icm(const DevModel& m, const DevState& s, const uint32_t* pos_of,
+                      uint32_t pair0, uint32_t pairs, uint32_t round, uint64_t seed,
+                      uint32_t* scratch, unsigned long long* flipped, hipStream_t stream);
+
 // Best-seen replica: energy, where it was found, and (separately) its bits.
 struct BestRecord {
     int64_t energy;

@@ -74,6 +74,9 @@ def load():
     lib.or_steepest_descent.restype = c_uint32
     lib.or_polish_1opt.argtypes = [c_uint32, u32p, u32p, i64p, i64p, u8p, u8p, u64p]
     lib.or_polish_1opt.restype = c_int64
+    lib.or_icm_sweep.argtypes = [c_uint32, u32p, u32p, i64p, i64p, c_uint32, c_uint64,
+                                 c_uint32, u64p, u32p, i64p, u32p]
+    lib.or_icm_sweep.restype = c_uint32
     lib.or_update_best.argtypes = [c_uint32, c_uint32, u64p, u32p, i64p, i64p, i32p, u8p]
     lib.or_update_best.restype = c_int
     _lib = lib
@@ -243,6 +246,7 @@ class OracleState:
         self.energy = csr.energies(self.words)
         self.sweep = 0
         self.xround = 0
+        self.icm_round = 0
         self.stats = np.zeros(2 + 2 * max(self.cells - 1, 0), dtype=np.uint64)
 
     def run(self, sweeps: int, exchange: bool, ladder=None, schedule=None):
@@ -263,6 +267,19 @@ class OracleState:
         self.sweep += sweeps
         if exchange and self.cells >= 2:
             self.xround += sweeps
+
+    def icm(self, rounds: int = 1):
+        """ICM sweeps (icm.rs:163-185), keyed on the state's ICM round counter.
+        Returns the per-pair cluster sizes of the last round ([cells][32])."""
+        sizes = np.zeros((self.cells, 32), dtype=np.uint32)
+        n, rp, ci, w = self.csr._args()
+        for _ in range(rounds):
+            load().or_icm_sweep(n, rp, ci, w, _p(self.csr.linear, c_int64), self.cells,
+                                self.seed, self.icm_round, _p(self.words, c_uint64),
+                                _p(self.temp_of, c_uint32), _p(self.energy, c_int64),
+                                _p(sizes, c_uint32))
+            self.icm_round += 1
+        return sizes
 
     def exchange(self, rounds: int, ladder):
         """Exchange-only rounds (ReplicaExchange::apply) on the tracked energies."""

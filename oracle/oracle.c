@@ -56,6 +56,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 #define TAG_EXCH 3u
 #define TAG_INIT 4u
 #define TAG_TAIL 5u
+#define TAG_ICM 6u
 /* Bit planes drawn for every replica before its tail word is consulted. */
 #define EAGER_PLANES 8u
 
@@ -589,4 +590,94 @@ int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint
     *best_e = e_min;
     *valid = 1;
     return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Isoenergetic cluster moves (icm.rs:91-185)                                 */
+/* ------------------------------------------------------------------------- */
+
+/* icm_sweep (icm.rs:163-185): for every temperature t and r < 32 a Houdayer
+ * move on the replicas (t, r) and (t, r + 32), found through temp_of (the
+ * configurations stay in their slots).  The seed is the floor(u |D| / 2^32)-th
+ * disagreeing site in site order, u = Philox(t, r, round, TAG_ICM<<24)[0];
+ * the cluster is the seed's connected component of disagreeing sites; both
+ * replicas flip it, E_A += dE_A and E_B -= dE_A (icm.rs:91-160).  words:
+ * [cells][n] site order.  flipped[cells*32] (or NULL) receives each pair's
+ * cluster size.  Returns the number of moves made. */
+uint32_t or_icm_sweep(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                      const int64_t *w, const int64_t *linear, uint32_t cells, uint64_t seed,
+                      uint32_t round, uint64_t *words, const uint32_t *temp_of,
+                      int64_t *energies, uint32_t *flipped) {
+    uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
+    uint8_t *in_c = (uint8_t *)malloc((size_t)n + 1);
+    uint8_t *dis = (uint8_t *)malloc((size_t)n + 1);
+    uint32_t *stack = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    uint32_t moves = 0;
+    for (uint32_t t = 0; t < cells; ++t)
+        for (uint32_t r = 0; r < 32; ++r) {
+            const uint32_t sa = slot_of[t * 64 + r], sb = slot_of[t * 64 + r + 32];
+            uint64_t *A = words + (uint64_t)sa * n, *B = words + (uint64_t)sb * n;
+            uint32_t count = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                dis[i] = (uint8_t)(bit(A[i], r) ^ bit(B[i], r + 32));
+                in_c[i] = 0;
+                count += dis[i];
+            }
+            if (flipped) flipped[t * 32 + r] = 0;
+            if (count == 0) continue;
+            uint32_t o[4];
+            philox_seeded(seed, t, r, round, TAG_ICM << 24, o);
+            uint32_t k = (uint32_t)(((uint64_t)o[0] * count) >> 32), sd = 0;
+            for (uint32_t i = 0; i < n; ++i)
+                if (dis[i] && k-- == 0) {
+                    sd = i;
+                    break;
+                }
+            uint32_t top = 0, size = 0;
+            stack[top++] = sd;
+            in_c[sd] = 1;
+            while (top) {
+                const uint32_t i = stack[--top];
+                ++size;
+                for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+                    const uint32_t j = col_idx[e];
+                    if (dis[j] && !in_c[j]) {
+                        in_c[j] = 1;
+                        stack[top++] = j;
+                    }
+                }
+            }
+            /* icm.rs:55-82: dE of the cluster flip in replica A */
+            int64_t de = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (!in_c[i]) continue;
+                const int64_t xi = bit(A[i], r);
+                de += linear[i] * (1 - 2 * xi);
+                for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+                    const uint32_t j = col_idx[e];
+                    const int64_t xj = bit(A[j], r);
+                    if (in_c[j]) {
+                        if (i < j) de += w[e] * ((1 - xi) * (1 - xj) - xi * xj);
+                    } else {
+                        de += w[e] * xj * (1 - 2 * xi);
+                    }
+                }
+            }
+            for (uint32_t i = 0; i < n; ++i)
+                if (in_c[i]) {
+                    A[i] ^= 1ull << r;
+                    B[i] ^= 1ull << (r + 32);
+                }
+            energies[sa * 64 + r] += de;
+            energies[sb * 64 + r + 32] -= de;
+            if (flipped) flipped[t * 32 + r] = size;
+            ++moves;
+        }
+    free(slot_of);
+    free(in_c);
+    free(dis);
+    free(stack);
+    return moves;
 }

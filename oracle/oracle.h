@@ -24,6 +24,8 @@
  *   or_steepest_descent  src/engine_v2/operators/steepest_descent.rs:80-124
  *   or_polish_1opt       src/solver/local_search.rs:19-60
  *   or_update_best       src/solver/ultimate.rs:397-431
+ *   or_icm_sweep         src/solver/icm.rs:91-185 (seed site and round keyed
+ *                        on Philox; see the function)
  *
  * Parity mode departs from the reference in exactly two declared places,
  * because the reference's ChaCha8 streams are a third-party dependency that
@@ -127,6 +129,12 @@ uint32_t or_steepest_descent(uint32_t n, const uint32_t *row_ptr, const uint32_t
 int64_t or_polish_1opt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
                        const int64_t *w, const int64_t *linear, const uint8_t *clamped,
                        uint8_t *x, uint64_t *flips);
+/* One ICM sweep (icm.rs:163-185) over every temperature's 32 lane pairs. */
+uint32_t or_icm_sweep(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                      const int64_t *w, const int64_t *linear, uint32_t cells, uint64_t seed,
+                      uint32_t round, uint64_t *words, const uint32_t *temp_of,
+                      int64_t *energies, uint32_t *flipped);
+
 /* Incumbent update from tracked energies; words [cells][n] site order. */
 int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint32_t *temp_of,
                    const int64_t *energies, int64_t *best_e, int32_t *valid, uint8_t *best_x);
