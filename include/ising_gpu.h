@@ -50,6 +50,9 @@
  *                           parse_rudy_maxcut / parse_orlib_bqp / parse_biqmac_sparse /
  *                           parse_qplib                src/benchmark/instances.rs:103-344
  *   ige_problem_get_info    ProblemIR::is_integral     src/engine_v2/ir.rs:143-148
+ *   ige_set_populations,    PA-PT hybrid of UltimateSolver::solve_population_annealing
+ *   ige_resample, ige_pa_*                             src/solver/ultimate.rs:559-852,
+ *                           apply_resample and helpers src/solver/population_annealing.rs:45-286
  *   ige_icm, IGE_RUN_ICM    icm_sweep / icm_pair_move  src/solver/icm.rs:91-185
  *                           (called per exchange round, src/solver/ultimate.rs:326-328)
  *   ige_steepest_descent    SteepestDescent::apply     src/engine_v2/operators/steepest_descent.rs:80-124
@@ -215,6 +218,33 @@ int32_t ige_steepest_descent(ige_state *state, uint32_t max_steps, const uint8_t
  * tracked energies move by +-dE exactly.  Move number k of a pair is keyed on
  * the state's ICM counter.  flipped: NULL or the total cluster size added up. */
 int32_t ige_icm(ige_state *state, uint32_t rounds, uint64_t *flipped);
+
+/* ---- population annealing (PA, and the PA-PT hybrid) ---- */
+/* Splits the cells into `pops` populations of T = cells / pops temperature
+ * indices each (population p owns p*T .. p*T+T-1; set the ladder to the same
+ * T betas for every population).  Exchange then pairs (t, t+1) only inside a
+ * population (engine.rs:713-716).  pops = 1 (the default) is plain PT. */
+int32_t ige_set_populations(ige_state *state, uint32_t pops);
+/* apply_resample (population_annealing.rs:229-286): member m = p*64 + r is
+ * lane r of population p across its T levels; afterwards the replica of
+ * member m at level t holds what member parents[m] held at level t, spins and
+ * tracked energy.  parents: pops*64 entries, each < pops*64. */
+int32_t ige_resample(ige_state *state, const uint32_t *parents);
+/* Host-side PA arithmetic (population_annealing.rs:45-124, :237-246), on
+ * count = pops*64 members.  normalize: log-sum-exp weights and ESS = 1/sum w^2;
+ * free_energy_increment: logsumexp(log_w) - ln count; systematic_resample:
+ * offspring counts from one offset u in [0, 1); parents: the parent list of
+ * the counts, shuffled (Fisher-Yates with Philox uniforms keyed on seed and
+ * the resampling event number); family_entropy: rho = count * sum_f (n_f /
+ * count)^2 and the number of surviving families. */
+int32_t ige_pa_normalize(const double *log_w, uint32_t count, double *w_norm, double *ess);
+int32_t ige_pa_free_energy_increment(const double *log_w, uint32_t count, double *out);
+int32_t ige_pa_systematic_resample(const double *w_norm, uint32_t count, double u,
+                                   uint32_t *counts);
+int32_t ige_pa_parents(const uint32_t *counts, uint32_t count, uint64_t seed, uint32_t event,
+                       uint32_t *parents);
+int32_t ige_pa_family_entropy(const uint32_t *family, uint32_t count, double *rho,
+                              uint32_t *families);
 
 /* Incumbent: the replica with the lowest tracked energy, ties to the lowest
  * (temperature index, lane), replaces the best-seen state when strictly lower

@@ -45,7 +45,9 @@ EXPORTED = (
     "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
     "ige_read_best", "ige_model_polish", "ige_server_solution", "ige_problem_parse",
     "ige_problem_destroy", "ige_problem_get_info", "ige_problem_pairs", "ige_problem_energy",
-    "ige_icm",
+    "ige_icm", "ige_set_populations", "ige_resample", "ige_pa_normalize",
+    "ige_pa_free_energy_increment", "ige_pa_systematic_resample", "ige_pa_parents",
+    "ige_pa_family_entropy",
 )
 
 FORMAT_RUDY = 0
@@ -193,6 +195,18 @@ def load() -> ctypes.CDLL:
                                            POINTER(DescentReport)]),
         "ige_update_best": (c_int32, [vp]),
         "ige_icm": (c_int32, [vp, c_uint32, POINTER(c_uint64)]),
+        "ige_set_populations": (c_int32, [vp, c_uint32]),
+        "ige_resample": (c_int32, [vp, POINTER(c_uint32)]),
+        "ige_pa_normalize": (c_int32, [POINTER(c_double), c_uint32, POINTER(c_double),
+                                       POINTER(c_double)]),
+        "ige_pa_free_energy_increment": (c_int32, [POINTER(c_double), c_uint32,
+                                                   POINTER(c_double)]),
+        "ige_pa_systematic_resample": (c_int32, [POINTER(c_double), c_uint32, c_double,
+                                                 POINTER(c_uint32)]),
+        "ige_pa_parents": (c_int32, [POINTER(c_uint32), c_uint32, c_uint64, c_uint32,
+                                     POINTER(c_uint32)]),
+        "ige_pa_family_entropy": (c_int32, [POINTER(c_uint32), c_uint32, POINTER(c_double),
+                                            POINTER(c_uint32)]),
         "ige_problem_parse": (c_int32, [c_char_p, c_uint64, c_uint32, c_uint32, POINTER(vp)]),
         "ige_problem_destroy": (c_int32, [vp]),
         "ige_problem_get_info": (c_int32, [vp, POINTER(ProblemInfo)]),
@@ -407,6 +421,14 @@ class State:
         check(self._lib.ige_icm(self.handle, int(rounds), ctypes.byref(f)))
         return f.value
 
+    def set_populations(self, pops: int) -> None:
+        check(self._lib.ige_set_populations(self.handle, int(pops)))
+        self.pops = int(pops)
+
+    def resample(self, parents) -> None:
+        par = np.ascontiguousarray(parents, dtype=np.uint32)
+        check(self._lib.ige_resample(self.handle, _ptr(par, c_uint32)))
+
     def update_best(self) -> None:
         check(self._lib.ige_update_best(self.handle))
 
@@ -480,6 +502,48 @@ class State:
             self.close()
         except Exception:
             pass
+
+
+def pa_normalize(log_w):
+    """(normalized weights, effective sample size) of accumulated log-weights."""
+    lw = np.ascontiguousarray(log_w, dtype=np.float64)
+    w = np.empty_like(lw)
+    ess = c_double()
+    check(load().ige_pa_normalize(_ptr(lw, c_double), lw.size, _ptr(w, c_double),
+                                  ctypes.byref(ess)))
+    return w, ess.value
+
+
+def pa_free_energy_increment(log_w) -> float:
+    lw = np.ascontiguousarray(log_w, dtype=np.float64)
+    out = c_double()
+    check(load().ige_pa_free_energy_increment(_ptr(lw, c_double), lw.size, ctypes.byref(out)))
+    return out.value
+
+
+def pa_systematic_resample(w_norm, u: float) -> np.ndarray:
+    w = np.ascontiguousarray(w_norm, dtype=np.float64)
+    counts = np.empty(w.size, dtype=np.uint32)
+    check(load().ige_pa_systematic_resample(_ptr(w, c_double), w.size, float(u),
+                                            _ptr(counts, c_uint32)))
+    return counts
+
+
+def pa_parents(counts, seed: int, event: int) -> np.ndarray:
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    out = np.empty(c.size, dtype=np.uint32)
+    check(load().ige_pa_parents(_ptr(c, c_uint32), c.size, int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                int(event), _ptr(out, c_uint32)))
+    return out
+
+
+def pa_family_entropy(family):
+    f = np.ascontiguousarray(family, dtype=np.uint32)
+    rho = c_double()
+    k = c_uint32()
+    check(load().ige_pa_family_entropy(_ptr(f, c_uint32), f.size, ctypes.byref(rho),
+                                       ctypes.byref(k)))
+    return rho.value, k.value
 
 
 def anneal_schedule(beta_lo: float, beta_hi: float, sweeps: int) -> np.ndarray:

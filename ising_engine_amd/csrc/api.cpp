@@ -123,6 +123,7 @@ struct ige_state {
     uint32_t sweep = 0;   // sweeps run so far: the counter sweep uniforms are keyed on
     uint32_t xround = 0;  // exchanges run so far: keys exchange uniforms, phase = xround & 1
     uint32_t icm_round = 0;  // ICM sweeps run so far: keys the cluster seeds
+    uint32_t pops = 1;       // populations (each a ladder of cells / pops levels)
     uint32_t dtab = 0;
     bool ladder_set = false;
     std::vector<double> ladder;
@@ -151,6 +152,7 @@ struct ige_state {
         s.slot_of = slot_of.p;
         s.accepted = counters.p;
         s.pair_stats = counters.p + 1;
+        s.pop_temps = cells / pops;
         return s;
     }
     ~ige_state() {
@@ -684,6 +686,42 @@ int32_t ige_steepest_descent(ige_state* s, uint32_t max_steps, const uint8_t* cl
             report->proposed = (static_cast<uint64_t>(r.steps) + r.converged) * s->model->ir.n *
                                64ull * s->cells;
         }
+    });
+}
+
+int32_t ige_set_populations(ige_state* s, uint32_t pops) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        require(pops >= 1 && s->cells % pops == 0, "pops must divide the number of cells");
+        s->pops = pops;
+    });
+}
+
+int32_t ige_resample(ige_state* s, const uint32_t* parents) {
+    return guarded([&] {
+        require(s && parents, "NULL argument");
+        const uint32_t members = s->pops * 64;
+        for (uint32_t m = 0; m < members; ++m)
+            require(parents[m] < members, "parent index out of range");
+        set_device(s->ctx);
+        const size_t words = static_cast<size_t>(s->cells) * s->model->ir.n;
+        DevBuf<uint64_t> old_spins;
+        DevBuf<int64_t> old_energy;
+        DevBuf<uint32_t> par;
+        old_spins.alloc(words);
+        old_energy.alloc(static_cast<size_t>(s->cells) * 64);
+        par.alloc(members);
+        check(hipMemcpyAsync(old_spins.p, s->spins.p, words * sizeof(uint64_t),
+                             hipMemcpyDeviceToDevice, s->stream),
+              "snapshot");
+        check(hipMemcpyAsync(old_energy.p, s->energy.p, old_energy.count * sizeof(int64_t),
+                             hipMemcpyDeviceToDevice, s->stream),
+              "snapshot");
+        par.upload(parents, members, s->stream);
+        check(ige::launch_resample(s->view(), s->model->ir.n, old_spins.p, old_energy.p, par.p,
+                                   s->stream),
+              "resample");
+        check(hipStreamSynchronize(s->stream), "resample");  // the snapshot is freed here
     });
 }
 

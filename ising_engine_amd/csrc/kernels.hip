@@ -168,10 +168,12 @@ __global__ __launch_bounds__(kBlock) void sweep_heatbath_kernel(
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void exchange_kernel(
     const double* __restrict__ beta, const int64_t* __restrict__ energy, uint32_t* temp_of,
-    uint32_t* slot_of, unsigned long long* pair_stats, uint32_t cells, uint32_t round,
-    uint32_t key0, uint32_t key1) {
+    uint32_t* slot_of, unsigned long long* pair_stats, uint32_t cells, uint32_t pop_temps,
+    uint32_t pairs_per_pop, uint32_t round, uint32_t key0, uint32_t key1) {
     const uint32_t r = threadIdx.x;
-    const uint32_t t = (round & 1u) + 2u * blockIdx.x;
+    // populations of pop_temps levels each: pairs stay inside one population
+    const uint32_t pop = blockIdx.x / pairs_per_pop, k = blockIdx.x % pairs_per_pop;
+    const uint32_t t = pop * pop_temps + (round & 1u) + 2u * k;
     if (t + 1 >= cells) return;
     const uint32_t s1 = slot_of[t * kWave + r];
     const uint32_t s2 = slot_of[(t + 1) * kWave + r];
@@ -678,13 +680,16 @@ hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* be
 
 hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round, uint64_t seed,
                            hipStream_t stream) {
-    if (s.cells < 2) return hipSuccess;
+    const uint32_t T = s.pop_temps ? s.pop_temps : s.cells;  // levels per population
+    if (T < 2 || s.cells % T) return s.cells % T ? hipErrorInvalidValue : hipSuccess;
     const uint32_t phase = round & 1u;
-    const uint32_t pairs = (s.cells - phase) / 2;  // t = phase, phase+2, ... with t+1 < cells
-    if (pairs == 0) return hipSuccess;
+    // per population t = phase, phase+2, ... with t+1 < T
+    const uint32_t per_pop = (T - phase) / 2;
+    if (per_pop == 0) return hipSuccess;
+    const uint32_t pairs = per_pop * (s.cells / T);
     hipLaunchKernelGGL(exchange_kernel, dim3(pairs), dim3(kWave), 0, stream, beta, s.energy,
-                       s.temp_of, s.slot_of, s.pair_stats, s.cells, round, (uint32_t)seed,
-                       (uint32_t)(seed >> 32));
+                       s.temp_of, s.slot_of, s.pair_stats, s.cells, T, per_pop, round,
+                       (uint32_t)seed, (uint32_t)(seed >> 32));
     return hipGetLastError();
 }
 

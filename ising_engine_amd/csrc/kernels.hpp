@@ -43,6 +43,7 @@ struct DevState {
     uint32_t* slot_of = nullptr;    // [cells][64] slot of (temperature, lane)
     unsigned long long* accepted = nullptr;    // 1 counter
     unsigned long long* pair_stats = nullptr;  // [cells-1][2] attempts, accepts
+    uint32_t pop_temps = 0;  // levels per population (0: one population of all cells)
 };
 
 // thr[t*(dtab+1) + d] = metropolis_threshold(beta[t], d) for d in [1, dtab].
@@ -83,7 +84,8 @@ constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
 hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
                           unsigned long long* qplanes, hipStream_t stream);
 
-// DEO exchange number `round`: pairs (t, t+1) with t = round mod 2.
+// DEO exchange number `round`: pairs (t, t+1) with t = round mod 2, inside
+// each population of s.pop_temps temperature indices (engine.rs:713-716).
 hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round, uint64_t seed,
                            hipStream_t stream);
 
@@ -137,6 +139,13 @@ uint64_t icm_scratch_words(uint32_t n);
 hipError_t launch_icm(const DevModel& m, const DevState& s, const uint32_t* pos_of,
                       uint32_t pair0, uint32_t pairs, uint32_t round, uint64_t seed,
                       uint32_t* scratch, unsigned long long* flipped, hipStream_t stream);
+
+// Population-annealing resampling: spins and energies gathered from the
+// snapshot (old_spins, old_energy) by member parents (pops*64 entries, with
+// pops = cells / s.pop_temps).
+hipError_t launch_resample(const DevState& s, uint32_t n, const uint64_t* old_spins,
+                           const int64_t* old_energy, const uint32_t* parents,
+                           hipStream_t stream);
 
 // Best-seen replica: energy, where it was found, and (separately) its bits.
 struct BestRecord {

@@ -59,10 +59,15 @@ def load():
     lib.or_init_words.argtypes = [c_uint32, c_uint32, c_uint64, u64p]
     lib.or_init_words.restype = None
     lib.or_run.argtypes = [c_uint32, u32p, u32p, i64p, i64p, c_uint32, c_uint64, c_uint32,
-                           c_uint32, c_uint32, c_uint32, c_uint32, dp, c_uint32, u64p, u32p,
-                           i64p, u64p]
+                           c_uint32, c_uint32, c_uint32, c_uint32, dp, c_uint32, c_uint32, u64p,
+                           u32p, i64p, u64p]
     lib.or_run.restype = c_int
     lib.or_exchange.argtypes = [c_uint32, c_uint64, c_uint32, dp, i64p, u32p, u64p]
+    lib.or_exchange_pops.argtypes = [c_uint32, c_uint32, c_uint64, c_uint32, dp, i64p, u32p,
+                                     u64p]
+    lib.or_exchange_pops.restype = None
+    lib.or_resample.argtypes = [c_uint32, c_uint32, c_uint32, u32p, u64p, u32p, i64p]
+    lib.or_resample.restype = None
     lib.or_exchange.restype = None
     lib.or_geometric_ladder.argtypes = [c_uint32, c_double, c_double, dp]
     lib.or_geometric_ladder.restype = None
@@ -247,6 +252,7 @@ class OracleState:
         self.sweep = 0
         self.xround = 0
         self.icm_round = 0
+        self.pops = 1
         self.stats = np.zeros(2 + 2 * max(self.cells - 1, 0), dtype=np.uint64)
 
     def run(self, sweeps: int, exchange: bool, ladder=None, schedule=None):
@@ -259,7 +265,8 @@ class OracleState:
         n, rp, ci, w = self.csr._args()
         rc = load().or_run(n, rp, ci, w, _p(self.csr.linear, c_int64), self.cells, self.seed,
                            self.view, 1 if exchange else 0, self.sweep, self.xround, sweeps,
-                           _p(betas, c_double), per, _p(self.words, c_uint64),
+                           _p(betas, c_double), per, self.cells // self.pops,
+                           _p(self.words, c_uint64),
                            _p(self.temp_of, c_uint32), _p(self.energy, c_int64),
                            _p(self.stats, c_uint64))
         if rc != 0:
@@ -285,10 +292,21 @@ class OracleState:
         """Exchange-only rounds (ReplicaExchange::apply) on the tracked energies."""
         if self.cells < 2:
             return
+        beta = np.ascontiguousarray(ladder, dtype=np.float64)
         for _ in range(rounds):
-            exchange(self.cells, self.seed, self.xround, ladder, self.energy, self.temp_of,
-                     self.stats)
+            load().or_exchange_pops(self.cells, self.cells // self.pops,
+                                    self.seed & 0xFFFFFFFFFFFFFFFF, self.xround,
+                                    _p(beta, c_double), _p(self.energy, c_int64),
+                                    _p(self.temp_of, c_uint32), _p(self.stats, c_uint64))
             self.xround += 1
+
+    def resample(self, parents):
+        """apply_resample: member m = p*64 + r takes member parents[m]'s configurations."""
+        par = np.ascontiguousarray(parents, dtype=np.uint32)
+        assert par.shape == (self.pops * 64,)
+        load().or_resample(self.csr.n, self.cells, self.cells // self.pops, _p(par, c_uint32),
+                           _p(self.words, c_uint64), _p(self.temp_of, c_uint32),
+                           _p(self.energy, c_int64))
 
 
 def exchange(cells, seed, round_, beta, energies, temp_of, stats):

@@ -319,13 +319,17 @@ static uint32_t uniform_exchange(uint64_t seed, uint32_t t, uint32_t lane, uint3
  * temperature t against lane r of temperature t+1; accept iff
  * delta = (beta_t - beta_{t+1}) (E_t - E_{t+1}) >= 0 or u < exp(delta).  The
  * configurations stay in their slots and the two lanes swap temperatures. */
-void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *beta,
-                 const int64_t *energies, uint32_t *temp_of, uint64_t *stats) {
+void or_exchange_pops(uint32_t cells, uint32_t pop_temps, uint64_t seed, uint32_t round,
+                      const double *beta, const int64_t *energies, uint32_t *temp_of,
+                      uint64_t *stats) {
     const uint32_t phase = round & 1u;
+    const uint32_t T = pop_temps ? pop_temps : cells;
     uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
     for (uint32_t s = 0; s < cells; ++s)
         for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
-    for (uint32_t t = phase; t + 1 < cells; t += 2) {
+    /* engine.rs:714-716: every population p, its pairs t = phase, phase + 2, ... */
+    for (uint32_t base = 0; base + T <= cells; base += T)
+    for (uint32_t t = base + phase; t + 1 < base + T; t += 2) {
         double dbeta = beta[t] - beta[t + 1];
         for (uint32_t r = 0; r < 64; ++r) {
             uint32_t s1 = slot_of[t * 64 + r], s2 = slot_of[(t + 1) * 64 + r];
@@ -345,6 +349,42 @@ void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *be
     free(slot_of);
 }
 
+void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *beta,
+                 const int64_t *energies, uint32_t *temp_of, uint64_t *stats) {
+    or_exchange_pops(cells, cells, seed, round, beta, energies, temp_of, stats);
+}
+
+/* apply_resample (population_annealing.rs:229-286) on MSC words: member
+ * m = p*64 + r is lane r of population p over its levels t < T (temperature
+ * index p*T + t, found through temp_of); afterwards the replica of member m
+ * at level t holds what member parents[m] held there, spins and energy.
+ * words [cells][n] site order. */
+void or_resample(uint32_t n, uint32_t cells, uint32_t pop_temps, const uint32_t *parents,
+                 uint64_t *words, const uint32_t *temp_of, int64_t *energies) {
+    const uint32_t T = pop_temps ? pop_temps : cells;
+    uint32_t *slot_of = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cells * 64);
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t r = 0; r < 64; ++r) slot_of[temp_of[s * 64 + r] * 64 + r] = s;
+    uint64_t *old = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)cells * n + 8);
+    int64_t *old_e = (int64_t *)malloc(sizeof(int64_t) * (size_t)cells * 64);
+    memcpy(old, words, sizeof(uint64_t) * (size_t)cells * n);
+    memcpy(old_e, energies, sizeof(int64_t) * (size_t)cells * 64);
+    for (uint32_t d = 0; d < cells; ++d)
+        for (uint32_t r = 0; r < 64; ++r) {
+            const uint32_t g = temp_of[d * 64 + r], p = g / T, t = g % T;
+            const uint32_t m = parents[p * 64 + r], rs = m % 64;
+            const uint32_t src = slot_of[((m / 64) * T + t) * 64 + rs];
+            energies[d * 64 + r] = old_e[src * 64 + rs];
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint64_t b = (old[(uint64_t)src * n + i] >> rs) & 1u;
+                words[(uint64_t)d * n + i] = (words[(uint64_t)d * n + i] & ~(1ull << r)) | (b << r);
+            }
+        }
+    free(slot_of);
+    free(old);
+    free(old_e);
+}
+
 /* ------------------------------------------------------------------------- */
 /* The run: metropolis_sweep.rs:95-146 in colour order, then engine.rs:713-818 */
 /* With view OR_HEATBATH the sweep is gibbs_color_sweep.rs:119-174 instead:    */
@@ -353,8 +393,8 @@ void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *be
 int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
            const int64_t *linear, uint32_t cells, uint64_t seed, uint32_t view, uint32_t exchange,
            uint32_t sweep0, uint32_t xround0, uint32_t sweeps, const double *betas,
-           uint32_t betas_per_sweep, uint64_t *words, uint32_t *temp_of, int64_t *energies,
-           uint64_t *stats) {
+           uint32_t betas_per_sweep, uint32_t pop_temps, uint64_t *words, uint32_t *temp_of,
+           int64_t *energies, uint64_t *stats) {
     if (cells == 0 || view > OR_HEATBATH) return -1;
     for (uint32_t i = 0; i < cells * 64u; ++i)
         if (temp_of[i] >= cells) return -1;
@@ -400,7 +440,7 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
             }
         }
         if (exchange && cells >= 2)
-            or_exchange(cells, seed, xround0 + k, beta, energies, temp_of, stats);
+            or_exchange_pops(cells, pop_temps, seed, xround0 + k, beta, energies, temp_of, stats);
     }
     free(colors);
     free(order);

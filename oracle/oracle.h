@@ -17,6 +17,7 @@
  *   or_run (sweep part)  src/engine_v2/operators/metropolis_sweep.rs:95-146
  *                        with ΔE = (1-2x)·h from sparse_bitslice.rs:247-255
  *   or_run (OR_HEATBATH) src/engine_v2/operators/gibbs_color_sweep.rs:119-174
+ *   or_resample          src/solver/population_annealing.rs:229-286
  *   or_run (exchange)    src/solver/engine.rs:713-818 (DEO even/odd phase,
  *                        lane r of temperature t against lane r of t+1)
  *   or_geometric_ladder  src/engine_v2/decision.rs:133-142
@@ -104,12 +105,20 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
            const int64_t *w, const int64_t *linear, uint32_t cells, uint64_t seed,
            uint32_t view, uint32_t exchange, uint32_t sweep0, uint32_t xround0,
            uint32_t sweeps, const double *betas, uint32_t betas_per_sweep,
+           uint32_t pop_temps /* levels per population, 0 = cells */,
            uint64_t *words, uint32_t *temp_of, int64_t *energies, uint64_t *stats);
 
 /* Exchange number `round` (phase = round & 1; engine.rs:713-818) on tracked
  * energies; stats as or_run. */
 void or_exchange(uint32_t cells, uint64_t seed, uint32_t round, const double *beta,
                  const int64_t *energies, uint32_t *temp_of, uint64_t *stats);
+/* The same inside each population of pop_temps levels (engine.rs:714-716). */
+void or_exchange_pops(uint32_t cells, uint32_t pop_temps, uint64_t seed, uint32_t round,
+                      const double *beta, const int64_t *energies, uint32_t *temp_of,
+                      uint64_t *stats);
+/* apply_resample (population_annealing.rs:229-286); parents [cells/T*64]. */
+void or_resample(uint32_t n, uint32_t cells, uint32_t pop_temps, const uint32_t *parents,
+                 uint64_t *words, const uint32_t *temp_of, int64_t *energies);
 
 /* ---- ladders (reference helpers) ---- */
 void or_geometric_ladder(uint32_t k, double t_hi, double t_lo, double *temps);
