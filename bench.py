@@ -322,6 +322,45 @@ def bench_descent(_lib, st, qb, cells: int, sweeps: int, steps: int = 200) -> di
             "sweeps_timed_with_and_without_tracking": sweeps}
 
 
+def bench_icm_pa(_lib, instances, ctx, st, qb, cells: int) -> dict:
+    """ICM sweeps (cells x 32 Houdayer moves each) on the timed parallel-tempering state, and
+    a PA-PT solve (4 populations x 8 temperatures, 50 stages of 10 PT steps, ICM after every
+    stage) on the Erdos-Renyi N=20000 d=6 +-J workload."""
+    from ising_engine_amd import population_annealing as pa_drv
+    out = {}
+    st.icm(1)
+    st.synchronize()
+    t0 = time.perf_counter()
+    flipped = st.icm(10)
+    dt = time.perf_counter() - t0
+    if not np.array_equal(st.energies(), st.recompute_energies()):
+        raise SystemExit("tracked energies drifted from recompute after ICM")
+    out["icm_3d"] = {"ms_per_icm_sweep": dt / 10 * 1e3, "moves_per_sweep": cells * 32,
+                     "mean_cluster_sites": flipped / (10 * cells * 32)}
+    log(f"[bench] ICM: {dt / 10 * 1e3:.2f} ms per sweep of {cells * 32} moves, "
+        f"mean cluster {flipped / (10 * cells * 32):.0f} sites")
+    wl = instances.workload_er_anneal()
+    q = wl["qubo"]
+    m = _lib.Model(ctx, q.n, q.offset, q.linear, q.pi, q.pj, q.q)
+    args = dict(num_pops=4, num_temps=8, stages=50, sweeps_per_stage=10,
+                temp_max=1.0 / wl["beta_lo"], temp_min=1.0 / wl["beta_hi"], use_icm=True)
+    pa_drv.solve(ctx, m, 1, **dict(args, stages=3))
+    t0 = time.perf_counter()
+    res = pa_drv.solve(ctx, m, 2, **args)
+    dt = time.perf_counter() - t0
+    if res.energy != int(q.energy(res.x)):
+        raise SystemExit("PA-PT solution energy does not match its state")
+    pt_steps = 50 * 10 + res.resample_events * 5
+    out["pa_pt_er_n20000"] = {"seconds": dt, "best_energy": res.energy,
+                              "resample_events": res.resample_events,
+                              "min_ess": float(min(res.ess_history)),
+                              "free_energy": res.free_energy, "pt_steps": pt_steps,
+                              "attempts_per_s": pt_steps * q.n * 64 * 32 / dt}
+    log(f"[bench] PA-PT ER N=20000: {dt:.2f} s, best {res.energy}, "
+        f"{res.resample_events} resamplings")
+    return out
+
+
 def bench_gset(_lib, ctx, cpu_s: float) -> dict:
     """solve_v2 (rudy file -> MaxCut QUBO -> heat-bath anneal of 256 all-zero replicas over
     200 temperatures, best state tracked every sweep) on the four G-Set instances the
@@ -448,6 +487,7 @@ def main():
     ap.add_argument("--no-workloads", action="store_true")
     ap.add_argument("--no-descent", action="store_true")
     ap.add_argument("--no-gset", action="store_true")
+    ap.add_argument("--no-icm-pa", action="store_true")
     ap.add_argument("--tts-runs", type=int, default=32)
     ap.add_argument("--sk-sweeps", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -522,6 +562,10 @@ def main():
     if not args.no_descent:
         descent = bench_descent(_lib, st, qb, cells, args.profile_steps)
 
+    icm_pa = None
+    if not args.no_icm_pa:
+        icm_pa = bench_icm_pa(_lib, instances, ctx, st, qb, cells)
+
     batched = None
     if not args.no_batch:
         batched = bench_batch(_lib, instances, ctx, args.batch_count)
@@ -566,6 +610,7 @@ def main():
         "workloads": workloads,
         "descent": descent,
         "gset_solve_v2": gset,
+        "icm_pa": icm_pa,
     }
     print(json.dumps(out), flush=True)
 

@@ -271,7 +271,8 @@ DescentResult descend(const ige_model* model, const ige::DevState& st, const uin
 
 // ICM sweeps (asynchronous).  The scratch, a visited bitmap and a queue per
 // move, is kept on the state and capped at 256 MiB; the moves of one sweep run
-// in as many launches as that needs.
+// in as many launches as that needs (at 3D L=64 x 32, 248 moves of 1 MiB each
+// per launch measured faster than all 1024 at once: 70 against 83 ms a sweep).
 void icm_rounds(ige_state* s, uint32_t rounds, unsigned long long* flipped) {
     const uint32_t n = s->model->ir.n;
     const uint32_t pairs = s->cells * 32;
@@ -781,9 +782,12 @@ int32_t ige_model_polish(const ige_model* model, uint32_t count, uint8_t* x,
         const uint32_t n = model->ir.n;
         const uint32_t cells = (count + 63) / 64;
         const auto& order = model->co.order;
+        // Lanes past `count` in the last cell repeat the cell's first state: they
+        // follow its path and add no descent steps.
         std::vector<uint64_t> w(static_cast<size_t>(cells) * n, 0);
-        for (uint32_t i = 0; i < count; ++i) {
-            const uint8_t* xi = x + static_cast<size_t>(i) * n;
+        for (uint32_t i = 0; i < cells * 64u; ++i) {
+            const uint32_t src = i < count ? i : (i / 64) * 64;
+            const uint8_t* xi = x + static_cast<size_t>(src) * n;
             uint64_t* wc = w.data() + static_cast<size_t>(i / 64) * n;
             const uint64_t bit = 1ull << (i % 64);
             for (uint32_t p = 0; p < n; ++p) {
