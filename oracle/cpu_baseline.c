@@ -122,9 +122,10 @@ static void sweep_cell(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_
     }
 }
 
-int cb_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const double *w,
-           const double *linear, uint32_t cells, const double *temps, uint32_t steps,
-           uint64_t seed, int threads, int8_t *spins, double *energies, int *deo_odd_phase) {
+static int run_impl(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                    const double *w, const double *linear, uint32_t cells, const double *temps,
+                    uint32_t steps, uint64_t seed, int threads, int8_t *spins, double *energies,
+                    int *deo_odd_phase, uint64_t *pair_stats) {
     if (cells == 0 || threads < 1) return -1;
     uint64_t master = seed;
     const uint64_t cell_len = (uint64_t)n * NR;
@@ -146,6 +147,10 @@ int cb_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const d
                 int s = ex >= 0.0 || splitmix_f64(&master) < exp(ex);
                 mask[r] = (int8_t)(-s);
                 any |= s;
+                if (pair_stats) {
+                    pair_stats[2 * t] += 1;
+                    pair_stats[2 * t + 1] += (uint64_t)s;
+                }
             }
             if (any) {
                 int8_t *c1 = spins + t * cell_len, *c2 = spins + (t + 1) * cell_len;
@@ -166,6 +171,21 @@ int cb_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const d
         *deo_odd_phase = !*deo_odd_phase;
     }
     return 0;
+}
+
+int cb_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const double *w,
+           const double *linear, uint32_t cells, const double *temps, uint32_t steps,
+           uint64_t seed, int threads, int8_t *spins, double *energies, int *deo_odd_phase) {
+    return run_impl(n, row_ptr, col_idx, w, linear, cells, temps, steps, seed, threads, spins,
+                    energies, deo_odd_phase, NULL);
+}
+
+int cb_run_stats(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const double *w,
+                 const double *linear, uint32_t cells, const double *temps, uint32_t steps,
+                 uint64_t seed, int threads, int8_t *spins, double *energies, int *deo_odd_phase,
+                 uint64_t *pair_stats) {
+    return run_impl(n, row_ptr, col_idx, w, linear, cells, temps, steps, seed, threads, spins,
+                    energies, deo_odd_phase, pair_stats);
 }
 
 void cb_energies(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const double *w,

@@ -23,6 +23,13 @@ int cb_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const d
            const double *linear, uint32_t cells, const double *temps, uint32_t steps,
            uint64_t seed, int threads, int8_t *spins, double *energies, int *deo_odd_phase);
 
+/* cb_run, adding per pair t (t, t+1) the exchange attempts and accepts to
+ * pair_stats[2t], pair_stats[2t+1]. */
+int cb_run_stats(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const double *w,
+                 const double *linear, uint32_t cells, const double *temps, uint32_t steps,
+                 uint64_t seed, int threads, int8_t *spins, double *energies, int *deo_odd_phase,
+                 uint64_t *pair_stats);
+
 /* calculate_replica_energies_local (engine.rs:333-418) for every cell. */
 void cb_energies(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const double *w,
                  const double *linear, uint32_t cells, const int8_t *spins, double *energies);
