@@ -162,12 +162,13 @@ def cpu_sample(qb, cells, temps, budget_s: float) -> float:
     return steps * qb.n * 64 * cells / dt
 
 
-def anneal_once(_lib, model, seed, sched, kernel=0):
-    """One 64-replica anneal over `sched` (one beta per sweep), timed on the device."""
+def anneal_once(_lib, model, seed, sched, kernel=0, eager=False):
+    """One 64-replica anneal over `sched` (one beta per sweep), timed on the device; the
+    sweep loop replays the state's captured graph unless `eager`."""
     st = _lib.State(model, 1, seed, kernel=kernel)
     st.synchronize()
     t0 = time.perf_counter()
-    st.run(len(sched), beta_schedule=sched[:, None])
+    st.run(len(sched), beta_schedule=sched[:, None], eager=eager)
     st.synchronize()
     dt = time.perf_counter() - t0
     if not np.array_equal(st.energies(), st.recompute_energies()):
@@ -187,8 +188,10 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
     sched = _lib.anneal_schedule(0.1, 3.0, 1000)
     anneal_once(_lib, m, 1, sched)
     dt, st = anneal_once(_lib, m, 2, sched)
+    dt_eager, _ = anneal_once(_lib, m, 2, sched, eager=True)
     out[wl["name"]] = {"value": 1000 * qb.n * 64 / dt, "unit": "spin-flip attempts/s",
-                       "seconds": dt, "kernel": kname[st.stats()[0].kernel],
+                       "seconds": dt, "seconds_eager_launches": dt_eager,
+                       "kernel": kname[st.stats()[0].kernel],
                        "best_energy_per_spin": float(st.energies().min()) / qb.n,
                        "cpu_1thread": cpu_sample(qb, 1, [1.0], cpu_s)}
     log(f"[bench] {wl['name']}: {out[wl['name']]['value']:.3e} attempts/s ({dt * 1e3:.1f} ms)")
@@ -201,6 +204,7 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
     sched = _lib.anneal_schedule(wl["beta_lo"], wl["beta_hi"], wl["sweeps"])
     anneal_once(_lib, m, 1, sched[:50])
     dt1, st = anneal_once(_lib, m, 7, sched)
+    dt1_eager, _ = anneal_once(_lib, m, 7, sched, eager=True)
     single = wl["sweeps"] * qb.n * 64 / dt1
     R = tts_runs
     st = _lib.State(m, R, 1000)
@@ -220,7 +224,8 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
     tts = lambda p: (t_run if p >= 1.0 else (float("inf") if p <= 0.0 else
                                               t_run * np.log(0.01) / np.log(1.0 - p)))
     out[wl["name"]] = {"value": single, "unit": "spin-flip attempts/s",
-                       "seconds_per_anneal": dt1, "kernel": kname[st.stats()[0].kernel],
+                       "seconds_per_anneal": dt1, "seconds_eager_launches": dt1_eager,
+                       "kernel": kname[st.stats()[0].kernel],
                        "restarts": R, "restarts_seconds": dtR,
                        "restarts_value": R * wl["sweeps"] * qb.n * 64 / dtR,
                        "t_run_per_restart_s": t_run, "best_energy": e_best,

@@ -92,6 +92,9 @@ extern "C" {
 #define IGE_RUN_HEATBATH 4u   /* heat-bath (Gibbs) sweeps instead of Metropolis:
                                  new spin = 1 with P = 1/(1+exp(beta h)),
                                  lane-per-replica kernel on any state       */
+#define IGE_RUN_EAGER 16u     /* launch every kernel from the host instead of
+                                 replaying the captured sweep-loop graph;
+                                 the results are the same bits either way  */
 
 #define IGE_STEPS_UNLIMITED 0xFFFFFFFFu /* descend until no replica can improve */
 
@@ -178,7 +181,10 @@ int32_t ige_state_destroy(ige_state *state);
 /* beta[t] for temperature index t (cells entries, finite, >= 0). */
 int32_t ige_set_ladder(ige_state *state, const double *beta);
 /* beta_schedule: NULL (use the ladder) or [sweeps][cells]; the ladder is left
- * at the last row.  Asynchronous on the state's stream. */
+ * at the last row.  Asynchronous on the state's stream.  Runs of 32 sweeps or
+ * more replay a hipGraph of 32 captured steps (kept with the state and
+ * captured again when the flags, populations or schedule buffer change); the
+ * rest, IGE_RUN_ICM, profiling and the dense kernel launch eagerly. */
 int32_t ige_run(ige_state *state, uint32_t sweeps, uint32_t flags, const double *beta_schedule);
 /* `rounds` DEO exchanges without sweeps, on the current ladder and tracked
  * energies.  Exchange number k uses pairs (t, t+1) with t = k mod 2 and is
@@ -285,7 +291,9 @@ typedef struct {
     uint32_t polish;
 } ige_instance_desc;
 
-#define IGE_BATCH_PERSISTENT 0u /* one launch, device work queue           */
+#define IGE_BATCH_PERSISTENT 0u /* one hipGraph launch (counter resets and
+                                   one persistent kernel over a device work
+                                   queue), captured once per (waves, k0, k1) */
 #define IGE_BATCH_PLAIN 1u      /* per instance, per sweep, per colour launches */
 
 /* beta_k = beta_hi * exp(-(sweeps-1-k) c), c = ln(beta_hi/beta_lo)/(sweeps-1),

@@ -29,6 +29,7 @@ RUN_EXCHANGE = 1
 RUN_TRACK_BEST = 2
 RUN_HEATBATH = 4
 RUN_ICM = 8
+RUN_EAGER = 16
 STEPS_UNLIMITED = 0xFFFFFFFF
 
 EXPORTED = (
@@ -395,7 +396,10 @@ class State:
         check(self._lib.ige_set_ladder(self.handle, _ptr(b, c_double)))
 
     def run(self, sweeps: int, exchange: bool = False, beta_schedule=None,
-            track_best: bool = False, heatbath: bool = False, icm: bool = False) -> None:
+            track_best: bool = False, heatbath: bool = False, icm: bool = False,
+            eager: bool = False) -> None:
+        """`sweeps` sweep steps (ige_run).  Runs of 32 or more replay the state's
+        captured sweep-loop graph unless `eager`; both give the same bits."""
         sched = None
         if beta_schedule is not None:
             sched = np.ascontiguousarray(beta_schedule, dtype=np.float64)
@@ -403,7 +407,8 @@ class State:
                 raise ValueError("beta_schedule must be [sweeps][cells]")
         self._sched_keepalive = sched
         flags = ((RUN_EXCHANGE if exchange else 0) | (RUN_TRACK_BEST if track_best else 0) |
-                 (RUN_HEATBATH if heatbath else 0) | (RUN_ICM if icm else 0))
+                 (RUN_HEATBATH if heatbath else 0) | (RUN_ICM if icm else 0) |
+                 (RUN_EAGER if eager else 0))
         check(self._lib.ige_run(self.handle, int(sweeps), flags, _ptr(sched, c_double)))
 
     def steepest_descent(self, max_steps: int = STEPS_UNLIMITED, clamped=None) -> DescentReport:

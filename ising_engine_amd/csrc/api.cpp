@@ -138,6 +138,13 @@ struct ige_state {
     DevBuf<uint32_t> icm_scratch;  // ICM bitmaps and queues, on first use
     DevBuf<uint8_t> best_x;        // n bytes, site order
     DevBuf<uint32_t> best_take;    // decision of the last update
+    // Captured sweep loop: kGraphSteps steps of ige_run replayed as one graph
+    // launch, its counters read from `clock`.  Recaptured when `graph_key`
+    // (flags, populations, buffers the launches name) changes.
+    DevBuf<uint32_t> clock;
+    DevBuf<double> sched_buf;  // schedule rows of the running ige_run, grown as needed
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_key[5] = {0, 0, 0, 0, 0};
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -156,6 +163,7 @@ struct ige_state {
         return s;
     }
     ~ige_state() {
+        if (graph) (void)hipGraphExecDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) {
@@ -222,11 +230,13 @@ void finish_model(ige_model* m) {
     upload_model(m);
 }
 
-void refresh_thresholds(ige_state* s, const double* dev_beta) {
+void refresh_thresholds(ige_state* s, const double* dev_beta, const uint32_t* clock = nullptr) {
     const auto m = s->model->view();
     auto st = s->view();
+    st.clock = clock;
     if (s->kernel == IGE_KERNEL_GENERIC)
-        check(ige::launch_thr_table(dev_beta, s->thr.p, s->cells, s->dtab, s->stream), "thr");
+        check(ige::launch_thr_table(dev_beta, s->thr.p, s->cells, s->dtab, s->stream, clock),
+              "thr");
     else if (s->kernel == IGE_KERNEL_MULTISPIN)
         check(ige::launch_qplanes(m, st, dev_beta, s->qplanes.p, s->stream), "qplanes");
     // the dense kernel evaluates its thresholds from beta directly
@@ -246,6 +256,127 @@ void track_best(ige_state* s) {
     check(ige::launch_track_best(s->model->view(), s->view(), s->sweep, s->best.p,
                                  s->best_take.p, s->best_x.p, s->stream),
           "track best");
+}
+
+void icm_rounds(ige_state* s, uint32_t rounds, unsigned long long* flipped);
+
+// What one step of ige_run launches; shared by the eager loop and the
+// captured graph.
+struct StepPlan {
+    bool exch = false, track = false, heat = false, icm = false, sched = false;
+    uint32_t hb_dtab = 0;
+};
+
+// Steps per replay of the captured sweep loop.
+constexpr uint32_t kGraphSteps = 32;
+
+// One sweep step on s->stream: the thresholds (when `refresh`), the colour
+// sweeps, the exchange, an ICM sweep and the incumbent update.  sweep and
+// round are absolute, or relative to the step clock when `clock` is set
+// (inside a graph capture; no host synchronisation happens then).
+void issue_step(ige_state* s, const StepPlan& P, const double* beta, uint32_t sweep,
+                uint32_t round, bool refresh, const uint32_t* clock) {
+    const auto m = s->model->view();
+    auto st = s->view();
+    st.clock = clock;
+    const auto& cs = s->model->co.class_start;
+    if (refresh) {
+        if (P.heat)
+            check(ige::launch_heatbath_table(beta, s->hb.p, s->cells, P.hb_dtab, s->stream, clock),
+                  "heat-bath table");
+        else
+            refresh_thresholds(s, beta, clock);
+    }
+    const bool prof = s->profile && !clock;
+    if (prof) check(hipEventRecord(s->ev0, s->stream), "event");
+    const uint32_t launches =
+        (s->kernel == IGE_KERNEL_DENSE && !P.heat) ? 1u : s->model->co.num_colors;
+    if (P.heat) {
+        for (uint32_t c = 0; c < s->model->co.num_colors; ++c)
+            check(ige::launch_sweep_heatbath(m, st, cs[c], cs[c + 1], sweep, s->seed, s->hb.p,
+                                             P.hb_dtab, s->stream),
+                  "sweep_heatbath");
+    } else if (s->kernel == IGE_KERNEL_DENSE) {
+        require(clock == nullptr, "the dense sweep is not captured");
+        check(ige::launch_sweep_dense(s->model->dense_view(), st, sweep, s->seed, beta, s->stream),
+              "sweep_dense");
+    } else {
+        for (uint32_t c = 0; c < s->model->co.num_colors; ++c) {
+            if (s->kernel == IGE_KERNEL_GENERIC)
+                check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], sweep, s->seed, s->thr.p,
+                                                s->dtab, s->stream),
+                      "sweep_generic");
+            else
+                check(ige::launch_sweep_multispin(m, st, cs[c], cs[c + 1], sweep, s->seed,
+                                                  s->qplanes.p, s->stream),
+                      "sweep_multispin");
+        }
+    }
+    const auto lap = [&](double& acc, uint64_t& count, uint64_t add) {
+        check(hipEventRecord(s->ev1, s->stream), "event");
+        check(hipEventSynchronize(s->ev1), "event sync");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
+        acc += ms;
+        count += add;
+    };
+    if (prof) lap(s->prof_ms, s->prof_launches, launches);
+    if (P.exch) {
+        if (prof) check(hipEventRecord(s->ev0, s->stream), "event");
+        check(ige::launch_exchange(st, beta, round, s->seed, s->stream), "exchange");
+        if (prof) lap(s->prof_other_ms, s->prof_other_launches, 1);
+    }
+    if (P.icm) {
+        require(clock == nullptr, "cluster moves are not captured");
+        icm_rounds(s, 1, nullptr);
+    }
+    if (P.track)
+        check(ige::launch_track_best(m, st, sweep + 1, s->best.p, s->best_take.p, s->best_x.p,
+                                     s->stream),
+              "track best");
+}
+
+// The state's captured sweep loop for plan P: kGraphSteps steps with counters
+// relative to the step clock, then a node that advances the clock.  Kernel
+// arguments name the state's buffers, so the graph is recaptured when one of
+// them (or the plan) changes.
+hipGraphExec_t sweep_graph(ige_state* s, const StepPlan& P) {
+    if (!s->clock.p) s->clock.alloc(ige::kClockWords);
+    const uint64_t key[5] = {
+        (P.exch ? 1u : 0u) | (P.track ? 2u : 0u) | (P.heat ? 4u : 0u) | (P.sched ? 8u : 0u) |
+            (static_cast<uint64_t>(s->pops) << 32),
+        reinterpret_cast<uint64_t>(P.sched ? s->sched_buf.p : nullptr),
+        reinterpret_cast<uint64_t>(s->hb.p), reinterpret_cast<uint64_t>(s->best.p), 1};
+    if (s->graph && std::equal(key, key + 5, s->graph_key)) return s->graph;
+    if (s->graph) {
+        check(hipGraphExecDestroy(s->graph), "graph destroy");
+        s->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+        const bool refresh = P.sched || (!P.heat && s->kernel == IGE_KERNEL_MULTISPIN && P.exch);
+        for (uint32_t k = 0; k < kGraphSteps; ++k) {
+            const double* beta =
+                P.sched ? s->sched_buf.p + static_cast<size_t>(k) * s->cells : s->beta.p;
+            issue_step(s, P, beta, k, k, refresh, s->clock.p);
+        }
+        check(ige::launch_clock_advance(s->clock.p, kGraphSteps, P.exch ? kGraphSteps : 0,
+                                        P.sched ? kGraphSteps : 0, s->stream),
+              "clock");
+    } catch (...) {
+        (void)hipStreamEndCapture(s->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check(hipStreamEndCapture(s->stream, &g), "end capture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    check(e, "graph instantiate");
+    s->graph = exec;
+    std::copy(key, key + 5, s->graph_key);
+    return exec;
 }
 
 using ige::DescentResult;
@@ -459,92 +590,63 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         require(s != nullptr, "NULL state");
         require(sched != nullptr || s->ladder_set, "no ladder set and no beta schedule given");
         require((flags & ~(IGE_RUN_EXCHANGE | IGE_RUN_TRACK_BEST | IGE_RUN_HEATBATH |
-                           IGE_RUN_ICM)) == 0,
+                           IGE_RUN_ICM | IGE_RUN_EAGER)) == 0,
                 "unknown flag");
         require(static_cast<uint64_t>(s->icm_round) + sweeps <= 0xFFFFFFFFull,
                 "ICM counter would exceed 2^32-1");
-        const bool icm = (flags & IGE_RUN_ICM) != 0;
         require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull &&
                     static_cast<uint64_t>(s->xround) + sweeps <= 0xFFFFFFFFull,
                 "sweep or exchange counter would exceed 2^32-1");
         set_device(s->ctx);
-        const bool exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
-        const bool track = (flags & IGE_RUN_TRACK_BEST) != 0;
-        if (track) ensure_best(s);
-        const bool heat = (flags & IGE_RUN_HEATBATH) != 0;
-        const uint32_t hb_dtab = static_cast<uint32_t>(s->model->max_local);
-        if (heat && !s->hb.p) s->hb.alloc(static_cast<size_t>(s->cells) * (2ull * hb_dtab + 1));
-        DevBuf<double> sched_dev;
+        StepPlan P;
+        P.exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
+        P.track = (flags & IGE_RUN_TRACK_BEST) != 0;
+        P.heat = (flags & IGE_RUN_HEATBATH) != 0;
+        P.icm = (flags & IGE_RUN_ICM) != 0;
+        P.sched = sched != nullptr;
+        P.hb_dtab = static_cast<uint32_t>(s->model->max_local);
+        if (P.track) ensure_best(s);
+        if (P.heat && !s->hb.p)
+            s->hb.alloc(static_cast<size_t>(s->cells) * (2ull * P.hb_dtab + 1));
         if (sched) {
             const size_t total = static_cast<size_t>(sweeps) * s->cells;
             for (size_t i = 0; i < total; ++i)
                 require(std::isfinite(sched[i]) && sched[i] >= 0.0,
                         "beta must be finite and >= 0");
-            sched_dev.alloc(total);
-            sched_dev.upload(sched, total, s->stream);
+            if (s->sched_buf.count < total) s->sched_buf.alloc(total);
+            s->sched_buf.upload(sched, total, s->stream);
         }
-        const auto m = s->model->view();
-        auto st = s->view();
-        const auto& cs = s->model->co.class_start;
-        for (uint32_t k = 0; k < sweeps; ++k) {
-            const double* beta = sched ? sched_dev.p + static_cast<size_t>(k) * s->cells : s->beta.p;
+        const auto row = [&](uint32_t k) {
+            return sched ? s->sched_buf.p + static_cast<size_t>(k) * s->cells : s->beta.p;
+        };
+        // Graph replays first: kGraphSteps steps per launch.
+        uint32_t done = 0;
+        const bool graphable = !s->profile && !P.icm && !(flags & IGE_RUN_EAGER) &&
+                               (P.heat || s->kernel != IGE_KERNEL_DENSE);
+        if (graphable && sweeps >= kGraphSteps) {
+            // Step 0's heat-bath table on a fixed ladder (the graph omits it).
+            if (P.heat && !sched)
+                check(ige::launch_heatbath_table(s->beta.p, s->hb.p, s->cells, P.hb_dtab,
+                                                 s->stream),
+                      "heat-bath table");
+            const hipGraphExec_t g = sweep_graph(s, P);
+            check(ige::launch_clock_set(s->clock.p, s->sweep, s->xround, 0, s->stream), "clock");
+            const uint32_t reps = sweeps / kGraphSteps;
+            for (uint32_t r = 0; r < reps; ++r) check(hipGraphLaunch(g, s->stream), "graph launch");
+            done = reps * kGraphSteps;
+            s->sweep += done;
+            if (P.exch) s->xround += done;
+        }
+        for (uint32_t k = done; k < sweeps; ++k) {
             // Thresholds depend on beta and, for the multispin planes, on
             // which temperature each (slot, lane) holds after the last swap.
             // Heat-bath tables are indexed by temperature: only beta matters.
-            if (heat) {
-                if (sched || k == 0)
-                    check(ige::launch_heatbath_table(beta, s->hb.p, s->cells, hb_dtab, s->stream),
-                          "heat-bath table");
-            } else if (sched || (s->kernel == IGE_KERNEL_MULTISPIN && exch && k > 0)) {
-                refresh_thresholds(s, beta);
-            }
-            if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
-            const uint32_t launches =
-                (s->kernel == IGE_KERNEL_DENSE && !heat) ? 1u : s->model->co.num_colors;
-            if (heat) {
-                for (uint32_t c = 0; c < s->model->co.num_colors; ++c)
-                    check(ige::launch_sweep_heatbath(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
-                                                     s->hb.p, hb_dtab, s->stream),
-                          "sweep_heatbath");
-            } else if (s->kernel == IGE_KERNEL_DENSE)
-                check(ige::launch_sweep_dense(s->model->dense_view(), st, s->sweep, s->seed, beta,
-                                              s->stream),
-                      "sweep_dense");
-            for (uint32_t c = 0;
-                 !heat && s->kernel != IGE_KERNEL_DENSE && c < s->model->co.num_colors; ++c) {
-                if (s->kernel == IGE_KERNEL_GENERIC)
-                    check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
-                                                    s->thr.p, s->dtab, s->stream),
-                          "sweep_generic");
-                else
-                    check(ige::launch_sweep_multispin(m, st, cs[c], cs[c + 1], s->sweep, s->seed,
-                                                      s->qplanes.p, s->stream),
-                          "sweep_multispin");
-            }
-            if (s->profile) {
-                check(hipEventRecord(s->ev1, s->stream), "event");
-                check(hipEventSynchronize(s->ev1), "event sync");
-                float ms = 0.f;
-                check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
-                s->prof_ms += ms;
-                s->prof_launches += launches;
-            }
-            if (exch) {
-                if (s->profile) check(hipEventRecord(s->ev0, s->stream), "event");
-                check(ige::launch_exchange(st, beta, s->xround, s->seed, s->stream), "exchange");
-                if (s->profile) {
-                    check(hipEventRecord(s->ev1, s->stream), "event");
-                    check(hipEventSynchronize(s->ev1), "event sync");
-                    float ms = 0.f;
-                    check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
-                    s->prof_other_ms += ms;
-                    s->prof_other_launches += 1;
-                }
-                s->xround += 1;
-            }
+            const bool refresh =
+                P.heat ? (P.sched || k == 0)
+                       : (P.sched || (s->kernel == IGE_KERNEL_MULTISPIN && P.exch && k > 0));
+            issue_step(s, P, row(k), s->sweep, s->xround, refresh, nullptr);
             s->sweep += 1;
-            if (icm) icm_rounds(s, 1, nullptr);
-            if (track) track_best(s);
+            if (P.exch) s->xround += 1;
         }
         if (sched) {
             // Leave the ladder at the schedule's last row.
@@ -554,9 +656,10 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                 s->beta.upload(s->ladder.data(), s->cells, s->stream);
                 s->ladder_set = true;
             }
-            check(hipStreamSynchronize(s->stream), "run");  // sched_dev is freed here
+            // the caller may free `sched` once we return
+            check(hipStreamSynchronize(s->stream), "run");
             if (sweeps > 0) refresh_thresholds(s, s->beta.p);
-        } else if (s->kernel == IGE_KERNEL_MULTISPIN && exch && sweeps > 0) {
+        } else if (s->kernel == IGE_KERNEL_MULTISPIN && P.exch && sweeps > 0) {
             refresh_thresholds(s, s->beta.p);
         }
     });

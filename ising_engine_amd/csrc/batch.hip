@@ -67,7 +67,17 @@ struct ige_batch {
     DevBuf<int64_t> d_energy;
     DevBuf<unsigned long long> d_accepted;
     DevBuf<unsigned int> d_next;
+    // The persistent anneal as one captured graph (progress fill, counter
+    // resets, persistent kernel), kept for the (waves, k0, k1) it was
+    // captured with; dropped when a new plan moves the tickets.
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_key[3] = {0, 0, 0};
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
     ~ige_batch() {
+        drop_graph();
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -319,8 +329,30 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
         } else {
             require(mode == IGE_BATCH_PERSISTENT, "unknown batch mode");
             const uint32_t w = waves ? waves : (b->tuned_waves ? b->tuned_waves : 8);
-            set_progress(b, k0);
-            ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+            require(w == 1 || w == 2 || w == 4 || w == 8 || w == 16,
+                    "waves per instance must be 1, 2, 4, 8 or 16");
+            const uint32_t key[3] = {w, k0, k1};
+            if (!b->graph || !std::equal(key, key + 3, b->graph_key)) {
+                b->drop_graph();
+                hipGraph_t g = nullptr;
+                // Relaxed: the launch sizes itself with an occupancy query.
+                check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed), "capture");
+                try {
+                    set_progress(b, k0);
+                    ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+                } catch (...) {
+                    (void)hipStreamEndCapture(b->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                check(hipStreamEndCapture(b->stream, &g), "end capture");
+                const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) b->graph = nullptr;
+                check(e, "graph instantiate");
+                std::copy(key, key + 3, b->graph_key);
+            }
+            check(hipGraphLaunch(b->graph, b->stream), "graph launch");
         }
     });
 }
@@ -378,6 +410,7 @@ int32_t ige_batch_configure(ige_batch* b, uint32_t max_gang, uint32_t gang_sites
         b->max_gang = max_gang;
         b->gang_sites = gang_sites ? gang_sites : ige::kGangSites;
         b->spin_limit = spin_limit;
+        b->drop_graph();
         plan(b);
     });
 }

@@ -183,7 +183,9 @@ constexpr uint32_t kTrackBlock = 1024;
 
 __global__ __launch_bounds__(kTrackBlock) void track_best_kernel(
     const int64_t* __restrict__ energy, const uint32_t* __restrict__ temp_of, uint32_t cells,
-    uint32_t sweep, BestRecord* __restrict__ best, uint32_t* __restrict__ take) {
+    uint32_t sweep_rel, BestRecord* __restrict__ best, uint32_t* __restrict__ take,
+    const uint32_t* __restrict__ clock) {
+    const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     __shared__ int64_t se[kTrackBlock];
     __shared__ uint32_t sk[kTrackBlock], sq[kTrackBlock];
     const uint32_t tid = threadIdx.x;
@@ -287,7 +289,7 @@ hipError_t launch_track_best(const DevModel& m, const DevState& s, uint32_t swee
                              BestRecord* best, uint32_t* take, uint8_t* best_x,
                              hipStream_t stream) {
     hipLaunchKernelGGL(track_best_kernel, dim3(1), dim3(kTrackBlock), 0, stream, s.energy,
-                       s.temp_of, s.cells, sweep, best, take);
+                       s.temp_of, s.cells, sweep, best, take, s.clock);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(track_copy_kernel, dim3(ceil_div(m.n, kBlock)), dim3(kBlock), 0, stream,

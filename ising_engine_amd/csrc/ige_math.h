@@ -24,6 +24,12 @@ struct U32x4 {
     uint32_t v[4];
 };
 
+// A counter of a sweep step: `rel` as launched, plus the step clock's word
+// when the launch belongs to a replayed graph (clock != NULL).
+__device__ __forceinline__ uint32_t clocked(const uint32_t* clock, uint32_t word, uint32_t rel) {
+    return clock ? rel + clock[word] : rel;
+}
+
 // a ^ b ^ c; one v_bitop3_b32 on the device.
 IGE_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 #ifdef __HIP_DEVICE_COMPILE__

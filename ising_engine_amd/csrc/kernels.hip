@@ -53,8 +53,9 @@ uint64_t lane_per_replica_chunk(uint32_t len, uint32_t cells) {
 // Threshold table: one thread per (t, d).
 // ---------------------------------------------------------------------------
 __global__ void thr_table_kernel(const double* __restrict__ beta, uint32_t* __restrict__ thr,
-                                 uint32_t cells, uint32_t dtab) {
+                                 uint32_t cells, uint32_t dtab, const uint32_t* __restrict__ clock) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    beta += (uint64_t)clocked(clock, kClockRow, 0) * cells;
     const uint64_t row = (uint64_t)dtab + 1;
     if (i >= row * cells) return;
     const uint32_t t = (uint32_t)(i / row);
@@ -73,8 +74,10 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
     const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr, uint32_t n, uint32_t cells,
     const uint32_t* __restrict__ temp_of, const uint32_t* __restrict__ thr, uint32_t dtab,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
-    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
+    const uint32_t* __restrict__ clock) {
     extern __shared__ uint32_t lds_thr[];
+    const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     if (kLdsThr) {
         const uint32_t total = cells * (dtab + 1);
@@ -112,8 +115,10 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
 // thresholds in [0, 2^32] (launch_heatbath_table).
 // ---------------------------------------------------------------------------
 __global__ void heatbath_table_kernel(const double* __restrict__ beta, uint64_t* __restrict__ hb,
-                                      uint32_t cells, uint32_t dtab) {
+                                      uint32_t cells, uint32_t dtab,
+                                      const uint32_t* __restrict__ clock) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    beta += (uint64_t)clocked(clock, kClockRow, 0) * cells;
     const uint64_t row = 2ull * dtab + 1;
     if (i >= row * cells) return;
     const uint32_t t = (uint32_t)(i / row);
@@ -128,8 +133,10 @@ __global__ __launch_bounds__(kBlock) void sweep_heatbath_kernel(
     const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr, uint32_t n, uint32_t cells,
     const uint32_t* __restrict__ temp_of, const uint64_t* __restrict__ hb, uint32_t dtab,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
-    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
+    const uint32_t* __restrict__ clock) {
     extern __shared__ uint64_t lds_hb[];
+    const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t row = 2 * dtab + 1;
     if (kLdsThr) {
@@ -164,17 +171,23 @@ __global__ __launch_bounds__(kBlock) void sweep_heatbath_kernel(
 // ---------------------------------------------------------------------------
 // DEO replica exchange number `round`: one wave per adjacent pair (t, t+1)
 // with t = round mod 2; lane r decides for replica lane r and swaps
-// temperature indices.
+// temperature indices.  Under a step clock the phase is known only on the
+// device: the grid then covers the larger phase and waves past the
+// population's last pair return.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void exchange_kernel(
     const double* __restrict__ beta, const int64_t* __restrict__ energy, uint32_t* temp_of,
     uint32_t* slot_of, unsigned long long* pair_stats, uint32_t cells, uint32_t pop_temps,
-    uint32_t pairs_per_pop, uint32_t round, uint32_t key0, uint32_t key1) {
+    uint32_t pairs_per_pop, uint32_t round_rel, uint32_t key0, uint32_t key1,
+    const uint32_t* __restrict__ clock) {
     const uint32_t r = threadIdx.x;
+    const uint32_t round = clocked(clock, kClockRound, round_rel);
+    beta += (uint64_t)clocked(clock, kClockRow, 0) * cells;
     // populations of pop_temps levels each: pairs stay inside one population
     const uint32_t pop = blockIdx.x / pairs_per_pop, k = blockIdx.x % pairs_per_pop;
-    const uint32_t t = pop * pop_temps + (round & 1u) + 2u * k;
-    if (t + 1 >= cells) return;
+    const uint32_t local = (round & 1u) + 2u * k;
+    if (local + 1 >= pop_temps) return;
+    const uint32_t t = pop * pop_temps + local;
     const uint32_t s1 = slot_of[t * kWave + r];
     const uint32_t s2 = slot_of[(t + 1) * kWave + r];
     const double delta = (beta[t] - beta[t + 1]) *
@@ -249,8 +262,10 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t* __restrict__ sgn, const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr,
     uint32_t n, uint32_t dmax, int32_t w0, const unsigned long long* __restrict__ qplanes,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
-    uint32_t per_lane, uint32_t sweep, uint32_t key0, uint32_t key1) {
+    uint32_t per_lane, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
+    const uint32_t* __restrict__ clock) {
     static_assert(kMaxPerLane <= 16 && W >= B + 4, "energy counters too narrow");
+    const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     constexpr int LMAX = DEG > 0 ? DEG - DEG / 2 : (1 << (B - 1));
     extern __shared__ unsigned long long lds_q[];
     __shared__ uint32_t queue[kWavesPerBlock][kQueueCap];
@@ -304,15 +319,6 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         uint64_t c[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) c[b] = 0;
-        auto add_neighbour = [&](uint32_t e) {
-            uint64_t carry = x ^ S[ci[e0 + e]] ^ (((sg >> e) & 1u) ? ~0ull : 0ull);
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const uint64_t s = c[b] ^ carry;
-                carry &= c[b];
-                c[b] = s;
-            }
-        };
         auto bond = [&](uint32_t e) {
             return x ^ S[ci[e0 + e]] ^ (((sg >> e) & 1u) ? ~0ull : 0ull);
         };
@@ -338,7 +344,33 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
                 c[2] = c1 & t;
             }
         } else {
-            for (uint32_t e = 0; e < d; ++e) add_neighbour(e);
+            // Rows of any length in groups of kRowGroup entries: a group's
+            // index loads issue together, then its word loads, so a row costs
+            // two memory round trips per group rather than two per neighbour
+            // (the sweep of a small colour class is one wave's walk along its
+            // rows).  Entries past the row's end repeat its last one and are
+            // masked out of the count.
+            constexpr uint32_t kRowGroup = 8;
+            for (uint32_t g = 0; g < d; g += kRowGroup) {
+                uint32_t j[kRowGroup];
+                uint64_t wj[kRowGroup];
+#pragma unroll
+                for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ci[e0 + min(g + q, d - 1)];
+#pragma unroll
+                for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = S[j[q]];
+#pragma unroll
+                for (uint32_t q = 0; q < kRowGroup; ++q) {
+                    const uint32_t e = g + q;
+                    uint64_t carry = (x ^ wj[q] ^ (((sg >> (e & 31u)) & 1u) ? ~0ull : 0ull)) &
+                                     (e < d ? ~0ull : 0ull);
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        const uint64_t s = c[b] ^ carry;
+                        carry &= c[b];
+                        c[b] = s;
+                    }
+                }
+            }
         }
         // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
         uint64_t lev[LMAX];
@@ -460,8 +492,10 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
 __global__ __launch_bounds__(kWave) void qplanes_kernel(const double* __restrict__ beta,
                                                         const uint32_t* __restrict__ temp_of,
                                                         unsigned long long* qplanes,
-                                                        uint32_t dmax, int32_t w0) {
+                                                        uint32_t dmax, int32_t w0,
+                                                        const uint32_t* __restrict__ clock) {
     const uint32_t r = threadIdx.x;
+    beta += (uint64_t)clocked(clock, kClockRow, 0) * gridDim.y;
     const uint32_t m = blockIdx.x;
     const uint32_t slot = blockIdx.y;
     const int64_t de = ((int64_t)m * w0) / 2;
@@ -477,6 +511,14 @@ __global__ __launch_bounds__(kWave) void qplanes_kernel(const double* __restrict
     if (r < kEagerPlanes) out[r] = mine;
     if (r == kEagerPlanes) out[kEagerPlanes] = nz;
     reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[r] = th;
+}
+
+__global__ __launch_bounds__(64) void clock_kernel(uint32_t* clock, uint32_t a, uint32_t b,
+                                                  uint32_t c, uint32_t add) {
+    if (threadIdx.x != 0) return;
+    clock[kClockSweep] = (add ? clock[kClockSweep] : 0u) + a;
+    clock[kClockRound] = (add ? clock[kClockRound] : 0u) + b;
+    clock[kClockRow] = (add ? clock[kClockRow] : 0u) + c;
 }
 
 __global__ void identity_perm_kernel(uint32_t* temp_of, uint32_t* slot_of, uint32_t cells) {
@@ -566,11 +608,24 @@ __global__ __launch_bounds__(kBlock) void fields_kernel(
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_thr_table(const double* beta, uint32_t* thr, uint32_t cells, uint32_t dtab,
+hipError_t launch_clock_set(uint32_t* clock, uint32_t sweep, uint32_t round, uint32_t row,
                             hipStream_t stream) {
+    hipLaunchKernelGGL(clock_kernel, dim3(1), dim3(64), 0, stream, clock, sweep, round, row, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_advance(uint32_t* clock, uint32_t dsweep, uint32_t dround, uint32_t drow,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(clock_kernel, dim3(1), dim3(64), 0, stream, clock, dsweep, dround, drow,
+                       1u);
+    return hipGetLastError();
+}
+
+hipError_t launch_thr_table(const double* beta, uint32_t* thr, uint32_t cells, uint32_t dtab,
+                            hipStream_t stream, const uint32_t* clock) {
     const uint64_t total = ((uint64_t)dtab + 1) * cells;
     hipLaunchKernelGGL(thr_table_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, stream,
-                       beta, thr, cells, dtab);
+                       beta, thr, cells, dtab, clock);
     return hipGetLastError();
 }
 
@@ -589,21 +644,21 @@ hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c
         hipLaunchKernelGGL(sweep_generic_kernel<true>, grid, dim3(kBlock), (uint32_t)lds_bytes,
                            stream, m.rp, m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells,
                            s.temp_of, thr, dtab, s.energy, s.accepted, c_begin, c_end,
-                           (uint32_t)chunk, waves_per_slot, sweep, key0, key1);
+                           (uint32_t)chunk, waves_per_slot, sweep, key0, key1, s.clock);
     } else {
         hipLaunchKernelGGL(sweep_generic_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
                            m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, thr,
                            dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
-                           waves_per_slot, sweep, key0, key1);
+                           waves_per_slot, sweep, key0, key1, s.clock);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_heatbath_table(const double* beta, uint64_t* hb, uint32_t cells, uint32_t dtab,
-                                 hipStream_t stream) {
+                                 hipStream_t stream, const uint32_t* clock) {
     const uint64_t total = (2ull * dtab + 1) * cells;
     hipLaunchKernelGGL(heatbath_table_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0,
-                       stream, beta, hb, cells, dtab);
+                       stream, beta, hb, cells, dtab, clock);
     return hipGetLastError();
 }
 
@@ -622,12 +677,12 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
         hipLaunchKernelGGL(sweep_heatbath_kernel<true>, grid, dim3(kBlock), (uint32_t)lds_bytes,
                            stream, m.rp, m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells,
                            s.temp_of, hb, dtab, s.energy, s.accepted, c_begin, c_end,
-                           (uint32_t)chunk, waves_per_slot, sweep, key0, key1);
+                           (uint32_t)chunk, waves_per_slot, sweep, key0, key1, s.clock);
     } else {
         hipLaunchKernelGGL(sweep_heatbath_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
                            m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, hb,
                            dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
-                           waves_per_slot, sweep, key0, key1);
+                           waves_per_slot, sweep, key0, key1, s.clock);
     }
     return hipGetLastError();
 }
@@ -651,7 +706,7 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
 #define IGE_MS_LAUNCH(B, W, DEG)                                                                   \
     hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp, \
                        m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,       \
-                       s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1)
+                       s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1, s.clock)
     // Counter width W = B + 4: at most kMaxPerLane = 16 sites per lane, each
     // adding k * flip <= 2^B - 1, so every sum stays below 2^(B+4).
     if (m.regular_degree == 6) {
@@ -674,7 +729,7 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
 hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
                           unsigned long long* qplanes, hipStream_t stream) {
     hipLaunchKernelGGL(qplanes_kernel, dim3(m.dmax + 1, s.cells), dim3(kWave), 0, stream, beta,
-                       s.temp_of, qplanes, m.dmax, m.w0);
+                       s.temp_of, qplanes, m.dmax, m.w0, s.clock);
     return hipGetLastError();
 }
 
@@ -682,14 +737,15 @@ hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round
                            hipStream_t stream) {
     const uint32_t T = s.pop_temps ? s.pop_temps : s.cells;  // levels per population
     if (T < 2 || s.cells % T) return s.cells % T ? hipErrorInvalidValue : hipSuccess;
-    const uint32_t phase = round & 1u;
-    // per population t = phase, phase+2, ... with t+1 < T
+    // per population t = phase, phase+2, ... with t+1 < T; under a clock the
+    // phase is the device's, so cover phase 0, which has the most pairs
+    const uint32_t phase = s.clock ? 0u : (round & 1u);
     const uint32_t per_pop = (T - phase) / 2;
     if (per_pop == 0) return hipSuccess;
     const uint32_t pairs = per_pop * (s.cells / T);
     hipLaunchKernelGGL(exchange_kernel, dim3(pairs), dim3(kWave), 0, stream, beta, s.energy,
                        s.temp_of, s.slot_of, s.pair_stats, s.cells, T, per_pop, round,
-                       (uint32_t)seed, (uint32_t)(seed >> 32));
+                       (uint32_t)seed, (uint32_t)(seed >> 32), s.clock);
     return hipGetLastError();
 }
 

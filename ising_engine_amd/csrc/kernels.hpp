@@ -35,6 +35,13 @@ struct DevDense {
     uint32_t dtab = 0;              // bound on |dE| (max local field)
 };
 
+// Step clock of a captured sweep loop (ige_run's graphs): kernels add
+// clock[kClockSweep] to their sweep number, clock[kClockRound] to their
+// exchange round and clock[kClockRow] * cells to their beta pointer, and a
+// final node of each replay advances the three.  NULL for eager launches,
+// whose arguments are absolute.
+enum : uint32_t { kClockSweep = 0, kClockRound = 1, kClockRow = 2, kClockWords = 4 };
+
 struct DevState {
     uint32_t cells = 0;
     uint64_t* spins = nullptr;      // [cells][n] MSC words, position order
@@ -44,11 +51,21 @@ struct DevState {
     unsigned long long* accepted = nullptr;    // 1 counter
     unsigned long long* pair_stats = nullptr;  // [cells-1][2] attempts, accepts
     uint32_t pop_temps = 0;  // levels per population (0: one population of all cells)
+    const uint32_t* clock = nullptr;  // kClockWords step counters, or NULL
 };
 
-// thr[t*(dtab+1) + d] = metropolis_threshold(beta[t], d) for d in [1, dtab].
-hipError_t launch_thr_table(const double* beta, uint32_t* thr, uint32_t cells, uint32_t dtab,
+// clock[kClockSweep, kClockRound, kClockRow] = (sweep, round, row), and
+// clock[...] += (dsweep, dround, drow): one thread each, no memset (these
+// run inside captured graphs).
+hipError_t launch_clock_set(uint32_t* clock, uint32_t sweep, uint32_t round, uint32_t row,
                             hipStream_t stream);
+hipError_t launch_clock_advance(uint32_t* clock, uint32_t dsweep, uint32_t dround, uint32_t drow,
+                                hipStream_t stream);
+
+// thr[t*(dtab+1) + d] = metropolis_threshold(beta[t], d) for d in [1, dtab].
+// clock: NULL, or a step clock whose row advances beta by rows of `cells`.
+hipError_t launch_thr_table(const double* beta, uint32_t* thr, uint32_t cells, uint32_t dtab,
+                            hipStream_t stream, const uint32_t* clock = nullptr);
 
 // Generic sweep of one colour class (lane = replica).
 hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c_begin,
@@ -59,7 +76,7 @@ hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c
 // for h in [-dtab, dtab], and the heat-bath sweep of one colour class
 // (lane = replica, the generic kernel's uniforms and mapping).
 hipError_t launch_heatbath_table(const double* beta, uint64_t* hb, uint32_t cells, uint32_t dtab,
-                                 hipStream_t stream);
+                                 hipStream_t stream, const uint32_t* clock = nullptr);
 hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t c_begin,
                                  uint32_t c_end, uint32_t sweep, uint64_t seed, const uint64_t* hb,
                                  uint32_t dtab, hipStream_t stream);

@@ -99,6 +99,24 @@ def test_split_runs_resume_exactly_and_autotune_resets(ctx):
         np.testing.assert_array_equal(got[1], full[i][1])
 
 
+def test_replayed_anneal_graph_reproduces_the_run(ctx):
+    """The persistent anneal is captured once per (waves, k0, k1) and replayed: replays after
+    a reset, with host reads in between, give the first run's results."""
+    qubos, seeds, sw, b_lo, b_hi = small_batch(count=7, sweeps=14, seed=8)
+    batch = _lib.Batch(ctx, qubos, seeds, sw, b_lo, b_hi)
+    batch.run(0, waves=2)
+    first = [batch.read(i) for i in range(7)]
+    for _ in range(3):
+        batch.reset()
+        batch.run(0, waves=2)
+        batch.synchronize()
+        for i in range(7):
+            got = batch.read(i)
+            np.testing.assert_array_equal(got[0], first[i][0])
+            np.testing.assert_array_equal(got[1], first[i][1])
+            assert got[2] == first[i][2]
+
+
 def test_single_state_with_the_schedule_reproduces_an_instance(ctx):
     qubos, seeds, sw, b_lo, b_hi = small_batch(count=4, sweeps=18, seed=9)
     batch = _lib.Batch(ctx, qubos, seeds, sw, b_lo, b_hi)
