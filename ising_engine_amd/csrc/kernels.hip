@@ -231,6 +231,43 @@ constexpr uint32_t kQueueCap = 1024; // deferred entries per wave
 constexpr uint32_t kMaxPerLane = 16; // sites per lane
 constexpr uint32_t kTagTail = 5u;
 
+// Bitwise functions of three 64-bit words, one v_bitop3_b32 per 32-bit half.
+// The truth-table byte of f(a, b, c) is f(0xF0, 0xCC, 0xAA): bit i of the
+// table is f of bits (i >> 2) & 1, (i >> 1) & 1 and i & 1 of the operands.
+constexpr uint32_t kXor3 = (0xF0 ^ 0xCC ^ 0xAA) & 0xFF;                         // a ^ b ^ c
+constexpr uint32_t kMaj = ((0xF0 & 0xCC) | (0xF0 & 0xAA) | (0xCC & 0xAA)) & 0xFF;  // majority
+constexpr uint32_t kAndOr = ((0xF0 & 0xCC) | 0xAA) & 0xFF;                      // (a & b) | c
+constexpr uint32_t kAndAndNot = (0xF0 & 0xCC & ~0xAA) & 0xFF;                   // a & b & ~c
+constexpr uint32_t kAndXnor = (0xF0 & ~(0xCC ^ 0xAA)) & 0xFF;                   // a & ~(b ^ c)
+constexpr uint32_t kNotOr = (~0xF0 | 0xCC) & 0xFF;                              // ~a | b
+static_assert(kXor3 == 0x96 && kMaj == 0xE8, "truth tables");
+
+// Truth table of "the three planes (a, b, c) spell the count k" (bit 0 in a).
+constexpr uint32_t count3_tt(uint32_t k) {
+    return ((k & 1u) ? 0xF0u : 0x0Fu) & ((k & 2u) ? 0xCCu : 0x33u) & ((k & 4u) ? 0xAAu : 0x55u);
+}
+
+template <uint32_t TT>
+__device__ __forceinline__ uint64_t bop3(uint64_t a, uint64_t b, uint64_t c) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, TT);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32),
+                                                    (uint32_t)(c >> 32), TT);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// As bop3 with the same 32-bit c in both halves.
+template <uint32_t TT>
+__device__ __forceinline__ uint64_t bop3s(uint64_t a, uint64_t b, uint32_t c) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, c, TT);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), c, TT);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// All ones if bit e of sg is set, else zero.
+__device__ __forceinline__ uint32_t bit_mask(uint32_t sg, uint32_t e) {
+    return (uint32_t)((int32_t)(sg << (31u - e)) >> 31);
+}
+
 // Sum over the wave's lanes of the per-lane bit-sliced counter acc[0..W-1],
 // returned in lane r for replica r.
 template <int W>
@@ -320,14 +357,13 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
 #pragma unroll
         for (int b = 0; b < B; ++b) c[b] = 0;
         auto bond = [&](uint32_t e) {
-            return x ^ S[ci[e0 + e]] ^ (((sg >> e) & 1u) ? ~0ull : 0ull);
+            return bop3s<kXor3>(x, S[ci[e0 + e]], bit_mask(sg, e));
         };
         if (DEG == 6 || DEG == 4) {
             // carry-save adder tree: c = popcount of the bond words, bit-sliced
             auto fa = [](uint64_t a, uint64_t b, uint64_t cin, uint64_t& sum, uint64_t& cout) {
-                const uint64_t t = a ^ b;
-                sum = t ^ cin;
-                cout = (a & b) | (cin & t);
+                sum = bop3<kXor3>(a, b, cin);
+                cout = bop3<kMaj>(a, b, cin);
             };
             uint64_t s1, c1;
             fa(bond(0), bond(1), bond(2), s1, c1);
@@ -361,7 +397,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
 #pragma unroll
                 for (uint32_t q = 0; q < kRowGroup; ++q) {
                     const uint32_t e = g + q;
-                    uint64_t carry = (x ^ wj[q] ^ (((sg >> (e & 31u)) & 1u) ? ~0ull : 0ull)) &
+                    uint64_t carry = bop3s<kXor3>(x, wj[q], bit_mask(sg, e & 31u)) &
                                      (e < d ? ~0ull : 0ull);
 #pragma unroll
                     for (int b = 0; b < B; ++b) {
@@ -376,16 +412,30 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         uint64_t lev[LMAX];
         uint32_t mrow[LMAX];
         uint64_t pos = 0, und = 0;
+        if constexpr (DEG == 6 || DEG == 4) {
+            // d fixed: each level is one three-input function of the count planes
+            static_assert(B == 3, "regular lattices use three count planes");
+            lev[0] = bop3<count3_tt(DEG / 2 + 1)>(c[0], c[1], c[2]);
+            lev[1] = bop3<count3_tt(DEG / 2 + 2)>(c[0], c[1], c[2]);
+            if constexpr (DEG == 6) lev[2] = bop3<count3_tt(DEG / 2 + 3)>(c[0], c[1], c[2]);
 #pragma unroll
-        for (int j = 0; j < LMAX; ++j) {
-            const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
-            uint64_t eq = k <= d ? ~0ull : 0ull;
+            for (int j = 0; j < LMAX; ++j) {
+                mrow[j] = (2 * (DEG / 2 + 1 + j) - DEG) * kQStride;
+                pos |= lev[j];
+                und = bop3<kAndOr>(lev[j], lds_q[mrow[j] + kEagerPlanes], und);
+            }
+        } else {
 #pragma unroll
-            for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
-            lev[j] = eq;
-            mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
-            pos |= eq;
-            und |= eq & lds_q[mrow[j] + kEagerPlanes];
+            for (int j = 0; j < LMAX; ++j) {
+                const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
+                uint64_t eq = k <= d ? ~0ull : 0ull;
+#pragma unroll
+                for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
+                lev[j] = eq;
+                mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
+                pos |= eq;
+                und = bop3<kAndOr>(eq, lds_q[mrow[j] + kEagerPlanes], und);
+            }
         }
         // kEagerPlanes lockstep bit planes, two per Philox call.
         uint64_t lt = 0;
@@ -397,24 +447,34 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
                 const uint32_t b = 2 * j + h;
                 const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
                                           : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
-                uint64_t T = 0;
+                uint64_t T = lev[0] & lds_q[mrow[0] + b];
 #pragma unroll
-                for (int l = 0; l < LMAX; ++l) T |= lev[l] & lds_q[mrow[l] + b];
-                lt |= und & T & ~P;
-                und &= ~(P ^ T);
+                for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], lds_q[mrow[l] + b], T);
+                lt |= bop3<kAndAndNot>(und, T, P);
+                und = bop3<kAndXnor>(und, P, T);
             }
         }
-        const uint64_t flip = ~pos | lt;
+        const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
         Wr[p] = x ^ flip;
         nacc += (uint32_t)__builtin_popcountll(flip);
         // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
         uint64_t ck = 0, cd = DEG > 0 ? flip : 0ull;
 #pragma unroll
         for (int b = 0; b < W; ++b) {
-            const uint64_t ak = b < B ? (c[b] & flip) : 0ull;
-            const uint64_t sk = kacc[b] ^ ak ^ ck;
-            ck = (kacc[b] & ak) | (ck & (kacc[b] ^ ak));
-            kacc[b] = sk;
+            if (b == 0) {  // half adder: no carry in
+                const uint64_t ak = c[0] & flip;
+                ck = kacc[0] & ak;
+                kacc[0] ^= ak;
+            } else if (b < B) {  // full adder
+                const uint64_t ak = c[b] & flip;
+                const uint64_t sk = bop3<kXor3>(kacc[b], ak, ck);
+                ck = bop3<kMaj>(kacc[b], ak, ck);
+                kacc[b] = sk;
+            } else {  // carry propagation only
+                const uint64_t sk = kacc[b] ^ ck;
+                ck &= kacc[b];
+                kacc[b] = sk;
+            }
             if (DEG > 0) {
                 if (b < kCnt) {  // dacc += flip (incrementer)
                     const uint64_t sd = dacc[b] ^ cd;
@@ -423,8 +483,8 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
                 }
             } else {
                 const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
-                const uint64_t sd = dacc[b] ^ ad ^ cd;
-                cd = (dacc[b] & ad) | (cd & (dacc[b] ^ ad));
+                const uint64_t sd = bop3<kXor3>(dacc[b], ad, cd);
+                cd = bop3<kMaj>(dacc[b], ad, cd);
                 dacc[b] = sd;
             }
         }
