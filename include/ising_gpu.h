@@ -120,6 +120,10 @@ typedef struct {
     uint64_t proposed; /* proposals (= sweeps * n * 64 * cells)         */
     uint32_t phase;    /* DEO phase of the next exchange (= exchanges & 1) */
     uint32_t kernel;   /* IGE_KERNEL_GENERIC, _MULTISPIN or _DENSE      */
+    uint64_t graph_sweeps;    /* sweeps run by replays of the captured loop */
+    uint64_t graph_fallbacks; /* ige_run calls whose capture was invalidated
+                                 (another thread synchronised the device
+                                 meanwhile) and that ran eagerly instead    */
 } ige_stats;
 
 /* ---- context ---- */

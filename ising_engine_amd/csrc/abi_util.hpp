@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -86,5 +87,16 @@ struct DevBuf {
 };
 
 inline int context_device(const ige_ctx* ctx) { return ctx->device; }
+
+// Sweep loops and batch anneals replay captured hipGraphs unless the
+// environment sets IGE_NO_GRAPH to a non-empty value other than "0" (for tools
+// that cannot follow graph launches); results are the same bits either way.
+inline bool graphs_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("IGE_NO_GRAPH");
+        return !(v && *v && !(v[0] == '0' && v[1] == '\0'));
+    }();
+    return on;
+}
 
 }  // namespace ige::abi

@@ -145,6 +145,8 @@ struct ige_state {
     DevBuf<double> sched_buf;  // schedule rows of the running ige_run, grown as needed
     hipGraphExec_t graph = nullptr;
     uint64_t graph_key[5] = {0, 0, 0, 0, 0};
+    uint64_t graph_fallbacks = 0;  // captures that failed (those calls ran eagerly)
+    uint64_t graph_sweeps = 0;     // sweeps run by graph replays
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -163,13 +165,11 @@ struct ige_state {
         return s;
     }
     ~ige_state() {
+        if (stream) (void)hipStreamSynchronize(stream);  // no replay still in flight
         if (graph) (void)hipGraphExecDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -349,11 +349,19 @@ hipGraphExec_t sweep_graph(ige_state* s, const StepPlan& P) {
         reinterpret_cast<uint64_t>(s->hb.p), reinterpret_cast<uint64_t>(s->best.p), 1};
     if (s->graph && std::equal(key, key + 5, s->graph_key)) return s->graph;
     if (s->graph) {
+        check(hipStreamSynchronize(s->stream), "graph retire");  // its replays have ended
         check(hipGraphExecDestroy(s->graph), "graph destroy");
         s->graph = nullptr;
     }
+    // A capture can be invalidated from outside (another thread's device-wide
+    // synchronisation while it runs); nothing was launched then, so the call
+    // falls back to eager launches and the next call captures again.
     hipGraph_t g = nullptr;
-    check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    bool recorded = true;
     try {
         const bool refresh = P.sched || (!P.heat && s->kernel == IGE_KERNEL_MULTISPIN && P.exch);
         for (uint32_t k = 0; k < kGraphSteps; ++k) {
@@ -364,16 +372,20 @@ hipGraphExec_t sweep_graph(ige_state* s, const StepPlan& P) {
         check(ige::launch_clock_advance(s->clock.p, kGraphSteps, P.exch ? kGraphSteps : 0,
                                         P.sched ? kGraphSteps : 0, s->stream),
               "clock");
-    } catch (...) {
-        (void)hipStreamEndCapture(s->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
+    } catch (const std::exception&) {
+        recorded = false;
     }
-    check(hipStreamEndCapture(s->stream, &g), "end capture");
+    const hipError_t end = hipStreamEndCapture(s->stream, &g);
     hipGraphExec_t exec = nullptr;
-    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    check(e, "graph instantiate");
+    const hipError_t inst = (recorded && end == hipSuccess && g)
+                                ? hipGraphInstantiate(&exec, g, nullptr, nullptr, 0)
+                                : hipErrorStreamCaptureInvalidated;
+    if (g) (void)hipGraphDestroy(g);
+    if (inst != hipSuccess) {
+        (void)hipGetLastError();
+        s->graph_fallbacks += 1;
+        return nullptr;
+    }
     s->graph = exec;
     std::copy(key, key + 5, s->graph_key);
     return exec;
@@ -621,21 +633,27 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         };
         // Graph replays first: kGraphSteps steps per launch.
         uint32_t done = 0;
-        const bool graphable = !s->profile && !P.icm && !(flags & IGE_RUN_EAGER) &&
+        const bool graphable = ige::abi::graphs_enabled() && !s->profile && !P.icm &&
+                               !(flags & IGE_RUN_EAGER) &&
                                (P.heat || s->kernel != IGE_KERNEL_DENSE);
         if (graphable && sweeps >= kGraphSteps) {
-            // Step 0's heat-bath table on a fixed ladder (the graph omits it).
-            if (P.heat && !sched)
-                check(ige::launch_heatbath_table(s->beta.p, s->hb.p, s->cells, P.hb_dtab,
-                                                 s->stream),
-                      "heat-bath table");
             const hipGraphExec_t g = sweep_graph(s, P);
-            check(ige::launch_clock_set(s->clock.p, s->sweep, s->xround, 0, s->stream), "clock");
-            const uint32_t reps = sweeps / kGraphSteps;
-            for (uint32_t r = 0; r < reps; ++r) check(hipGraphLaunch(g, s->stream), "graph launch");
-            done = reps * kGraphSteps;
-            s->sweep += done;
-            if (P.exch) s->xround += done;
+            if (g) {
+                // Step 0's heat-bath table on a fixed ladder (the graph omits it).
+                if (P.heat && !sched)
+                    check(ige::launch_heatbath_table(s->beta.p, s->hb.p, s->cells, P.hb_dtab,
+                                                     s->stream),
+                          "heat-bath table");
+                check(ige::launch_clock_set(s->clock.p, s->sweep, s->xround, 0, s->stream),
+                      "clock");
+                const uint32_t reps = sweeps / kGraphSteps;
+                for (uint32_t r = 0; r < reps; ++r)
+                    check(hipGraphLaunch(g, s->stream), "graph launch");
+                done = reps * kGraphSteps;
+                s->graph_sweeps += done;
+                s->sweep += done;
+                if (P.exch) s->xround += done;
+            }
         }
         for (uint32_t k = done; k < sweeps; ++k) {
             // Thresholds depend on beta and, for the multispin planes, on
@@ -766,6 +784,8 @@ int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
         stats->proposed = static_cast<uint64_t>(s->sweep) * s->model->ir.n * 64ull * s->cells;
         stats->phase = s->xround & 1u;
         stats->kernel = s->kernel;
+        stats->graph_sweeps = s->graph_sweeps;
+        stats->graph_fallbacks = s->graph_fallbacks;
         for (uint32_t t = 0; t + 1 < s->cells; ++t) {
             if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];
             if (pair_accepts) pair_accepts[t] = c[2 + 2 * t];

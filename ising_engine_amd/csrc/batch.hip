@@ -73,7 +73,9 @@ struct ige_batch {
     hipGraphExec_t graph = nullptr;
     uint32_t graph_key[3] = {0, 0, 0};
     void drop_graph() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        if (!graph) return;
+        (void)hipStreamSynchronize(stream);  // its launches have ended
+        (void)hipGraphExecDestroy(graph);
         graph = nullptr;
     }
     ~ige_batch() {
@@ -140,10 +142,14 @@ void plan(ige_batch* b) {
 
 void check_error_flag(ige_batch* b) {
     uint32_t e = 0;
-    check(hipMemcpy(&e, b->d_err.p, sizeof(e), hipMemcpyDeviceToHost), "error flag");
+    check(hipMemcpyAsync(&e, b->d_err.p, sizeof(e), hipMemcpyDeviceToHost, b->stream),
+          "error flag");
+    check(hipStreamSynchronize(b->stream), "error flag");
     if (e) {
         const uint32_t z = 0;
-        check(hipMemcpy(b->d_err.p, &z, sizeof(z), hipMemcpyHostToDevice), "error flag");
+        check(hipMemcpyAsync(b->d_err.p, &z, sizeof(z), hipMemcpyHostToDevice, b->stream),
+              "error flag");
+        check(hipStreamSynchronize(b->stream), "error flag");
         throw Failure(IGE_ERR_HIP, "a gang barrier wait exceeded its spin limit; results invalid");
     }
 }
@@ -332,24 +338,39 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
             require(w == 1 || w == 2 || w == 4 || w == 8 || w == 16,
                     "waves per instance must be 1, 2, 4, 8 or 16");
             const uint32_t key[3] = {w, k0, k1};
+            if (!ige::abi::graphs_enabled()) {
+                set_progress(b, k0);
+                ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+                return;
+            }
             if (!b->graph || !std::equal(key, key + 3, b->graph_key)) {
                 b->drop_graph();
+                // Relaxed: the launch sizes itself with an occupancy query.  A
+                // capture invalidated from outside (another thread synchronising
+                // the device meanwhile) launched nothing: launch directly then.
                 hipGraph_t g = nullptr;
-                // Relaxed: the launch sizes itself with an occupancy query.
-                check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed), "capture");
-                try {
+                bool recorded = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed) ==
+                                hipSuccess;
+                if (recorded) {
+                    try {
+                        set_progress(b, k0);
+                        ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+                    } catch (const std::exception&) {
+                        recorded = false;
+                    }
+                    recorded = hipStreamEndCapture(b->stream, &g) == hipSuccess && g && recorded;
+                }
+                hipGraphExec_t exec = nullptr;
+                if (recorded && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
+                    exec = nullptr;
+                if (g) (void)hipGraphDestroy(g);
+                if (!exec) {
+                    (void)hipGetLastError();
                     set_progress(b, k0);
                     ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
-                } catch (...) {
-                    (void)hipStreamEndCapture(b->stream, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
+                    return;
                 }
-                check(hipStreamEndCapture(b->stream, &g), "end capture");
-                const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (e != hipSuccess) b->graph = nullptr;
-                check(e, "graph instantiate");
+                b->graph = exec;
                 std::copy(key, key + 3, b->graph_key);
             }
             check(hipGraphLaunch(b->graph, b->stream), "graph launch");
@@ -432,20 +453,25 @@ int32_t ige_batch_read(ige_batch* b, uint32_t index, uint64_t* words, int64_t* e
         const ige::InstDesc& D = b->desc[index];
         check(hipStreamSynchronize(b->stream), "batch read");
         check_error_flag(b);
-        if (words) {
-            std::vector<uint64_t> w(L.n);
-            check(hipMemcpy(w.data(), D.spins, L.n * sizeof(uint64_t), hipMemcpyDeviceToHost),
+        // Copies on the batch's own stream (never the null stream, which
+        // would order against other streams of the process).
+        std::vector<uint64_t> w(words ? L.n : 0);
+        unsigned long long a = 0;
+        if (words)
+            check(hipMemcpyAsync(w.data(), D.spins, L.n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 b->stream),
                   "read words");
-            for (uint32_t p = 0; p < L.n; ++p) words[L.order[p]] = w[p];
-        }
         if (energies)
-            check(hipMemcpy(energies, D.energy, 64 * sizeof(int64_t), hipMemcpyDeviceToHost),
+            check(hipMemcpyAsync(energies, D.energy, 64 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                 b->stream),
                   "read energies");
-        if (accepted) {
-            unsigned long long a = 0;
-            check(hipMemcpy(&a, D.accepted, sizeof(a), hipMemcpyDeviceToHost), "read accepted");
-            *accepted = a;
-        }
+        if (accepted)
+            check(hipMemcpyAsync(&a, D.accepted, sizeof(a), hipMemcpyDeviceToHost, b->stream),
+                  "read accepted");
+        check(hipStreamSynchronize(b->stream), "batch read");
+        if (words)
+            for (uint32_t p = 0; p < L.n; ++p) words[L.order[p]] = w[p];
+        if (accepted) *accepted = a;
     });
 }
 

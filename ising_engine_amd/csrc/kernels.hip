@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(
 // theirs through an LDS accumulator.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kQueueCap = 1024; // deferred entries per wave
-constexpr uint32_t kMaxPerLane = 16; // sites per lane
+constexpr uint32_t kMaxPerLane = 8;  // sites per lane
 constexpr uint32_t kTagTail = 5u;
 
 // Bitwise functions of three 64-bit words, one v_bitop3_b32 per 32-bit half.
@@ -240,6 +240,9 @@ constexpr uint32_t kAndOr = ((0xF0 & 0xCC) | 0xAA) & 0xFF;                      
 constexpr uint32_t kAndAndNot = (0xF0 & 0xCC & ~0xAA) & 0xFF;                   // a & b & ~c
 constexpr uint32_t kAndXnor = (0xF0 & ~(0xCC ^ 0xAA)) & 0xFF;                   // a & ~(b ^ c)
 constexpr uint32_t kNotOr = (~0xF0 | 0xCC) & 0xFF;                              // ~a | b
+constexpr uint32_t kNotAnd = (~0xF0 & 0xCC) & 0xFF;                             // ~a & b
+constexpr uint32_t kXorAnd = ((0xF0 ^ 0xCC) & 0xAA) & 0xFF;                     // (a ^ b) & c
+constexpr uint32_t kXnorAnd = (~(0xF0 ^ 0xCC) & 0xAA) & 0xFF;                   // ~(a ^ b) & c
 static_assert(kXor3 == 0x96 && kMaj == 0xE8, "truth tables");
 
 // Truth table of "the three planes (a, b, c) spell the count k" (bit 0 in a).
@@ -279,6 +282,19 @@ __device__ __forceinline__ int64_t wave_column_sum(const uint64_t* acc, uint32_t
     return tot;
 }
 
+// As wave_column_sum for W-plane two's-complement counters (plane W-1 weighs
+// -2^(W-1)).
+template <int W>
+__device__ __forceinline__ int64_t wave_column_sum_signed(const uint64_t* acc, uint32_t lane) {
+    int64_t tot = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+        const int64_t cnt = (int64_t)__builtin_popcountll(wave_transpose(acc[b], lane)) << b;
+        tot += b == W - 1 ? -cnt : cnt;
+    }
+    return tot;
+}
+
 // Resolves one queued replica: u_r's kEagerPlanes leading bits tie with
 // thr_r's, so u_r < thr_r iff tail >> kEagerPlanes < thr_r's low bits.
 __device__ __forceinline__ bool resolve_tail(uint32_t p, uint32_t slot, uint32_t sweep,
@@ -301,7 +317,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
     uint32_t per_lane, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
     const uint32_t* __restrict__ clock) {
-    static_assert(kMaxPerLane <= 16 && W >= B + 4, "energy counters too narrow");
+    static_assert(kMaxPerLane <= 8 && W >= B + 3, "energy counters too narrow");
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     constexpr int LMAX = DEG > 0 ? DEG - DEG / 2 : (1 << (B - 1));
     extern __shared__ unsigned long long lds_q[];
@@ -323,11 +339,17 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         if (threadIdx.x == 0) blk_done = blk_acc = 0;
         __syncthreads();
     }
-    const uint32_t wave = uniform(blockIdx.x * kWavesPerBlock + wv);
+    // Workgroups go to the 8 XCDs round robin by dispatch order; when the row
+    // of blocks divides by 8, XCD k takes the k-th eighth of the class, so a
+    // block's neighbour words (the adjacent position ranges) sit in the same
+    // L2 as its own.
+    const uint32_t gx = gridDim.x;
+    const uint32_t bx = (gx & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (gx >> 3) + (blockIdx.x >> 3);
+    const uint32_t wave = uniform(bx * kWavesPerBlock + wv);
     const uint32_t base = c_begin + wave * kWave * per_lane;
     if (base >= c_end) return;
     // waves of this block that have sites (the rest returned above)
-    const uint32_t first_wave = blockIdx.x * kWavesPerBlock;
+    const uint32_t first_wave = bx * kWavesPerBlock;
     const uint32_t span = kWave * per_lane;
     const uint32_t active = min(kWavesPerBlock, (c_end - c_begin + span - 1) / span - first_wave);
     const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
@@ -337,13 +359,18 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         return (r & 1u) ? (uint32_t)(w >> 32) : (uint32_t)w;
     };
 
-    // Bit-sliced per-replica sums over this lane's sites: kacc = sum k * flip,
-    // dacc = sum d * flip (for a regular lattice d is fixed, and dacc counts
-    // flips in its low kCnt planes instead).
-    constexpr int kCnt = 5;  // flips per lane <= kMaxPerLane = 16
-    uint64_t kacc[W], dacc[W];
+    // Bit-sliced per-replica sums over this lane's sites.  Any degree:
+    // kacc = sum k * flip and dacc = sum d * flip, so dE = w0 (2 kacc - dacc) / 2.
+    // Regular lattices (d = DEG even): sacc = sum (k - DEG/2) * flip in kSW-plane
+    // two's complement (each term in [-3, 3], at most kMaxPerLane of them), so
+    // dE = w0 sacc.
+    constexpr int kSW = 6;
+    static_assert(3 * kMaxPerLane < (1u << (kSW - 1)), "signed counters too narrow");
+    uint64_t kacc[W], dacc[W], sacc[kSW];
 #pragma unroll
     for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
+#pragma unroll
+    for (int b = 0; b < kSW; ++b) sacc[b] = 0;
     uint32_t nacc = 0;
 
     for (uint32_t it = 0; it < per_lane; ++it) {
@@ -457,31 +484,48 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
         Wr[p] = x ^ flip;
         nacc += (uint32_t)__builtin_popcountll(flip);
-        // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
-        uint64_t ck = 0, cd = DEG > 0 ? flip : 0ull;
-#pragma unroll
-        for (int b = 0; b < W; ++b) {
-            if (b == 0) {  // half adder: no carry in
-                const uint64_t ak = c[0] & flip;
-                ck = kacc[0] & ak;
-                kacc[0] ^= ak;
-            } else if (b < B) {  // full adder
-                const uint64_t ak = c[b] & flip;
-                const uint64_t sk = bop3<kXor3>(kacc[b], ak, ck);
-                ck = bop3<kMaj>(kacc[b], ak, ck);
-                kacc[b] = sk;
-            } else {  // carry propagation only
-                const uint64_t sk = kacc[b] ^ ck;
-                ck &= kacc[b];
-                kacc[b] = sk;
-            }
-            if (DEG > 0) {
-                if (b < kCnt) {  // dacc += flip (incrementer)
-                    const uint64_t sd = dacc[b] ^ cd;
-                    cd &= dacc[b];
-                    dacc[b] = sd;
-                }
+        if constexpr (DEG > 0) {
+            // (k - DEG/2) * flip as a three-plane two's-complement addend:
+            // k + (8 - DEG/2) mod 8, i.e. + 101b for DEG 6 and + 110b for DEG 4.
+            uint64_t a0, a1, a2;
+            if constexpr (DEG == 6) {
+                a0 = bop3<kNotAnd>(c[0], flip, 0ull);              // ~c0, carry c0
+                a1 = bop3<kXorAnd>(c[1], c[0], flip);              // c1 ^ c0, carry c1 c0
+                a2 = bop3<kXnorAnd>(c[2], c[1] & c[0], flip);      // c2 + 1 + carry
             } else {
+                a0 = c[0] & flip;                                  // c0 + 0
+                a1 = bop3<kNotAnd>(c[1], flip, 0ull);              // ~c1, carry c1
+                a2 = bop3<kXnorAnd>(c[2], c[1], flip);             // c2 + 1 + carry
+            }
+            // sacc += a, a sign-extended from plane 2 (ripple add)
+            uint64_t cs = sacc[0] & a0;
+            sacc[0] ^= a0;
+#pragma unroll
+            for (int b = 1; b < kSW; ++b) {
+                const uint64_t ab = b == 1 ? a1 : a2;
+                const uint64_t sb = bop3<kXor3>(sacc[b], ab, cs);
+                if (b + 1 < kSW) cs = bop3<kMaj>(sacc[b], ab, cs);
+                sacc[b] = sb;
+            }
+        } else {
+            // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
+            uint64_t ck = 0, cd = 0;
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                if (b == 0) {  // half adder: no carry in
+                    const uint64_t ak = c[0] & flip;
+                    ck = kacc[0] & ak;
+                    kacc[0] ^= ak;
+                } else if (b < B) {  // full adder
+                    const uint64_t ak = c[b] & flip;
+                    const uint64_t sk = bop3<kXor3>(kacc[b], ak, ck);
+                    ck = bop3<kMaj>(kacc[b], ak, ck);
+                    kacc[b] = sk;
+                } else {  // carry propagation only
+                    const uint64_t sk = kacc[b] ^ ck;
+                    ck &= kacc[b];
+                    kacc[b] = sk;
+                }
                 const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
                 const uint64_t sd = bop3<kXor3>(dacc[b], ad, cd);
                 cd = bop3<kMaj>(dacc[b], ad, cd);
@@ -525,10 +569,14 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     }
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
-    const int64_t ksum = wave_column_sum<W>(kacc, lane);
-    const int64_t dsum = DEG > 0 ? (int64_t)DEG * wave_column_sum<kCnt>(dacc, lane)
-                                 : wave_column_sum<W>(dacc, lane);
-    const int64_t de = ((int64_t)w0 * (2 * ksum - dsum)) / 2 + (int64_t)edef[wv][lane];
+    int64_t de;
+    if constexpr (DEG > 0) {
+        de = (int64_t)w0 * wave_column_sum_signed<kSW>(sacc, lane) + (int64_t)edef[wv][lane];
+    } else {
+        const int64_t ksum = wave_column_sum<W>(kacc, lane);
+        const int64_t dsum = wave_column_sum<W>(dacc, lane);
+        de = ((int64_t)w0 * (2 * ksum - dsum)) / 2 + (int64_t)edef[wv][lane];
+    }
     for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
     // Combine the block's waves in LDS; the last wave to finish publishes.
     atomicAdd(&blk_e[lane], (unsigned long long)de);
@@ -752,11 +800,12 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
                                   const unsigned long long* qplanes, hipStream_t stream) {
     if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
-    // Sites per lane: about 4096 waves (four per SIMD, all resident at once,
-    // which measured fastest on 3D L=64 x 32 slots: 16 sites per lane) and at
-    // most kMaxPerLane so the bit-sliced energy counters stay narrow.
-    uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 4096 - 1) /
-                        ((uint64_t)kWave * 4096);
+    // Sites per lane: about 8192 waves (eight per SIMD; 8 sites per lane on
+    // 3D L=64 x 32 slots, which measured as fast as 4096 waves of 16 and lets
+    // the counters be one plane narrower) and at most kMaxPerLane so the
+    // bit-sliced energy counters stay narrow.
+    uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 8192 - 1) /
+                        ((uint64_t)kWave * 8192);
     if (per_lane < 1) per_lane = 1;
     if (per_lane > kMaxPerLane) per_lane = kMaxPerLane;
     const uint32_t waves = ceil_div(len, kWave * per_lane);
@@ -767,18 +816,18 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp, \
                        m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,       \
                        s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1, s.clock)
-    // Counter width W = B + 4: at most kMaxPerLane = 16 sites per lane, each
-    // adding k * flip <= 2^B - 1, so every sum stays below 2^(B+4).
+    // Counter width W = B + 3: at most kMaxPerLane = 8 sites per lane, each
+    // adding k * flip <= 2^B - 1, so every sum stays below 2^(B+3).
     if (m.regular_degree == 6) {
-        IGE_MS_LAUNCH(3, 7, 6);
+        IGE_MS_LAUNCH(3, 6, 6);
     } else if (m.regular_degree == 4) {
-        IGE_MS_LAUNCH(3, 7, 4);
+        IGE_MS_LAUNCH(3, 6, 4);
     } else if (m.dmax < 8) {
-        IGE_MS_LAUNCH(3, 7, 0);
+        IGE_MS_LAUNCH(3, 6, 0);
     } else if (m.dmax < 16) {
-        IGE_MS_LAUNCH(4, 8, 0);
+        IGE_MS_LAUNCH(4, 7, 0);
     } else if (m.dmax < 32) {
-        IGE_MS_LAUNCH(5, 9, 0);
+        IGE_MS_LAUNCH(5, 8, 0);
     } else {
         return hipErrorInvalidValue;
     }

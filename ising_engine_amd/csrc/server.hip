@@ -472,16 +472,19 @@ void serve_loop(ige_server* s) {
             check(hipEventSynchronize(e1), "round");
             {
                 uint32_t e = 0;
-                check(hipMemcpy(&e, s->d_err.p, sizeof(e), hipMemcpyDeviceToHost), "error flag");
+                check(hipMemcpyAsync(&e, s->d_err.p, sizeof(e), hipMemcpyDeviceToHost, s->stream),
+                      "error flag");
+                check(hipStreamSynchronize(s->stream), "error flag");
                 if (e) throw Failure(IGE_ERR_HIP, "a gang barrier wait exceeded its spin limit");
             }
             float ms = 0.f;
             check(hipEventElapsedTime(&ms, e0, e1), "event");
             // per-request device time of this round -> its rate for the next one
             s->h_ticks.resize(L);
-            check(hipMemcpy(s->h_ticks.data(), s->d_ticks.p, L * sizeof(uint64_t),
-                            hipMemcpyDeviceToHost),
+            check(hipMemcpyAsync(s->h_ticks.data(), s->d_ticks.p, L * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost, s->stream),
                   "round ticks");
+            check(hipStreamSynchronize(s->stream), "round ticks");
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
                 const double done = static_cast<double>(r.L.n) * 64.0 * r.D.round;

@@ -85,6 +85,9 @@ def test_graph_replays_equal_eager_launches(ctx, case):
     np.testing.assert_array_equal(g.recompute_energies(), g.energies())
     s = g.stats()[0]
     assert 0 < s.accepted < s.proposed
+    # the replays really ran: 32 + 64 + 0 + 32 sweeps of the 142 came from the graph
+    assert (s.graph_sweeps, s.graph_fallbacks) == (128, 0)
+    assert e.stats()[0].graph_sweeps == 0
 
 
 def test_graph_replays_match_the_oracle(ctx):
@@ -106,3 +109,4 @@ def test_graph_replays_match_the_oracle(ctx):
         assert s.accepted == int(ost.stats[0]) and s.proposed == int(ost.stats[1])
         np.testing.assert_array_equal(pa, ost.stats[2::2])
         np.testing.assert_array_equal(pc, ost.stats[3::2])
+    assert st.stats()[0].graph_sweeps == 64 + 32
