@@ -227,7 +227,8 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(
 // integers by one 64x64 bit transpose per counter plane; queued flips add
 // theirs through an LDS accumulator.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kQueueCap = 1024; // deferred entries per wave
+constexpr uint32_t kQueueCap = 1024; // deferred replicas per wave (any degree)
+constexpr uint32_t kRecCap = 128;    // deferred sites per wave (regular lattices)
 constexpr uint32_t kMaxPerLane = 8;  // sites per lane
 constexpr uint32_t kTagTail = 5u;
 
@@ -264,6 +265,19 @@ __device__ __forceinline__ uint64_t bop3s(uint64_t a, uint64_t b, uint32_t c) {
     const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, c, TT);
     const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), c, TT);
     return ((uint64_t)hi << 32) | lo;
+}
+
+// base[i] with a 32-bit byte offset (the caller guarantees i * sizeof(T) <
+// 2^32), so the load takes the uniform base in SGPRs and one offset VGPR
+// instead of a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
 // All ones if bit e of sg is set, else zero.
@@ -321,7 +335,14 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     constexpr int LMAX = DEG > 0 ? DEG - DEG / 2 : (1 << (B - 1));
     extern __shared__ unsigned long long lds_q[];
-    __shared__ uint32_t queue[kWavesPerBlock][kQueueCap];
+    // Ties wait in a per-wave LDS queue: one entry per replica for any degree;
+    // on a regular lattice one record per site instead (its tied replicas, the
+    // ones among them at levels 1 and 2, and where the site is).
+    __shared__ uint32_t queue[kWavesPerBlock][DEG > 0 ? 1 : kQueueCap];
+    __shared__ unsigned long long rec_und[kWavesPerBlock][DEG > 0 ? kRecCap : 1];
+    __shared__ unsigned long long rec_l1[kWavesPerBlock][DEG > 0 ? kRecCap : 1];
+    __shared__ unsigned long long rec_l2[kWavesPerBlock][DEG == 6 ? kRecCap : 1];
+    __shared__ uint32_t rec_pos[kWavesPerBlock][DEG > 0 ? kRecCap : 1];
     __shared__ uint32_t qcount[kWavesPerBlock];
     __shared__ unsigned long long edef[kWavesPerBlock][kWave];
     __shared__ unsigned long long blk_e[kWave];
@@ -376,15 +397,17 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     for (uint32_t it = 0; it < per_lane; ++it) {
         const uint32_t p = base + it * kWave + lane;
         if (p >= c_end) break;
-        const uint64_t x = S[p];
-        const uint32_t e0 = DEG > 0 ? p * (uint32_t)DEG : rp[p];
-        const uint32_t d = DEG > 0 ? (uint32_t)DEG : rp[p + 1] - e0;
-        const uint32_t sg = sgn[p];
+        // word and entry byte offsets fit 32 bits (multispin_shape: n < 2^29,
+        // nnz < 2^30)
+        const uint64_t x = ld32(S, p);
+        const uint32_t e0 = DEG > 0 ? p * (uint32_t)DEG : ld32(rp, p);
+        const uint32_t d = DEG > 0 ? (uint32_t)DEG : ld32(rp, p + 1) - e0;
+        const uint32_t sg = ld32(sgn, p);
         uint64_t c[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) c[b] = 0;
         auto bond = [&](uint32_t e) {
-            return bop3s<kXor3>(x, S[ci[e0 + e]], bit_mask(sg, e));
+            return bop3s<kXor3>(x, ld32(S, ld32(ci, e0 + e)), bit_mask(sg, e));
         };
         if (DEG == 6 || DEG == 4) {
             // carry-save adder tree: c = popcount of the bond words, bit-sliced
@@ -418,9 +441,9 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
                 uint32_t j[kRowGroup];
                 uint64_t wj[kRowGroup];
 #pragma unroll
-                for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ci[e0 + min(g + q, d - 1)];
+                for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ld32(ci, e0 + min(g + q, d - 1));
 #pragma unroll
-                for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = S[j[q]];
+                for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld32(S, j[q]);
 #pragma unroll
                 for (uint32_t q = 0; q < kRowGroup; ++q) {
                     const uint32_t e = g + q;
@@ -482,7 +505,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
             }
         }
         const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
-        Wr[p] = x ^ flip;
+        st32(Wr, p, x ^ flip);
         nacc += (uint32_t)__builtin_popcountll(flip);
         if constexpr (DEG > 0) {
             // (k - DEG/2) * flip as a three-plane two's-complement addend:
@@ -533,38 +556,87 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
             }
         }
         // Queue the replicas whose leading bits tie.
-        while (und) {
-            const uint32_t r = (uint32_t)__builtin_ctzll(und);
-            und &= und - 1;
-            uint32_t m = 0;
+        if constexpr (DEG > 0) {
+            if (und) {
+                const uint32_t at = atomicAdd(&qcount[wv], 1u);
+                if (at < kRecCap) {
+                    rec_und[wv][at] = und;
+                    rec_l1[wv][at] = und & lev[1];
+                    if constexpr (DEG == 6) rec_l2[wv][at] = und & lev[2];
+                    rec_pos[wv][at] = (it << 6) | lane;
+                } else {
+                    // queue full: resolve in place; level j has m = 2 + 2j
+                    __threadfence_block();
+                    while (und) {
+                        const uint32_t r = (uint32_t)__builtin_ctzll(und);
+                        und &= und - 1;
+                        uint32_t m = 2;
 #pragma unroll
-            for (int l = 0; l < LMAX; ++l)
-                if ((lev[l] >> r) & 1u) m = mrow[l] / kQStride;
-            const uint32_t at = atomicAdd(&qcount[wv], 1u);
-            if (at < kQueueCap) {
-                queue[wv][at] = (it << 17) | (lane << 11) | (r << 5) | m;
-            } else if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
-                // queue full: resolve in place
-                __threadfence_block();
-                atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-                atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
-                nacc += 1;
+                        for (int l = 1; l < LMAX; ++l)
+                            if ((lev[l] >> r) & 1u) m = 2 + 2 * l;
+                        if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
+                            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+                            atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
+                            nacc += 1;
+                        }
+                    }
+                }
+            }
+        } else {
+            while (und) {
+                const uint32_t r = (uint32_t)__builtin_ctzll(und);
+                und &= und - 1;
+                uint32_t m = 0;
+#pragma unroll
+                for (int l = 0; l < LMAX; ++l)
+                    if ((lev[l] >> r) & 1u) m = mrow[l] / kQStride;
+                const uint32_t at = atomicAdd(&qcount[wv], 1u);
+                if (at < kQueueCap) {
+                    queue[wv][at] = (it << 17) | (lane << 11) | (r << 5) | m;
+                } else if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
+                    // queue full: resolve in place
+                    __threadfence_block();
+                    atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+                    atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
+                    nacc += 1;
+                }
             }
         }
     }
     // Drain the queue with every lane of the wave.
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
-    const uint32_t queued = min(qcount[wv], kQueueCap);
-    for (uint32_t q = lane; q < queued; q += kWave) {
-        const uint32_t code = queue[wv][q];
-        const uint32_t it = code >> 17, sl = (code >> 11) & 63u, r = (code >> 5) & 63u,
-                       m = code & 31u;
-        const uint32_t p = base + it * kWave + sl;
-        if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
-            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-            atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
-            nacc += 1;
+    if constexpr (DEG > 0) {
+        const uint32_t queued = min(qcount[wv], kRecCap);
+        for (uint32_t q = lane; q < queued; q += kWave) {
+            uint64_t u = rec_und[wv][q];
+            const uint64_t l1 = rec_l1[wv][q];
+            const uint64_t l2 = DEG == 6 ? rec_l2[wv][q] : 0ull;
+            const uint32_t pos = rec_pos[wv][q];
+            const uint32_t p = base + (pos >> 6) * kWave + (pos & 63u);
+            while (u) {
+                const uint32_t r = (uint32_t)__builtin_ctzll(u);
+                u &= u - 1;
+                const uint32_t m = ((l1 >> r) & 1u) ? 4u : ((l2 >> r) & 1u) ? 6u : 2u;
+                if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
+                    atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+                    atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
+                    nacc += 1;
+                }
+            }
+        }
+    } else {
+        const uint32_t queued = min(qcount[wv], kQueueCap);
+        for (uint32_t q = lane; q < queued; q += kWave) {
+            const uint32_t code = queue[wv][q];
+            const uint32_t it = code >> 17, sl = (code >> 11) & 63u, r = (code >> 5) & 63u,
+                           m = code & 31u;
+            const uint32_t p = base + it * kWave + sl;
+            if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
+                atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+                atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
+                nacc += 1;
+            }
         }
     }
     __threadfence_block();

@@ -134,6 +134,8 @@ MultispinShape multispin_shape(const ProblemIR& ir) {
         if (2 * static_cast<int64_t>(ir.linear[i]) != -sum) return s;
     }
     if (ir.max_degree() > 31) return s;
+    // the kernel addresses words and CSR entries with 32-bit byte offsets
+    if (ir.n >= (1u << 29) || ir.nnz() >= (1ull << 30)) return s;
     s.eligible = true;
     s.w0 = static_cast<int32_t>(w0);
     return s;
