@@ -130,7 +130,8 @@ struct ige_state {
     DevBuf<uint64_t> spins;
     DevBuf<int64_t> energy, scratch_e;
     DevBuf<uint32_t> temp_of, slot_of, thr;
-    DevBuf<unsigned long long> counters, qplanes;  // counters: accepted, then pair stats
+    DevBuf<unsigned long long> counters;  // accepted, then pair stats
+    DevBuf<uint32_t> lev_thr;             // multispin thresholds per temperature and level
     DevBuf<double> beta;                             // current ladder (+ schedule rows)
     DevBuf<uint64_t> hb;                             // heat-bath thresholds, on first use
     DevBuf<int32_t> fields;
@@ -238,7 +239,8 @@ void refresh_thresholds(ige_state* s, const double* dev_beta, const uint32_t* cl
         check(ige::launch_thr_table(dev_beta, s->thr.p, s->cells, s->dtab, s->stream, clock),
               "thr");
     else if (s->kernel == IGE_KERNEL_MULTISPIN)
-        check(ige::launch_qplanes(m, st, dev_beta, s->qplanes.p, s->stream), "qplanes");
+        check(ige::launch_level_thresholds(m, st, dev_beta, s->lev_thr.p, s->stream),
+              "level thresholds");
     // the dense kernel evaluates its thresholds from beta directly
 }
 
@@ -308,7 +310,7 @@ void issue_step(ige_state* s, const StepPlan& P, const double* beta, uint32_t sw
                       "sweep_generic");
             else
                 check(ige::launch_sweep_multispin(m, st, cs[c], cs[c + 1], sweep, s->seed,
-                                                  s->qplanes.p, s->stream),
+                                                  s->lev_thr.p, s->stream),
                       "sweep_multispin");
         }
     }
@@ -363,7 +365,7 @@ hipGraphExec_t sweep_graph(ige_state* s, const StepPlan& P) {
     }
     bool recorded = true;
     try {
-        const bool refresh = P.sched || (!P.heat && s->kernel == IGE_KERNEL_MULTISPIN && P.exch);
+        const bool refresh = P.sched;
         for (uint32_t k = 0; k < kGraphSteps; ++k) {
             const double* beta =
                 P.sched ? s->sched_buf.p + static_cast<size_t>(k) * s->cells : s->beta.p;
@@ -554,7 +556,7 @@ int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, u
         } else if (s->kernel == IGE_KERNEL_DENSE) {
             model->ensure_dense();
         } else {
-            s->qplanes.alloc(static_cast<size_t>(cells) * (model->dmax + 1) * ige::kQStride);
+            s->lev_thr.alloc(static_cast<size_t>(cells) * (model->dmax + 1));
         }
         std::vector<unsigned long long> zeros(s->counters.count, 0);
         s->counters.upload(zeros.data(), zeros.size(), s->stream);
@@ -656,12 +658,10 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             }
         }
         for (uint32_t k = done; k < sweeps; ++k) {
-            // Thresholds depend on beta and, for the multispin planes, on
-            // which temperature each (slot, lane) holds after the last swap.
-            // Heat-bath tables are indexed by temperature: only beta matters.
-            const bool refresh =
-                P.heat ? (P.sched || k == 0)
-                       : (P.sched || (s->kernel == IGE_KERNEL_MULTISPIN && P.exch && k > 0));
+            // Threshold tables are indexed by temperature, so only a new beta
+            // (a schedule row) needs them rebuilt; the heat-bath table is
+            // built on the first step of a call.
+            const bool refresh = P.heat ? (P.sched || k == 0) : P.sched;
             issue_step(s, P, row(k), s->sweep, s->xround, refresh, nullptr);
             s->sweep += 1;
             if (P.exch) s->xround += 1;
@@ -677,8 +677,6 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             // the caller may free `sched` once we return
             check(hipStreamSynchronize(s->stream), "run");
             if (sweeps > 0) refresh_thresholds(s, s->beta.p);
-        } else if (s->kernel == IGE_KERNEL_MULTISPIN && P.exch && sweeps > 0) {
-            refresh_thresholds(s, s->beta.p);
         }
     });
 }
@@ -696,7 +694,6 @@ int32_t ige_exchange(ige_state* s, uint32_t rounds) {
             check(ige::launch_exchange(st, s->beta.p, s->xround, s->seed, s->stream), "exchange");
             s->xround += 1;
         }
-        if (s->kernel == IGE_KERNEL_MULTISPIN) refresh_thresholds(s, s->beta.p);
     });
 }
 

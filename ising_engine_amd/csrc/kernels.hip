@@ -327,7 +327,8 @@ template <int B, int W, int DEG>
 __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const uint32_t* __restrict__ sgn, const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr,
-    uint32_t n, uint32_t dmax, int32_t w0, const unsigned long long* __restrict__ qplanes,
+    uint32_t n, uint32_t dmax, int32_t w0, const uint32_t* __restrict__ lev_thr,
+    const uint32_t* __restrict__ temp_of,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
     uint32_t per_lane, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
     const uint32_t* __restrict__ clock) {
@@ -351,9 +352,27 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t wv = uniform(threadIdx.x >> 6);
     const uint32_t slot = blockIdx.y;
     {
-        const uint32_t total = (dmax + 1) * kQStride;
-        const unsigned long long* src = qplanes + (uint64_t)slot * total;
-        for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_q[i] = src[i];
+        // The slot's threshold records, one per level m <= dmax, kQStride u64
+        // each: bit r of plane b (b < kEagerPlanes) is bit 31 - b of thr_r, the
+        // threshold of the temperature that lane r of this slot holds; then
+        // the mask of nonzero thr_r and the 64 thr_r themselves.
+        // With a fixed even degree only the levels m = 2, 4, ..., DEG are read.
+        const uint32_t tr = temp_of[slot * kWave + lane];
+        constexpr uint32_t m_step = DEG > 0 ? 2 : 1;
+        for (uint32_t m = (DEG > 0 ? 2 : 0) + m_step * wv; m <= dmax; m += m_step * kWavesPerBlock) {
+            const uint32_t th = lev_thr[tr * (dmax + 1) + m];
+            unsigned long long* out = lds_q + m * kQStride;
+            uint64_t mine = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < kEagerPlanes; ++b) {
+                const uint64_t plane = ballot((th >> (31u - b)) & 1u);
+                if (lane == b) mine = plane;
+            }
+            const uint64_t nz = ballot(th != 0);
+            if (lane < kEagerPlanes) out[lane] = mine;
+            if (lane == kEagerPlanes) out[kEagerPlanes] = nz;
+            reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[lane] = th;
+        }
         edef[wv][lane] = 0;
         if (lane == 0) qcount[wv] = 0;
         if (wv == 0) blk_e[lane] = 0;
@@ -667,30 +686,16 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     }
 }
 
-// One wave per (slot, m), m in [0, dmax]: lane r holds the threshold of the
-// temperature that (slot, r) currently carries.
-__global__ __launch_bounds__(kWave) void qplanes_kernel(const double* __restrict__ beta,
-                                                        const uint32_t* __restrict__ temp_of,
-                                                        unsigned long long* qplanes,
-                                                        uint32_t dmax, int32_t w0,
-                                                        const uint32_t* __restrict__ clock) {
-    const uint32_t r = threadIdx.x;
-    beta += (uint64_t)clocked(clock, kClockRow, 0) * gridDim.y;
-    const uint32_t m = blockIdx.x;
-    const uint32_t slot = blockIdx.y;
-    const int64_t de = ((int64_t)m * w0) / 2;
-    const uint32_t th = m == 0 ? 0u : metropolis_threshold(beta[temp_of[slot * kWave + r]], de);
-    unsigned long long* out = qplanes + ((uint64_t)slot * (dmax + 1) + m) * kQStride;
-    uint64_t mine = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < kEagerPlanes; ++b) {
-        const uint64_t plane = ballot((th >> (31u - b)) & 1u);
-        if (r == b) mine = plane;
-    }
-    const uint64_t nz = ballot(th != 0);
-    if (r < kEagerPlanes) out[r] = mine;
-    if (r == kEagerPlanes) out[kEagerPlanes] = nz;
-    reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[r] = th;
+// One thread per (temperature t, level m), m in [0, dmax]: the threshold of
+// dE = m w0 / 2 at beta[t] (0 for m = 0, whose moves are accepted outright).
+__global__ void level_thr_kernel(const double* __restrict__ beta, uint32_t* __restrict__ lev_thr,
+                                 uint32_t cells, uint32_t dmax, int32_t w0,
+                                 const uint32_t* __restrict__ clock) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    beta += (uint64_t)clocked(clock, kClockRow, 0) * cells;
+    if (i >= cells * (dmax + 1)) return;
+    const uint32_t t = i / (dmax + 1), m = i % (dmax + 1);
+    lev_thr[i] = m == 0 ? 0u : metropolis_threshold(beta[t], ((int64_t)m * w0) / 2);
 }
 
 __global__ __launch_bounds__(64) void clock_kernel(uint32_t* clock, uint32_t a, uint32_t b,
@@ -869,7 +874,7 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
 
 hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
                                   uint32_t c_end, uint32_t sweep, uint64_t seed,
-                                  const unsigned long long* qplanes, hipStream_t stream) {
+                                  const uint32_t* lev_thr, hipStream_t stream) {
     if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
     // Sites per lane: about 8192 waves (eight per SIMD; 8 sites per lane on
@@ -886,7 +891,8 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
 #define IGE_MS_LAUNCH(B, W, DEG)                                                                   \
     hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp, \
-                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, qplanes, s.energy,       \
+                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, lev_thr, s.temp_of,      \
+                       s.energy,                                                                 \
                        s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1, s.clock)
     // Counter width W = B + 3: at most kMaxPerLane = 8 sites per lane, each
     // adding k * flip <= 2^B - 1, so every sum stays below 2^(B+3).
@@ -907,10 +913,11 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
-                          unsigned long long* qplanes, hipStream_t stream) {
-    hipLaunchKernelGGL(qplanes_kernel, dim3(m.dmax + 1, s.cells), dim3(kWave), 0, stream, beta,
-                       s.temp_of, qplanes, m.dmax, m.w0, s.clock);
+hipError_t launch_level_thresholds(const DevModel& m, const DevState& s, const double* beta,
+                                   uint32_t* lev_thr, hipStream_t stream) {
+    const uint32_t total = s.cells * (m.dmax + 1);
+    hipLaunchKernelGGL(level_thr_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, stream,
+                       beta, lev_thr, s.cells, m.dmax, m.w0, s.clock);
     return hipGetLastError();
 }
 

@@ -82,24 +82,25 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
                                  uint32_t dtab, hipStream_t stream);
 
 // Multispin sweep of one colour class (lane = site, bit-sliced +-J).
-// qplanes: [cells][dmax+1][33] threshold bit planes, see launch_qplanes.
+// lev_thr: [cells][dmax+1] thresholds per temperature and level
+// (launch_level_thresholds); each workgroup turns its slot's into bit planes
+// through temp_of.
 hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
                                   uint32_t c_end, uint32_t sweep, uint64_t seed,
-                                  const unsigned long long* qplanes, hipStream_t stream);
+                                  const uint32_t* lev_thr, hipStream_t stream);
 
 // Uniforms of the multispin kernel: kEagerPlanes leading bits as bit planes
 // (two per Philox call), the rest from a tail word drawn only on a tie.
 constexpr uint32_t kEagerPlanes = 8;
-// Per (slot, level m): the threshold's kEagerPlanes leading bit planes, the
-// nonzero mask and the 64 thresholds themselves (two per u64).
+// Per (slot, level m), in the sweep's LDS: the threshold's kEagerPlanes
+// leading bit planes, the nonzero mask and the 64 thresholds (two per u64).
 constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
 
-// Per (slot, m), m <= dmax, a record of kQStride u64 at qplanes[(slot*(dmax+1)+m)*kQStride]:
-// words 0..kEagerPlanes-1: bit r of word b = bit (31-b) of thr_r =
-// metropolis_threshold(beta of the temperature (slot, r) holds, m*w0/2);
-// word kEagerPlanes: bit r set iff thr_r != 0; then the 64 thr_r as u32.
-hipError_t launch_qplanes(const DevModel& m, const DevState& s, const double* beta,
-                          unsigned long long* qplanes, hipStream_t stream);
+// lev_thr[t*(dmax+1) + m] = metropolis_threshold(beta[t], m*w0/2) for m in
+// [1, dmax], 0 for m = 0.  It depends on the ladder alone, so an exchange
+// (which moves temperatures between slots) needs no refresh.
+hipError_t launch_level_thresholds(const DevModel& m, const DevState& s, const double* beta,
+                                   uint32_t* lev_thr, hipStream_t stream);
 
 // DEO exchange number `round`: pairs (t, t+1) with t = round mod 2, inside
 // each population of s.pop_temps temperature indices (engine.rs:713-716).
