@@ -92,9 +92,10 @@ extern "C" {
 #define IGE_RUN_HEATBATH 4u   /* heat-bath (Gibbs) sweeps instead of Metropolis:
                                  new spin = 1 with P = 1/(1+exp(beta h)),
                                  lane-per-replica kernel on any state       */
-#define IGE_RUN_EAGER 16u     /* launch every kernel from the host instead of
-                                 replaying the captured sweep-loop graph;
-                                 the results are the same bits either way  */
+#define IGE_RUN_EAGER 16u     /* one launch per colour class and sweep from
+                                 the host: no captured sweep-loop graph and
+                                 no resident run; the results are the same
+                                 bits either way                            */
 
 #define IGE_STEPS_UNLIMITED 0xFFFFFFFFu /* descend until no replica can improve */
 
@@ -124,6 +125,9 @@ typedef struct {
     uint64_t graph_fallbacks; /* ige_run calls whose capture was invalidated
                                  (another thread synchronised the device
                                  meanwhile) and that ran eagerly instead    */
+    uint64_t resident_sweeps; /* sweeps run with the slot's words in LDS for
+                                 the whole call (small regular lattices,
+                                 no exchange)                               */
 } ige_stats;
 
 /* ---- context ---- */
@@ -185,10 +189,13 @@ int32_t ige_state_destroy(ige_state *state);
 /* beta[t] for temperature index t (cells entries, finite, >= 0). */
 int32_t ige_set_ladder(ige_state *state, const double *beta);
 /* beta_schedule: NULL (use the ladder) or [sweeps][cells]; the ladder is left
- * at the last row.  Asynchronous on the state's stream.  Runs of 32 sweeps or
- * more replay a hipGraph of 32 captured steps (kept with the state and
- * captured again when the flags, populations or schedule buffer change); the
- * rest, IGE_RUN_ICM, profiling and the dense kernel launch eagerly. */
+ * at the last row.  Asynchronous on the state's stream.  A multispin state of
+ * a regular lattice (degree 4 or 6, n <= 4096) run without exchange,
+ * incumbent, heat-bath or ICM does the whole call in one resident launch (one
+ * workgroup per slot, words in LDS).  Otherwise runs of 32 sweeps or more
+ * replay a hipGraph of 32 captured steps (kept with the state and captured
+ * again when the flags, populations or schedule buffer change); the rest,
+ * IGE_RUN_ICM, profiling and the dense kernel launch eagerly. */
 int32_t ige_run(ige_state *state, uint32_t sweeps, uint32_t flags, const double *beta_schedule);
 /* `rounds` DEO exchanges without sweeps, on the current ladder and tracked
  * energies.  Exchange number k uses pairs (t, t+1) with t = k mod 2 and is

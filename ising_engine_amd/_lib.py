@@ -80,7 +80,7 @@ class Stats(ctypes.Structure):
         ("sweeps", c_uint64), ("exchanges", c_uint64), ("accepted", c_uint64),
         ("proposed", c_uint64),
         ("phase", c_uint32), ("kernel", c_uint32),
-        ("graph_sweeps", c_uint64), ("graph_fallbacks", c_uint64),
+        ("graph_sweeps", c_uint64), ("graph_fallbacks", c_uint64), ("resident_sweeps", c_uint64),
     ]
 
 

@@ -45,7 +45,7 @@ struct ige_model {
     int64_t max_local = 0;
     uint32_t dmax = 0;
     uint32_t regular_degree = 0;
-    DevBuf<uint32_t> rp, ci, order, pos_of, sgn;
+    DevBuf<uint32_t> rp, ci, order, pos_of, sgn, cstart;  // cstart: colour class bounds
     DevBuf<int32_t> wt, lin;
     // Dense int8 matrix, built on the first dense state of this model.
     mutable std::mutex dense_mu;
@@ -148,6 +148,8 @@ struct ige_state {
     uint64_t graph_key[5] = {0, 0, 0, 0, 0};
     uint64_t graph_fallbacks = 0;  // captures that failed (those calls ran eagerly)
     uint64_t graph_sweeps = 0;     // sweeps run by graph replays
+    uint64_t resident_sweeps = 0;  // sweeps run by resident (LDS) runs
+    DevBuf<uint32_t> lev_rows;     // per-sweep level thresholds of a resident run
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -211,6 +213,8 @@ void upload_model(ige_model* m) {
     m->order.upload(co.order.data(), n, nullptr);
     m->pos_of.upload(co.pos_of.data(), n, nullptr);
     m->sgn.upload(sgn.data(), n, nullptr);
+    m->cstart.alloc(co.class_start.size());
+    m->cstart.upload(co.class_start.data(), co.class_start.size(), nullptr);
     check(hipStreamSynchronize(nullptr), "model upload");
 }
 
@@ -633,12 +637,41 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         const auto row = [&](uint32_t k) {
             return sched ? s->sched_buf.p + static_cast<size_t>(k) * s->cells : s->beta.p;
         };
-        // Graph replays first: kGraphSteps steps per launch.
         uint32_t done = 0;
+        // Small regular lattices without exchange: the whole call in one
+        // resident launch (words in LDS, one workgroup per slot).
+        const bool resident = !(flags & IGE_RUN_EAGER) && !s->profile &&
+                              s->kernel == IGE_KERNEL_MULTISPIN && !P.exch && !P.track &&
+                              !P.heat && !P.icm && sweeps >= 2 &&
+                              s->model->ir.n <= ige::kResidentMaxSites &&
+                              (s->model->regular_degree == 4 || s->model->regular_degree == 6);
+        if (resident) {
+            const auto m = s->model->view();
+            const auto st = s->view();
+            const uint32_t row = s->cells * (s->model->dmax + 1);
+            const uint32_t* rows = s->lev_thr.p;
+            uint32_t stride = 0;
+            if (sched) {
+                const size_t need = static_cast<size_t>(sweeps) * row;
+                if (s->lev_rows.count < need) s->lev_rows.alloc(need);
+                check(ige::launch_level_rows(m, s->cells, s->sched_buf.p, sweeps, s->lev_rows.p,
+                                             s->stream),
+                      "level rows");
+                rows = s->lev_rows.p;
+                stride = row;
+            }
+            check(ige::launch_resident_sweeps(m, st, s->model->cstart.p, s->model->co.num_colors,
+                                              rows, stride, s->sweep, sweeps, s->seed, s->stream),
+                  "resident sweeps");
+            s->sweep += sweeps;
+            s->resident_sweeps += sweeps;
+            done = sweeps;
+        }
+        // Graph replays next: kGraphSteps steps per launch.
         const bool graphable = ige::abi::graphs_enabled() && !s->profile && !P.icm &&
                                !(flags & IGE_RUN_EAGER) &&
                                (P.heat || s->kernel != IGE_KERNEL_DENSE);
-        if (graphable && sweeps >= kGraphSteps) {
+        if (graphable && sweeps - done >= kGraphSteps) {
             const hipGraphExec_t g = sweep_graph(s, P);
             if (g) {
                 // Step 0's heat-bath table on a fixed ladder (the graph omits it).
@@ -783,6 +816,7 @@ int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
         stats->kernel = s->kernel;
         stats->graph_sweeps = s->graph_sweeps;
         stats->graph_fallbacks = s->graph_fallbacks;
+        stats->resident_sweeps = s->resident_sweeps;
         for (uint32_t t = 0; t + 1 < s->cells; ++t) {
             if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];
             if (pair_accepts) pair_accepts[t] = c[2 + 2 * t];

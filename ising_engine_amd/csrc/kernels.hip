@@ -15,13 +15,15 @@
 
 #include "ige_math.h"
 #include "kernels.hpp"
+#include "multispin_ops.h"
 #include "sweep_generic.h"
 
 namespace ige {
 
 namespace {
 
-constexpr uint32_t kWave = 64;
+using namespace msc;  // kWave, the bit operations and the site updates
+
 constexpr uint32_t kBlock = 256;  // 4 waves
 constexpr uint32_t kWavesPerBlock = kBlock / kWave;
 
@@ -230,94 +232,130 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(
 constexpr uint32_t kQueueCap = 1024; // deferred replicas per wave (any degree)
 constexpr uint32_t kRecCap = 128;    // deferred sites per wave (regular lattices)
 constexpr uint32_t kMaxPerLane = 8;  // sites per lane
-constexpr uint32_t kTagTail = 5u;
 
-// Bitwise functions of three 64-bit words, one v_bitop3_b32 per 32-bit half.
-// The truth-table byte of f(a, b, c) is f(0xF0, 0xCC, 0xAA): bit i of the
-// table is f of bits (i >> 2) & 1, (i >> 1) & 1 and i & 1 of the operands.
-constexpr uint32_t kXor3 = (0xF0 ^ 0xCC ^ 0xAA) & 0xFF;                         // a ^ b ^ c
-constexpr uint32_t kMaj = ((0xF0 & 0xCC) | (0xF0 & 0xAA) | (0xCC & 0xAA)) & 0xFF;  // majority
-constexpr uint32_t kAndOr = ((0xF0 & 0xCC) | 0xAA) & 0xFF;                      // (a & b) | c
-constexpr uint32_t kAndAndNot = (0xF0 & 0xCC & ~0xAA) & 0xFF;                   // a & b & ~c
-constexpr uint32_t kAndXnor = (0xF0 & ~(0xCC ^ 0xAA)) & 0xFF;                   // a & ~(b ^ c)
-constexpr uint32_t kNotOr = (~0xF0 | 0xCC) & 0xFF;                              // ~a | b
-constexpr uint32_t kNotAnd = (~0xF0 & 0xCC) & 0xFF;                             // ~a & b
-constexpr uint32_t kXorAnd = ((0xF0 ^ 0xCC) & 0xAA) & 0xFF;                     // (a ^ b) & c
-constexpr uint32_t kXnorAnd = (~(0xF0 ^ 0xCC) & 0xAA) & 0xFF;                   // ~(a ^ b) & c
-static_assert(kXor3 == 0x96 && kMaj == 0xE8, "truth tables");
-
-// Truth table of "the three planes (a, b, c) spell the count k" (bit 0 in a).
-constexpr uint32_t count3_tt(uint32_t k) {
-    return ((k & 1u) ? 0xF0u : 0x0Fu) & ((k & 2u) ? 0xCCu : 0x33u) & ((k & 4u) ? 0xAAu : 0x55u);
-}
-
-template <uint32_t TT>
-__device__ __forceinline__ uint64_t bop3(uint64_t a, uint64_t b, uint64_t c) {
-    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, TT);
-    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32),
-                                                    (uint32_t)(c >> 32), TT);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// As bop3 with the same 32-bit c in both halves.
-template <uint32_t TT>
-__device__ __forceinline__ uint64_t bop3s(uint64_t a, uint64_t b, uint32_t c) {
-    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, c, TT);
-    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), c, TT);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// base[i] with a 32-bit byte offset (the caller guarantees i * sizeof(T) <
-// 2^32), so the load takes the uniform base in SGPRs and one offset VGPR
-// instead of a 64-bit address per lane.
-template <typename T>
-__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
-                                       i * (uint32_t)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
-}
-
-// All ones if bit e of sg is set, else zero.
-__device__ __forceinline__ uint32_t bit_mask(uint32_t sg, uint32_t e) {
-    return (uint32_t)((int32_t)(sg << (31u - e)) >> 31);
-}
-
-// Sum over the wave's lanes of the per-lane bit-sliced counter acc[0..W-1],
-// returned in lane r for replica r.
-template <int W>
-__device__ __forceinline__ int64_t wave_column_sum(const uint64_t* acc, uint32_t lane) {
-    int64_t tot = 0;
+// The update of position p for any degree d < 2^B (row p at rp[p]): count
+// planes from the row in groups, a level per k > d/2, the same lockstep
+// planes as regular_site, unsigned counters kacc += k flip and dacc += d flip,
+// and one queue entry per tied replica ((it << 17) | (lane << 11) | (r << 5) |
+// m), resolved in place when the queue is full.
+template <int B, int W>
+__device__ __forceinline__ void irregular_site(
+    const uint64_t* S, uint64_t* Wr, uint32_t p, const uint32_t* rp, const uint32_t* ci,
+    const uint32_t* sgn, const unsigned long long* recs, uint32_t slot, uint32_t sweep,
+    uint32_t key0, uint32_t key1, int32_t w0, uint64_t (&kacc)[W], uint64_t (&dacc)[W],
+    uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t it, uint32_t lane,
+    unsigned long long* edef) {
+    constexpr int LMAX = 1 << (B - 1);
+    const uint64_t x = ld32(S, p);
+    const uint32_t e0 = ld32(rp, p);
+    const uint32_t d = ld32(rp, p + 1) - e0;
+    const uint32_t sg = ld32(sgn, p);
+    uint64_t c[B];
 #pragma unroll
-    for (int b = 0; b < W; ++b)
-        tot += (int64_t)__builtin_popcountll(wave_transpose(acc[b], lane)) << b;
-    return tot;
-}
-
-// As wave_column_sum for W-plane two's-complement counters (plane W-1 weighs
-// -2^(W-1)).
-template <int W>
-__device__ __forceinline__ int64_t wave_column_sum_signed(const uint64_t* acc, uint32_t lane) {
-    int64_t tot = 0;
+    for (int b = 0; b < B; ++b) c[b] = 0;
+    // Rows in groups of kRowGroup entries: a group's index loads issue
+    // together, then its word loads, so a row costs two memory round trips per
+    // group rather than two per neighbour (the sweep of a small colour class
+    // is one wave's walk along its rows).  Entries past the row's end repeat
+    // its last one and are masked out of the count.
+    constexpr uint32_t kRowGroup = 8;
+    for (uint32_t g = 0; g < d; g += kRowGroup) {
+        uint32_t j[kRowGroup];
+        uint64_t wj[kRowGroup];
+#pragma unroll
+        for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ld32(ci, e0 + min(g + q, d - 1));
+#pragma unroll
+        for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld32(S, j[q]);
+#pragma unroll
+        for (uint32_t q = 0; q < kRowGroup; ++q) {
+            const uint32_t e = g + q;
+            uint64_t carry =
+                bop3s<kXor3>(x, wj[q], bit_mask(sg, e & 31u)) & (e < d ? ~0ull : 0ull);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const uint64_t s = c[b] ^ carry;
+                carry &= c[b];
+                c[b] = s;
+            }
+        }
+    }
+    // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
+    uint64_t lev[LMAX];
+    uint32_t mrow[LMAX];
+    uint64_t pos = 0, und = 0;
+#pragma unroll
+    for (int j = 0; j < LMAX; ++j) {
+        const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
+        uint64_t eq = k <= d ? ~0ull : 0ull;
+#pragma unroll
+        for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
+        lev[j] = eq;
+        mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
+        pos |= eq;
+        und = bop3<kAndOr>(eq, recs[mrow[j] + kEagerPlanes], und);
+    }
+    // kEagerPlanes lockstep bit planes, two per Philox call.
+    uint64_t lt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
+        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t b = 2 * j + h;
+            const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
+                                      : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
+            uint64_t T = lev[0] & recs[mrow[0] + b];
+#pragma unroll
+            for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], recs[mrow[l] + b], T);
+            lt |= bop3<kAndAndNot>(und, T, P);
+            und = bop3<kAndXnor>(und, P, T);
+        }
+    }
+    const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
+    st32(Wr, p, x ^ flip);
+    nacc += (uint32_t)__builtin_popcountll(flip);
+    // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
+    uint64_t ck = 0, cd = 0;
 #pragma unroll
     for (int b = 0; b < W; ++b) {
-        const int64_t cnt = (int64_t)__builtin_popcountll(wave_transpose(acc[b], lane)) << b;
-        tot += b == W - 1 ? -cnt : cnt;
+        if (b == 0) {  // half adder: no carry in
+            const uint64_t ak = c[0] & flip;
+            ck = kacc[0] & ak;
+            kacc[0] ^= ak;
+        } else if (b < B) {  // full adder
+            const uint64_t ak = c[b] & flip;
+            const uint64_t sk = bop3<kXor3>(kacc[b], ak, ck);
+            ck = bop3<kMaj>(kacc[b], ak, ck);
+            kacc[b] = sk;
+        } else {  // carry propagation only
+            const uint64_t sk = kacc[b] ^ ck;
+            ck &= kacc[b];
+            kacc[b] = sk;
+        }
+        const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
+        const uint64_t sd = bop3<kXor3>(dacc[b], ad, cd);
+        cd = bop3<kMaj>(dacc[b], ad, cd);
+        dacc[b] = sd;
     }
-    return tot;
-}
-
-// Resolves one queued replica: u_r's kEagerPlanes leading bits tie with
-// thr_r's, so u_r < thr_r iff tail >> kEagerPlanes < thr_r's low bits.
-__device__ __forceinline__ bool resolve_tail(uint32_t p, uint32_t slot, uint32_t sweep,
-                                             uint32_t key0, uint32_t key1, uint32_t r,
-                                             uint32_t thr) {
-    const U32x4 o = philox4x32_10(p, slot, sweep, (kTagTail << 24) | (r >> 2), key0, key1);
-    const uint32_t q = r & 3u;
-    const uint32_t w = q == 0 ? o.v[0] : q == 1 ? o.v[1] : q == 2 ? o.v[2] : o.v[3];
-    return (w >> kEagerPlanes) < (thr & ((1u << (32 - kEagerPlanes)) - 1u));
+    // Queue the replicas whose leading bits tie.
+    while (und) {
+        const uint32_t r = (uint32_t)__builtin_ctzll(und);
+        und &= und - 1;
+        uint32_t m = 0;
+#pragma unroll
+        for (int l = 0; l < LMAX; ++l)
+            if ((lev[l] >> r) & 1u) m = mrow[l] / kQStride;
+        const uint32_t at = atomicAdd(qcount, 1u);
+        if (at < kQueueCap) {
+            queue[at] = (it << 17) | (lane << 11) | (r << 5) | m;
+        } else if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) {
+            // queue full: resolve in place
+            __threadfence_block();
+            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+            atomicAdd(&edef[r], (unsigned long long)(((int64_t)m * w0) / 2));
+            nacc += 1;
+        }
+    }
 }
 
 // B: bit planes of the neighbour count (degree < 2^B).  W: width of the
@@ -334,7 +372,6 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t* __restrict__ clock) {
     static_assert(kMaxPerLane <= 8 && W >= B + 3, "energy counters too narrow");
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
-    constexpr int LMAX = DEG > 0 ? DEG - DEG / 2 : (1 << (B - 1));
     extern __shared__ unsigned long long lds_q[];
     // Ties wait in a per-wave LDS queue: one entry per replica for any degree;
     // on a regular lattice one record per site instead (its tied replicas, the
@@ -359,20 +396,8 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         // With a fixed even degree only the levels m = 2, 4, ..., DEG are read.
         const uint32_t tr = temp_of[slot * kWave + lane];
         constexpr uint32_t m_step = DEG > 0 ? 2 : 1;
-        for (uint32_t m = (DEG > 0 ? 2 : 0) + m_step * wv; m <= dmax; m += m_step * kWavesPerBlock) {
-            const uint32_t th = lev_thr[tr * (dmax + 1) + m];
-            unsigned long long* out = lds_q + m * kQStride;
-            uint64_t mine = 0;
-#pragma unroll
-            for (uint32_t b = 0; b < kEagerPlanes; ++b) {
-                const uint64_t plane = ballot((th >> (31u - b)) & 1u);
-                if (lane == b) mine = plane;
-            }
-            const uint64_t nz = ballot(th != 0);
-            if (lane < kEagerPlanes) out[lane] = mine;
-            if (lane == kEagerPlanes) out[kEagerPlanes] = nz;
-            reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[lane] = th;
-        }
+        build_records(lds_q, lev_thr, tr, dmax, (DEG > 0 ? 2 : 0) + m_step * wv,
+                      m_step * kWavesPerBlock, lane);
         edef[wv][lane] = 0;
         if (lane == 0) qcount[wv] = 0;
         if (wv == 0) blk_e[lane] = 0;
@@ -394,17 +419,14 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t active = min(kWavesPerBlock, (c_end - c_begin + span - 1) / span - first_wave);
     const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
     uint64_t* Wr = spins_wr + (uint64_t)slot * n;
-    auto thr_of = [&](uint32_t m, uint32_t r) -> uint32_t {
-        const unsigned long long w = lds_q[m * kQStride + kEagerPlanes + 1 + (r >> 1)];
-        return (r & 1u) ? (uint32_t)(w >> 32) : (uint32_t)w;
-    };
+    auto thr_of = [&](uint32_t m, uint32_t r) -> uint32_t { return record_thr(lds_q, m, r); };
+    const TieRecords tq{rec_und[wv], rec_l1[wv], rec_l2[wv], rec_pos[wv], &qcount[wv], kRecCap};
 
     // Bit-sliced per-replica sums over this lane's sites.  Any degree:
     // kacc = sum k * flip and dacc = sum d * flip, so dE = w0 (2 kacc - dacc) / 2.
     // Regular lattices (d = DEG even): sacc = sum (k - DEG/2) * flip in kSW-plane
     // two's complement (each term in [-3, 3], at most kMaxPerLane of them), so
     // dE = w0 sacc.
-    constexpr int kSW = 6;
     static_assert(3 * kMaxPerLane < (1u << (kSW - 1)), "signed counters too narrow");
     uint64_t kacc[W], dacc[W], sacc[kSW];
 #pragma unroll
@@ -416,234 +438,19 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     for (uint32_t it = 0; it < per_lane; ++it) {
         const uint32_t p = base + it * kWave + lane;
         if (p >= c_end) break;
-        // word and entry byte offsets fit 32 bits (multispin_shape: n < 2^29,
-        // nnz < 2^30)
-        const uint64_t x = ld32(S, p);
-        const uint32_t e0 = DEG > 0 ? p * (uint32_t)DEG : ld32(rp, p);
-        const uint32_t d = DEG > 0 ? (uint32_t)DEG : ld32(rp, p + 1) - e0;
-        const uint32_t sg = ld32(sgn, p);
-        uint64_t c[B];
-#pragma unroll
-        for (int b = 0; b < B; ++b) c[b] = 0;
-        auto bond = [&](uint32_t e) {
-            return bop3s<kXor3>(x, ld32(S, ld32(ci, e0 + e)), bit_mask(sg, e));
-        };
-        if (DEG == 6 || DEG == 4) {
-            // carry-save adder tree: c = popcount of the bond words, bit-sliced
-            auto fa = [](uint64_t a, uint64_t b, uint64_t cin, uint64_t& sum, uint64_t& cout) {
-                sum = bop3<kXor3>(a, b, cin);
-                cout = bop3<kMaj>(a, b, cin);
-            };
-            uint64_t s1, c1;
-            fa(bond(0), bond(1), bond(2), s1, c1);
-            if (DEG == 6) {
-                uint64_t s2, c2;
-                fa(bond(3), bond(4), bond(5), s2, c2);
-                c[0] = s1 ^ s2;
-                fa(c1, c2, s1 & s2, c[1], c[2]);
-            } else {
-                const uint64_t k3 = bond(3);
-                c[0] = s1 ^ k3;
-                const uint64_t t = s1 & k3;
-                c[1] = c1 ^ t;
-                c[2] = c1 & t;
-            }
-        } else {
-            // Rows of any length in groups of kRowGroup entries: a group's
-            // index loads issue together, then its word loads, so a row costs
-            // two memory round trips per group rather than two per neighbour
-            // (the sweep of a small colour class is one wave's walk along its
-            // rows).  Entries past the row's end repeat its last one and are
-            // masked out of the count.
-            constexpr uint32_t kRowGroup = 8;
-            for (uint32_t g = 0; g < d; g += kRowGroup) {
-                uint32_t j[kRowGroup];
-                uint64_t wj[kRowGroup];
-#pragma unroll
-                for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ld32(ci, e0 + min(g + q, d - 1));
-#pragma unroll
-                for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld32(S, j[q]);
-#pragma unroll
-                for (uint32_t q = 0; q < kRowGroup; ++q) {
-                    const uint32_t e = g + q;
-                    uint64_t carry = bop3s<kXor3>(x, wj[q], bit_mask(sg, e & 31u)) &
-                                     (e < d ? ~0ull : 0ull);
-#pragma unroll
-                    for (int b = 0; b < B; ++b) {
-                        const uint64_t s = c[b] ^ carry;
-                        carry &= c[b];
-                        c[b] = s;
-                    }
-                }
-            }
-        }
-        // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
-        uint64_t lev[LMAX];
-        uint32_t mrow[LMAX];
-        uint64_t pos = 0, und = 0;
-        if constexpr (DEG == 6 || DEG == 4) {
-            // d fixed: each level is one three-input function of the count planes
-            static_assert(B == 3, "regular lattices use three count planes");
-            lev[0] = bop3<count3_tt(DEG / 2 + 1)>(c[0], c[1], c[2]);
-            lev[1] = bop3<count3_tt(DEG / 2 + 2)>(c[0], c[1], c[2]);
-            if constexpr (DEG == 6) lev[2] = bop3<count3_tt(DEG / 2 + 3)>(c[0], c[1], c[2]);
-#pragma unroll
-            for (int j = 0; j < LMAX; ++j) {
-                mrow[j] = (2 * (DEG / 2 + 1 + j) - DEG) * kQStride;
-                pos |= lev[j];
-                und = bop3<kAndOr>(lev[j], lds_q[mrow[j] + kEagerPlanes], und);
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < LMAX; ++j) {
-                const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
-                uint64_t eq = k <= d ? ~0ull : 0ull;
-#pragma unroll
-                for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
-                lev[j] = eq;
-                mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
-                pos |= eq;
-                und = bop3<kAndOr>(eq, lds_q[mrow[j] + kEagerPlanes], und);
-            }
-        }
-        // kEagerPlanes lockstep bit planes, two per Philox call.
-        uint64_t lt = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
-            const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
-#pragma unroll
-            for (uint32_t h = 0; h < 2; ++h) {
-                const uint32_t b = 2 * j + h;
-                const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
-                                          : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
-                uint64_t T = lev[0] & lds_q[mrow[0] + b];
-#pragma unroll
-                for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], lds_q[mrow[l] + b], T);
-                lt |= bop3<kAndAndNot>(und, T, P);
-                und = bop3<kAndXnor>(und, P, T);
-            }
-        }
-        const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
-        st32(Wr, p, x ^ flip);
-        nacc += (uint32_t)__builtin_popcountll(flip);
         if constexpr (DEG > 0) {
-            // (k - DEG/2) * flip as a three-plane two's-complement addend:
-            // k + (8 - DEG/2) mod 8, i.e. + 101b for DEG 6 and + 110b for DEG 4.
-            uint64_t a0, a1, a2;
-            if constexpr (DEG == 6) {
-                a0 = bop3<kNotAnd>(c[0], flip, 0ull);              // ~c0, carry c0
-                a1 = bop3<kXorAnd>(c[1], c[0], flip);              // c1 ^ c0, carry c1 c0
-                a2 = bop3<kXnorAnd>(c[2], c[1] & c[0], flip);      // c2 + 1 + carry
-            } else {
-                a0 = c[0] & flip;                                  // c0 + 0
-                a1 = bop3<kNotAnd>(c[1], flip, 0ull);              // ~c1, carry c1
-                a2 = bop3<kXnorAnd>(c[2], c[1], flip);             // c2 + 1 + carry
-            }
-            // sacc += a, a sign-extended from plane 2 (ripple add)
-            uint64_t cs = sacc[0] & a0;
-            sacc[0] ^= a0;
-#pragma unroll
-            for (int b = 1; b < kSW; ++b) {
-                const uint64_t ab = b == 1 ? a1 : a2;
-                const uint64_t sb = bop3<kXor3>(sacc[b], ab, cs);
-                if (b + 1 < kSW) cs = bop3<kMaj>(sacc[b], ab, cs);
-                sacc[b] = sb;
-            }
+            regular_site<DEG>(S, Wr, p, ci, sgn, lds_q, slot, sweep, key0, key1, w0, sacc, nacc,
+                              tq, edef[wv]);
         } else {
-            // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
-            uint64_t ck = 0, cd = 0;
-#pragma unroll
-            for (int b = 0; b < W; ++b) {
-                if (b == 0) {  // half adder: no carry in
-                    const uint64_t ak = c[0] & flip;
-                    ck = kacc[0] & ak;
-                    kacc[0] ^= ak;
-                } else if (b < B) {  // full adder
-                    const uint64_t ak = c[b] & flip;
-                    const uint64_t sk = bop3<kXor3>(kacc[b], ak, ck);
-                    ck = bop3<kMaj>(kacc[b], ak, ck);
-                    kacc[b] = sk;
-                } else {  // carry propagation only
-                    const uint64_t sk = kacc[b] ^ ck;
-                    ck &= kacc[b];
-                    kacc[b] = sk;
-                }
-                const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
-                const uint64_t sd = bop3<kXor3>(dacc[b], ad, cd);
-                cd = bop3<kMaj>(dacc[b], ad, cd);
-                dacc[b] = sd;
-            }
-        }
-        // Queue the replicas whose leading bits tie.
-        if constexpr (DEG > 0) {
-            if (und) {
-                const uint32_t at = atomicAdd(&qcount[wv], 1u);
-                if (at < kRecCap) {
-                    rec_und[wv][at] = und;
-                    rec_l1[wv][at] = und & lev[1];
-                    if constexpr (DEG == 6) rec_l2[wv][at] = und & lev[2];
-                    rec_pos[wv][at] = (it << 6) | lane;
-                } else {
-                    // queue full: resolve in place; level j has m = 2 + 2j
-                    __threadfence_block();
-                    while (und) {
-                        const uint32_t r = (uint32_t)__builtin_ctzll(und);
-                        und &= und - 1;
-                        uint32_t m = 2;
-#pragma unroll
-                        for (int l = 1; l < LMAX; ++l)
-                            if ((lev[l] >> r) & 1u) m = 2 + 2 * l;
-                        if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
-                            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-                            atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
-                            nacc += 1;
-                        }
-                    }
-                }
-            }
-        } else {
-            while (und) {
-                const uint32_t r = (uint32_t)__builtin_ctzll(und);
-                und &= und - 1;
-                uint32_t m = 0;
-#pragma unroll
-                for (int l = 0; l < LMAX; ++l)
-                    if ((lev[l] >> r) & 1u) m = mrow[l] / kQStride;
-                const uint32_t at = atomicAdd(&qcount[wv], 1u);
-                if (at < kQueueCap) {
-                    queue[wv][at] = (it << 17) | (lane << 11) | (r << 5) | m;
-                } else if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
-                    // queue full: resolve in place
-                    __threadfence_block();
-                    atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-                    atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
-                    nacc += 1;
-                }
-            }
+            irregular_site<B, W>(S, Wr, p, rp, ci, sgn, lds_q, slot, sweep, key0, key1, w0, kacc,
+                                 dacc, nacc, queue[wv], &qcount[wv], it, lane, edef[wv]);
         }
     }
     // Drain the queue with every lane of the wave.
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
     if constexpr (DEG > 0) {
-        const uint32_t queued = min(qcount[wv], kRecCap);
-        for (uint32_t q = lane; q < queued; q += kWave) {
-            uint64_t u = rec_und[wv][q];
-            const uint64_t l1 = rec_l1[wv][q];
-            const uint64_t l2 = DEG == 6 ? rec_l2[wv][q] : 0ull;
-            const uint32_t pos = rec_pos[wv][q];
-            const uint32_t p = base + (pos >> 6) * kWave + (pos & 63u);
-            while (u) {
-                const uint32_t r = (uint32_t)__builtin_ctzll(u);
-                u &= u - 1;
-                const uint32_t m = ((l1 >> r) & 1u) ? 4u : ((l2 >> r) & 1u) ? 6u : 2u;
-                if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
-                    atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-                    atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
-                    nacc += 1;
-                }
-            }
-        }
+        drain_ties<DEG>(tq, Wr, lds_q, slot, sweep, key0, key1, w0, edef[wv], lane, nacc);
     } else {
         const uint32_t queued = min(qcount[wv], kQueueCap);
         for (uint32_t q = lane; q < queued; q += kWave) {

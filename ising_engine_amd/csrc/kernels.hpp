@@ -96,6 +96,22 @@ constexpr uint32_t kEagerPlanes = 8;
 // leading bit planes, the nonzero mask and the 64 thresholds (two per u64).
 constexpr uint32_t kQStride = kEagerPlanes + 1 + 32;
 
+// Whole runs on small regular lattices (degree 4 or 6, n <= kResidentMaxSites,
+// no exchange): one workgroup per slot keeps the slot's words in LDS for all
+// `sweeps` sweeps (numbered sweep0, sweep0 + 1, ...), the same updates as the
+// colour-class launches.  class_start: ncol + 1 class bounds on the device.
+// Sweep k reads its thresholds from lev_rows + k * row_stride, laid out as
+// lev_thr below (row_stride 0: one row for every sweep).
+constexpr uint32_t kResidentMaxSites = 4096;
+hipError_t launch_resident_sweeps(const DevModel& m, const DevState& s,
+                                  const uint32_t* class_start, uint32_t ncol,
+                                  const uint32_t* lev_rows, uint32_t row_stride, uint32_t sweep0,
+                                  uint32_t sweeps, uint64_t seed, hipStream_t stream);
+// lev_thr rows for `rows` ladders beta_rows[k * cells + t] into out (rows x
+// cells x (dmax + 1)).
+hipError_t launch_level_rows(const DevModel& m, uint32_t cells, const double* beta_rows,
+                             uint32_t rows, uint32_t* out, hipStream_t stream);
+
 // lev_thr[t*(dmax+1) + m] = metropolis_threshold(beta[t], m*w0/2) for m in
 // [1, dmax], 0 for m = 0.  It depends on the ladder alone, so an exchange
 // (which moves temperatures between slots) needs no refresh.

@@ -1,0 +1,310 @@
+// multispin_ops.h -- the bit-sliced pieces of the multispin sweep (lane =
+// site, one u64 word = 64 replicas), shared by the colour-class launch
+// (sweep_multispin_kernel, kernels.hip) and the resident anneal
+// (resident.hip).
+//
+// Model restriction (checked on the host): |q_ij| = w0 for every bond and
+// lin_i = -1/2 sum_j q_ij, so dE_i(replica r) = w0 (k_r - d_i / 2) with
+// k_r = #{ j : bit r of (x_i ^ x_j ^ [q_ij < 0]) }.  Replicas with dE <= 0
+// are accepted outright; the rest compare their uniform u_r with the
+// threshold thr_r on its kEagerPlanes leading bits (bit planes from Philox,
+// every lane in lockstep), and the replicas whose leading bits tie are
+// resolved later from a tail word.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ige_math.h"
+#include "kernels.hpp"
+#include "sweep_generic.h"
+
+namespace ige {
+namespace msc {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kTagTail = 5u;
+constexpr int kSW = 6;  // planes of the signed energy counter of regular lattices
+
+// Bitwise functions of three 64-bit words, one v_bitop3_b32 per 32-bit half.
+// The truth-table byte of f(a, b, c) is f(0xF0, 0xCC, 0xAA): bit i of the
+// table is f of bits (i >> 2) & 1, (i >> 1) & 1 and i & 1 of the operands.
+constexpr uint32_t kXor3 = (0xF0 ^ 0xCC ^ 0xAA) & 0xFF;                         // a ^ b ^ c
+constexpr uint32_t kMaj = ((0xF0 & 0xCC) | (0xF0 & 0xAA) | (0xCC & 0xAA)) & 0xFF;  // majority
+constexpr uint32_t kAndOr = ((0xF0 & 0xCC) | 0xAA) & 0xFF;                      // (a & b) | c
+constexpr uint32_t kAndAndNot = (0xF0 & 0xCC & ~0xAA) & 0xFF;                   // a & b & ~c
+constexpr uint32_t kAndXnor = (0xF0 & ~(0xCC ^ 0xAA)) & 0xFF;                   // a & ~(b ^ c)
+constexpr uint32_t kNotOr = (~0xF0 | 0xCC) & 0xFF;                              // ~a | b
+constexpr uint32_t kNotAnd = (~0xF0 & 0xCC) & 0xFF;                             // ~a & b
+constexpr uint32_t kXorAnd = ((0xF0 ^ 0xCC) & 0xAA) & 0xFF;                     // (a ^ b) & c
+constexpr uint32_t kXnorAnd = (~(0xF0 ^ 0xCC) & 0xAA) & 0xFF;                   // ~(a ^ b) & c
+static_assert(kXor3 == 0x96 && kMaj == 0xE8, "truth tables");
+
+// Truth table of "the three planes (a, b, c) spell the count k" (bit 0 in a).
+constexpr uint32_t count3_tt(uint32_t k) {
+    return ((k & 1u) ? 0xF0u : 0x0Fu) & ((k & 2u) ? 0xCCu : 0x33u) & ((k & 4u) ? 0xAAu : 0x55u);
+}
+
+template <uint32_t TT>
+__device__ __forceinline__ uint64_t bop3(uint64_t a, uint64_t b, uint64_t c) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, TT);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32),
+                                                    (uint32_t)(c >> 32), TT);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// As bop3 with the same 32-bit c in both halves.
+template <uint32_t TT>
+__device__ __forceinline__ uint64_t bop3s(uint64_t a, uint64_t b, uint32_t c) {
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, c, TT);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), c, TT);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// base[i] with a 32-bit byte offset (the caller guarantees i * sizeof(T) <
+// 2^32), so a global load takes the uniform base in SGPRs and one offset VGPR
+// instead of a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
+// All ones if bit e of sg is set, else zero.
+__device__ __forceinline__ uint32_t bit_mask(uint32_t sg, uint32_t e) {
+    return (uint32_t)((int32_t)(sg << (31u - e)) >> 31);
+}
+
+// Sum over the wave's lanes of the per-lane bit-sliced counter acc[0..W-1],
+// returned in lane r for replica r.
+template <int W>
+__device__ __forceinline__ int64_t wave_column_sum(const uint64_t* acc, uint32_t lane) {
+    int64_t tot = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b)
+        tot += (int64_t)__builtin_popcountll(wave_transpose(acc[b], lane)) << b;
+    return tot;
+}
+
+// As wave_column_sum for W-plane two's-complement counters (plane W-1 weighs
+// -2^(W-1)).
+template <int W>
+__device__ __forceinline__ int64_t wave_column_sum_signed(const uint64_t* acc, uint32_t lane) {
+    int64_t tot = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+        const int64_t cnt = (int64_t)__builtin_popcountll(wave_transpose(acc[b], lane)) << b;
+        tot += b == W - 1 ? -cnt : cnt;
+    }
+    return tot;
+}
+
+// Resolves one tied replica: u_r's kEagerPlanes leading bits tie with thr_r's,
+// so u_r < thr_r iff tail >> kEagerPlanes < thr_r's low bits.
+__device__ __forceinline__ bool resolve_tail(uint32_t p, uint32_t slot, uint32_t sweep,
+                                             uint32_t key0, uint32_t key1, uint32_t r,
+                                             uint32_t thr) {
+    const U32x4 o = philox4x32_10(p, slot, sweep, (kTagTail << 24) | (r >> 2), key0, key1);
+    const uint32_t q = r & 3u;
+    const uint32_t w = q == 0 ? o.v[0] : q == 1 ? o.v[1] : q == 2 ? o.v[2] : o.v[3];
+    return (w >> kEagerPlanes) < (thr & ((1u << (32 - kEagerPlanes)) - 1u));
+}
+
+// Threshold of replica r at level m from a slot's threshold records (kQStride
+// u64 per level: the kEagerPlanes leading bit planes, the nonzero mask, then
+// the 64 thresholds two per word).
+__device__ __forceinline__ uint32_t record_thr(const unsigned long long* recs, uint32_t m,
+                                               uint32_t r) {
+    const unsigned long long w = recs[m * kQStride + kEagerPlanes + 1 + (r >> 1)];
+    return (r & 1u) ? (uint32_t)(w >> 32) : (uint32_t)w;
+}
+
+// Builds a slot's threshold records in LDS for levels m = m0, m0 + step, ...,
+// <= dmax: lane r reads thr(temperature of lane r, m) from lev_thr ([cells][dmax
+// + 1]) and nine ballots make the planes.  Every lane of the wave takes part.
+__device__ __forceinline__ void build_records(unsigned long long* recs, const uint32_t* lev_thr,
+                                              uint32_t tr, uint32_t dmax, uint32_t m0,
+                                              uint32_t step, uint32_t lane) {
+    for (uint32_t m = m0; m <= dmax; m += step) {
+        const uint32_t th = lev_thr[tr * (dmax + 1) + m];
+        unsigned long long* out = recs + m * kQStride;
+        uint64_t mine = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < kEagerPlanes; ++b) {
+            const uint64_t plane = wave_ballot((th >> (31u - b)) & 1u);
+            if (lane == b) mine = plane;
+        }
+        const uint64_t nz = wave_ballot(th != 0);
+        if (lane < kEagerPlanes) out[lane] = mine;
+        if (lane == kEagerPlanes) out[kEagerPlanes] = nz;
+        reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[lane] = th;
+    }
+}
+
+// One wave's tie records on a regular lattice: per site with tied replicas,
+// the tied mask, its subsets at levels 1 and 2 (level j has m = 2 + 2j, dE =
+// w0 m / 2) and the position.  All in LDS; *count is the wave's fill.
+struct TieRecords {
+    unsigned long long* und;
+    unsigned long long* l1;
+    unsigned long long* l2;  // degree 6 only
+    uint32_t* pos;
+    uint32_t* count;
+    uint32_t cap;
+};
+
+// Resolves the tied replicas u of position p: flips the accepted ones in Wr
+// (atomic XOR: other lanes own other words of the same wave) and adds their
+// dE to edef[r].
+template <int DEG>
+__device__ __forceinline__ void resolve_site_ties(uint64_t u, uint64_t l1, uint64_t l2, uint32_t p,
+                                                  uint64_t* Wr, const unsigned long long* recs,
+                                                  uint32_t slot, uint32_t sweep, uint32_t key0,
+                                                  uint32_t key1, int32_t w0,
+                                                  unsigned long long* edef, uint32_t& nacc) {
+    while (u) {
+        const uint32_t r = (uint32_t)__builtin_ctzll(u);
+        u &= u - 1;
+        const uint32_t m = ((l1 >> r) & 1u) ? 4u : (DEG == 6 && ((l2 >> r) & 1u)) ? 6u : 2u;
+        if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) {
+            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
+            atomicAdd(&edef[r], (unsigned long long)(((int64_t)m * w0) / 2));
+            nacc += 1;
+        }
+    }
+}
+
+// Drains a wave's tie records with all 64 lanes.
+template <int DEG>
+__device__ __forceinline__ void drain_ties(const TieRecords& tq, uint64_t* Wr,
+                                           const unsigned long long* recs, uint32_t slot,
+                                           uint32_t sweep, uint32_t key0, uint32_t key1,
+                                           int32_t w0, unsigned long long* edef, uint32_t lane,
+                                           uint32_t& nacc) {
+    const uint32_t queued = min(*tq.count, tq.cap);
+    for (uint32_t q = lane; q < queued; q += kWave)
+        resolve_site_ties<DEG>(tq.und[q], tq.l1[q], DEG == 6 ? tq.l2[q] : 0ull, tq.pos[q], Wr,
+                               recs, slot, sweep, key0, key1, w0, edef, nacc);
+}
+
+// The update of position p on a regular lattice (degree DEG in {4, 6}, row p
+// at p * DEG, count planes B = 3): reads the word and its neighbours from S,
+// writes the new word to Wr, adds (k - DEG/2) * flip to the signed counter
+// sacc and the flip count to nacc, and records the tied replicas in tq (or,
+// when tq is full, resolves them in place).
+template <int DEG>
+__device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, uint32_t p,
+                                             const uint32_t* ci, const uint32_t* sgn,
+                                             const unsigned long long* recs, uint32_t slot,
+                                             uint32_t sweep, uint32_t key0, uint32_t key1,
+                                             int32_t w0, uint64_t (&sacc)[kSW], uint32_t& nacc,
+                                             const TieRecords& tq, unsigned long long* edef) {
+    static_assert(DEG == 4 || DEG == 6, "regular lattices of degree 4 or 6");
+    constexpr int LMAX = DEG - DEG / 2;
+    // word and entry byte offsets fit 32 bits (multispin_shape: n < 2^29,
+    // nnz < 2^30)
+    const uint64_t x = ld32(S, p);
+    const uint32_t e0 = p * (uint32_t)DEG;
+    const uint32_t sg = ld32(sgn, p);
+    auto bond = [&](uint32_t e) {
+        return bop3s<kXor3>(x, ld32(S, ld32(ci, e0 + e)), bit_mask(sg, e));
+    };
+    // carry-save adder tree: c = popcount of the bond words, bit-sliced
+    auto fa = [](uint64_t a, uint64_t b, uint64_t cin, uint64_t& sum, uint64_t& cout) {
+        sum = bop3<kXor3>(a, b, cin);
+        cout = bop3<kMaj>(a, b, cin);
+    };
+    uint64_t c[3];
+    uint64_t s1, c1;
+    fa(bond(0), bond(1), bond(2), s1, c1);
+    if constexpr (DEG == 6) {
+        uint64_t s2, c2;
+        fa(bond(3), bond(4), bond(5), s2, c2);
+        c[0] = s1 ^ s2;
+        fa(c1, c2, s1 & s2, c[1], c[2]);
+    } else {
+        const uint64_t k3 = bond(3);
+        c[0] = s1 ^ k3;
+        const uint64_t t = s1 & k3;
+        c[1] = c1 ^ t;
+        c[2] = c1 & t;
+    }
+    // Level masks for k = DEG/2 + 1 + j (level m = 2k - DEG = 2 + 2j), each one
+    // three-input function of the count planes.
+    uint64_t lev[LMAX];
+    lev[0] = bop3<count3_tt(DEG / 2 + 1)>(c[0], c[1], c[2]);
+    lev[1] = bop3<count3_tt(DEG / 2 + 2)>(c[0], c[1], c[2]);
+    if constexpr (DEG == 6) lev[2] = bop3<count3_tt(DEG / 2 + 3)>(c[0], c[1], c[2]);
+    uint64_t pos = 0, und = 0;
+#pragma unroll
+    for (int j = 0; j < LMAX; ++j) {
+        pos |= lev[j];
+        und = bop3<kAndOr>(lev[j], recs[(2 + 2 * j) * kQStride + kEagerPlanes], und);
+    }
+    // kEagerPlanes lockstep bit planes, two per Philox call.
+    uint64_t lt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
+        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t b = 2 * j + h;
+            const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
+                                      : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
+            uint64_t T = lev[0] & recs[2 * kQStride + b];
+#pragma unroll
+            for (int l = 1; l < LMAX; ++l)
+                T = bop3<kAndOr>(lev[l], recs[(2 + 2 * l) * kQStride + b], T);
+            lt |= bop3<kAndAndNot>(und, T, P);
+            und = bop3<kAndXnor>(und, P, T);
+        }
+    }
+    const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
+    st32(Wr, p, x ^ flip);
+    nacc += (uint32_t)__builtin_popcountll(flip);
+    // (k - DEG/2) * flip as a three-plane two's-complement addend:
+    // k + (8 - DEG/2) mod 8, i.e. + 101b for DEG 6 and + 110b for DEG 4.
+    uint64_t a0, a1, a2;
+    if constexpr (DEG == 6) {
+        a0 = bop3<kNotAnd>(c[0], flip, 0ull);          // ~c0, carry c0
+        a1 = bop3<kXorAnd>(c[1], c[0], flip);          // c1 ^ c0, carry c1 c0
+        a2 = bop3<kXnorAnd>(c[2], c[1] & c[0], flip);  // c2 + 1 + carry
+    } else {
+        a0 = c[0] & flip;                              // c0 + 0
+        a1 = bop3<kNotAnd>(c[1], flip, 0ull);          // ~c1, carry c1
+        a2 = bop3<kXnorAnd>(c[2], c[1], flip);         // c2 + 1 + carry
+    }
+    // sacc += a, a sign-extended from plane 2 (ripple add)
+    uint64_t cs = sacc[0] & a0;
+    sacc[0] ^= a0;
+#pragma unroll
+    for (int b = 1; b < kSW; ++b) {
+        const uint64_t ab = b == 1 ? a1 : a2;
+        const uint64_t sb = bop3<kXor3>(sacc[b], ab, cs);
+        if (b + 1 < kSW) cs = bop3<kMaj>(sacc[b], ab, cs);
+        sacc[b] = sb;
+    }
+    // Record the replicas whose leading bits tie.
+    if (und) {
+        const uint32_t at = atomicAdd(tq.count, 1u);
+        const uint64_t l1 = und & lev[1];
+        const uint64_t l2 = DEG == 6 ? (und & lev[LMAX - 1]) : 0ull;
+        if (at < tq.cap) {
+            tq.und[at] = und;
+            tq.l1[at] = l1;
+            if constexpr (DEG == 6) tq.l2[at] = l2;
+            tq.pos[at] = p;
+        } else {
+            __threadfence_block();
+            resolve_site_ties<DEG>(und, l1, l2, p, Wr, recs, slot, sweep, key0, key1, w0, edef,
+                                   nacc);
+        }
+    }
+}
+
+}  // namespace msc
+}  // namespace ige
