@@ -45,6 +45,9 @@ struct ige_model {
     int64_t max_local = 0;
     uint32_t dmax = 0;
     uint32_t regular_degree = 0;
+    // Multispin sweeps run classes [fuse_from, num_colors) in one launch (each
+    // at most kFuseMaxClassSites sites, at least two classes); num_colors: none.
+    uint32_t fuse_from = 0;
     DevBuf<uint32_t> rp, ci, order, pos_of, sgn, cstart;  // cstart: colour class bounds
     DevBuf<int32_t> wt, lin;
     // Dense int8 matrix, built on the first dense state of this model.
@@ -226,6 +229,12 @@ void finish_model(ige_model* m) {
     m->dmax = m->ir.max_degree();
     m->colors = m->ir.greedy_coloring();
     m->co = ige::color_order(m->colors);
+    {
+        const auto& cs = m->co.class_start;
+        uint32_t c = m->co.num_colors;
+        while (c > 0 && cs[c] - cs[c - 1] <= ige::kFuseMaxClassSites) --c;
+        m->fuse_from = m->co.num_colors - c >= 2 ? c : m->co.num_colors;
+    }
     m->ms = ige::multispin_shape(m->ir);
     m->ds = ige::dense_shape(m->ir);
     m->regular_degree = m->dmax;
@@ -270,6 +279,7 @@ void icm_rounds(ige_state* s, uint32_t rounds, unsigned long long* flipped);
 // captured graph.
 struct StepPlan {
     bool exch = false, track = false, heat = false, icm = false, sched = false;
+    bool per_class = false;  // one launch per colour class (no fused tail)
     uint32_t hb_dtab = 0;
 };
 
@@ -306,17 +316,21 @@ void issue_step(ige_state* s, const StepPlan& P, const double* beta, uint32_t sw
         require(clock == nullptr, "the dense sweep is not captured");
         check(ige::launch_sweep_dense(s->model->dense_view(), st, sweep, s->seed, beta, s->stream),
               "sweep_dense");
+    } else if (s->kernel == IGE_KERNEL_GENERIC) {
+        for (uint32_t c = 0; c < s->model->co.num_colors; ++c)
+            check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], sweep, s->seed, s->thr.p,
+                                            s->dtab, s->stream),
+                  "sweep_generic");
     } else {
-        for (uint32_t c = 0; c < s->model->co.num_colors; ++c) {
-            if (s->kernel == IGE_KERNEL_GENERIC)
-                check(ige::launch_sweep_generic(m, st, cs[c], cs[c + 1], sweep, s->seed, s->thr.p,
-                                                s->dtab, s->stream),
-                      "sweep_generic");
-            else
-                check(ige::launch_sweep_multispin(m, st, cs[c], cs[c + 1], sweep, s->seed,
-                                                  s->lev_thr.p, s->stream),
-                      "sweep_multispin");
-        }
+        const uint32_t ncol = s->model->co.num_colors;
+        const uint32_t fuse = P.per_class ? ncol : s->model->fuse_from;
+        for (uint32_t c = 0; c < fuse; ++c)
+            check(ige::launch_sweep_multispin(m, st, cs[c], cs[c + 1], sweep, s->seed,
+                                              s->lev_thr.p, s->stream),
+                  "sweep_multispin");
+        check(ige::launch_classes_sweep(m, st, s->model->cstart.p, fuse, ncol, sweep, s->seed,
+                                        s->lev_thr.p, s->stream),
+              "classes_sweep");
     }
     const auto lap = [&](double& acc, uint64_t& count, uint64_t add) {
         check(hipEventRecord(s->ev1, s->stream), "event");
@@ -622,6 +636,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         P.heat = (flags & IGE_RUN_HEATBATH) != 0;
         P.icm = (flags & IGE_RUN_ICM) != 0;
         P.sched = sched != nullptr;
+        P.per_class = (flags & IGE_RUN_EAGER) || s->profile;
         P.hb_dtab = static_cast<uint32_t>(s->model->max_local);
         if (P.track) ensure_best(s);
         if (P.heat && !s->hb.p)

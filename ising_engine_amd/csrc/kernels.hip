@@ -233,131 +233,6 @@ constexpr uint32_t kQueueCap = 1024; // deferred replicas per wave (any degree)
 constexpr uint32_t kRecCap = 128;    // deferred sites per wave (regular lattices)
 constexpr uint32_t kMaxPerLane = 8;  // sites per lane
 
-// The update of position p for any degree d < 2^B (row p at rp[p]): count
-// planes from the row in groups, a level per k > d/2, the same lockstep
-// planes as regular_site, unsigned counters kacc += k flip and dacc += d flip,
-// and one queue entry per tied replica ((it << 17) | (lane << 11) | (r << 5) |
-// m), resolved in place when the queue is full.
-template <int B, int W>
-__device__ __forceinline__ void irregular_site(
-    const uint64_t* S, uint64_t* Wr, uint32_t p, const uint32_t* rp, const uint32_t* ci,
-    const uint32_t* sgn, const unsigned long long* recs, uint32_t slot, uint32_t sweep,
-    uint32_t key0, uint32_t key1, int32_t w0, uint64_t (&kacc)[W], uint64_t (&dacc)[W],
-    uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t it, uint32_t lane,
-    unsigned long long* edef) {
-    constexpr int LMAX = 1 << (B - 1);
-    const uint64_t x = ld32(S, p);
-    const uint32_t e0 = ld32(rp, p);
-    const uint32_t d = ld32(rp, p + 1) - e0;
-    const uint32_t sg = ld32(sgn, p);
-    uint64_t c[B];
-#pragma unroll
-    for (int b = 0; b < B; ++b) c[b] = 0;
-    // Rows in groups of kRowGroup entries: a group's index loads issue
-    // together, then its word loads, so a row costs two memory round trips per
-    // group rather than two per neighbour (the sweep of a small colour class
-    // is one wave's walk along its rows).  Entries past the row's end repeat
-    // its last one and are masked out of the count.
-    constexpr uint32_t kRowGroup = 8;
-    for (uint32_t g = 0; g < d; g += kRowGroup) {
-        uint32_t j[kRowGroup];
-        uint64_t wj[kRowGroup];
-#pragma unroll
-        for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ld32(ci, e0 + min(g + q, d - 1));
-#pragma unroll
-        for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld32(S, j[q]);
-#pragma unroll
-        for (uint32_t q = 0; q < kRowGroup; ++q) {
-            const uint32_t e = g + q;
-            uint64_t carry =
-                bop3s<kXor3>(x, wj[q], bit_mask(sg, e & 31u)) & (e < d ? ~0ull : 0ull);
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const uint64_t s = c[b] ^ carry;
-                carry &= c[b];
-                c[b] = s;
-            }
-        }
-    }
-    // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
-    uint64_t lev[LMAX];
-    uint32_t mrow[LMAX];
-    uint64_t pos = 0, und = 0;
-#pragma unroll
-    for (int j = 0; j < LMAX; ++j) {
-        const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
-        uint64_t eq = k <= d ? ~0ull : 0ull;
-#pragma unroll
-        for (int b = 0; b < B; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
-        lev[j] = eq;
-        mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
-        pos |= eq;
-        und = bop3<kAndOr>(eq, recs[mrow[j] + kEagerPlanes], und);
-    }
-    // kEagerPlanes lockstep bit planes, two per Philox call.
-    uint64_t lt = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
-        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
-#pragma unroll
-        for (uint32_t h = 0; h < 2; ++h) {
-            const uint32_t b = 2 * j + h;
-            const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
-                                      : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
-            uint64_t T = lev[0] & recs[mrow[0] + b];
-#pragma unroll
-            for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], recs[mrow[l] + b], T);
-            lt |= bop3<kAndAndNot>(und, T, P);
-            und = bop3<kAndXnor>(und, P, T);
-        }
-    }
-    const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
-    st32(Wr, p, x ^ flip);
-    nacc += (uint32_t)__builtin_popcountll(flip);
-    // kacc += k * flip, dacc += d * flip (bit-sliced ripple adds).
-    uint64_t ck = 0, cd = 0;
-#pragma unroll
-    for (int b = 0; b < W; ++b) {
-        if (b == 0) {  // half adder: no carry in
-            const uint64_t ak = c[0] & flip;
-            ck = kacc[0] & ak;
-            kacc[0] ^= ak;
-        } else if (b < B) {  // full adder
-            const uint64_t ak = c[b] & flip;
-            const uint64_t sk = bop3<kXor3>(kacc[b], ak, ck);
-            ck = bop3<kMaj>(kacc[b], ak, ck);
-            kacc[b] = sk;
-        } else {  // carry propagation only
-            const uint64_t sk = kacc[b] ^ ck;
-            ck &= kacc[b];
-            kacc[b] = sk;
-        }
-        const uint64_t ad = (b < B && ((d >> b) & 1u)) ? flip : 0ull;
-        const uint64_t sd = bop3<kXor3>(dacc[b], ad, cd);
-        cd = bop3<kMaj>(dacc[b], ad, cd);
-        dacc[b] = sd;
-    }
-    // Queue the replicas whose leading bits tie.
-    while (und) {
-        const uint32_t r = (uint32_t)__builtin_ctzll(und);
-        und &= und - 1;
-        uint32_t m = 0;
-#pragma unroll
-        for (int l = 0; l < LMAX; ++l)
-            if ((lev[l] >> r) & 1u) m = mrow[l] / kQStride;
-        const uint32_t at = atomicAdd(qcount, 1u);
-        if (at < kQueueCap) {
-            queue[at] = (it << 17) | (lane << 11) | (r << 5) | m;
-        } else if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) {
-            // queue full: resolve in place
-            __threadfence_block();
-            atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-            atomicAdd(&edef[r], (unsigned long long)(((int64_t)m * w0) / 2));
-            nacc += 1;
-        }
-    }
-}
-
 // B: bit planes of the neighbour count (degree < 2^B).  W: width of the
 // energy counters.  DEG > 0: every site has exactly DEG neighbours and row p
 // starts at p * DEG (regular lattices); DEG == 0: any degree < 2^B.
@@ -419,7 +294,6 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t active = min(kWavesPerBlock, (c_end - c_begin + span - 1) / span - first_wave);
     const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
     uint64_t* Wr = spins_wr + (uint64_t)slot * n;
-    auto thr_of = [&](uint32_t m, uint32_t r) -> uint32_t { return record_thr(lds_q, m, r); };
     const TieRecords tq{rec_und[wv], rec_l1[wv], rec_l2[wv], rec_pos[wv], &qcount[wv], kRecCap};
 
     // Bit-sliced per-replica sums over this lane's sites.  Any degree:
@@ -442,8 +316,9 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
             regular_site<DEG>(S, Wr, p, ci, sgn, lds_q, slot, sweep, key0, key1, w0, sacc, nacc,
                               tq, edef[wv]);
         } else {
-            irregular_site<B, W>(S, Wr, p, rp, ci, sgn, lds_q, slot, sweep, key0, key1, w0, kacc,
-                                 dacc, nacc, queue[wv], &qcount[wv], it, lane, edef[wv]);
+            irregular_site<B, W>(S, Wr, p, p - base, rp, ci, sgn, lds_q, slot, sweep, key0, key1,
+                                 w0, kacc, dacc, nacc, queue[wv], &qcount[wv], kQueueCap,
+                                 edef[wv]);
         }
     }
     // Drain the queue with every lane of the wave.
@@ -452,18 +327,8 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     if constexpr (DEG > 0) {
         drain_ties<DEG>(tq, Wr, lds_q, slot, sweep, key0, key1, w0, edef[wv], lane, nacc);
     } else {
-        const uint32_t queued = min(qcount[wv], kQueueCap);
-        for (uint32_t q = lane; q < queued; q += kWave) {
-            const uint32_t code = queue[wv][q];
-            const uint32_t it = code >> 17, sl = (code >> 11) & 63u, r = (code >> 5) & 63u,
-                           m = code & 31u;
-            const uint32_t p = base + it * kWave + sl;
-            if (resolve_tail(p, slot, sweep, key0, key1, r, thr_of(m, r))) {
-                atomicXor(reinterpret_cast<unsigned long long*>(Wr + p), 1ull << r);
-                atomicAdd(&edef[wv][r], (unsigned long long)(((int64_t)m * w0) / 2));
-                nacc += 1;
-            }
-        }
+        drain_queue(queue[wv], &qcount[wv], kQueueCap, base, Wr, lds_q, slot, sweep, key0, key1,
+                    w0, edef[wv], lane, nacc);
     }
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();

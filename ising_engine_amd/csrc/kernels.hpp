@@ -107,6 +107,18 @@ hipError_t launch_resident_sweeps(const DevModel& m, const DevState& s,
                                   const uint32_t* class_start, uint32_t ncol,
                                   const uint32_t* lev_rows, uint32_t row_stride, uint32_t sweep0,
                                   uint32_t sweeps, uint64_t seed, hipStream_t stream);
+// Colour classes [c0, c1) of sweep `sweep` in one workgroup per slot (any
+// multispin model), one class after the other: the same updates as
+// launch_sweep_multispin class by class.  For trailing classes too small to
+// be worth a launch each: every fused class is one pass of the workgroup's
+// kFuseMaxClassSites lanes (a class that needs several passes is a chain of
+// dependent memory round trips per pass and runs no faster fused).
+// lev_thr as for launch_sweep_multispin.
+constexpr uint32_t kFuseMaxClassSites = 512;
+hipError_t launch_classes_sweep(const DevModel& m, const DevState& s,
+                                const uint32_t* class_start, uint32_t c0, uint32_t c1,
+                                uint32_t sweep, uint64_t seed, const uint32_t* lev_thr,
+                                hipStream_t stream);
 // lev_thr rows for `rows` ladders beta_rows[k * cells + t] into out (rows x
 // cells x (dmax + 1)).
 hipError_t launch_level_rows(const DevModel& m, uint32_t cells, const double* beta_rows,

@@ -1,8 +1,9 @@
 """Resident runs (a small regular lattice's whole ige_run call in one workgroup per slot, words
-in LDS) against the colour-class launches of the same sweeps (IGE_RUN_EAGER), bit for bit after
-every call: words, int64 energies, temperatures, flip counts.  Covers both degrees, schedules and
-fixed ladders, several slots, the 4096-site limit, calls mixed with exchange runs (which take the
-other paths), and one run against the oracle directly."""
+in LDS) and fused small colour classes (trailing classes of a sweep in one launch) against the
+colour-class launches of the same sweeps (IGE_RUN_EAGER), bit for bit after every call: words,
+int64 energies, temperatures, flip counts.  Covers both degrees and irregular graphs, schedules
+and fixed ladders, several slots, the 4096-site limit, calls mixed with exchange runs, and runs
+against the oracle directly."""
 import numpy as np
 import pytest
 
@@ -91,3 +92,46 @@ def test_resident_run_matches_the_oracle(ctx):
         s = st.stats()[0]
         assert s.accepted == int(ost.stats[0]) and s.proposed == int(ost.stats[1])
     assert st.stats()[0].resident_sweeps == 67
+
+
+@pytest.mark.parametrize("name,factory,cells,exchange", [
+    ("er_pmJ_n20000_tail_of_small_classes", lambda: instances.erdos_renyi_pm_j(20000, 6, 86), 1,
+     False),
+    ("er_pmJ_n1500_tail_fused_pt", lambda: instances.erdos_renyi_pm_j(1500, 6, 87), 4, True),
+    ("pmJ_2d_L16_both_classes_fused_pt", lambda: instances.hypercubic_pm_j(16, 2, 88), 4, True),
+])
+def test_fused_small_classes_equal_per_class_launches(ctx, name, factory, cells, exchange):
+    """Trailing colour classes of at most 512 sites each run in one launch (one workgroup
+    per slot, class after class); IGE_RUN_EAGER launches each class alone."""
+    print(f"[fused] {name}", flush=True)
+    qb = factory()
+    model = make(ctx, qb)
+    seed = 0x5E51_2000 + len(name)
+    f = _lib.State(model, cells, seed, kernel=_lib.KERNEL_MULTISPIN)
+    e = _lib.State(model, cells, seed, kernel=_lib.KERNEL_MULTISPIN)
+    t_max, t_min = instances.suggested_temp_range(qb)
+    ladder = instances.geometric_betas(t_max, t_min, cells)
+    f.set_ladder(ladder)
+    e.set_ladder(ladder)
+    sched = np.repeat(instances.anneal_schedule(1.0 / t_max, 1.0 / t_min, 45), cells, axis=1)
+    for k0, k1 in ((0, 7), (7, 45)):
+        f.run(k1 - k0, exchange=exchange, beta_schedule=sched[k0:k1])
+        e.run(k1 - k0, exchange=exchange, beta_schedule=sched[k0:k1], eager=True)
+        same(f, e)
+    np.testing.assert_array_equal(f.recompute_energies(), f.energies())
+
+
+def test_fused_tail_matches_the_oracle(ctx):
+    qb = instances.erdos_renyi_pm_j(20000, 6, 89)
+    csr = ob.CSR.from_qubo(qb)
+    model = make(ctx, qb)
+    st = _lib.State(model, 1, 0x5E51_3000, kernel=_lib.KERNEL_MULTISPIN)
+    ost = ob.OracleState(csr, 1, 0x5E51_3000, view=ob.VIEW_PLANE)
+    t_max, t_min = instances.suggested_temp_range(qb)
+    sched = instances.anneal_schedule(1.0 / t_max, 1.0 / t_min, 6)
+    st.run(6, beta_schedule=sched)
+    ost.run(6, exchange=False, schedule=sched)
+    np.testing.assert_array_equal(st.spins(), ost.words)
+    np.testing.assert_array_equal(st.energies(), ost.energy)
+    s = st.stats()[0]
+    assert s.accepted == int(ost.stats[0])
