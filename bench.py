@@ -268,6 +268,20 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
     dt_pol = time.perf_counter() - t0
     ops = 2.0 * qb.n * qb.n * 64  # int8 multiply-adds of one full field recompute, x2
     mfma_tops = ops / (ms / 1e3 / launches) / 1e12
+    # The same window on R independent restarts (R cells of one state, one workgroup each,
+    # one per CU): a sweep is one chain per slot, so restarts fill the other CUs.
+    R = 256
+    stR = _lib.State(m, R, 11)
+    winR = np.repeat(win, R, axis=1)
+    stR.run(2, beta_schedule=winR[:2])
+    stR.synchronize()
+    t0 = time.perf_counter()
+    stR.run(len(winR), beta_schedule=winR)
+    stR.synchronize()
+    dtR = time.perf_counter() - t0
+    restarts_value = R * qb.n * 64 * len(winR) / dtR
+    restarts_tops = R * ops * len(winR) / dtR / 1e12
+    stR.close()
     out[wl["name"]] = {"value": qb.n * 64 / per_sweep, "unit": "spin-flip attempts/s",
                        "ms_per_sweep": per_sweep * 1e3,
                        "projected_anneal_s": per_sweep * wl["sweeps"],
@@ -283,8 +297,13 @@ def bench_workloads(_lib, instances, ctx, tts_runs: int, sk_sweeps: int, cpu_s: 
                                     "frac": mfma_tops / 5000.0,
                                     "frac_of_one_cu": mfma_tops / (5000.0 / 256),
                                     "ops_per_sweep": ops},
+                       "restarts": R, "restarts_ms_per_sweep": dtR / len(winR) * 1e3,
+                       "restarts_value": restarts_value,
+                       "restarts_field_tops": restarts_tops,
                        "cpu_1thread": cpu_sample(qb, 1, [1.0], cpu_s)}
-    log(f"[bench] {wl['name']}: {per_sweep * 1e3:.2f} ms/sweep, {mfma_tops:.2f} TOP/s int8")
+    log(f"[bench] {wl['name']}: {per_sweep * 1e3:.2f} ms/sweep, {mfma_tops:.2f} TOP/s int8; "
+        f"{R} restarts {dtR / len(winR) * 1e3:.2f} ms/sweep, {restarts_value:.3e} attempts/s, "
+        f"{restarts_tops:.1f} TOP/s int8")
     return out
 
 
