@@ -38,12 +38,12 @@ constexpr uint32_t kNotOr = (~0xF0 | 0xCC) & 0xFF;                              
 constexpr uint32_t kNotAnd = (~0xF0 & 0xCC) & 0xFF;                             // ~a & b
 constexpr uint32_t kXorAnd = ((0xF0 ^ 0xCC) & 0xAA) & 0xFF;                     // (a ^ b) & c
 constexpr uint32_t kXnorAnd = (~(0xF0 ^ 0xCC) & 0xAA) & 0xFF;                   // ~(a ^ b) & c
-static_assert(kXor3 == 0x96 && kMaj == 0xE8, "truth tables");
-
-// Truth table of "the three planes (a, b, c) spell the count k" (bit 0 in a).
-constexpr uint32_t count3_tt(uint32_t k) {
-    return ((k & 1u) ? 0xF0u : 0x0Fu) & ((k & 2u) ? 0xCCu : 0x33u) & ((k & 4u) ? 0xAAu : 0x55u);
-}
+constexpr uint32_t kMux = ((0xF0 & 0xCC) | (~0xF0 & 0xAA)) & 0xFF;              // a ? b : c
+constexpr uint32_t kAnd3 = (0xF0 & 0xCC & 0xAA) & 0xFF;                         // a & b & c
+// One step of a least-significant-first comparison u < t: with a = t's bit,
+// b = u's bit and c = the verdict of the lower bits, (a ^ b) ? a : c.
+constexpr uint32_t kLtStep = ((0xF0 & ~0xCC) | (~(0xF0 ^ 0xCC) & 0xAA)) & 0xFF;
+static_assert(kXor3 == 0x96 && kMaj == 0xE8 && kMux == 0xCA && kLtStep == 0xB2, "truth tables");
 
 template <uint32_t TT>
 __device__ __forceinline__ uint64_t bop3(uint64_t a, uint64_t b, uint64_t c) {
@@ -204,7 +204,6 @@ __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, ui
                                              int32_t w0, uint64_t (&sacc)[kSW], uint32_t& nacc,
                                              const TieRecords& tq, unsigned long long* edef) {
     static_assert(DEG == 4 || DEG == 6, "regular lattices of degree 4 or 6");
-    constexpr int LMAX = DEG - DEG / 2;
     // word and entry byte offsets fit 32 bits (multispin_shape: n < 2^29,
     // nnz < 2^30)
     const uint64_t x = ld32(S, p);
@@ -233,36 +232,45 @@ __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, ui
         c[1] = c1 ^ t;
         c[2] = c1 & t;
     }
-    // Level masks for k = DEG/2 + 1 + j (level m = 2k - DEG = 2 + 2j), each one
-    // three-input function of the count planes.
-    uint64_t lev[LMAX];
-    lev[0] = bop3<count3_tt(DEG / 2 + 1)>(c[0], c[1], c[2]);
-    lev[1] = bop3<count3_tt(DEG / 2 + 2)>(c[0], c[1], c[2]);
-    if constexpr (DEG == 6) lev[2] = bop3<count3_tt(DEG / 2 + 3)>(c[0], c[1], c[2]);
-    uint64_t pos = 0, und = 0;
+    // dE > 0 for k > DEG/2, at level m = 2k - DEG.  Degree 6 (k <= 6): pos =
+    // c2, and k = 4, 5, 6 (c = 100, 101, 110) is level 2, 4, 6 = c0 ? 4 : c1 ?
+    // 6 : 2.  Degree 4 (k <= 4): pos = c2 | c1 c0, and k = 3, 4 (011, 100) is
+    // level c2 ? 4 : 2.  Outside pos the selected record is arbitrary: those
+    // replicas are accepted whatever the comparison says.
+    const uint64_t pos = DEG == 6 ? c[2] : bop3<kAndOr>(c[1], c[0], c[2]);
+    auto level_row = [&](uint32_t b) {
+        const unsigned long long* r = recs + b;
+        if constexpr (DEG == 6)
+            return bop3<kMux>(c[0], r[4 * kQStride], bop3<kMux>(c[1], r[6 * kQStride],
+                                                                r[2 * kQStride]));
+        else
+            return bop3<kMux>(c[2], r[4 * kQStride], r[2 * kQStride]);
+    };
+    // kEagerPlanes lockstep bit planes, two per Philox call, compared least
+    // significant first: lt = u < thr on the leading bits (0 where they tie),
+    // eq = they tie.
+    uint64_t lt = 0, eq = 0;
 #pragma unroll
-    for (int j = 0; j < LMAX; ++j) {
-        pos |= lev[j];
-        und = bop3<kAndOr>(lev[j], recs[(2 + 2 * j) * kQStride + kEagerPlanes], und);
-    }
-    // kEagerPlanes lockstep bit planes, two per Philox call.
-    uint64_t lt = 0;
+    for (int j = kEagerPlanes / 2 - 1; j >= 0; --j) {
+        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | (uint32_t)j, key0,
+                                      key1);
 #pragma unroll
-    for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
-        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
-#pragma unroll
-        for (uint32_t h = 0; h < 2; ++h) {
-            const uint32_t b = 2 * j + h;
+        for (int h = 1; h >= 0; --h) {
+            const uint32_t b = 2 * (uint32_t)j + (uint32_t)h;
             const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
                                       : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
-            uint64_t T = lev[0] & recs[2 * kQStride + b];
-#pragma unroll
-            for (int l = 1; l < LMAX; ++l)
-                T = bop3<kAndOr>(lev[l], recs[(2 + 2 * l) * kQStride + b], T);
-            lt |= bop3<kAndAndNot>(und, T, P);
-            und = bop3<kAndXnor>(und, P, T);
+            const uint64_t T = level_row(b);
+            if (b == kEagerPlanes - 1) {
+                lt = T & ~P;
+                eq = ~(T ^ P);
+            } else {
+                lt = bop3<kLtStep>(T, P, lt);
+                eq = bop3<kAndXnor>(eq, T, P);
+            }
         }
     }
+    // ties of replicas with dE > 0 and a nonzero threshold go to the tail word
+    const uint64_t und = bop3<kAnd3>(eq, pos, level_row(kEagerPlanes));
     const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
     st32(Wr, p, x ^ flip);
     nacc += (uint32_t)__builtin_popcountll(flip);
@@ -291,8 +299,10 @@ __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, ui
     // Record the replicas whose leading bits tie.
     if (und) {
         const uint32_t at = atomicAdd(tq.count, 1u);
-        const uint64_t l1 = und & lev[1];
-        const uint64_t l2 = DEG == 6 ? (und & lev[LMAX - 1]) : 0ull;
+        // level 4 is k = 5 (c0 set) on degree 6, k = 4 (c2 set) on degree 4;
+        // level 6 is k = 6 (c1 set)
+        const uint64_t l1 = und & (DEG == 6 ? c[0] : c[2]);
+        const uint64_t l2 = DEG == 6 ? (und & c[1]) : 0ull;
         if (at < tq.cap) {
             tq.und[at] = und;
             tq.l1[at] = l1;
@@ -356,7 +366,7 @@ __device__ __forceinline__ void irregular_site(
     // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
     uint64_t lev[LMAX];
     uint32_t mrow[LMAX];
-    uint64_t pos = 0, und = 0;
+    uint64_t pos = 0;
 #pragma unroll
     for (int j = 0; j < LMAX; ++j) {
         const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
@@ -366,25 +376,39 @@ __device__ __forceinline__ void irregular_site(
         lev[j] = eq;
         mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
         pos |= eq;
-        und = bop3<kAndOr>(eq, recs[mrow[j] + kEagerPlanes], und);
     }
-    // kEagerPlanes lockstep bit planes, two per Philox call.
-    uint64_t lt = 0;
+    // Row b of each replica's level record (zero outside pos).
+    auto level_row = [&](uint32_t b) {
+        uint64_t T = lev[0] & recs[mrow[0] + b];
 #pragma unroll
-    for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
-        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
+        for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], recs[mrow[l] + b], T);
+        return T;
+    };
+    // kEagerPlanes lockstep bit planes, two per Philox call, compared least
+    // significant first: lt = u < thr on the leading bits (0 where they tie),
+    // eq = they tie.
+    uint64_t lt = 0, eq = 0;
 #pragma unroll
-        for (uint32_t h = 0; h < 2; ++h) {
-            const uint32_t b = 2 * j + h;
+    for (int j = kEagerPlanes / 2 - 1; j >= 0; --j) {
+        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | (uint32_t)j, key0,
+                                      key1);
+#pragma unroll
+        for (int h = 1; h >= 0; --h) {
+            const uint32_t b = 2 * (uint32_t)j + (uint32_t)h;
             const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
                                       : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
-            uint64_t T = lev[0] & recs[mrow[0] + b];
-#pragma unroll
-            for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], recs[mrow[l] + b], T);
-            lt |= bop3<kAndAndNot>(und, T, P);
-            und = bop3<kAndXnor>(und, P, T);
+            const uint64_t T = level_row(b);
+            if (b == kEagerPlanes - 1) {
+                lt = T & ~P;
+                eq = ~(T ^ P);
+            } else {
+                lt = bop3<kLtStep>(T, P, lt);
+                eq = bop3<kAndXnor>(eq, T, P);
+            }
         }
     }
+    // ties of replicas with dE > 0 and a nonzero threshold go to the queue
+    uint64_t und = eq & level_row(kEagerPlanes);
     const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
     st32(Wr, p, x ^ flip);
     nacc += (uint32_t)__builtin_popcountll(flip);
