@@ -309,13 +309,26 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     for (int b = 0; b < kSW; ++b) sacc[b] = 0;
     uint32_t nacc = 0;
 
-    for (uint32_t it = 0; it < per_lane; ++it) {
-        const uint32_t p = base + it * kWave + lane;
-        if (p >= c_end) break;
-        if constexpr (DEG > 0) {
-            regular_site<DEG>(S, Wr, p, ci, sgn, lds_q, slot, sweep, key0, key1, w0, sacc, nacc,
-                              tq, edef[wv]);
-        } else {
+    if constexpr (DEG > 0) {
+        // The next site's row loads while this one updates (the class's sites
+        // share no bonds, so no update writes a word another row reads).
+        uint32_t p = base + lane;
+        SiteRow<DEG> row{};
+        if (p < c_end) row = load_row<DEG>(S, p, ci, sgn);
+        for (uint32_t it = 0; it < per_lane && p < c_end; ++it) {
+            const uint32_t pn = p + kWave;
+            const bool more = it + 1 < per_lane && pn < c_end;
+            SiteRow<DEG> next = row;
+            if (more) next = load_row<DEG>(S, pn, ci, sgn);
+            regular_site<DEG>(S, Wr, p, row, lds_q, slot, sweep, key0, key1, w0, sacc, nacc, tq,
+                              edef[wv]);
+            row = next;
+            p = pn;
+        }
+    } else {
+        for (uint32_t it = 0; it < per_lane; ++it) {
+            const uint32_t p = base + it * kWave + lane;
+            if (p >= c_end) break;
             irregular_site<B, W>(S, Wr, p, p - base, rp, ci, sgn, lds_q, slot, sweep, key0, key1,
                                  w0, kacc, dacc, nacc, queue[wv], &qcount[wv], kQueueCap,
                                  edef[wv]);

@@ -192,26 +192,46 @@ __device__ __forceinline__ void drain_ties(const TieRecords& tq, uint64_t* Wr,
 }
 
 // The update of position p on a regular lattice (degree DEG in {4, 6}, row p
-// at p * DEG, count planes B = 3): reads the word and its neighbours from S,
+// at p * DEG, count planes B = 3): reads the neighbours of row from S,
 // writes the new word to Wr, adds (k - DEG/2) * flip to the signed counter
 // sacc and the flip count to nacc, and records the tied replicas in tq (or,
 // when tq is full, resolves them in place).
+// What a regular-lattice update reads before its neighbour words: the word
+// at p, the coupling sign bits and the DEG neighbour positions.  A caller
+// loads the next site's row while it updates the current one, so the index
+// loads' latency is spent on the current site's arithmetic.
+template <int DEG>
+struct SiteRow {
+    uint64_t x;
+    uint32_t sg;
+    uint32_t j[DEG];
+};
+
+template <int DEG>
+__device__ __forceinline__ SiteRow<DEG> load_row(const uint64_t* S, uint32_t p,
+                                                 const uint32_t* ci, const uint32_t* sgn) {
+    // word and entry byte offsets fit 32 bits (multispin_shape: n < 2^29,
+    // nnz < 2^30)
+    SiteRow<DEG> r;
+    r.x = ld32(S, p);
+    r.sg = ld32(sgn, p);
+    const uint32_t e0 = p * (uint32_t)DEG;
+#pragma unroll
+    for (int e = 0; e < DEG; ++e) r.j[e] = ld32(ci, e0 + (uint32_t)e);
+    return r;
+}
+
 template <int DEG>
 __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, uint32_t p,
-                                             const uint32_t* ci, const uint32_t* sgn,
+                                             const SiteRow<DEG>& row,
                                              const unsigned long long* recs, uint32_t slot,
                                              uint32_t sweep, uint32_t key0, uint32_t key1,
                                              int32_t w0, uint64_t (&sacc)[kSW], uint32_t& nacc,
                                              const TieRecords& tq, unsigned long long* edef) {
     static_assert(DEG == 4 || DEG == 6, "regular lattices of degree 4 or 6");
-    // word and entry byte offsets fit 32 bits (multispin_shape: n < 2^29,
-    // nnz < 2^30)
-    const uint64_t x = ld32(S, p);
-    const uint32_t e0 = p * (uint32_t)DEG;
-    const uint32_t sg = ld32(sgn, p);
-    auto bond = [&](uint32_t e) {
-        return bop3s<kXor3>(x, ld32(S, ld32(ci, e0 + e)), bit_mask(sg, e));
-    };
+    const uint64_t x = row.x;
+    const uint32_t sg = row.sg;
+    auto bond = [&](uint32_t e) { return bop3s<kXor3>(x, ld32(S, row.j[e]), bit_mask(sg, e)); };
     // carry-save adder tree: c = popcount of the bond words, bit-sliced
     auto fa = [](uint64_t a, uint64_t b, uint64_t cin, uint64_t& sum, uint64_t& cout) {
         sum = bop3<kXor3>(a, b, cin);

@@ -77,8 +77,8 @@ __global__ __launch_bounds__(kResThreads) void resident_sweeps_kernel(
             for (uint32_t base = cb + wv * kWave; base < ce; base += kResThreads) {
                 const uint32_t p = base + lane;
                 if (p < ce)
-                    regular_site<DEG>(words, words, p, ci, sgn, recs, slot, sweep, key0, key1, w0,
-                                      sacc, nacc, tq, edef[wv]);
+                    regular_site<DEG>(words, words, p, load_row<DEG>(words, p, ci, sgn), recs,
+                                      slot, sweep, key0, key1, w0, sacc, nacc, tq, edef[wv]);
                 if (++pending == kFlushSites) {  // wave-uniform
                     e_lane += (int64_t)w0 * wave_column_sum_signed<kSW>(sacc, lane);
 #pragma unroll
@@ -180,8 +180,8 @@ __global__ __launch_bounds__(kClsThreads) void classes_sweep_kernel(
             const uint32_t p = base + lane;
             if (p < ce) {
                 if constexpr (DEG > 0)
-                    regular_site<DEG>(S, S, p, ci, sgn, recs, slot, sweep, key0, key1, w0, sacc,
-                                      nacc, tq, edef[wv]);
+                    regular_site<DEG>(S, S, p, load_row<DEG>(S, p, ci, sgn), recs, slot, sweep,
+                                      key0, key1, w0, sacc, nacc, tq, edef[wv]);
                 else
                     irregular_site<B, W>(S, S, p, p - cb, rp, ci, sgn, recs, slot, sweep, key0,
                                          key1, w0, kacc, dacc, nacc, queue[wv], &qcount[wv],
