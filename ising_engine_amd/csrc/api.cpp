@@ -559,6 +559,7 @@ int32_t ige_state_create(ige_ctx* ctx, const ige_model* model, uint32_t cells, u
                         : (model->ds.eligible && model->ds.dense) ? IGE_KERNEL_DENSE
                                                                   : IGE_KERNEL_GENERIC;
         check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (s->kernel == IGE_KERNEL_MULTISPIN) ige::prepare_sweep_multispin(model->view());
         check(hipEventCreate(&s->ev0), "hipEventCreate");
         check(hipEventCreate(&s->ev1), "hipEventCreate");
         const uint32_t n = model->ir.n;

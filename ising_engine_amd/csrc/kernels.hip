@@ -13,6 +13,8 @@
 //                    all 64 replicas with bitwise arithmetic (+-J models).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "ige_math.h"
 #include "kernels.hpp"
 #include "multispin_ops.h"
@@ -231,7 +233,9 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(
 // ---------------------------------------------------------------------------
 constexpr uint32_t kQueueCap = 1024; // deferred replicas per wave (any degree)
 constexpr uint32_t kRecCap = 128;    // deferred sites per wave (regular lattices)
-constexpr uint32_t kMaxPerLane = 8;  // sites per lane
+constexpr uint32_t kMaxPerLane = 8;  // sites per lane, any degree
+// Regular lattices: a lane's signed counter (kSW planes) holds 10 terms in [-3, 3].
+constexpr uint32_t kMaxPerLaneRegular = 10;
 
 // B: bit planes of the neighbour count (degree < 2^B).  W: width of the
 // energy counters.  DEG > 0: every site has exactly DEG neighbours and row p
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     uint32_t n, uint32_t dmax, int32_t w0, const uint32_t* __restrict__ lev_thr,
     const uint32_t* __restrict__ temp_of,
     int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
-    uint32_t per_lane, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
+    uint32_t per_lane, uint32_t span, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
     const uint32_t* __restrict__ clock) {
     static_assert(kMaxPerLane <= 8 && W >= B + 3, "energy counters too narrow");
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
@@ -286,11 +290,13 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t gx = gridDim.x;
     const uint32_t bx = (gx & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (gx >> 3) + (blockIdx.x >> 3);
     const uint32_t wave = uniform(bx * kWavesPerBlock + wv);
-    const uint32_t base = c_begin + wave * kWave * per_lane;
+    // Wave w takes positions [base, wave_end) of the class, span of them
+    // (span <= kWave * per_lane); lane l takes base + l + kWave * it.
+    const uint32_t base = c_begin + wave * span;
     if (base >= c_end) return;
+    const uint32_t wave_end = min(c_end, base + span);
     // waves of this block that have sites (the rest returned above)
     const uint32_t first_wave = bx * kWavesPerBlock;
-    const uint32_t span = kWave * per_lane;
     const uint32_t active = min(kWavesPerBlock, (c_end - c_begin + span - 1) / span - first_wave);
     const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
     uint64_t* Wr = spins_wr + (uint64_t)slot * n;
@@ -299,9 +305,9 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     // Bit-sliced per-replica sums over this lane's sites.  Any degree:
     // kacc = sum k * flip and dacc = sum d * flip, so dE = w0 (2 kacc - dacc) / 2.
     // Regular lattices (d = DEG even): sacc = sum (k - DEG/2) * flip in kSW-plane
-    // two's complement (each term in [-3, 3], at most kMaxPerLane of them), so
+    // two's complement (each term in [-3, 3], at most kMaxPerLaneRegular of them), so
     // dE = w0 sacc.
-    static_assert(3 * kMaxPerLane < (1u << (kSW - 1)), "signed counters too narrow");
+    static_assert(3 * kMaxPerLaneRegular < (1u << (kSW - 1)), "signed counters too narrow");
     uint64_t kacc[W], dacc[W], sacc[kSW];
 #pragma unroll
     for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
@@ -314,10 +320,10 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         // share no bonds, so no update writes a word another row reads).
         uint32_t p = base + lane;
         SiteRow<DEG> row{};
-        if (p < c_end) row = load_row<DEG>(S, p, ci, sgn);
-        for (uint32_t it = 0; it < per_lane && p < c_end; ++it) {
+        if (p < wave_end) row = load_row<DEG>(S, p, ci, sgn);
+        for (uint32_t it = 0; it < per_lane && p < wave_end; ++it) {
             const uint32_t pn = p + kWave;
-            const bool more = it + 1 < per_lane && pn < c_end;
+            const bool more = it + 1 < per_lane && pn < wave_end;
             SiteRow<DEG> next = row;
             if (more) next = load_row<DEG>(S, pn, ci, sgn);
             regular_site<DEG>(S, Wr, p, row, lds_q, slot, sweep, key0, key1, w0, sacc, nacc, tq,
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     } else {
         for (uint32_t it = 0; it < per_lane; ++it) {
             const uint32_t p = base + it * kWave + lane;
-            if (p >= c_end) break;
+            if (p >= wave_end) break;
             irregular_site<B, W>(S, Wr, p, p - base, rp, ci, sgn, lds_q, slot, sweep, key0, key1,
                                  w0, kacc, dacc, nacc, queue[wv], &qcount[wv], kQueueCap,
                                  edef[wv]);
@@ -557,44 +563,117 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
+// Workgroups of one sweep_multispin_kernel instantiation resident per CU at
+// once (occupancy query, once per process and dynamic LDS size) times the CU
+// count of the current device: the waves of one round.
+template <int B, int W, int DEG>
+static uint32_t resident_waves(uint32_t lds) {
+    static std::atomic<uint64_t> per_cu{0};  // (lds << 32) | blocks, 0 = not asked yet
+    uint64_t v = per_cu.load(std::memory_order_relaxed);
+    if (v == 0 || (uint32_t)(v >> 32) != lds) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void*>(sweep_multispin_kernel<B, W, DEG>), kBlock,
+                lds) != hipSuccess || nb < 1)
+            nb = 0;
+        v = ((uint64_t)lds << 32) | (uint32_t)nb;
+        per_cu.store(v, std::memory_order_relaxed);
+    }
+    constexpr int kDevices = 64;
+    static std::atomic<int> cus[kDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) return 0;
+    int ncu = cus[dev].load(std::memory_order_relaxed);
+    if (ncu == 0) {
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu < 1)
+            return 0;
+        cus[dev].store(ncu, std::memory_order_relaxed);
+    }
+    return (uint32_t)v * (uint32_t)ncu * kWavesPerBlock;
+}
+
+template <int B, int W, int DEG>
+static void launch_multispin_grid(const DevModel& m, const DevState& s, uint32_t c_begin,
                                   uint32_t c_end, uint32_t sweep, uint64_t seed,
                                   const uint32_t* lev_thr, hipStream_t stream) {
-    if (c_end <= c_begin) return hipSuccess;
     const uint32_t len = c_end - c_begin;
-    // Sites per lane: about 8192 waves (eight per SIMD; 8 sites per lane on
-    // 3D L=64 x 32 slots, which measured as fast as 4096 waves of 16 and lets
-    // the counters be one plane narrower) and at most kMaxPerLane so the
-    // bit-sliced energy counters stay narrow.
+    const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
+    // Sites per lane: about 8192 waves (eight per SIMD) and at most
+    // kMaxPerLane so the bit-sliced energy counters stay narrow; wave w takes
+    // kWave * per_lane consecutive positions.
     uint64_t per_lane = ((uint64_t)len * s.cells + (uint64_t)kWave * 8192 - 1) /
                         ((uint64_t)kWave * 8192);
     if (per_lane < 1) per_lane = 1;
     if (per_lane > kMaxPerLane) per_lane = kMaxPerLane;
-    const uint32_t waves = ceil_div(len, kWave * per_lane);
+    uint32_t span = kWave * (uint32_t)per_lane;
+    uint32_t waves = ceil_div(len, span);
+    if (DEG > 0) {
+        // A regular lattice whose waves would not all be resident at once
+        // takes one round instead: each wave a contiguous span of up to
+        // kWave * kMaxPerLaneRegular positions, a multiple of 8 workgroups per
+        // slot (the XCD eighths).  Every wave then starts together and none
+        // runs alone behind a partial first round; the sites of a class are
+        // independent, so the split does not change any result.
+        // It is taken when the grid above would lose the XCD eighths (a row
+        // of workgroups not a multiple of 8) or when it costs no more wave
+        // iterations (waves x sites per lane); on 3D L=64 x 32 slots the grid
+        // above packs every lane with 8 sites and measured faster (56.0 us)
+        // than one round of 10-site lanes (58.2 us), while 28 slots went from
+        // 56.2 to 50.0 us.
+        const uint32_t round = resident_waves<B, W, DEG>(lds);
+        constexpr uint32_t kUnit = 8 * kWavesPerBlock;
+        if (round > 0 && (uint64_t)waves * s.cells > round) {
+            const uint32_t per_slot = round / s.cells / kUnit * kUnit;
+            if (per_slot > 0) {
+                const uint32_t sp = ceil_div(len, per_slot);
+                const uint32_t pl = ceil_div(sp, kWave);
+                const uint32_t nw = ceil_div(len, sp);
+                const bool eighths = ceil_div(waves, kWavesPerBlock) % 8u == 0;
+                if (pl <= kMaxPerLaneRegular &&
+                    (!eighths || (uint64_t)nw * pl <= (uint64_t)waves * per_lane)) {
+                    span = sp;
+                    per_lane = pl;
+                    waves = nw;
+                }
+            }
+        }
+    }
     const dim3 grid(ceil_div(waves, kWavesPerBlock), s.cells);
-    const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
-#define IGE_MS_LAUNCH(B, W, DEG)                                                                   \
-    hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp, \
-                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, lev_thr, s.temp_of,      \
-                       s.energy,                                                                 \
-                       s.accepted, c_begin, c_end, (uint32_t)per_lane, sweep, key0, key1, s.clock)
+    hipLaunchKernelGGL((sweep_multispin_kernel<B, W, DEG>), grid, dim3(kBlock), lds, stream, m.rp,
+                       m.ci, m.sgn, s.spins, s.spins, m.n, m.dmax, m.w0, lev_thr, s.temp_of,
+                       s.energy, s.accepted, c_begin, c_end, (uint32_t)per_lane, span, sweep,
+                       key0, key1, s.clock);
+}
+
+void prepare_sweep_multispin(const DevModel& m) {
+    const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
+    if (m.regular_degree == 6)
+        (void)resident_waves<3, 6, 6>(lds);
+    else if (m.regular_degree == 4)
+        (void)resident_waves<3, 6, 4>(lds);
+}
+
+hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
+                                  uint32_t c_end, uint32_t sweep, uint64_t seed,
+                                  const uint32_t* lev_thr, hipStream_t stream) {
+    if (c_end <= c_begin) return hipSuccess;
     // Counter width W = B + 3: at most kMaxPerLane = 8 sites per lane, each
     // adding k * flip <= 2^B - 1, so every sum stays below 2^(B+3).
     if (m.regular_degree == 6) {
-        IGE_MS_LAUNCH(3, 6, 6);
+        launch_multispin_grid<3, 6, 6>(m, s, c_begin, c_end, sweep, seed, lev_thr, stream);
     } else if (m.regular_degree == 4) {
-        IGE_MS_LAUNCH(3, 6, 4);
+        launch_multispin_grid<3, 6, 4>(m, s, c_begin, c_end, sweep, seed, lev_thr, stream);
     } else if (m.dmax < 8) {
-        IGE_MS_LAUNCH(3, 6, 0);
+        launch_multispin_grid<3, 6, 0>(m, s, c_begin, c_end, sweep, seed, lev_thr, stream);
     } else if (m.dmax < 16) {
-        IGE_MS_LAUNCH(4, 7, 0);
+        launch_multispin_grid<4, 7, 0>(m, s, c_begin, c_end, sweep, seed, lev_thr, stream);
     } else if (m.dmax < 32) {
-        IGE_MS_LAUNCH(5, 8, 0);
+        launch_multispin_grid<5, 8, 0>(m, s, c_begin, c_end, sweep, seed, lev_thr, stream);
     } else {
         return hipErrorInvalidValue;
     }
-#undef IGE_MS_LAUNCH
     return hipGetLastError();
 }
 

@@ -88,6 +88,9 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
 hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t c_begin,
                                   uint32_t c_end, uint32_t sweep, uint64_t seed,
                                   const uint32_t* lev_thr, hipStream_t stream);
+// Asks (once) how many workgroups of the model's multispin kernel fit a CU,
+// which sizes its grids; called at state creation, outside any capture.
+void prepare_sweep_multispin(const DevModel& m);
 
 // Uniforms of the multispin kernel: kEagerPlanes leading bits as bit planes
 // (two per Philox call), the rest from a tail word drawn only on a tie.
