@@ -267,6 +267,24 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wv = uniform(threadIdx.x >> 6);
     const uint32_t slot = blockIdx.y;
+    // Workgroups go to the 8 XCDs round robin by dispatch order; when the row
+    // of blocks divides by 8, XCD k takes the k-th eighth of the class, so a
+    // block's neighbour words (the adjacent position ranges) sit in the same
+    // L2 as its own.
+    const uint32_t gx = gridDim.x;
+    const uint32_t bx = (gx & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (gx >> 3) + (blockIdx.x >> 3);
+    const uint32_t wave = uniform(bx * kWavesPerBlock + wv);
+    // Wave w takes positions [base, wave_end) of the class, span of them
+    // (span <= kWave * per_lane); lane l takes base + l + kWave * it.
+    const uint32_t base = c_begin + wave * span;
+    const uint32_t wave_end = min(c_end, base + span);
+    const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
+    // Regular lattices: the first site's row is in flight while the records
+    // are built.
+    SiteRow<(DEG > 0 ? DEG : 4)> row{};
+    if constexpr (DEG > 0) {
+        if (base + lane < wave_end) row = load_row<DEG>(S, base + lane, ci, sgn);
+    }
     {
         // The slot's threshold records, one per level m <= dmax, kQStride u64
         // each: bit r of plane b (b < kEagerPlanes) is bit 31 - b of thr_r, the
@@ -283,22 +301,10 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         if (threadIdx.x == 0) blk_done = blk_acc = 0;
         __syncthreads();
     }
-    // Workgroups go to the 8 XCDs round robin by dispatch order; when the row
-    // of blocks divides by 8, XCD k takes the k-th eighth of the class, so a
-    // block's neighbour words (the adjacent position ranges) sit in the same
-    // L2 as its own.
-    const uint32_t gx = gridDim.x;
-    const uint32_t bx = (gx & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (gx >> 3) + (blockIdx.x >> 3);
-    const uint32_t wave = uniform(bx * kWavesPerBlock + wv);
-    // Wave w takes positions [base, wave_end) of the class, span of them
-    // (span <= kWave * per_lane); lane l takes base + l + kWave * it.
-    const uint32_t base = c_begin + wave * span;
     if (base >= c_end) return;
-    const uint32_t wave_end = min(c_end, base + span);
     // waves of this block that have sites (the rest returned above)
     const uint32_t first_wave = bx * kWavesPerBlock;
     const uint32_t active = min(kWavesPerBlock, (c_end - c_begin + span - 1) / span - first_wave);
-    const uint64_t* __restrict__ S = spins_rd + (uint64_t)slot * n;
     uint64_t* Wr = spins_wr + (uint64_t)slot * n;
     const TieRecords tq{rec_und[wv], rec_l1[wv], rec_l2[wv], rec_pos[wv], &qcount[wv], kRecCap};
 
@@ -319,8 +325,6 @@ __global__ __launch_bounds__(kBlock, (DEG > 0 ? 5 : 1)) void sweep_multispin_ker
         // The next site's row loads while this one updates (the class's sites
         // share no bonds, so no update writes a word another row reads).
         uint32_t p = base + lane;
-        SiteRow<DEG> row{};
-        if (p < wave_end) row = load_row<DEG>(S, p, ci, sgn);
         for (uint32_t it = 0; it < per_lane && p < wave_end; ++it) {
             const uint32_t pn = p + kWave;
             const bool more = it + 1 < per_lane && pn < wave_end;
