@@ -386,7 +386,10 @@ __device__ __forceinline__ void irregular_site(
     // Level masks for k = floor(d/2) + 1 + j, k <= d; level m = 2k - d.
     uint64_t lev[LMAX];
     uint32_t mrow[LMAX];
-    uint64_t pos = 0;
+    // Most significant plane first (a least-significant-first comparison,
+    // as on regular lattices, needs more registers here and costs the
+    // degree < 32 kernel an occupancy step).
+    uint64_t pos = 0, und = 0;
 #pragma unroll
     for (int j = 0; j < LMAX; ++j) {
         const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
@@ -396,39 +399,25 @@ __device__ __forceinline__ void irregular_site(
         lev[j] = eq;
         mrow[j] = (k <= d ? 2 * k - d : 0) * kQStride;
         pos |= eq;
+        und = bop3<kAndOr>(eq, recs[mrow[j] + kEagerPlanes], und);
     }
-    // Row b of each replica's level record (zero outside pos).
-    auto level_row = [&](uint32_t b) {
-        uint64_t T = lev[0] & recs[mrow[0] + b];
+    // kEagerPlanes lockstep bit planes, two per Philox call.
+    uint64_t lt = 0;
 #pragma unroll
-        for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], recs[mrow[l] + b], T);
-        return T;
-    };
-    // kEagerPlanes lockstep bit planes, two per Philox call, compared least
-    // significant first: lt = u < thr on the leading bits (0 where they tie),
-    // eq = they tie.
-    uint64_t lt = 0, eq = 0;
+    for (uint32_t j = 0; j < kEagerPlanes / 2; ++j) {
+        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | j, key0, key1);
 #pragma unroll
-    for (int j = kEagerPlanes / 2 - 1; j >= 0; --j) {
-        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | (uint32_t)j, key0,
-                                      key1);
-#pragma unroll
-        for (int h = 1; h >= 0; --h) {
-            const uint32_t b = 2 * (uint32_t)j + (uint32_t)h;
+        for (uint32_t h = 0; h < 2; ++h) {
+            const uint32_t b = 2 * j + h;
             const uint64_t P = h == 0 ? ((uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32))
                                       : ((uint64_t)o.v[2] | ((uint64_t)o.v[3] << 32));
-            const uint64_t T = level_row(b);
-            if (b == kEagerPlanes - 1) {
-                lt = T & ~P;
-                eq = ~(T ^ P);
-            } else {
-                lt = bop3<kLtStep>(T, P, lt);
-                eq = bop3<kAndXnor>(eq, T, P);
-            }
+            uint64_t T = lev[0] & recs[mrow[0] + b];
+#pragma unroll
+            for (int l = 1; l < LMAX; ++l) T = bop3<kAndOr>(lev[l], recs[mrow[l] + b], T);
+            lt |= bop3<kAndAndNot>(und, T, P);
+            und = bop3<kAndXnor>(und, P, T);
         }
     }
-    // ties of replicas with dE > 0 and a nonzero threshold go to the queue
-    uint64_t und = eq & level_row(kEagerPlanes);
     const uint64_t flip = bop3<kNotOr>(pos, lt, 0ull);
     st32(Wr, p, x ^ flip);
     nacc += (uint32_t)__builtin_popcountll(flip);
