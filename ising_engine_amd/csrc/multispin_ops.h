@@ -191,11 +191,6 @@ __device__ __forceinline__ void drain_ties(const TieRecords& tq, uint64_t* Wr,
                                recs, slot, sweep, key0, key1, w0, edef, nacc);
 }
 
-// The update of position p on a regular lattice (degree DEG in {4, 6}, row p
-// at p * DEG, count planes B = 3): reads the neighbours of row from S,
-// writes the new word to Wr, adds (k - DEG/2) * flip to the signed counter
-// sacc and the flip count to nacc, and records the tied replicas in tq (or,
-// when tq is full, resolves them in place).
 // What a regular-lattice update reads before its neighbour words: the word
 // at p, the coupling sign bits and the DEG neighbour positions.  A caller
 // loads the next site's row while it updates the current one, so the index
@@ -221,6 +216,11 @@ __device__ __forceinline__ SiteRow<DEG> load_row(const uint64_t* S, uint32_t p,
     return r;
 }
 
+// The update of position p on a regular lattice (degree DEG in {4, 6}, row p
+// at p * DEG, count planes B = 3): reads the neighbours of row from S,
+// writes the new word to Wr, adds (k - DEG/2) * flip to the signed counter
+// sacc and the flip count to nacc, and records the tied replicas in tq (or,
+// when tq is full, resolves them in place).
 template <int DEG>
 __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, uint32_t p,
                                              const SiteRow<DEG>& row,
