@@ -37,6 +37,23 @@ def bytes_per_attempt(cells: int, deg: float, w_b: int = 1, w_l: int = 1) -> flo
     return (16.0 * cells + 4 + deg * (4 + w_b) + w_l) / (64.0 * cells)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(cells: int) -> int:
+    """Host threads for a CPU baseline: the reference parallelises over cells only, so at
+    most one per cell, and at most the 16 host threads one GPU's share of the box has."""
+    return max(1, min(cells, 16, os.cpu_count() or 1))
+
+
 def measured_counters(kernel: str, workload: str) -> dict:
     """The committed counter summary of `kernel` on `workload` (HBM bytes per launch from
     FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy; VALU
@@ -51,13 +68,21 @@ def measured_counters(kernel: str, workload: str) -> dict:
     return {}
 
 
-def counter_probe(_lib, instances, L: int, temps: int, sweeps: int, workload: str = "pt"):
-    """The program the counter passes run: `sweeps` PT steps of the workload (or, for
-    workload "sk", sweeps of the dense SK N=4096 anneal), then one 1 GiB device-to-device
-    copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters' units."""
+def counter_probe(_lib, instances, L: int, temps: int, sweeps: int, workload: str = "pt",
+                  waves: int = 8):
+    """The program the counter passes run: `sweeps` PT steps of the workload (for workload
+    "sk", sweeps of the dense SK N=4096 anneal; for "batch", sweeps [0, sweeps) of the
+    4096-instance QUBO batch in one persistent launch of `waves` waves per workgroup), then
+    one 1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the
+    counters' units."""
     import torch
     ctx = _lib.Context(0)
-    if workload == "sk":
+    if workload == "batch":
+        wl = instances.workload_qubo_batch()
+        st = _lib.Batch(ctx, wl["qubos"], wl["seeds"], wl["sweeps"], wl["beta_lo"],
+                        wl["beta_hi"])
+        st.run(_lib.BATCH_PERSISTENT, waves=waves, k0=0, k1=sweeps)
+    elif workload == "sk":
         wl = instances.workload_sk_anneal()
         qb = wl["qubo"]
         m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
@@ -103,7 +128,7 @@ def cpu_baseline(qb, cells, ladder, budget_s: float):
     spins = rng.integers(0, 2, size=(cells, n, 64), dtype=np.int8)
     energies = np.zeros((cells, 64), dtype=np.float64)
     phase = c_int(0)
-    threads = max(1, min(cells, 16, os.cpu_count() or 1))
+    threads = cpu_threads(cells)
     P = lambda a, t: a.ctypes.data_as(POINTER(t))
     args = lambda steps: (n, P(rp, c_uint32), P(ci, c_uint32), P(w, c_double), P(lin, c_double),
                           cells, P(temps, c_double), steps, 7, threads, P(spins, c_int8),
@@ -121,7 +146,7 @@ def cpu_baseline(qb, cells, ladder, budget_s: float):
     per_step = float(np.median(times))
     attempts = n * 64 * cells
     return dict(value=attempts / per_step, unit="spin-flip attempts/s", cores=threads,
-                kind="port",
+                kind="port", nproc=os.cpu_count(), cpu_model=cpu_model(),
                 sample=f"engine.rs::step port (i8 lanes, f64 dE, xoshiro256++x8, fast_exp, DEO), "
                        f"same 3D L={round(n ** (1 / 3))} {cells}-temperature workload, "
                        f"{3 * max(1, steps // 3)} steps, median of 3 repetitions")
@@ -423,9 +448,72 @@ def bench_gset(_lib, ctx, cpu_s: float) -> dict:
     return out
 
 
-def bench_batch(_lib, instances, ctx, count: int) -> dict:
+def cpu_batch_baseline(wl, budget_s: float) -> dict:
+    """The reference's production sweep (engine.rs::step port) on the batch workload with
+    instances spread across host threads, one instance per thread at a time (BASELINE's
+    harness choice for the batch: the reference parallelises over cells only, and a batch
+    instance is one cell).  A bounded sample: random instances of the batch, each run for a
+    fixed number of sweeps at its mid-anneal temperature, until about budget_s of wall time."""
+    import ctypes
+    import threading
+    from ctypes import POINTER, c_double, c_int, c_int8, c_uint32, c_uint64
+    from oracle import binding as ob
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "libcpu_baseline.so"))
+    lib.cb_run.argtypes = [c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double),
+                           POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, c_uint64,
+                           c_int, POINTER(c_int8), POINTER(c_double), POINTER(c_int)]
+    lib.cb_run.restype = c_int
+    P = lambda a, t: a.ctypes.data_as(POINTER(t))
+    threads = cpu_threads(10 ** 6)
+    sweeps = 100
+    rng = np.random.default_rng(29)
+    order = rng.permutation(len(wl["qubos"]))
+    lock = threading.Lock()
+    done = {"attempts": 0, "instances": 0, "next": 0}
+    t_end = time.perf_counter() + budget_s
+
+    def worker(seed):
+        while time.perf_counter() < t_end:
+            with lock:
+                if done["next"] >= len(order):
+                    return
+                i = int(order[done["next"]])
+                done["next"] += 1
+            qb = wl["qubos"][i]
+            csr = ob.CSR.from_qubo(qb)
+            w = csr.w[: csr.nnz].astype(np.float64)
+            lin = csr.linear.astype(np.float64)
+            temp = np.array([2.0 / (wl["beta_lo"][i] + wl["beta_hi"][i])])
+            spins = np.random.default_rng(seed + i).integers(0, 2, size=(1, qb.n, 64),
+                                                             dtype=np.int8)
+            e = np.zeros((1, 64), dtype=np.float64)
+            phase = c_int(0)
+            lib.cb_run(qb.n, P(csr.row_ptr, c_uint32), P(csr.col_idx, c_uint32), P(w, c_double),
+                       P(lin, c_double), 1, P(temp, c_double), sweeps, 13 + i, 1,
+                       P(spins, c_int8), P(e, c_double), ctypes.byref(phase))
+            with lock:
+                done["attempts"] += qb.n * 64 * sweeps
+                done["instances"] += 1
+
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=worker, args=(k,)) for k in range(threads)]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    dt = time.perf_counter() - t0
+    return dict(value=done["attempts"] / dt, unit="spin-flip attempts/s", cores=threads,
+                kind="port", nproc=os.cpu_count(), cpu_model=cpu_model(),
+                sample=f"engine.rs::step port, {done['instances']} random instances of the "
+                       f"batch x {sweeps} sweeps at the mid-anneal temperature, one instance "
+                       f"per thread at a time, {dt:.1f} s wall")
+
+
+def bench_batch(_lib, instances, ctx, count: int, cpu_s: float) -> dict:
     """Independent sparse QUBO instances (N log-uniform 64..8192, degree 8, int weights),
-    2000-sweep anneals: one persistent launch vs plain per-instance launches."""
+    2000-sweep anneals: one persistent launch vs plain per-instance launches.  After the
+    timed launch every instance's tracked energies must equal a recompute from its words."""
     wl = instances.workload_qubo_batch(count=count)
     b = _lib.Batch(ctx, wl["qubos"], wl["seeds"], wl["sweeps"], wl["beta_lo"], wl["beta_hi"])
     waves, probe_ms = b.autotune(probe_sweeps=10)
@@ -435,6 +523,21 @@ def bench_batch(_lib, instances, ctx, count: int) -> dict:
     b.synchronize()
     dt = time.perf_counter() - t0
     persistent = b.total_attempts / dt
+    launch_s = b.launch_ms() / 1e3
+    tracked, recomputed = b.energies()
+    if not np.array_equal(tracked, recomputed):
+        bad = int(np.sum(np.any(tracked != recomputed, axis=1)))
+        raise SystemExit(f"batch: tracked energies of {bad} instances differ from a recompute")
+    inst_ms = b.timing_ms()
+    ns = np.array([q.n for q in wl["qubos"]], dtype=np.int64)
+    nnz = np.array([2 * len(q.q) for q in wl["qubos"]], dtype=np.int64)
+    sweeps = wl["sweeps"][0]
+    # algorithmic bytes: per (instance, site, sweep) the word in and out (16 B), the row
+    # pointer (4 B), deg x (4 B column + 2 B int16 coupling), the int16 linear term (2 B)
+    alg_bytes = sweeps * (22 * int(ns.sum()) + 6 * int(nnz.sum()))
+    bpa = alg_bytes / b.total_attempts
+    achieved = alg_bytes / launch_s / 1e9
+    counters = measured_counters("persistent_anneal_kernel", wl["name"])
     # plain launches: the first 3 sweeps of every instance (the per-sweep cost of the plain
     # path does not depend on the sweep index), scaled to attempts/s
     b.reset()
@@ -445,12 +548,33 @@ def bench_batch(_lib, instances, ctx, count: int) -> dict:
     dp = time.perf_counter() - t0
     plain = b.total_attempts * 3 / wl["sweeps"][0] / dp
     log(f"[bench] batch {count}: persistent {persistent:.4e} attempts/s ({dt:.2f} s, waves={waves}),"
-        f" plain {plain:.4e} attempts/s -> {persistent / plain:.1f}x")
+        f" plain {plain:.4e} attempts/s -> {persistent / plain:.1f}x; energies == recompute "
+        f"for all {count}; launch {launch_s * 1e3:.1f} ms, largest instance "
+        f"{inst_ms[ns.argmax()]:.1f} ms, slowest {inst_ms.max():.1f} ms")
     b.close()
-    return {"workload": wl["name"], "instances": count, "sweeps": wl["sweeps"][0],
+    cpu = cpu_batch_baseline(wl, cpu_s) if cpu_s > 0 else None
+    if cpu:
+        log(f"[bench] batch cpu baseline {cpu['value']:.3e} attempts/s on {cpu['cores']} threads")
+    return {"workload": wl["name"], "instances": count, "sweeps": sweeps,
             "value": persistent, "unit": "spin-flip attempts/s", "seconds": dt,
             "plain_value": plain, "plain_sample": "first 3 sweeps of every instance",
-            "ratio_vs_plain": persistent / plain, "waves_per_instance": waves}
+            "ratio_vs_plain": persistent / plain, "waves_per_instance": waves,
+            "autotune_ms": probe_ms,
+            "energies_verified": f"tracked == device recompute for all {count} instances",
+            "critical_path": {"largest_instance_ms": float(inst_ms[ns.argmax()]),
+                              "slowest_instance_ms": float(inst_ms.max()),
+                              "median_instance_ms": float(np.median(inst_ms)),
+                              "sum_instance_ms": float(inst_ms.sum()),
+                              "launch_ms": launch_s * 1e3},
+            "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
+                             unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5),
+                             traffic=counters.get("traffic_bytes_per_launch"),
+                             valu_busy=counters.get("valu_busy"),
+                             algorithmic_bytes_per_launch=alg_bytes,
+                             kernel=f"persistent_anneal_kernel<{waves}>",
+                             avg_launch_us=round(launch_s * 1e6, 1),
+                             bytes_per_attempt=round(bpa, 4)),
+            "cpu_baseline": cpu}
 
 
 def bench_serving(_lib, instances, ctx, rate: float, count: int, track_every: int = 100,
@@ -521,7 +645,9 @@ def main():
     ap.add_argument("--counter-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS steps of --probe-workload and a 1 GiB copy "
                          "(for rocprofv3 --pmc passes)")
-    ap.add_argument("--probe-workload", choices=["pt", "sk"], default="pt")
+    ap.add_argument("--probe-workload", choices=["pt", "sk", "batch"], default="pt")
+    ap.add_argument("--probe-waves", type=int, default=8,
+                    help="waves per workgroup of the batch counter probe")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -531,7 +657,8 @@ def main():
     from ising_engine_amd import _lib, instances
 
     if args.counter_probe:
-        counter_probe(_lib, instances, args.L, args.temps, args.counter_probe, args.probe_workload)
+        counter_probe(_lib, instances, args.L, args.temps, args.counter_probe, args.probe_workload,
+                      args.probe_waves)
         return
     kernel = {"auto": _lib.KERNEL_AUTO, "generic": _lib.KERNEL_GENERIC,
               "multispin": _lib.KERNEL_MULTISPIN}[args.kernel]
@@ -592,7 +719,8 @@ def main():
 
     batched = None
     if not args.no_batch:
-        batched = bench_batch(_lib, instances, ctx, args.batch_count)
+        batched = bench_batch(_lib, instances, ctx, args.batch_count,
+                              cpu_s=0.0 if args.no_cpu_baseline else args.cpu_seconds)
     workloads = None
     if not args.no_workloads:
         workloads = bench_workloads(_lib, instances, ctx, args.tts_runs, args.sk_sweeps,

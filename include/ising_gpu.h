@@ -15,6 +15,8 @@
  * Reference interfaces replaced (paths relative to the reference checkout):
  *   ige_model_from_pairs    ProblemIR::from_pairs      src/engine_v2/ir.rs:30-69
  *   ige_model_coloring      ProblemIR::greedy_coloring src/engine_v2/ir.rs:114-139
+ *   ige_model_order         MetropolisSweep::ensure_order
+ *                                                      src/engine_v2/operators/metropolis_sweep.rs:40-58
  *   ige_model_get_info      ProblemIR::is_integral + DecisionEngine backend choice
  *                                                      src/engine_v2/ir.rs:143-148,
  *                                                      src/engine_v2/decision.rs:115-124
@@ -143,6 +145,12 @@ int32_t ige_model_from_pairs(ige_ctx *ctx, uint32_t n, int64_t offset, const int
 int32_t ige_model_destroy(ige_model *model);
 int32_t ige_model_get_info(const ige_model *model, ige_model_info *info);
 int32_t ige_model_coloring(const ige_model *model, uint32_t *colors /* n */);
+/* The sweep's position order (MetropolisSweep::ensure_order,
+ * src/engine_v2/operators/metropolis_sweep.rs:40-58): order[p] = the site
+ * visited p-th (colour 0's sites ascending, then colour 1's, ...);
+ * class_start: num_colors + 1 class bounds in position space (or NULL). */
+int32_t ige_model_order(const ige_model *model, uint32_t *order /* n */,
+                        uint32_t *class_start /* num_colors+1 */);
 int32_t ige_model_csr(const ige_model *model, uint32_t *row_ptr /* n+1 */,
                       uint32_t *col_idx /* nnz */, int32_t *w /* nnz */);
 
@@ -338,6 +346,19 @@ int32_t ige_batch_synchronize(ige_batch *batch);
 int32_t ige_batch_info(const ige_batch *batch, uint64_t *total_sites, uint64_t *total_attempts);
 int32_t ige_batch_read(ige_batch *batch, uint32_t index, uint64_t *words /* n, site order */,
                        int64_t *energies /* 64 */, uint64_t *accepted);
+/* Every instance's 64 tracked energies (instance i at i*64) and, computed
+ * afresh on the device from its words and CSR, its exact energies
+ * (SpinState::audit's comparison, sparse_bitslice.rs:317-346, for a whole
+ * batch at once).  Either pointer may be NULL. */
+int32_t ige_batch_energies(ige_batch *batch, int64_t *tracked /* [count][64] */,
+                           int64_t *recomputed /* [count][64] */);
+/* launch_ms: duration of the last IGE_BATCH_PERSISTENT run, from HIP events
+ * recorded on the batch's stream around its launch.  ms: device time of each
+ * instance's sweeps in the last persistent run that advanced it (wall clock
+ * of its first workgroup, first sweep to last); the batch's critical path is
+ * the largest of them.  Either pointer may be NULL.  No reference counterpart
+ * (the reference times whole solves, src/bin/host_timing_calibration.rs). */
+int32_t ige_batch_timing(ige_batch *batch, double *launch_ms, double *ms /* [count] */);
 
 /* ---- serving: continuous batching of solve requests ---- */
 /* Request statuses. */

@@ -35,13 +35,13 @@ STEPS_UNLIMITED = 0xFFFFFFFF
 EXPORTED = (
     "ige_context_create", "ige_context_destroy", "ige_last_error", "ige_version",
     "ige_model_from_pairs", "ige_model_destroy", "ige_model_get_info",
-    "ige_model_coloring", "ige_model_csr", "ige_state_create", "ige_state_destroy",
+    "ige_model_coloring", "ige_model_order", "ige_model_csr", "ige_state_create", "ige_state_destroy",
     "ige_set_ladder", "ige_run", "ige_exchange", "ige_synchronize", "ige_read_spins",
     "ige_read_energies", "ige_read_temp_of", "ige_read_fields",
     "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
     "ige_profile_read", "ige_anneal_schedule", "ige_batch_create", "ige_batch_destroy",
     "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
-    "ige_batch_info", "ige_batch_read", "ige_server_create", "ige_server_destroy",
+    "ige_batch_info", "ige_batch_read", "ige_batch_energies", "ige_batch_timing", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
     "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
     "ige_read_best", "ige_model_polish", "ige_server_solution", "ige_problem_parse",
@@ -157,6 +157,7 @@ def load() -> ctypes.CDLL:
         "ige_model_destroy": (c_int32, [vp]),
         "ige_model_get_info": (c_int32, [vp, POINTER(ModelInfo)]),
         "ige_model_coloring": (c_int32, [vp, POINTER(c_uint32)]),
+        "ige_model_order": (c_int32, [vp, POINTER(c_uint32), POINTER(c_uint32)]),
         "ige_model_csr": (c_int32, [vp, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]),
         "ige_state_create": (c_int32, [vp, vp, c_uint32, c_uint64, POINTER(c_uint64), c_uint32,
                                        POINTER(vp)]),
@@ -182,6 +183,8 @@ def load() -> ctypes.CDLL:
         "ige_batch_info": (c_int32, [vp, POINTER(c_uint64), POINTER(c_uint64)]),
         "ige_batch_read": (c_int32, [vp, c_uint32, POINTER(c_uint64), POINTER(c_int64),
                                      POINTER(c_uint64)]),
+        "ige_batch_energies": (c_int32, [vp, POINTER(c_int64), POINTER(c_int64)]),
+        "ige_batch_timing": (c_int32, [vp, POINTER(c_double), POINTER(c_double)]),
         "ige_server_create": (c_int32, [vp, c_uint64, c_double, c_uint32, POINTER(vp)]),
         "ige_server_destroy": (c_int32, [vp]),
         "ige_server_submit": (c_int32, [vp, POINTER(InstanceDesc), POINTER(c_uint64)]),
@@ -283,6 +286,13 @@ class Model:
         out = np.empty(self.n, dtype=np.uint32)
         check(self._lib.ige_model_coloring(self.handle, _ptr(out, c_uint32)))
         return out
+
+    def order(self):
+        """(order, class_start): the sweep's colour-major position order."""
+        order = np.empty(self.n, dtype=np.uint32)
+        cs = np.empty(self.info.num_colors + 1, dtype=np.uint32)
+        check(self._lib.ige_model_order(self.handle, _ptr(order, c_uint32), _ptr(cs, c_uint32)))
+        return order, cs
 
     def csr(self):
         rp = np.empty(self.n + 1, dtype=np.uint32)
@@ -692,6 +702,26 @@ class Batch:
         check(self._lib.ige_batch_read(self.handle, i, _ptr(words, c_uint64), _ptr(e, c_int64),
                                        ctypes.byref(acc)))
         return words, e, acc.value
+
+    def energies(self):
+        """(tracked, recomputed): [count][64] tracked energies and a fresh device
+        recompute from the words."""
+        t = np.empty((self.count, 64), dtype=np.int64)
+        r = np.empty((self.count, 64), dtype=np.int64)
+        check(self._lib.ige_batch_energies(self.handle, _ptr(t, c_int64), _ptr(r, c_int64)))
+        return t, r
+
+    def timing_ms(self):
+        """Per-instance device time (ms) of the last persistent run that advanced it."""
+        ms = np.empty(self.count, dtype=np.float64)
+        check(self._lib.ige_batch_timing(self.handle, None, _ptr(ms, c_double)))
+        return ms
+
+    def launch_ms(self):
+        """Duration (ms) of the last persistent run, from HIP events on the batch's stream."""
+        v = c_double()
+        check(self._lib.ige_batch_timing(self.handle, ctypes.byref(v), None))
+        return v.value
 
     def close(self):
         if self.handle:

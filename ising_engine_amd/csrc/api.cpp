@@ -526,6 +526,16 @@ int32_t ige_model_coloring(const ige_model* m, uint32_t* colors) {
     });
 }
 
+int32_t ige_model_order(const ige_model* m, uint32_t* order, uint32_t* class_start) {
+    return guarded([&] {
+        require(m && order, "NULL argument");
+        std::memcpy(order, m->co.order.data(), m->co.order.size() * sizeof(uint32_t));
+        if (class_start)
+            std::memcpy(class_start, m->co.class_start.data(),
+                        m->co.class_start.size() * sizeof(uint32_t));
+    });
+}
+
 int32_t ige_model_csr(const ige_model* m, uint32_t* row_ptr, uint32_t* col_idx, int32_t* w) {
     return guarded([&] {
         require(m && row_ptr && (m->ir.nnz() == 0 || (col_idx && w)), "NULL argument");
@@ -717,8 +727,8 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         }
         if (sched) {
             // Leave the ladder at the schedule's last row.
-            const double* last = sched + static_cast<size_t>(sweeps - 1) * s->cells;
             if (sweeps > 0) {
+                const double* last = sched + static_cast<size_t>(sweeps - 1) * s->cells;
                 s->ladder.assign(last, last + s->cells);
                 s->beta.upload(s->ladder.data(), s->cells, s->stream);
                 s->ladder_set = true;

@@ -58,13 +58,14 @@ struct ige_batch {
     std::vector<uint32_t> gang;
     std::vector<uint32_t> tickets;
     int num_cus = 0;
+    int wall_khz = 0;  // device wall-clock rate (wall_clock64 ticks per ms)
     DevBuf<ige::InstDesc> d_desc;
     DevBuf<uint32_t> d_queue, d_rp, d_ci, d_cs, d_order, d_progress, d_zero_temp, d_thr,
         d_list, d_bars, d_err;
     DevBuf<const uint32_t*> d_orders;
     DevBuf<int32_t> d_wt, d_lin;
-    DevBuf<uint64_t> d_spins;
-    DevBuf<int64_t> d_energy;
+    DevBuf<uint64_t> d_spins, d_ticks;
+    DevBuf<int64_t> d_energy, d_recompute;
     DevBuf<unsigned long long> d_accepted;
     DevBuf<unsigned int> d_next;
     // The persistent anneal as one captured graph (progress fill, counter
@@ -72,6 +73,9 @@ struct ige_batch {
     // captured with; dropped when a new plan moves the tickets.
     hipGraphExec_t graph = nullptr;
     uint32_t graph_key[3] = {0, 0, 0};
+    // HIP events on the batch's stream around the last persistent run.
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
     void drop_graph() {
         if (!graph) return;
         (void)hipStreamSynchronize(stream);  // its launches have ended
@@ -84,6 +88,8 @@ struct ige_batch {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
     }
 };
 
@@ -94,7 +100,7 @@ void batch_reset(ige_batch* b) {
                        b->d_desc.p, b->d_list.p, b->d_orders.p);
     check(hipGetLastError(), "batch init");
     hipLaunchKernelGGL(ige::instance_energy_kernel, dim3(b->count), dim3(256), 0, b->stream,
-                       b->d_desc.p, b->d_list.p);
+                       b->d_desc.p, b->d_list.p, (int64_t*)nullptr);
     check(hipGetLastError(), "batch energy");
     check(hipStreamSynchronize(b->stream), "batch reset");
 }
@@ -126,7 +132,7 @@ void plan(ige_batch* b) {
         ige::InstDesc& D = b->desc[i];
         D.gang = b->gang[i];
         D.bar = b->d_bars.p + i;
-        D.ticks = nullptr;
+        D.ticks = b->d_ticks.p + i;
         D.thr_mode = ige::choose_thr_mode(L.dtab);
     }
     // largest work per workgroup first; a gang's tickets stay consecutive
@@ -215,7 +221,11 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         check(hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount,
                                     b->device),
               "device attribute");
+        check(hipDeviceGetAttribute(&b->wall_khz, hipDeviceAttributeWallClockRate, b->device),
+              "device attribute");
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+        check(hipEventCreate(&b->ev0), "event");
+        check(hipEventCreate(&b->ev1), "event");
         b->count = count;
         b->inst.reserve(count);
         uint64_t n_tot = 0, e_tot = 0, c_tot = 0;
@@ -242,6 +252,7 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         b->d_accepted.alloc(count);
         b->d_progress.alloc(count);
         b->d_bars.alloc(count);
+        b->d_ticks.alloc(count);
         b->d_err.alloc(1);
         b->d_thr.alloc(static_cast<size_t>(b->max_dtab) + 1);
         std::vector<uint32_t> rp, ci, cs, order;
@@ -303,6 +314,7 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
         b->d_zero_temp.upload(zt.data(), 64, b->stream);
         const uint32_t zero = 0;
         b->d_err.upload(&zero, 1, b->stream);
+        check(hipMemsetAsync(b->d_ticks.p, 0, sizeof(uint64_t) * count, b->stream), "ticks");
         check(hipStreamSynchronize(b->stream), "batch upload");
         plan(b.get());
         batch_reset(b.get());
@@ -338,6 +350,12 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
             require(w == 1 || w == 2 || w == 4 || w == 8 || w == 16,
                     "waves per instance must be 1, 2, 4, 8 or 16");
             const uint32_t key[3] = {w, k0, k1};
+            check(hipEventRecord(b->ev0, b->stream), "event");
+            b->timed = false;
+            struct EndEvent {  // records the closing event on every exit path
+                ige_batch* b;
+                ~EndEvent() { b->timed = hipEventRecord(b->ev1, b->stream) == hipSuccess; }
+            } end_event{b};
             if (!ige::abi::graphs_enabled()) {
                 set_progress(b, k0);
                 ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
@@ -441,6 +459,47 @@ int32_t ige_batch_info(const ige_batch* b, uint64_t* total_sites, uint64_t* tota
         require(b != nullptr, "NULL batch");
         if (total_sites) *total_sites = b->total_sites;
         if (total_attempts) *total_attempts = b->total_attempts;
+    });
+}
+
+int32_t ige_batch_energies(ige_batch* b, int64_t* tracked, int64_t* recomputed) {
+    return guarded([&] {
+        require(b != nullptr && (tracked || recomputed), "bad arguments");
+        check(hipSetDevice(b->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->stream), "batch energies");
+        check_error_flag(b);
+        const size_t cnt = static_cast<size_t>(b->count) * 64;
+        if (tracked) b->d_energy.download(tracked, cnt, b->stream);
+        if (recomputed) {
+            // Instance i's energies are the 64 entries from i * 64 in both
+            // arrays, and d_list is the identity, so out[list index] = out[i].
+            if (!b->d_recompute.p) b->d_recompute.alloc(cnt);
+            hipLaunchKernelGGL(ige::instance_energy_kernel, dim3(b->count), dim3(256), 0,
+                               b->stream, b->d_desc.p, b->d_list.p, b->d_recompute.p);
+            check(hipGetLastError(), "batch recompute");
+            b->d_recompute.download(recomputed, cnt, b->stream);
+        }
+        check(hipStreamSynchronize(b->stream), "batch energies");
+    });
+}
+
+int32_t ige_batch_timing(ige_batch* b, double* launch_ms, double* ms) {
+    return guarded([&] {
+        require(b != nullptr && (launch_ms || ms), "bad arguments");
+        check(hipSetDevice(b->device), "hipSetDevice");
+        check(hipStreamSynchronize(b->stream), "batch timing");
+        if (launch_ms) {
+            require(b->timed, "no persistent run timed yet");
+            float f = 0.f;
+            check(hipEventElapsedTime(&f, b->ev0, b->ev1), "event");
+            *launch_ms = f;
+        }
+        if (!ms) return;
+        std::vector<uint64_t> t(b->count);
+        b->d_ticks.download(t.data(), b->count, b->stream);
+        check(hipStreamSynchronize(b->stream), "batch timing");
+        const double per_ms = b->wall_khz > 0 ? static_cast<double>(b->wall_khz) : 1.0e5;
+        for (uint32_t i = 0; i < b->count; ++i) ms[i] = static_cast<double>(t[i]) / per_ms;
     });
 }
 

@@ -123,7 +123,11 @@ __global__ __launch_bounds__(kIcmBlock) void icm_kernel(
     }
     __syncthreads();
     for (;;) {
+        // Every thread reads this level's bounds before any wave appends to
+        // the queue: without the barrier a late wave could see a tail that
+        // already counts this level's appends and load slots not yet stored.
         const uint32_t head = s_head, end = s_tail;
+        __syncthreads();
         if (head == end) break;
         for (uint32_t q = head + tid; q < end; q += kIcmBlock) {
             const uint32_t i = load_l2(queue + q);

@@ -268,8 +268,11 @@ static __global__ __launch_bounds__(256) void instance_init_kernel(const InstDes
     }
 }
 
+// Exact energies of the listed instances from their words: into their tracked
+// energies, or (out != NULL) into out[list index][64], leaving them untouched.
 static __global__ __launch_bounds__(256) void instance_energy_kernel(const InstDesc* __restrict__ insts,
-                                                              const uint32_t* __restrict__ list) {
+                                                              const uint32_t* __restrict__ list,
+                                                              int64_t* out) {
     __shared__ int64_t part[4][64];
     const InstDesc I = insts[list[blockIdx.x]];
     const uint32_t lane = threadIdx.x & 63u;
@@ -282,8 +285,14 @@ static __global__ __launch_bounds__(256) void instance_energy_kernel(const InstD
     }
     part[wv][lane] = a;
     __syncthreads();
-    if (threadIdx.x < 64)
-        I.energy[lane] = I.offset + (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / 2;
+    if (threadIdx.x < 64) {
+        const int64_t e =
+            I.offset + (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / 2;
+        if (out)
+            out[(uint64_t)blockIdx.x * 64 + lane] = e;
+        else
+            I.energy[lane] = e;
+    }
 }
 
 }  // namespace ige

@@ -452,7 +452,7 @@ void serve_loop(ige_server* s) {
                 hipLaunchKernelGGL(ige::instance_init_kernel, dim3(list.size()), dim3(256), 0,
                                    s->stream, s->d_desc.p, s->d_list.p, s->d_orders.p);
                 hipLaunchKernelGGL(ige::instance_energy_kernel, dim3(list.size()), dim3(256), 0,
-                                   s->stream, s->d_desc.p, s->d_list.p);
+                                   s->stream, s->d_desc.p, s->d_list.p, (int64_t*)nullptr);
                 check(hipGetLastError(), "request init");
             }
             check(hipEventRecord(e0, s->stream), "event");

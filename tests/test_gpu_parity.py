@@ -77,6 +77,11 @@ def test_parity_with_oracle(ctx, case):
     np.testing.assert_array_equal(ci, csr.col_idx[: csr.nnz])
     np.testing.assert_array_equal(w.astype(np.int64), csr.w[: csr.nnz])
     np.testing.assert_array_equal(model.coloring(), csr.coloring())
+    # the sweep's position order equals the oracle's (pinned by test_oracle_order.py)
+    order, cs = model.order()
+    o_order, o_cs = csr.color_order()
+    np.testing.assert_array_equal(order, o_order)
+    np.testing.assert_array_equal(cs, o_cs)
     seed = 0x1234_5678_9ABC_DEF0 + len(name)
     st = _lib.State(model, cells, seed, kernel=kernel)
     ost = ob.OracleState(csr, cells, seed, view=view)
