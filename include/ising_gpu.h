@@ -236,6 +236,18 @@ typedef struct {
  * Synchronous. */
 int32_t ige_steepest_descent(ige_state *state, uint32_t max_steps, const uint8_t *clamped,
                              ige_descent_report *report);
+/* ige_steepest_descent without any allocation, for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  Same arguments, steps
+ * and results.  Its scratch (per-replica argmin keys, control words, flip
+ * counts, the clamp map and, when the model's dense matrix already exists,
+ * the matrix-core fields) lives in one fixed 16 MiB device workspace per GPU,
+ * held from library load until the process ends; a state needing more is
+ * IGE_ERR_INVALID.  Calls on one GPU share that workspace: calls from host
+ * threads of this process take turns, and each call has finished all its
+ * device work when it returns. */
+int32_t ige_steepest_descent_noalloc(ige_state *state, uint32_t max_steps, const uint8_t *clamped,
+                                     ige_descent_report *report);
 
 /* `rounds` ICM sweeps (Houdayer moves, icm.rs:163-185): for every temperature t
  * and r < 32 the replicas (t, r) and (t, r + 32) flip, both, the connected

@@ -43,7 +43,7 @@ EXPORTED = (
     "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
     "ige_batch_info", "ige_batch_read", "ige_batch_energies", "ige_batch_timing", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
-    "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_update_best",
+    "ige_batch_configure", "ige_server_configure", "ige_steepest_descent", "ige_steepest_descent_noalloc", "ige_update_best",
     "ige_read_best", "ige_model_polish", "ige_server_solution", "ige_problem_parse",
     "ige_problem_destroy", "ige_problem_get_info", "ige_problem_pairs", "ige_problem_energy",
     "ige_icm", "ige_set_populations", "ige_resample", "ige_pa_normalize",
@@ -198,6 +198,8 @@ def load() -> ctypes.CDLL:
                                        POINTER(c_double), POINTER(c_uint64)]),
         "ige_steepest_descent": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
                                            POINTER(DescentReport)]),
+        "ige_steepest_descent_noalloc": (c_int32, [vp, c_uint32, POINTER(ctypes.c_uint8),
+                                                   POINTER(DescentReport)]),
         "ige_update_best": (c_int32, [vp]),
         "ige_icm": (c_int32, [vp, c_uint32, POINTER(c_uint64)]),
         "ige_set_populations": (c_int32, [vp, c_uint32]),
@@ -232,6 +234,11 @@ def load() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def last_error() -> str:
+    """Message of this thread's last failed call."""
+    return load().ige_last_error().decode(errors="replace")
 
 
 def check(code: int) -> None:

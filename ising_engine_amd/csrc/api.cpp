@@ -22,6 +22,7 @@
 #include "descent_host.hpp"
 #include "ige_math.h"
 #include "kernels.hpp"
+#include "noalloc.hpp"
 #include "problem_ir.hpp"
 
 namespace {
@@ -55,6 +56,10 @@ struct ige_model {
     mutable DevBuf<int8_t> dense_a;
     mutable DevBuf<int32_t> dense_lin, dense_wdiag;
     mutable uint32_t n_pad = 0;
+    bool dense_ready() const {
+        std::lock_guard<std::mutex> lock(dense_mu);
+        return n_pad != 0;
+    }
     void ensure_dense() const {
         std::lock_guard<std::mutex> lock(dense_mu);
         if (n_pad) return;
@@ -427,9 +432,8 @@ DescentResult descend(const ige_model* model, const ige::DevState& st, const uin
         dv = model->dense_view();
         fields.alloc(static_cast<size_t>(st.cells) * n * 64);
     }
-    return ige::run_descent(model->view(), model->pos_of.p, model->co.order.data(),
-                            dense ? &dv : nullptr, fields.p, st, clamped, max_steps, flips,
-                            stream);
+    return ige::run_descent(model->view(), model->pos_of.p, dense ? &dv : nullptr, fields.p, st,
+                            clamped, max_steps, flips, stream);
 }
 
 // ICM sweeps (asynchronous).  The scratch, a visited bitmap and a queue per
@@ -473,6 +477,8 @@ int32_t ige_context_create(int32_t device, ige_ctx** out) {
         int count = 0;
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         require(device >= 0 && device < count, "device index out of range");
+        check(hipSetDevice(device), "hipSetDevice");
+        check(ige::noalloc_prepare(device), "no-alloc workspace address");
         auto* c = new ige_ctx;
         c->device = device;
         *out = c;
@@ -866,6 +872,57 @@ int32_t ige_steepest_descent(ige_state* s, uint32_t max_steps, const uint8_t* cl
             report->flips = r.flips;
             report->proposed = (static_cast<uint64_t>(r.steps) + r.converged) * s->model->ir.n *
                                64ull * s->cells;
+        }
+    });
+}
+
+int32_t ige_steepest_descent_noalloc(ige_state* s, uint32_t max_steps, const uint8_t* clamped,
+                                     ige_descent_report* report) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        set_device(s->ctx);
+        const int dev = s->ctx->device;
+        uint8_t* base = static_cast<uint8_t*>(ige::noalloc_region(dev));
+        require(base != nullptr, "the no-alloc workspace of this device is not set up");
+        const ige_model* model = s->model;
+        const uint64_t n = model->ir.n, cells = s->cells;
+        // dE from the matrix-core fields only when the model's dense matrix
+        // already exists (building it allocates); the CSR scan gives the same
+        // exact dE and so the same steps.
+        const bool dense = model->ds.eligible && model->ds.dense && model->dense_ready() &&
+                           cells * n * 64 * sizeof(int32_t) <= (1ull << 32);
+        auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+        const uint64_t o_keys = 0;
+        const uint64_t o_ctl = o_keys + up(cells * 64 * sizeof(unsigned long long));
+        const uint64_t o_flips = o_ctl + up(ige::kDescentCtlWords * sizeof(unsigned long long));
+        const uint64_t o_cl = o_flips + up(cells * 64 * sizeof(uint32_t));
+        const uint64_t o_aux = o_cl + (clamped ? up(n) : 0);
+        const uint64_t o_fields = o_aux + (clamped ? up(n * sizeof(uint32_t)) : 0);
+        const uint64_t need = o_fields + (dense ? up(cells * n * 64 * sizeof(int32_t)) : 0);
+        if (need > ige::kNoAllocScratchBytes)
+            throw Failure(IGE_ERR_INVALID,
+                          "state needs " + std::to_string(need) +
+                              " bytes of descent workspace; the no-alloc limit is " +
+                              std::to_string(ige::kNoAllocScratchBytes));
+        std::lock_guard<std::mutex> lock(ige::noalloc_mutex(dev));
+        ige::DescentWorkspace ws;
+        ws.keys = reinterpret_cast<unsigned long long*>(base + o_keys);
+        ws.ctl = reinterpret_cast<unsigned long long*>(base + o_ctl);
+        ws.clamped = clamped ? base + o_cl : nullptr;
+        ws.site_aux = clamped ? reinterpret_cast<uint32_t*>(base + o_aux) : nullptr;
+        uint32_t* flips = reinterpret_cast<uint32_t*>(base + o_flips);
+        check(hipMemsetAsync(flips, 0, cells * 64 * sizeof(uint32_t), s->stream), "clear flips");
+        ige::DevDense dv;
+        if (dense) dv = model->dense_view();
+        const DescentResult r = ige::run_descent_in(
+            model->view(), model->pos_of.p, dense ? &dv : nullptr,
+            dense ? reinterpret_cast<int32_t*>(base + o_fields) : nullptr, s->view(), clamped,
+            max_steps, flips, ws, s->stream);
+        if (report) {
+            report->steps = r.steps;
+            report->converged = r.converged;
+            report->flips = r.flips;
+            report->proposed = (static_cast<uint64_t>(r.steps) + r.converged) * n * 64ull * cells;
         }
     });
 }

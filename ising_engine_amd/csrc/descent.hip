@@ -168,6 +168,15 @@ __global__ void descent_finish_kernel(unsigned long long* ctl) {
     ctl[kDescentStepFlips] = 0;
 }
 
+// site_aux[p] = order[p] | (clamped[order[p]] != 0) << 31: the sparse scan's
+// position -> site map with the clamp flag.
+__global__ void descent_aux_kernel(const uint32_t* __restrict__ order,
+                                   const uint8_t* __restrict__ clamped, uint32_t n,
+                                   uint32_t* __restrict__ site_aux) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) site_aux[p] = order[p] | (clamped[order[p]] ? 0x80000000u : 0u);
+}
+
 __global__ void descent_reset_kernel(unsigned long long* keys, uint64_t count,
                                      unsigned long long* ctl) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,6 +254,13 @@ __global__ __launch_bounds__(kBlock) void track_copy_kernel(
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
 }  // namespace
+
+hipError_t launch_descent_aux(const uint32_t* order, const uint8_t* clamped, uint32_t n,
+                              uint32_t* site_aux, hipStream_t stream) {
+    hipLaunchKernelGGL(descent_aux_kernel, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, stream,
+                       order, clamped, n, site_aux);
+    return hipGetLastError();
+}
 
 hipError_t launch_descent_reset(unsigned long long* keys, uint32_t cells, unsigned long long* ctl,
                                 hipStream_t stream) {

@@ -169,6 +169,10 @@ enum : uint32_t {
 };
 hipError_t launch_descent_reset(unsigned long long* keys, uint32_t cells, unsigned long long* ctl,
                                 hipStream_t stream);
+// site_aux[p] = order[p] | (clamped[order[p]] != 0) << 31 for the n positions
+// (order: position -> site; clamped: n bytes in site order, on the device).
+hipError_t launch_descent_aux(const uint32_t* order, const uint8_t* clamped, uint32_t n,
+                              uint32_t* site_aux, hipStream_t stream);
 // One step.  dense_fields: NULL (dE from the CSR rows) or [cells][n][64]
 // fields of the current spins in site order (launch_dense_fields).
 // pos_of: n, site -> position.  clamped: NULL or n bytes in site order (read

@@ -298,9 +298,9 @@ void polish_state(ige_server* s, Request& r, std::vector<uint8_t>& x, int64_t& e
     st.spins = s->d_pol_words.p;
     st.energy = s->d_pol_energy.p;
     check(ige::launch_energy(m, st, st.energy, s->stream), "polish energy");
-    (void)ige::run_descent(
-        m, reinterpret_cast<const uint32_t*>(base + lay.pos_of), L.order.data(), nullptr, nullptr,
-        st, nullptr, IGE_STEPS_UNLIMITED, s->d_pol_flips.p, s->stream);
+    (void)ige::run_descent(m, reinterpret_cast<const uint32_t*>(base + lay.pos_of), nullptr,
+                           nullptr, st, nullptr, IGE_STEPS_UNLIMITED, s->d_pol_flips.p,
+                           s->stream);
     int64_t e0 = 0;
     uint32_t f0 = 0;
     s->d_pol_words.download(w.data(), L.n, s->stream);
