@@ -164,6 +164,56 @@ __device__ __forceinline__ void track_point(const InstDesc& I, uint32_t sweep, u
     }
 }
 
+// Sweeps [k0, k1) of one instance by one member of its gang, with the
+// thresholds of mode MODE (kThrLds: a per-sweep table in thr_lds; kThrBand:
+// threshold_band, no table).  Adds the member's energy changes to eacc (per
+// lane) and its accepted count to nacc.
+template <int WAVES, uint32_t MODE>
+__device__ __forceinline__ void anneal_sweeps(const InstDesc& I, uint32_t k0, uint32_t k1,
+                                              uint32_t member, uint32_t gang, uint32_t lane,
+                                              uint32_t wave, uint32_t* thr_lds,
+                                              int64_t (*part)[64], uint32_t& bar_target,
+                                              uint32_t spin_limit, uint32_t* err, int64_t& eacc,
+                                              uint32_t& nacc) {
+    for (uint32_t k = k0; k < k1; ++k) {
+        const double beta = anneal_beta(I.beta_hi, I.c, I.sweeps, k);
+        const uint32_t dlim = threshold_limit(beta, I.dtab);
+        if constexpr (MODE == kThrLds) {
+            for (uint32_t d = threadIdx.x; d <= dlim; d += WAVES * 64)
+                thr_lds[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
+            __syncthreads();
+        }
+        const double nbd = -beta * 1.4426950408889634;
+        auto accept = [&](uint32_t uu, int32_t de) -> bool {
+            if ((uint32_t)de > dlim) return false;
+            if constexpr (MODE == kThrLds)
+                return uu < thr_lds[de];
+            else
+                return band_accept(uu, de, nbd, beta);
+        };
+        const uint32_t stride = gang * WAVES;
+        const uint32_t lane_of_gang = member * WAVES + wave;
+        for (uint32_t c = 0; c < I.ncol; ++c) {
+            const uint32_t cb = __builtin_amdgcn_readfirstlane(I.cs[c]);
+            const uint32_t ce = __builtin_amdgcn_readfirstlane(I.cs[c + 1]);
+            if (ce > cb) {
+                for (uint32_t g = (cb >> 2) + lane_of_gang; g <= ((ce - 1) >> 2); g += stride) {
+                    const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
+                    generic_group(I.rp, I.ci, I.wt, I.lin, I.spins, I.spins, p0, p1, 0u, k,
+                                  I.key0, I.key1, lane, accept, eacc, nacc);
+                }
+            }
+            if (gang > 1)
+                gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+            else
+                __syncthreads();
+        }
+        if (I.track_every && ((k + 1) % I.track_every == 0 || k + 1 == I.sweeps))
+            track_point<WAVES>(I, k + 1, member, gang, lane, wave, eacc, part, bar_target,
+                               spin_limit, err);
+    }
+}
+
 // queue[t] = instance << 5 | member.  A gang's tickets are consecutive, so at
 // most one gang is incomplete at any time and it is completed by the next
 // workgroups to finish their work: with every workgroup of the launch
@@ -190,43 +240,17 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         uint32_t bar_target = 0;
         const uint32_t mode = I.thr_mode;
         const uint64_t t_start = wall_clock64();
-        const uint32_t k0 = *I.progress;
+        const uint32_t k0 = __builtin_amdgcn_readfirstlane(*I.progress);
         const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
         int64_t eacc = 0;
         uint32_t nacc = 0;
-        for (uint32_t k = k0; k < k1; ++k) {
-            const double beta = anneal_beta(I.beta_hi, I.c, I.sweeps, k);
-            const uint32_t dlim = threshold_limit(beta, I.dtab);
-            if (mode == kThrLds) {
-                for (uint32_t d = threadIdx.x; d <= dlim; d += WAVES * 64)
-                    thr_lds[d] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(beta, (int64_t)d);
-                __syncthreads();
-            }
-            const double nbd = -beta * 1.4426950408889634;
-            auto accept = [&](uint32_t uu, int32_t de) -> bool {
-                if ((uint32_t)de > dlim) return false;
-                return mode == kThrLds ? uu < thr_lds[de] : band_accept(uu, de, nbd, beta);
-            };
-            const uint32_t stride = gang * WAVES;
-            const uint32_t lane_of_gang = member * WAVES + wave;
-            for (uint32_t c = 0; c < I.ncol; ++c) {
-                const uint32_t cb = I.cs[c], ce = I.cs[c + 1];
-                if (ce > cb) {
-                    for (uint32_t g = (cb >> 2) + lane_of_gang; g <= ((ce - 1) >> 2); g += stride) {
-                        const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
-                        generic_group(I.rp, I.ci, I.wt, I.lin, I.spins, I.spins, p0, p1, 0u, k,
-                                      I.key0, I.key1, lane, accept, eacc, nacc);
-                    }
-                }
-                if (gang > 1)
-                    gang_barrier(I.bar, bar_target, gang, spin_limit, err);
-                else
-                    __syncthreads();
-            }
-            if (I.track_every && ((k + 1) % I.track_every == 0 || k + 1 == I.sweeps))
-                track_point<WAVES>(I, k + 1, member, gang, lane, wave, eacc, part, bar_target,
-                                   spin_limit, err);
-        }
+        // One code path per threshold mode, so neither carries the other's registers.
+        if (mode == kThrLds)
+            anneal_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds, part,
+                                          bar_target, spin_limit, err, eacc, nacc);
+        else
+            anneal_sweeps<WAVES, kThrBand>(I, k0, k1, member, gang, lane, wave, thr_lds, part,
+                                           bar_target, spin_limit, err, eacc, nacc);
         part[wave][lane] = eacc;
         // nacc is already the wave's total (a popcount of the accept ballot).
         if (lane == 0 && nacc) atomicAdd(I.accepted, (unsigned long long)nacc);

@@ -130,10 +130,12 @@ __device__ __forceinline__ void generic_group(
     uint32_t key1, uint32_t lane, Acc&& accept, int64_t& eacc, uint32_t& nacc) {
     const U32x4 u = philox4x32_10(p0 >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
     const uint32_t cnt = p1 - p0;  // 1..4
-    // row bounds of the group's positions (uniform)
+    // row bounds of the group's positions (wave-uniform: declared so, so that
+    // the entry loops below run on scalar counters)
     uint32_t rb[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) rb[q] = rp[p0 + min((uint32_t)q, cnt)];
+    for (int q = 0; q < 5; ++q)
+        rb[q] = __builtin_amdgcn_readfirstlane(rp[p0 + min((uint32_t)q, cnt)]);
     const uint64_t xw = lane < cnt ? S[p0 + lane] : 0ull;
     int32_t h[4];
 #pragma unroll
@@ -153,7 +155,12 @@ __device__ __forceinline__ void generic_group(
             for (uint32_t k = a; k < b; ++k) {
                 const uint64_t word = readlane64(nw, k - base);
                 const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
-                h[q] += lane_bit(word) ? w : 0;
+                int32_t hw = h[q] + w;
+                // An empty statement that keeps hw opaque: without it the select
+                // below is folded into h + (bit ? w : 0), which needs w copied
+                // into a VGPR first (one more VALU instruction per entry).
+                asm volatile("" : "+v"(hw));
+                h[q] = lane_bit(word) ? hw : h[q];
             }
         }
     }
