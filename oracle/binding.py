@@ -62,6 +62,13 @@ def load():
                            c_uint32, c_uint32, c_uint32, c_uint32, dp, c_uint32, c_uint32, u64p,
                            u32p, i64p, u64p]
     lib.or_run.restype = c_int
+    lib.or_run_temps.argtypes = [c_uint32, u32p, u32p, i64p, i64p, c_uint32, c_uint32, c_uint64,
+                                 c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, dp, c_uint32,
+                                 c_uint32, u64p, u32p, i64p, u64p]
+    lib.or_run_temps.restype = c_int
+    lib.or_exchange_replicas.argtypes = [c_uint32, c_uint64, c_uint32, c_uint32, dp, i64p, u32p,
+                                         u64p]
+    lib.or_exchange_replicas.restype = None
     lib.or_exchange.argtypes = [c_uint32, c_uint64, c_uint32, dp, i64p, u32p, u64p]
     lib.or_exchange_pops.argtypes = [c_uint32, c_uint32, c_uint64, c_uint32, dp, i64p, u32p,
                                      u64p]
@@ -84,6 +91,9 @@ def load():
     lib.or_icm_sweep.restype = c_uint32
     lib.or_update_best.argtypes = [c_uint32, c_uint32, u64p, u32p, i64p, i64p, i32p, u8p]
     lib.or_update_best.restype = c_int
+    lib.or_update_best_view.argtypes = [c_uint32, c_uint32, c_uint32, u64p, u32p, i64p, i64p,
+                                        i32p, u8p]
+    lib.or_update_best_view.restype = c_int
     _lib = lib
     return lib
 
@@ -221,14 +231,15 @@ class Incumbent:
         self.valid = ctypes.c_int32(0)
         self.x = np.zeros(n, dtype=np.uint8)
 
-    def update(self, words, temp_of, energies) -> bool:
+    def update(self, words, temp_of, energies, replica_view=False) -> bool:
         words = np.ascontiguousarray(words, dtype=np.uint64)
         temp_of = np.ascontiguousarray(temp_of, dtype=np.uint32)
         energies = np.ascontiguousarray(energies, dtype=np.int64)
-        return bool(load().or_update_best(self.n, words.shape[0], _p(words, c_uint64),
-                                          _p(temp_of, c_uint32), _p(energies, c_int64),
-                                          ctypes.byref(self.energy), ctypes.byref(self.valid),
-                                          _p(self.x, ctypes.c_uint8)))
+        return bool(load().or_update_best_view(self.n, words.shape[0], 1 if replica_view else 0,
+                                               _p(words, c_uint64), _p(temp_of, c_uint32),
+                                               _p(energies, c_int64), ctypes.byref(self.energy),
+                                               ctypes.byref(self.valid),
+                                               _p(self.x, ctypes.c_uint8)))
 
 
 def init_words(n: int, cells: int, seed: int) -> np.ndarray:
@@ -254,8 +265,44 @@ class OracleState:
         self.icm_round = 0
         self.pops = 1
         self.stats = np.zeros(2 + 2 * max(self.cells - 1, 0), dtype=np.uint64)
+        self.replica_betas = None  # the replica view's beta per replica index, once set
+
+    def set_replica_temperatures(self, temps):
+        """The replica view (metropolis_sweep.rs:115, replica_exchange.rs:78-85): replica
+        rho = slot * 64 + lane runs at T = temps[rho % len(temps)].  The first call makes
+        temp_of the identity over replica indices and resizes the pair statistics."""
+        temps = np.asarray(temps, dtype=np.float64)
+        R = self.cells * 64
+        if self.replica_betas is None:
+            self.temp_of = np.arange(R, dtype=np.uint32).reshape(self.cells, 64).copy()
+            self.stats = np.concatenate([self.stats[:2],
+                                         np.zeros(2 * (R - 1), dtype=np.uint64)])
+        self.replica_betas = np.ascontiguousarray(1.0 / temps[np.arange(R) % temps.size])
+
+    def exchange_replicas(self, rounds: int):
+        """ReplicaExchange::apply with budget `rounds`: round k of this call pairs from
+        start k & 1; the uniforms are keyed on the state's exchange counter."""
+        for k in range(rounds):
+            load().or_exchange_replicas(self.cells, self.seed, self.xround, k & 1,
+                                        _p(self.replica_betas, c_double),
+                                        _p(self.energy, c_int64), _p(self.temp_of, c_uint32),
+                                        _p(self.stats, c_uint64))
+            self.xround += 1
 
     def run(self, sweeps: int, exchange: bool, ladder=None, schedule=None):
+        if self.replica_betas is not None:
+            assert schedule is None and not exchange
+            n, rp, ci, w = self.csr._args()
+            R = self.cells * 64
+            rc = load().or_run_temps(n, rp, ci, w, _p(self.csr.linear, c_int64), self.cells, R,
+                                     self.seed, self.view, 0, self.sweep, self.xround, sweeps,
+                                     _p(self.replica_betas, c_double), 0, self.cells,
+                                     _p(self.words, c_uint64), _p(self.temp_of, c_uint32),
+                                     _p(self.energy, c_int64), _p(self.stats, c_uint64))
+            if rc != 0:
+                raise ValueError("or_run_temps rejected its arguments")
+            self.sweep += sweeps
+            return
         if schedule is not None:
             betas = np.ascontiguousarray(schedule, dtype=np.float64).reshape(sweeps, self.cells)
             per = 1

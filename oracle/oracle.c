@@ -57,6 +57,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 #define TAG_INIT 4u
 #define TAG_TAIL 5u
 #define TAG_ICM 6u
+#define TAG_REPX 8u
 /* Bit planes drawn for every replica before its tail word is consulted. */
 #define EAGER_PLANES 8u
 
@@ -390,14 +391,16 @@ void or_resample(uint32_t n, uint32_t cells, uint32_t pop_temps, const uint32_t 
 /* With view OR_HEATBATH the sweep is gibbs_color_sweep.rs:119-174 instead:    */
 /* the new spin is u < or_heatbath_threshold(beta, h), word-view uniforms.     */
 /* ------------------------------------------------------------------------- */
-int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
-           const int64_t *linear, uint32_t cells, uint64_t seed, uint32_t view, uint32_t exchange,
-           uint32_t sweep0, uint32_t xround0, uint32_t sweeps, const double *betas,
-           uint32_t betas_per_sweep, uint32_t pop_temps, uint64_t *words, uint32_t *temp_of,
-           int64_t *energies, uint64_t *stats) {
-    if (cells == 0 || view > OR_HEATBATH) return -1;
+int or_run_temps(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+                 const int64_t *linear, uint32_t cells, uint32_t temps, uint64_t seed,
+                 uint32_t view, uint32_t exchange, uint32_t sweep0, uint32_t xround0,
+                 uint32_t sweeps, const double *betas, uint32_t betas_per_sweep,
+                 uint32_t pop_temps, uint64_t *words, uint32_t *temp_of, int64_t *energies,
+                 uint64_t *stats) {
+    if (cells == 0 || view > OR_HEATBATH || (temps != cells && temps != cells * 64u)) return -1;
+    if (temps != cells && exchange) return -1; /* the replica view exchanges separately */
     for (uint32_t i = 0; i < cells * 64u; ++i)
-        if (temp_of[i] >= cells) return -1;
+        if (temp_of[i] >= temps) return -1;
     uint32_t *colors = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
     uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
     uint32_t *cstart = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 2));
@@ -407,7 +410,7 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
 
     for (uint32_t k = 0; k < sweeps; ++k) {
         uint32_t sweep = sweep0 + k;
-        const double *beta = betas + (betas_per_sweep ? (uint64_t)k * cells : 0);
+        const double *beta = betas + (betas_per_sweep ? (uint64_t)k * temps : 0);
         for (uint32_t c = 0; c < ncol; ++c) {
             for (uint32_t pos = cstart[c]; pos < cstart[c + 1]; ++pos) {
                 uint32_t site = order[pos];
@@ -446,6 +449,51 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const i
     free(order);
     free(cstart);
     return 0;
+}
+
+int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+           const int64_t *linear, uint32_t cells, uint64_t seed, uint32_t view, uint32_t exchange,
+           uint32_t sweep0, uint32_t xround0, uint32_t sweeps, const double *betas,
+           uint32_t betas_per_sweep, uint32_t pop_temps, uint64_t *words, uint32_t *temp_of,
+           int64_t *energies, uint64_t *stats) {
+    return or_run_temps(n, row_ptr, col_idx, w, linear, cells, cells, seed, view, exchange,
+                        sweep0, xround0, sweeps, betas, betas_per_sweep, pop_temps, words,
+                        temp_of, energies, stats);
+}
+
+/* replica_exchange.rs:61-109 in the replica view: R = cells * 64 replicas,
+ * replica rho at temperature 1 / beta[rho]; the configuration of replica rho
+ * sits at position q = slot * 64 + lane with temp_of[q] = rho.  One round with
+ * start `phase`: pairs (i, i + 1) for i = phase, phase + 2, ..., i + 1 < R,
+ * delta = (beta_i - beta_{i+1}) (E_i - E_{i+1}), accept iff delta >= 0 or
+ * u < exp(delta) with u = Philox(i >> 2, 0, xround, TAG_REPX << 24)[i & 3];
+ * an accepted pair swaps the two configurations' replica indices (the
+ * reference's swap_replicas, seen through the permutation).  The pairs of a
+ * round are disjoint, so their order does not matter.
+ * stats[2 + 2i] / [3 + 2i]: attempts / accepts of pair i. */
+void or_exchange_replicas(uint32_t cells, uint64_t seed, uint32_t xround, uint32_t phase,
+                          const double *beta, const int64_t *energies, uint32_t *temp_of,
+                          uint64_t *stats) {
+    const uint32_t R = cells * 64u;
+    uint32_t *loc = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)R);
+    for (uint32_t q = 0; q < R; ++q) loc[temp_of[q]] = q;
+    for (uint32_t i = phase & 1u; i + 1 < R; i += 2) {
+        const uint32_t qi = loc[i], qj = loc[i + 1];
+        const double delta = (beta[i] - beta[i + 1]) * (double)(energies[qi] - energies[qj]);
+        uint32_t o[4];
+        philox_seeded(seed, i >> 2, 0, xround, TAG_REPX << 24, o);
+        const uint32_t uu = o[i & 3u];
+        const int accept = (delta >= 0.0) || (uu < or_exchange_threshold(delta));
+        stats[2 + 2 * (uint64_t)i] += 1;
+        if (accept) {
+            stats[3 + 2 * (uint64_t)i] += 1;
+            temp_of[qi] = i + 1;
+            temp_of[qj] = i;
+            loc[i] = qj;
+            loc[i + 1] = qi;
+        }
+    }
+    free(loc);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -608,15 +656,18 @@ int64_t or_polish_1opt(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_
  * the incumbent when its energy is strictly lower (or no incumbent exists).
  * words are [cells][n] in site order; best_x receives n bytes.  Returns 1 if
  * the incumbent changed. */
-int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint32_t *temp_of,
-                   const int64_t *energies, int64_t *best_e, int32_t *valid, uint8_t *best_x) {
+int or_update_best_view(uint32_t n, uint32_t cells, uint32_t replica_view, const uint64_t *words,
+                        const uint32_t *temp_of, const int64_t *energies, int64_t *best_e,
+                        int32_t *valid, uint8_t *best_x) {
     uint64_t arg = UINT64_MAX;
     int64_t e_min = 0;
-    uint64_t key_min = UINT64_MAX; /* temperature * 64 + lane of the current minimum */
+    /* the replica index of the current minimum: temperature * 64 + lane, or
+     * in the replica view temp_of itself */
+    uint64_t key_min = UINT64_MAX;
     for (uint32_t s = 0; s < cells; ++s)
         for (uint32_t r = 0; r < 64; ++r) {
             const uint64_t q = (uint64_t)s * 64 + r;
-            const uint64_t key = (uint64_t)temp_of[q] * 64 + r;
+            const uint64_t key = replica_view ? (uint64_t)temp_of[q] : (uint64_t)temp_of[q] * 64 + r;
             if (arg == UINT64_MAX || energies[q] < e_min ||
                 (energies[q] == e_min && key < key_min)) {
                 arg = q;
@@ -630,6 +681,11 @@ int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint
     *best_e = e_min;
     *valid = 1;
     return 1;
+}
+
+int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint32_t *temp_of,
+                   const int64_t *energies, int64_t *best_e, int32_t *valid, uint8_t *best_x) {
+    return or_update_best_view(n, cells, 0, words, temp_of, energies, best_e, valid, best_x);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -20,6 +20,10 @@
  *   or_resample          src/solver/population_annealing.rs:229-286
  *   or_run (exchange)    src/solver/engine.rs:713-818 (DEO even/odd phase,
  *                        lane r of temperature t against lane r of t+1)
+ *   or_run_temps         or_run with one temperature per replica
+ *                        (metropolis_sweep.rs:115, T = temps[rep % len])
+ *   or_exchange_replicas src/engine_v2/operators/replica_exchange.rs:61-109
+ *                        (adjacent replica indices, start = round & 1)
  *   or_geometric_ladder  src/engine_v2/decision.rs:133-142
  *   or_suggested_range   src/solver/schedule.rs:21-43
  *   or_steepest_descent  src/engine_v2/operators/steepest_descent.rs:80-124
@@ -107,6 +111,20 @@ int or_run(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
            uint32_t sweeps, const double *betas, uint32_t betas_per_sweep,
            uint32_t pop_temps /* levels per population, 0 = cells */,
            uint64_t *words, uint32_t *temp_of, int64_t *energies, uint64_t *stats);
+/* or_run with `temps` betas per row: temps == cells (a temperature per slot
+ * index, as or_run), or temps == cells * 64 (the replica view: temp_of holds
+ * replica indices and replica rho runs at beta[rho]; exchange must be 0). */
+int or_run_temps(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const int64_t *w,
+                 const int64_t *linear, uint32_t cells, uint32_t temps, uint64_t seed,
+                 uint32_t view, uint32_t exchange, uint32_t sweep0, uint32_t xround0,
+                 uint32_t sweeps, const double *betas, uint32_t betas_per_sweep,
+                 uint32_t pop_temps, uint64_t *words, uint32_t *temp_of, int64_t *energies,
+                 uint64_t *stats);
+/* One ReplicaExchange round in the replica view (pairs from `phase`, uniforms
+ * keyed on xround); beta [cells*64]; stats [2 + 2*(cells*64-1)]. */
+void or_exchange_replicas(uint32_t cells, uint64_t seed, uint32_t xround, uint32_t phase,
+                          const double *beta, const int64_t *energies, uint32_t *temp_of,
+                          uint64_t *stats);
 
 /* Exchange number `round` (phase = round & 1; engine.rs:713-818) on tracked
  * energies; stats as or_run. */
@@ -147,6 +165,10 @@ uint32_t or_icm_sweep(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_i
 /* Incumbent update from tracked energies; words [cells][n] site order. */
 int or_update_best(uint32_t n, uint32_t cells, const uint64_t *words, const uint32_t *temp_of,
                    const int64_t *energies, int64_t *best_e, int32_t *valid, uint8_t *best_x);
+/* The same with ties broken by replica index in the replica view. */
+int or_update_best_view(uint32_t n, uint32_t cells, uint32_t replica_view, const uint64_t *words,
+                        const uint32_t *temp_of, const int64_t *energies, int64_t *best_e,
+                        int32_t *valid, uint8_t *best_x);
 
 #ifdef __cplusplus
 }
