@@ -15,6 +15,8 @@
  * Reference interfaces replaced (paths relative to the reference checkout):
  *   ige_model_from_pairs    ProblemIR::from_pairs      src/engine_v2/ir.rs:30-69
  *   ige_model_coloring      ProblemIR::greedy_coloring src/engine_v2/ir.rs:114-139
+ *   ige_model_csr,          ProblemIR row_ptr / col_idx / weights, linear, offset
+ *   ige_model_linear                                   src/engine_v2/ir.rs:14-24
  *   ige_model_order         MetropolisSweep::ensure_order
  *                                                      src/engine_v2/operators/metropolis_sweep.rs:40-58
  *   ige_model_get_info      ProblemIR::is_integral + DecisionEngine backend choice
@@ -28,8 +30,13 @@
  *                           engine::step (sweep + DEO) src/solver/engine.rs:573-819
  *                           with IGE_RUN_HEATBATH: GibbsColorSweep::apply
  *                                                      src/engine_v2/operators/gibbs_color_sweep.rs:119-174
- *   ige_exchange            ReplicaExchange::apply     src/engine_v2/operators/replica_exchange.rs:61-109
- *                           (DEO schedule of engine.rs:713-818)
+ *   ige_exchange            engine::step Phase 2 (DEO schedule)
+ *                                                      src/solver/engine.rs:713-818
+ *   ige_set_replica_temperatures
+ *                           RuntimeView.temperatures as MetropolisSweep reads them,
+ *                           T = temps[rep % temps.len()]
+ *                                                      src/engine_v2/operators/metropolis_sweep.rs:109-116
+ *   ige_exchange_replicas   ReplicaExchange::apply     src/engine_v2/operators/replica_exchange.rs:61-109
  *   ige_read_energies       SpinState::energies_into   src/engine_v2/backends/sparse_bitslice.rs:273-278
  *   ige_read_fields         SparseBitSlice fields ledger; on a dense state the fields
  *                           are recomputed on int8 matrix cores (the dense SK / QUBO
@@ -39,6 +46,7 @@
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:114-137,317-346
  *   ige_read_spins          SpinState::spin / extract_into
  *                                                      src/engine_v2/backends/sparse_bitslice.rs:280-289
+ *   ige_write_spins         SparseBitSlice::rebuild    src/engine_v2/backends/sparse_bitslice.rs:70-95
  *   ige_server_*            POST /api/v1/solve request loop
  *                                                      src/bin/server_api.rs:111-213;
  *                           ige_server_solution: the solution UltimateSolver::solve returns
@@ -153,6 +161,9 @@ int32_t ige_model_order(const ige_model *model, uint32_t *order /* n */,
                         uint32_t *class_start /* num_colors+1 */);
 int32_t ige_model_csr(const ige_model *model, uint32_t *row_ptr /* n+1 */,
                       uint32_t *col_idx /* nnz */, int32_t *w /* nnz */);
+/* ProblemIR's linear terms and offset (src/engine_v2/ir.rs:14-24); either
+ * pointer may be NULL. */
+int32_t ige_model_linear(const ige_model *model, int32_t *linear /* n */, int64_t *offset);
 
 /* ---- instance formats (host only, no device) ---- */
 typedef struct ige_problem ige_problem;
@@ -209,15 +220,44 @@ int32_t ige_run(ige_state *state, uint32_t sweeps, uint32_t flags, const double 
  * energies.  Exchange number k uses pairs (t, t+1) with t = k mod 2 and is
  * keyed on k, whether it runs here or after a sweep in ige_run. */
 int32_t ige_exchange(ige_state *state, uint32_t rounds);
+
+/* The replica view, the engine_v2 operators' own semantics.  Replica
+ * rho runs at T = temps[rho % len] (finite, > 0), as MetropolisSweep::apply
+ * reads RuntimeView.temperatures.  The first call on a state in the
+ * slot-ladder view switches it: the configuration at (temperature index t,
+ * lane) becomes replica t * 64 + lane (slot * 64 + lane on a state that never
+ * exchanged), and temp_of (ige_read_temp_of) then holds replica indices;
+ * ige_run sweeps on those temperatures (no
+ * schedule, IGE_RUN_EXCHANGE or IGE_RUN_ICM), ige_update_best breaks energy
+ * ties by replica index and ige_read_stats reports the cells*64 - 1 replica
+ * pairs.  Later calls change the temperatures only.  ige_set_ladder switches
+ * back (temperature indices = slot indices again, identity permutation);
+ * ICM, populations and resampling need the slot-ladder view. */
+int32_t ige_set_replica_temperatures(ige_state *state, const double *temps, uint32_t len);
+/* ReplicaExchange::apply with budget.sweeps = rounds, in the replica view:
+ * round k of this call pairs replicas (i, i + 1) for i = (k & 1), (k & 1) + 2,
+ * ...; delta = (beta_i - beta_{i+1}) (E_i - E_{i+1}); accept iff delta >= 0 or
+ * u < floor(exp(delta) 2^32), u = Philox4x32-10(i >> 2, 0, x, 8 << 24)[i & 3]
+ * with x the state's exchange counter (one per round, shared with
+ * ige_exchange); an accepted pair swaps the two configurations' replica
+ * indices.  Asynchronous. */
+int32_t ige_exchange_replicas(ige_state *state, uint32_t rounds);
 int32_t ige_synchronize(ige_state *state);
 
 int32_t ige_read_spins(ige_state *state, uint64_t *words /* [cells][n] */);
+/* Replaces every slot's words ([cells][n], site order) and recomputes the
+ * tracked energies exactly (SparseBitSlice::rebuild,
+ * src/engine_v2/backends/sparse_bitslice.rs:70-95); the temperature
+ * permutation, counters and ladder are kept.  Synchronous. */
+int32_t ige_write_spins(ige_state *state, const uint64_t *words);
 int32_t ige_read_energies(ige_state *state, int64_t *energies /* [cells][64] */);
 int32_t ige_read_temp_of(ige_state *state, uint32_t *temp_of /* [cells][64] */);
 int32_t ige_read_fields(ige_state *state, int32_t *fields /* [cells][n][64] */);
 int32_t ige_recompute_energies(ige_state *state, int64_t *energies /* [cells][64] */);
+/* pair_attempts / pair_accepts: one entry per adjacent pair of temperature
+ * indices, cells - 1 (cells*64 - 1 in the replica view) each, or NULL. */
 int32_t ige_read_stats(ige_state *state, ige_stats *stats, uint64_t *pair_attempts,
-                       uint64_t *pair_accepts /* cells-1 each, or NULL */);
+                       uint64_t *pair_accepts);
 
 /* ---- deterministic descent and the incumbent ---- */
 typedef struct {

@@ -35,8 +35,9 @@ STEPS_UNLIMITED = 0xFFFFFFFF
 EXPORTED = (
     "ige_context_create", "ige_context_destroy", "ige_last_error", "ige_version",
     "ige_model_from_pairs", "ige_model_destroy", "ige_model_get_info",
-    "ige_model_coloring", "ige_model_order", "ige_model_csr", "ige_state_create", "ige_state_destroy",
-    "ige_set_ladder", "ige_run", "ige_exchange", "ige_synchronize", "ige_read_spins",
+    "ige_model_coloring", "ige_model_order", "ige_model_csr", "ige_model_linear", "ige_state_create", "ige_state_destroy",
+    "ige_set_ladder", "ige_set_replica_temperatures", "ige_exchange_replicas", "ige_run",
+    "ige_exchange", "ige_synchronize", "ige_read_spins", "ige_write_spins",
     "ige_read_energies", "ige_read_temp_of", "ige_read_fields",
     "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
     "ige_profile_read", "ige_anneal_schedule", "ige_batch_create", "ige_batch_destroy",
@@ -158,15 +159,19 @@ def load() -> ctypes.CDLL:
         "ige_model_get_info": (c_int32, [vp, POINTER(ModelInfo)]),
         "ige_model_coloring": (c_int32, [vp, POINTER(c_uint32)]),
         "ige_model_order": (c_int32, [vp, POINTER(c_uint32), POINTER(c_uint32)]),
+        "ige_model_linear": (c_int32, [vp, POINTER(c_int32), POINTER(c_int64)]),
         "ige_model_csr": (c_int32, [vp, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]),
         "ige_state_create": (c_int32, [vp, vp, c_uint32, c_uint64, POINTER(c_uint64), c_uint32,
                                        POINTER(vp)]),
         "ige_state_destroy": (c_int32, [vp]),
         "ige_set_ladder": (c_int32, [vp, POINTER(c_double)]),
+        "ige_set_replica_temperatures": (c_int32, [vp, POINTER(c_double), c_uint32]),
+        "ige_exchange_replicas": (c_int32, [vp, c_uint32]),
         "ige_run": (c_int32, [vp, c_uint32, c_uint32, POINTER(c_double)]),
         "ige_exchange": (c_int32, [vp, c_uint32]),
         "ige_synchronize": (c_int32, [vp]),
         "ige_read_spins": (c_int32, [vp, POINTER(c_uint64)]),
+        "ige_write_spins": (c_int32, [vp, POINTER(c_uint64)]),
         "ige_read_energies": (c_int32, [vp, POINTER(c_int64)]),
         "ige_read_temp_of": (c_int32, [vp, POINTER(c_uint32)]),
         "ige_read_fields": (c_int32, [vp, POINTER(c_int32)]),
@@ -412,6 +417,17 @@ class State:
         if b.shape != (self.cells,):
             raise ValueError("ladder needs one beta per cell")
         check(self._lib.ige_set_ladder(self.handle, _ptr(b, c_double)))
+        self.replica_view = False
+
+    def set_replica_temperatures(self, temps) -> None:
+        """Replica rho = slot * 64 + lane runs at T = temps[rho % len(temps)]."""
+        t = np.ascontiguousarray(temps, dtype=np.float64).reshape(-1)
+        check(self._lib.ige_set_replica_temperatures(self.handle, _ptr(t, c_double), t.size))
+        self.replica_view = True
+
+    def exchange_replicas(self, rounds: int) -> None:
+        """ReplicaExchange::apply with a budget of `rounds`."""
+        check(self._lib.ige_exchange_replicas(self.handle, int(rounds)))
 
     def run(self, sweeps: int, exchange: bool = False, beta_schedule=None,
             track_best: bool = False, heatbath: bool = False, icm: bool = False,
@@ -475,6 +491,11 @@ class State:
         check(self._lib.ige_read_spins(self.handle, _ptr(out, c_uint64)))
         return out
 
+    def write_spins(self, words) -> None:
+        """Replace the words ([cells][n], site order); tracked energies are recomputed."""
+        w = np.ascontiguousarray(words, dtype=np.uint64).reshape(self.cells, self.n)
+        check(self._lib.ige_write_spins(self.handle, _ptr(w, c_uint64)))
+
     def energies(self) -> np.ndarray:
         out = np.empty((self.cells, 64), dtype=np.int64)
         check(self._lib.ige_read_energies(self.handle, _ptr(out, c_int64)))
@@ -497,11 +518,12 @@ class State:
 
     def stats(self):
         st = Stats()
-        pa = np.zeros(max(self.cells - 1, 1), dtype=np.uint64)
-        pc = np.zeros(max(self.cells - 1, 1), dtype=np.uint64)
+        pairs = (self.cells * 64 if getattr(self, "replica_view", False) else self.cells) - 1
+        pa = np.zeros(max(pairs, 1), dtype=np.uint64)
+        pc = np.zeros(max(pairs, 1), dtype=np.uint64)
         check(self._lib.ige_read_stats(self.handle, ctypes.byref(st), _ptr(pa, c_uint64),
                                        _ptr(pc, c_uint64)))
-        return st, pa[: self.cells - 1], pc[: self.cells - 1]
+        return st, pa[:pairs], pc[:pairs]
 
     def profile(self, on: bool) -> None:
         check(self._lib.ige_profile_enable(self.handle, 1 if on else 0))

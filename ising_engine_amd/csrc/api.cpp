@@ -134,6 +134,10 @@ struct ige_state {
     uint32_t pops = 1;       // populations (each a ladder of cells / pops levels)
     uint32_t dtab = 0;
     bool ladder_set = false;
+    // Replica view (ige_set_replica_temperatures): one temperature per replica
+    // index instead of one per slot index.
+    bool replica_view = false;
+    uint32_t temps() const { return replica_view ? cells * 64u : cells; }
     std::vector<double> ladder;
     DevBuf<uint64_t> spins;
     DevBuf<int64_t> energy, scratch_e;
@@ -166,6 +170,8 @@ struct ige_state {
     ige::DevState view() {
         ige::DevState s;
         s.cells = cells;
+        s.temps = temps();
+        s.replica_view = replica_view ? 1u : 0u;
         s.spins = spins.p;
         s.energy = energy.p;
         s.temp_of = temp_of.p;
@@ -254,12 +260,49 @@ void refresh_thresholds(ige_state* s, const double* dev_beta, const uint32_t* cl
     auto st = s->view();
     st.clock = clock;
     if (s->kernel == IGE_KERNEL_GENERIC)
-        check(ige::launch_thr_table(dev_beta, s->thr.p, s->cells, s->dtab, s->stream, clock),
+        check(ige::launch_thr_table(dev_beta, s->thr.p, s->temps(), s->dtab, s->stream, clock),
               "thr");
     else if (s->kernel == IGE_KERNEL_MULTISPIN)
         check(ige::launch_level_thresholds(m, st, dev_beta, s->lev_thr.p, s->stream),
               "level thresholds");
     // the dense kernel evaluates its thresholds from beta directly
+}
+
+// Switches between the slot-ladder view (one temperature per slot index) and
+// the replica view (one per replica index).  The temperature tables, the
+// ladder and the pair statistics are resized and the captured sweep loop
+// (which names those buffers) is dropped; the accepted count is kept.  Into
+// the replica view, the configuration at (temperature t, lane) becomes
+// replica t * 64 + lane; back into the slot-ladder view, the permutation
+// becomes the identity (a general permutation of replicas cannot be one of
+// lane-preserving temperature indices).
+void set_view(ige_state* s, bool replica) {
+    if (s->replica_view == replica) return;
+    check(hipStreamSynchronize(s->stream), "view switch");
+    if (s->graph) {
+        check(hipGraphExecDestroy(s->graph), "graph destroy");
+        s->graph = nullptr;
+    }
+    unsigned long long accepted = 0;
+    s->counters.download(&accepted, 1, s->stream);
+    check(hipStreamSynchronize(s->stream), "view switch");
+    s->replica_view = replica;
+    const size_t T = s->temps();
+    s->beta.alloc(T);
+    if (s->thr.p) s->thr.alloc(T * (s->dtab + 1));
+    if (s->lev_thr.p) s->lev_thr.alloc(T * (s->model->dmax + 1));
+    if (s->hb.p) s->hb.alloc(T * (2ull * static_cast<uint64_t>(s->model->max_local) + 1));
+    s->counters.alloc(1 + 2 * T);
+    std::vector<unsigned long long> zeros(s->counters.count, 0);
+    zeros[0] = accepted;
+    s->counters.upload(zeros.data(), zeros.size(), s->stream);
+    s->ladder.clear();
+    s->ladder_set = false;
+    if (replica)
+        check(ige::launch_replica_index_perm(s->view(), s->stream), "perm");
+    else
+        check(ige::launch_identity_perm(s->view(), s->stream), "perm");
+    check(hipStreamSynchronize(s->stream), "view switch");
 }
 
 void ensure_best(ige_state* s) {
@@ -303,7 +346,8 @@ void issue_step(ige_state* s, const StepPlan& P, const double* beta, uint32_t sw
     const auto& cs = s->model->co.class_start;
     if (refresh) {
         if (P.heat)
-            check(ige::launch_heatbath_table(beta, s->hb.p, s->cells, P.hb_dtab, s->stream, clock),
+            check(ige::launch_heatbath_table(beta, s->hb.p, s->temps(), P.hb_dtab, s->stream,
+                                             clock),
                   "heat-bath table");
         else
             refresh_thresholds(s, beta, clock);
@@ -369,7 +413,7 @@ hipGraphExec_t sweep_graph(ige_state* s, const StepPlan& P) {
     if (!s->clock.p) s->clock.alloc(ige::kClockWords);
     const uint64_t key[5] = {
         (P.exch ? 1u : 0u) | (P.track ? 2u : 0u) | (P.heat ? 4u : 0u) | (P.sched ? 8u : 0u) |
-            (static_cast<uint64_t>(s->pops) << 32),
+            (s->replica_view ? 16u : 0u) | (static_cast<uint64_t>(s->pops) << 32),
         reinterpret_cast<uint64_t>(P.sched ? s->sched_buf.p : nullptr),
         reinterpret_cast<uint64_t>(s->hb.p), reinterpret_cast<uint64_t>(s->best.p), 1};
     if (s->graph && std::equal(key, key + 5, s->graph_key)) return s->graph;
@@ -532,6 +576,15 @@ int32_t ige_model_coloring(const ige_model* m, uint32_t* colors) {
     });
 }
 
+int32_t ige_model_linear(const ige_model* m, int32_t* linear, int64_t* offset) {
+    return guarded([&] {
+        require(m && (linear || offset), "NULL argument");
+        if (linear)
+            for (uint32_t i = 0; i < m->ir.n; ++i) linear[i] = m->ir.linear[i];
+        if (offset) *offset = m->ir.offset;
+    });
+}
+
 int32_t ige_model_order(const ige_model* m, uint32_t* order, uint32_t* class_start) {
     return guarded([&] {
         require(m && order, "NULL argument");
@@ -626,6 +679,7 @@ int32_t ige_set_ladder(ige_state* s, const double* beta) {
         for (uint32_t t = 0; t < s->cells; ++t)
             require(std::isfinite(beta[t]) && beta[t] >= 0.0, "beta must be finite and >= 0");
         set_device(s->ctx);
+        set_view(s, false);
         s->ladder.assign(beta, beta + s->cells);
         s->beta.upload(s->ladder.data(), s->cells, s->stream);
         s->ladder_set = true;
@@ -646,6 +700,10 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         require(static_cast<uint64_t>(s->sweep) + sweeps <= 0xFFFFFFFFull &&
                     static_cast<uint64_t>(s->xround) + sweeps <= 0xFFFFFFFFull,
                 "sweep or exchange counter would exceed 2^32-1");
+        if (s->replica_view)
+            require(!(flags & (IGE_RUN_EXCHANGE | IGE_RUN_ICM)) && sched == nullptr,
+                    "a state in the replica view runs sweeps on its replica temperatures: no "
+                    "schedule, no IGE_RUN_EXCHANGE (use ige_exchange_replicas), no IGE_RUN_ICM");
         set_device(s->ctx);
         StepPlan P;
         P.exch = (flags & IGE_RUN_EXCHANGE) && s->cells >= 2;
@@ -657,7 +715,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         P.hb_dtab = static_cast<uint32_t>(s->model->max_local);
         if (P.track) ensure_best(s);
         if (P.heat && !s->hb.p)
-            s->hb.alloc(static_cast<size_t>(s->cells) * (2ull * P.hb_dtab + 1));
+            s->hb.alloc(static_cast<size_t>(s->temps()) * (2ull * P.hb_dtab + 1));
         if (sched) {
             const size_t total = static_cast<size_t>(sweeps) * s->cells;
             for (size_t i = 0; i < total; ++i)
@@ -708,7 +766,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             if (g) {
                 // Step 0's heat-bath table on a fixed ladder (the graph omits it).
                 if (P.heat && !sched)
-                    check(ige::launch_heatbath_table(s->beta.p, s->hb.p, s->cells, P.hb_dtab,
+                    check(ige::launch_heatbath_table(s->beta.p, s->hb.p, s->temps(), P.hb_dtab,
                                                      s->stream),
                           "heat-bath table");
                 check(ige::launch_clock_set(s->clock.p, s->sweep, s->xround, 0, s->stream),
@@ -746,9 +804,48 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
     });
 }
 
+int32_t ige_set_replica_temperatures(ige_state* s, const double* temps, uint32_t len) {
+    return guarded([&] {
+        require(s && temps, "NULL argument");
+        require(len >= 1, "need at least one temperature");
+        for (uint32_t k = 0; k < len; ++k)
+            require(std::isfinite(temps[k]) && temps[k] > 0.0,
+                    "temperatures must be finite and > 0");
+        require(s->pops == 1, "the replica view has one population");
+        set_device(s->ctx);
+        set_view(s, true);
+        const uint32_t R = s->temps();
+        s->ladder.resize(R);
+        for (uint32_t r = 0; r < R; ++r) s->ladder[r] = 1.0 / temps[r % len];
+        s->beta.upload(s->ladder.data(), R, s->stream);
+        s->ladder_set = true;
+        refresh_thresholds(s, s->beta.p);
+        check(hipStreamSynchronize(s->stream), "set replica temperatures");
+    });
+}
+
+int32_t ige_exchange_replicas(ige_state* s, uint32_t rounds) {
+    return guarded([&] {
+        require(s != nullptr, "NULL state");
+        require(s->replica_view, "not in the replica view: call ige_set_replica_temperatures");
+        require(static_cast<uint64_t>(s->xround) + rounds <= 0xFFFFFFFFull,
+                "exchange counter would exceed 2^32-1");
+        set_device(s->ctx);
+        const auto st = s->view();
+        for (uint32_t k = 0; k < rounds; ++k) {
+            check(ige::launch_exchange_replicas(st, s->beta.p, k & 1u, s->xround, s->seed,
+                                                s->stream),
+                  "exchange replicas");
+            s->xround += 1;
+        }
+    });
+}
+
 int32_t ige_exchange(ige_state* s, uint32_t rounds) {
     return guarded([&] {
         require(s != nullptr, "NULL state");
+        require(!s->replica_view, "a state in the replica view exchanges with "
+                                  "ige_exchange_replicas");
         require(s->ladder_set, "no ladder set");
         require(static_cast<uint64_t>(s->xround) + rounds <= 0xFFFFFFFFull,
                 "exchange counter would exceed 2^32-1");
@@ -782,6 +879,22 @@ int32_t ige_read_spins(ige_state* s, uint64_t* words) {
         for (uint32_t c = 0; c < s->cells; ++c)
             for (uint32_t p = 0; p < n; ++p)
                 words[static_cast<size_t>(c) * n + order[p]] = w[static_cast<size_t>(c) * n + p];
+    });
+}
+
+int32_t ige_write_spins(ige_state* s, const uint64_t* w) {
+    return guarded([&] {
+        require(s && w, "NULL argument");
+        set_device(s->ctx);
+        const uint32_t n = s->model->ir.n;
+        const auto& order = s->model->co.order;
+        std::vector<uint64_t> pw(static_cast<size_t>(s->cells) * n);
+        for (uint32_t c = 0; c < s->cells; ++c)
+            for (uint32_t p = 0; p < n; ++p)
+                pw[static_cast<size_t>(c) * n + p] = w[static_cast<size_t>(c) * n + order[p]];
+        s->spins.upload(pw.data(), pw.size(), s->stream);
+        recompute_energy_into_state(s);
+        check(hipStreamSynchronize(s->stream), "write spins");
     });
 }
 
@@ -849,7 +962,7 @@ int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
         stats->graph_sweeps = s->graph_sweeps;
         stats->graph_fallbacks = s->graph_fallbacks;
         stats->resident_sweeps = s->resident_sweeps;
-        for (uint32_t t = 0; t + 1 < s->cells; ++t) {
+        for (uint32_t t = 0; t + 1 < s->temps(); ++t) {
             if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];
             if (pair_accepts) pair_accepts[t] = c[2 + 2 * t];
         }
@@ -931,6 +1044,7 @@ int32_t ige_set_populations(ige_state* s, uint32_t pops) {
     return guarded([&] {
         require(s != nullptr, "NULL state");
         require(pops >= 1 && s->cells % pops == 0, "pops must divide the number of cells");
+        require(pops == 1 || !s->replica_view, "the replica view has one population");
         s->pops = pops;
     });
 }
@@ -938,6 +1052,7 @@ int32_t ige_set_populations(ige_state* s, uint32_t pops) {
 int32_t ige_resample(ige_state* s, const uint32_t* parents) {
     return guarded([&] {
         require(s && parents, "NULL argument");
+        require(!s->replica_view, "resampling works on the slot ladder, not the replica view");
         const uint32_t members = s->pops * 64;
         for (uint32_t m = 0; m < members; ++m)
             require(parents[m] < members, "parent index out of range");
@@ -968,6 +1083,8 @@ int32_t ige_icm(ige_state* s, uint32_t rounds, uint64_t* flipped) {
         require(s != nullptr, "NULL state");
         require(static_cast<uint64_t>(s->icm_round) + rounds <= 0xFFFFFFFFull,
                 "ICM counter would exceed 2^32-1");
+        require(!s->replica_view, "cluster moves pair lanes of one temperature: not in the "
+                                  "replica view");
         set_device(s->ctx);
         DevBuf<unsigned long long> count;
         if (flipped) {

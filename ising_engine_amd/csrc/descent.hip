@@ -192,8 +192,8 @@ constexpr uint32_t kTrackBlock = 1024;
 
 __global__ __launch_bounds__(kTrackBlock) void track_best_kernel(
     const int64_t* __restrict__ energy, const uint32_t* __restrict__ temp_of, uint32_t cells,
-    uint32_t sweep_rel, BestRecord* __restrict__ best, uint32_t* __restrict__ take,
-    const uint32_t* __restrict__ clock) {
+    uint32_t replica_view, uint32_t sweep_rel, BestRecord* __restrict__ best,
+    uint32_t* __restrict__ take, const uint32_t* __restrict__ clock) {
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     __shared__ int64_t se[kTrackBlock];
     __shared__ uint32_t sk[kTrackBlock], sq[kTrackBlock];
@@ -202,7 +202,8 @@ __global__ __launch_bounds__(kTrackBlock) void track_best_kernel(
     uint32_t k = ~0u, qq = ~0u;
     for (uint32_t q = tid; q < cells * kWave; q += kTrackBlock) {
         const int64_t eq = energy[q];
-        const uint32_t kq = temp_of[q] * kWave + (q & (kWave - 1));
+        // the replica index: (temperature, lane), or temp_of itself in the replica view
+        const uint32_t kq = replica_view ? temp_of[q] : temp_of[q] * kWave + (q & (kWave - 1));
         if (qq == ~0u || eq < e || (eq == e && kq < k)) {
             e = eq;
             k = kq;
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(kTrackBlock) void track_best_kernel(
         best->energy = se[0];
         best->valid = 1;
         best->sweep = sweep;
-        best->temp = sk[0] / kWave;
+        best->temp = replica_view ? sk[0] : sk[0] / kWave;
         best->lane = sq[0] % kWave;
     }
 }
@@ -305,7 +306,7 @@ hipError_t launch_track_best(const DevModel& m, const DevState& s, uint32_t swee
                              BestRecord* best, uint32_t* take, uint8_t* best_x,
                              hipStream_t stream) {
     hipLaunchKernelGGL(track_best_kernel, dim3(1), dim3(kTrackBlock), 0, stream, s.energy,
-                       s.temp_of, s.cells, sweep, best, take, s.clock);
+                       s.temp_of, s.cells, s.replica_view, sweep, best, take, s.clock);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(track_copy_kernel, dim3(ceil_div(m.n, kBlock)), dim3(kBlock), 0, stream,

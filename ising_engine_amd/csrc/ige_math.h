@@ -19,6 +19,7 @@ constexpr uint32_t kTagPlane = 2u;  // bit-plane sweep uniforms (multispin)
 constexpr uint32_t kTagExch = 3u;   // replica-exchange uniforms
 constexpr uint32_t kTagInit = 4u;   // default initial spins
 constexpr uint32_t kTagIcm = 6u;    // cluster-move seed sites
+constexpr uint32_t kTagRepx = 8u;   // replica-index exchange uniforms (replica view)
 
 struct U32x4 {
     uint32_t v[4];

@@ -76,15 +76,15 @@ __global__ __launch_bounds__(kBlock) void sweep_generic_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
     const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr, uint32_t n, uint32_t cells,
-    const uint32_t* __restrict__ temp_of, const uint32_t* __restrict__ thr, uint32_t dtab,
-    int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
-    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
-    const uint32_t* __restrict__ clock) {
+    uint32_t temps, const uint32_t* __restrict__ temp_of, const uint32_t* __restrict__ thr,
+    uint32_t dtab, int64_t* energy, unsigned long long* accepted, uint32_t c_begin,
+    uint32_t c_end, uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep_rel, uint32_t key0,
+    uint32_t key1, const uint32_t* __restrict__ clock) {
     extern __shared__ uint32_t lds_thr[];
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     if (kLdsThr) {
-        const uint32_t total = cells * (dtab + 1);
+        const uint32_t total = temps * (dtab + 1);
         for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_thr[i] = thr[i];
         __syncthreads();
     }
@@ -135,16 +135,16 @@ __global__ __launch_bounds__(kBlock) void sweep_heatbath_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const int32_t* __restrict__ wt, const int32_t* __restrict__ lin,
     const uint64_t* __restrict__ spins_rd, uint64_t* spins_wr, uint32_t n, uint32_t cells,
-    const uint32_t* __restrict__ temp_of, const uint64_t* __restrict__ hb, uint32_t dtab,
-    int64_t* energy, unsigned long long* accepted, uint32_t c_begin, uint32_t c_end,
-    uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep_rel, uint32_t key0, uint32_t key1,
-    const uint32_t* __restrict__ clock) {
+    uint32_t temps, const uint32_t* __restrict__ temp_of, const uint64_t* __restrict__ hb,
+    uint32_t dtab, int64_t* energy, unsigned long long* accepted, uint32_t c_begin,
+    uint32_t c_end, uint32_t chunk, uint32_t waves_per_slot, uint32_t sweep_rel, uint32_t key0,
+    uint32_t key1, const uint32_t* __restrict__ clock) {
     extern __shared__ uint64_t lds_hb[];
     const uint32_t sweep = clocked(clock, kClockSweep, sweep_rel);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t row = 2 * dtab + 1;
     if (kLdsThr) {
-        const uint32_t total = cells * row;
+        const uint32_t total = temps * row;
         for (uint32_t i = threadIdx.x; i < total; i += kBlock) lds_hb[i] = hb[i];
         __syncthreads();
     }
@@ -401,11 +401,50 @@ __global__ __launch_bounds__(64) void clock_kernel(uint32_t* clock, uint32_t a, 
     clock[kClockRow] = (add ? clock[kClockRow] : 0u) + c;
 }
 
-__global__ void identity_perm_kernel(uint32_t* temp_of, uint32_t* slot_of, uint32_t cells) {
+__global__ void identity_perm_kernel(uint32_t* temp_of, uint32_t* slot_of, uint32_t cells,
+                                     uint32_t replica_view) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cells * kWave) return;
-    temp_of[i] = i / kWave;
-    slot_of[i] = i / kWave;
+    temp_of[i] = replica_view ? i : i / kWave;
+    slot_of[i] = replica_view ? i : i / kWave;
+}
+
+// Slot-ladder view -> replica view: the configuration at (temperature t,
+// lane) becomes replica t * 64 + lane, so replica identities survive the
+// switch.  One thread per position; rho is a bijection, so no two threads
+// write the same entry.
+__global__ void replica_index_perm_kernel(uint32_t* temp_of, uint32_t* slot_of, uint32_t cells) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cells * kWave) return;
+    const uint32_t rho = temp_of[q] * kWave + (q & (kWave - 1));
+    temp_of[q] = rho;
+    slot_of[rho] = q;
+}
+
+// ReplicaExchange round in the replica view (replica_exchange.rs:61-109): one
+// thread per pair (i, i + 1), i = phase + 2 k.  The pairs of a round are
+// disjoint, so each thread owns its two replicas and its statistics.
+__global__ void exchange_replicas_kernel(const double* __restrict__ beta,
+                                         const int64_t* __restrict__ energy, uint32_t* temp_of,
+                                         uint32_t* slot_of, unsigned long long* pair_stats,
+                                         uint32_t replicas, uint32_t phase, uint32_t round,
+                                         uint32_t key0, uint32_t key1) {
+    const uint32_t i = phase + 2u * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i + 1 >= replicas) return;
+    const uint32_t qi = slot_of[i], qj = slot_of[i + 1];
+    const double delta = (beta[i] - beta[i + 1]) * (double)(energy[qi] - energy[qj]);
+    const U32x4 u = philox4x32_10(i >> 2, 0u, round, kTagRepx << 24, key0, key1);
+    const uint32_t w = i & 3u;
+    const uint32_t uu = w == 0 ? u.v[0] : w == 1 ? u.v[1] : w == 2 ? u.v[2] : u.v[3];
+    const bool acc = delta >= 0.0 || uu < exchange_threshold(delta);
+    pair_stats[2ull * i] += 1;
+    if (acc) {
+        pair_stats[2ull * i + 1] += 1;
+        slot_of[i] = qj;
+        slot_of[i + 1] = qi;
+        temp_of[qi] = i + 1;
+        temp_of[qj] = i;
+    }
 }
 
 __global__ void init_spins_kernel(const uint32_t* __restrict__ order, uint64_t* spins, uint32_t n,
@@ -518,17 +557,18 @@ hipError_t launch_sweep_generic(const DevModel& m, const DevState& s, uint32_t c
     const uint32_t waves_per_slot = ceil_div(len, chunk);
     const uint64_t waves = (uint64_t)waves_per_slot * s.cells;
     const dim3 grid(ceil_div(waves, kWavesPerBlock));
-    const uint64_t lds_bytes = (uint64_t)s.cells * (dtab + 1) * sizeof(uint32_t);
+    const uint32_t temps = s.temps ? s.temps : s.cells;
+    const uint64_t lds_bytes = (uint64_t)temps * (dtab + 1) * sizeof(uint32_t);
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
     if (lds_bytes <= 32768) {
         hipLaunchKernelGGL(sweep_generic_kernel<true>, grid, dim3(kBlock), (uint32_t)lds_bytes,
                            stream, m.rp, m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells,
-                           s.temp_of, thr, dtab, s.energy, s.accepted, c_begin, c_end,
+                           temps, s.temp_of, thr, dtab, s.energy, s.accepted, c_begin, c_end,
                            (uint32_t)chunk, waves_per_slot, sweep, key0, key1, s.clock);
     } else {
         hipLaunchKernelGGL(sweep_generic_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
-                           m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, thr,
-                           dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
+                           m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, temps, s.temp_of,
+                           thr, dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
                            waves_per_slot, sweep, key0, key1, s.clock);
     }
     return hipGetLastError();
@@ -551,17 +591,18 @@ hipError_t launch_sweep_heatbath(const DevModel& m, const DevState& s, uint32_t 
     const uint32_t waves_per_slot = ceil_div(len, chunk);
     const uint64_t waves = (uint64_t)waves_per_slot * s.cells;
     const dim3 grid(ceil_div(waves, kWavesPerBlock));
-    const uint64_t lds_bytes = (uint64_t)s.cells * (2ull * dtab + 1) * sizeof(uint64_t);
+    const uint32_t temps = s.temps ? s.temps : s.cells;
+    const uint64_t lds_bytes = (uint64_t)temps * (2ull * dtab + 1) * sizeof(uint64_t);
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
     if (lds_bytes <= 32768) {
         hipLaunchKernelGGL(sweep_heatbath_kernel<true>, grid, dim3(kBlock), (uint32_t)lds_bytes,
                            stream, m.rp, m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells,
-                           s.temp_of, hb, dtab, s.energy, s.accepted, c_begin, c_end,
+                           temps, s.temp_of, hb, dtab, s.energy, s.accepted, c_begin, c_end,
                            (uint32_t)chunk, waves_per_slot, sweep, key0, key1, s.clock);
     } else {
         hipLaunchKernelGGL(sweep_heatbath_kernel<false>, grid, dim3(kBlock), 0, stream, m.rp,
-                           m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, s.temp_of, hb,
-                           dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
+                           m.ci, m.wt, m.lin, s.spins, s.spins, m.n, s.cells, temps, s.temp_of,
+                           hb, dtab, s.energy, s.accepted, c_begin, c_end, (uint32_t)chunk,
                            waves_per_slot, sweep, key0, key1, s.clock);
     }
     return hipGetLastError();
@@ -683,9 +724,10 @@ hipError_t launch_sweep_multispin(const DevModel& m, const DevState& s, uint32_t
 
 hipError_t launch_level_thresholds(const DevModel& m, const DevState& s, const double* beta,
                                    uint32_t* lev_thr, hipStream_t stream) {
-    const uint32_t total = s.cells * (m.dmax + 1);
+    const uint32_t temps = s.temps ? s.temps : s.cells;
+    const uint32_t total = temps * (m.dmax + 1);
     hipLaunchKernelGGL(level_thr_kernel, dim3(ceil_div(total, kBlock)), dim3(kBlock), 0, stream,
-                       beta, lev_thr, s.cells, m.dmax, m.w0, s.clock);
+                       beta, lev_thr, temps, m.dmax, m.w0, s.clock);
     return hipGetLastError();
 }
 
@@ -707,7 +749,25 @@ hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round
 
 hipError_t launch_identity_perm(const DevState& s, hipStream_t stream) {
     hipLaunchKernelGGL(identity_perm_kernel, dim3(ceil_div((uint64_t)s.cells * kWave, kBlock)),
-                       dim3(kBlock), 0, stream, s.temp_of, s.slot_of, s.cells);
+                       dim3(kBlock), 0, stream, s.temp_of, s.slot_of, s.cells, s.replica_view);
+    return hipGetLastError();
+}
+
+hipError_t launch_replica_index_perm(const DevState& s, hipStream_t stream) {
+    hipLaunchKernelGGL(replica_index_perm_kernel,
+                       dim3(ceil_div((uint64_t)s.cells * kWave, kBlock)), dim3(kBlock), 0, stream,
+                       s.temp_of, s.slot_of, s.cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_exchange_replicas(const DevState& s, const double* beta, uint32_t phase,
+                                    uint32_t round, uint64_t seed, hipStream_t stream) {
+    const uint32_t replicas = s.cells * kWave;
+    const uint32_t pairs = (replicas - (phase & 1u)) / 2;
+    if (pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(exchange_replicas_kernel, dim3(ceil_div(pairs, kBlock)), dim3(kBlock), 0,
+                       stream, beta, s.energy, s.temp_of, s.slot_of, s.pair_stats, replicas,
+                       phase & 1u, round, (uint32_t)seed, (uint32_t)(seed >> 32));
     return hipGetLastError();
 }
 

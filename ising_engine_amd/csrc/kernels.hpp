@@ -44,10 +44,17 @@ enum : uint32_t { kClockSweep = 0, kClockRound = 1, kClockRow = 2, kClockWords =
 
 struct DevState {
     uint32_t cells = 0;
+    // Temperature rows (entries of beta and of the threshold tables): cells, or
+    // cells * 64 in the replica view, where temp_of holds replica indices
+    // rho = slot * 64 + lane (identity at the switch) and slot_of[rho] the
+    // position slot * 64 + lane of replica rho.
+    uint32_t temps = 0;
+    uint32_t replica_view = 0;
     uint64_t* spins = nullptr;      // [cells][n] MSC words, position order
     int64_t* energy = nullptr;      // [cells][64]
-    uint32_t* temp_of = nullptr;    // [cells][64] temperature index of (slot, lane)
-    uint32_t* slot_of = nullptr;    // [cells][64] slot of (temperature, lane)
+    uint32_t* temp_of = nullptr;    // [cells][64] temperature (replica) index of (slot, lane)
+    uint32_t* slot_of = nullptr;    // [cells][64] slot of (temperature, lane), or position of
+                                    // each replica in the replica view
     unsigned long long* accepted = nullptr;    // 1 counter
     unsigned long long* pair_stats = nullptr;  // [cells-1][2] attempts, accepts
     uint32_t pop_temps = 0;  // levels per population (0: one population of all cells)
@@ -137,6 +144,12 @@ hipError_t launch_level_thresholds(const DevModel& m, const DevState& s, const d
 // each population of s.pop_temps temperature indices (engine.rs:713-716).
 hipError_t launch_exchange(const DevState& s, const double* beta, uint32_t round, uint64_t seed,
                            hipStream_t stream);
+// One ReplicaExchange round in the replica view (replica_exchange.rs:61-109):
+// pairs of replica indices (i, i + 1) for i = phase, phase + 2, ... < cells*64 - 1,
+// uniforms keyed on `round`; beta [cells * 64].  Pair i's attempts and accepts
+// are added to pair_stats[2i], [2i + 1].
+hipError_t launch_exchange_replicas(const DevState& s, const double* beta, uint32_t phase,
+                                    uint32_t round, uint64_t seed, hipStream_t stream);
 
 // Dense sweep: all positions in order, one workgroup per slot, fields of each
 // block of 64 positions recomputed on int8 matrix cores.
@@ -155,6 +168,10 @@ hipError_t launch_energy(const DevModel& m, const DevState& s, int64_t* out, hip
 // Local fields, out[(slot*n + site)*64 + lane] (site order).
 hipError_t launch_fields(const DevModel& m, const DevState& s, int32_t* out, hipStream_t stream);
 hipError_t launch_identity_perm(const DevState& s, hipStream_t stream);
+// temp_of[q] = temp_of[q] * 64 + q % 64 and slot_of its inverse: the
+// slot-ladder permutation expressed as replica indices (the switch to the
+// replica view).
+hipError_t launch_replica_index_perm(const DevState& s, hipStream_t stream);
 
 // Steepest descent (every replica flips its most negative-dE site per step).
 // ctl: kDescentCtlWords counters, cleared by launch_descent_reset with the

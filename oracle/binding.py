@@ -274,7 +274,9 @@ class OracleState:
         temps = np.asarray(temps, dtype=np.float64)
         R = self.cells * 64
         if self.replica_betas is None:
-            self.temp_of = np.arange(R, dtype=np.uint32).reshape(self.cells, 64).copy()
+            # the configuration at (temperature t, lane) becomes replica t * 64 + lane
+            self.temp_of = (self.temp_of.astype(np.uint32) * 64
+                            + np.arange(64, dtype=np.uint32)[None, :]).astype(np.uint32)
             self.stats = np.concatenate([self.stats[:2],
                                          np.zeros(2 * (R - 1), dtype=np.uint64)])
         self.replica_betas = np.ascontiguousarray(1.0 / temps[np.arange(R) % temps.size])
