@@ -51,6 +51,8 @@
  *                                                      src/bin/server_api.rs:111-213;
  *                           ige_server_solution: the solution UltimateSolver::solve returns
  *                                                      src/solver/ultimate.rs:299-374
+ *   ige_ultimate_ladder     UltimateSolver's geometric ladder
+ *                                                      src/solver/ultimate.rs:237-246
  *   ige_batch_*             no single counterpart: many UltimateSolver-style anneals
  *                           (src/solver/ultimate.rs:234-336) in one launch
  *   ige_read_stats          Report{accepted,proposed} + StepScratch swap statistics
@@ -360,6 +362,18 @@ typedef struct {
      * the solution is polished to a 1-opt local minimum (ultimate.rs:374). */
     uint32_t track_every;
     uint32_t polish;
+    /* Serving only: num_temps >= 1 runs UltimateSolver's static-ladder loop
+     * (ultimate.rs:234-374) instead of the anneal: num_temps cells of 64
+     * replicas at the ladder of ige_ultimate_ladder(temp_max, temp_min,
+     * num_temps), a DEO exchange after every sweep, the incumbent updated
+     * after every track_every-th sweep (sweeps_per_exchange; it must divide
+     * sweeps = sweeps_per_exchange * exchanges), and the reference's solution:
+     * the incumbent when strictly below the best final replica, else that
+     * replica, then polish.  beta_lo and beta_hi are not used then.  0: the
+     * anneal (a batch takes only anneals). */
+    uint32_t num_temps;
+    double temp_max;
+    double temp_min;
 } ige_instance_desc;
 
 #define IGE_BATCH_PERSISTENT 0u /* one hipGraph launch (counter resets and
@@ -372,6 +386,12 @@ typedef struct {
  * instance follows exactly this schedule, so ige_run with it on a one-cell
  * state of the same seed reproduces the instance bit for bit. */
 int32_t ige_anneal_schedule(double beta_lo, double beta_hi, uint32_t sweeps, double *beta_out);
+/* UltimateSolver's temperature ladder (ultimate.rs:237-246):
+ * T_t = temp_max * f^t, f = (temp_min / temp_max)^(1 / (num_temps - 1))
+ * (f = 1 for one temperature), f^t by binary exponentiation as Rust's powi;
+ * beta_out[t] = 1 / T_t, hottest first. */
+int32_t ige_ultimate_ladder(double temp_max, double temp_min, uint32_t num_temps,
+                            double *beta_out);
 int32_t ige_batch_create(ige_ctx *ctx, uint32_t count, const ige_instance_desc *desc,
                          ige_batch **out);
 int32_t ige_batch_destroy(ige_batch *batch);
@@ -425,9 +445,9 @@ typedef struct {
     uint64_t id;
     int32_t status;        /* IGE_REQ_*                                     */
     uint32_t n;
-    int64_t best_energy;   /* lowest of the 64 replica energies at the end  */
-    uint32_t best_replica;
-    uint32_t reserved;
+    int64_t best_energy;   /* lowest of the final replica energies          */
+    uint32_t best_replica; /* its slot * 64 + lane                          */
+    uint32_t cells;        /* cells of 64 replicas (num_temps, 1 for an anneal) */
     uint64_t accepted;
     double latency_ms;     /* submit -> result ready                        */
     double queue_ms;       /* submit -> admitted to the device              */
@@ -455,17 +475,21 @@ int32_t ige_server_configure(ige_server *server, uint32_t max_gang, uint32_t gan
 /* Queues one request (validated like POST /api/v1/solve, server_api.rs:70-99);
  * thread-safe.  The schedule is that of ige_anneal_schedule. */
 int32_t ige_server_submit(ige_server *server, const ige_instance_desc *request, uint64_t *id);
-/* Status and, once done, the final spins of all 64 replicas (site order) and
- * their energies.  release != 0 forgets a finished request. */
+/* Status and, once done, the final words of every cell ([cells][n], site
+ * order; cells in info) and their energies ([cells][64]).  release != 0
+ * forgets a finished request. */
 int32_t ige_server_poll(ige_server *server, uint64_t id, int32_t release, ige_request_info *info,
-                        uint64_t *words /* n, or NULL */, int64_t *energies /* 64, or NULL */);
+                        uint64_t *words /* cells*n, or NULL */,
+                        int64_t *energies /* cells*64, or NULL */);
 int32_t ige_server_wait(ige_server *server, uint64_t id, double timeout_ms,
                         ige_request_info *info);
 /* The request's solution as UltimateSolver::solve returns it
- * (ultimate.rs:336-374): the best state seen at the incumbent updates
- * (track_every > 0) or else the lowest-energy final replica (lowest lane on
- * a tie), then 1-opt polished when the request asked for it.  x: n bytes in
- * site order. */
+ * (ultimate.rs:336-374): the lowest-energy final replica (ties to the lowest
+ * temperature index, then lane), replaced by the best state seen at the
+ * incumbent updates (track_every > 0) when that is strictly lower, then
+ * 1-opt polished when the request asked for it.  With num_temps >= 1 the
+ * request ran UltimateSolver's loop itself; with num_temps = 0 it ran a
+ * one-cell anneal.  x: n bytes in site order. */
 int32_t ige_server_solution(ige_server *server, uint64_t id, uint8_t *x, int64_t *energy);
 int32_t ige_server_stats(ige_server *server, ige_server_counters *stats);
 

@@ -40,7 +40,7 @@ EXPORTED = (
     "ige_exchange", "ige_synchronize", "ige_read_spins", "ige_write_spins",
     "ige_read_energies", "ige_read_temp_of", "ige_read_fields",
     "ige_recompute_energies", "ige_read_stats", "ige_profile_enable",
-    "ige_profile_read", "ige_anneal_schedule", "ige_batch_create", "ige_batch_destroy",
+    "ige_profile_read", "ige_anneal_schedule", "ige_ultimate_ladder", "ige_batch_create", "ige_batch_destroy",
     "ige_batch_reset", "ige_batch_run", "ige_batch_autotune", "ige_batch_synchronize",
     "ige_batch_info", "ige_batch_read", "ige_batch_energies", "ige_batch_timing", "ige_server_create", "ige_server_destroy",
     "ige_server_submit", "ige_server_poll", "ige_server_wait", "ige_server_stats",
@@ -91,14 +91,15 @@ class InstanceDesc(ctypes.Structure):
         ("npairs", c_uint64), ("pi", POINTER(c_uint32)), ("pj", POINTER(c_uint32)),
         ("q", POINTER(c_int32)), ("seed", c_uint64), ("sweeps", c_uint32),
         ("beta_lo", c_double), ("beta_hi", c_double), ("track_every", c_uint32),
-        ("polish", c_uint32),
+        ("polish", c_uint32), ("num_temps", c_uint32), ("temp_max", c_double),
+        ("temp_min", c_double),
     ]
 
 
 class RequestInfo(ctypes.Structure):
     _fields_ = [
         ("id", c_uint64), ("status", c_int32), ("n", c_uint32), ("best_energy", c_int64),
-        ("best_replica", c_uint32), ("reserved", c_uint32), ("accepted", c_uint64),
+        ("best_replica", c_uint32), ("cells", c_uint32), ("accepted", c_uint64),
         ("latency_ms", c_double), ("queue_ms", c_double), ("solution_energy", c_int64),
         ("solution_sweep", c_uint32), ("polish_flips", c_uint32),
     ]
@@ -179,6 +180,7 @@ def load() -> ctypes.CDLL:
         "ige_read_stats": (c_int32, [vp, POINTER(Stats), POINTER(c_uint64), POINTER(c_uint64)]),
         "ige_profile_enable": (c_int32, [vp, c_int32]),
         "ige_anneal_schedule": (c_int32, [c_double, c_double, c_uint32, POINTER(c_double)]),
+        "ige_ultimate_ladder": (c_int32, [c_double, c_double, c_uint32, POINTER(c_double)]),
         "ige_batch_create": (c_int32, [vp, c_uint32, POINTER(InstanceDesc), POINTER(vp)]),
         "ige_batch_destroy": (c_int32, [vp]),
         "ige_batch_reset": (c_int32, [vp]),
@@ -591,6 +593,14 @@ def pa_family_entropy(family):
     return rho.value, k.value
 
 
+def ultimate_ladder(temp_max: float, temp_min: float, num_temps: int) -> np.ndarray:
+    """UltimateSolver's betas (hottest first) for a PT solve request."""
+    out = np.empty(num_temps, dtype=np.float64)
+    check(load().ige_ultimate_ladder(float(temp_max), float(temp_min), int(num_temps),
+                                     _ptr(out, c_double)))
+    return out
+
+
 def anneal_schedule(beta_lo: float, beta_hi: float, sweeps: int) -> np.ndarray:
     """The deterministic geometric schedule batch instances follow."""
     out = np.empty(sweeps, dtype=np.float64)
@@ -633,23 +643,31 @@ class Server:
     def configure(self, max_gang=16, gang_sites=2048, spin_limit=1 << 24):
         check(self._lib.ige_server_configure(self.handle, max_gang, gang_sites, spin_limit))
 
-    def submit(self, qb, seed, sweeps, beta_lo, beta_hi, track_every: int = 0,
-               polish: bool = False) -> int:
+    def submit(self, qb, seed, sweeps, beta_lo=0.0, beta_hi=0.0, track_every: int = 0,
+               polish: bool = False, num_temps: int = 0, temp_max: float = 0.0,
+               temp_min: float = 0.0) -> int:
+        """An anneal (num_temps 0, beta_lo..beta_hi over `sweeps`) or UltimateSolver's
+        loop (num_temps cells on the ladder temp_max..temp_min, `sweeps` steps, the
+        incumbent every track_every steps)."""
         d = InstanceDesc()
         keep = []
         _fill_desc(d, qb, seed, sweeps, beta_lo, beta_hi, keep)
         d.track_every = int(track_every)
         d.polish = 1 if polish else 0
+        d.num_temps = int(num_temps)
+        d.temp_max = float(temp_max)
+        d.temp_min = float(temp_min)
         rid = c_uint64()
         check(self._lib.ige_server_submit(self.handle, ctypes.byref(d), ctypes.byref(rid)))
-        self.sizes[rid.value] = qb.n
+        self.sizes[rid.value] = (qb.n, max(1, int(num_temps)))
         return rid.value
 
     def poll(self, rid: int, release: bool = False, want_words: bool = True):
+        """(info, words [cells][n], energies [cells][64]) of a request."""
         info = RequestInfo()
-        n = self.sizes[rid]
-        words = np.empty(n, dtype=np.uint64) if want_words else None
-        e = np.empty(64, dtype=np.int64) if want_words else None
+        n, cells = self.sizes[rid]
+        words = np.empty((cells, n), dtype=np.uint64) if want_words else None
+        e = np.empty((cells, 64), dtype=np.int64) if want_words else None
         check(self._lib.ige_server_poll(self.handle, rid, 1 if release else 0, ctypes.byref(info),
                                         _ptr(words, c_uint64), _ptr(e, c_int64)))
         if release and info.status == REQ_DONE:
@@ -658,7 +676,7 @@ class Server:
 
     def solution(self, rid: int):
         """(x bytes in site order, energy) of a finished request."""
-        x = np.empty(self.sizes[rid], dtype=np.uint8)
+        x = np.empty(self.sizes[rid][0], dtype=np.uint8)
         e = c_int64()
         check(self._lib.ige_server_solution(self.handle, rid, _ptr(x, ctypes.c_uint8),
                                             ctypes.byref(e)))

@@ -211,10 +211,24 @@ int32_t ige_anneal_schedule(double beta_lo, double beta_hi, uint32_t sweeps, dou
     });
 }
 
+int32_t ige_ultimate_ladder(double temp_max, double temp_min, uint32_t num_temps, double* out) {
+    return guarded([&] {
+        require(out != nullptr && num_temps >= 1, "bad arguments");
+        require(std::isfinite(temp_max) && std::isfinite(temp_min) && temp_max > 0.0 &&
+                    temp_min > 0.0,
+                "temperatures must be finite and > 0");
+        const std::vector<double> b = ige::ultimate_ladder(temp_max, temp_min, num_temps);
+        std::copy(b.begin(), b.end(), out);
+    });
+}
+
 int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* desc,
                          ige_batch** out) {
     return guarded([&] {
         require(ctx && desc && out && count >= 1, "bad arguments");
+        for (uint32_t i = 0; i < count; ++i)
+            if (desc[i].num_temps != 0)
+                throw Failure(IGE_ERR_UNSUPPORTED, "a batch runs anneals (num_temps = 0)");
         auto b = std::make_unique<ige_batch>();
         b->device = ige::abi::context_device(ctx);
         check(hipSetDevice(b->device), "hipSetDevice");
@@ -286,6 +300,11 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
             D.beta_hi = L.beta_hi;
             D.c = L.c;
             D.offset = L.offset;
+            D.cells = 1;
+            D.pt = 0;
+            D.ladder = nullptr;
+            D.temp_of = nullptr;
+            D.slot_of = nullptr;
             for (uint32_t r : L.rp) rp.push_back(e_base + r);
             ci.insert(ci.end(), L.ci.begin(), L.ci.end());
             wt.insert(wt.end(), L.wt.begin(), L.wt.end());

@@ -32,10 +32,34 @@ using ige::abi::require;
 
 namespace ige {
 
+// UltimateSolver's ladder (ultimate.rs:237-246): T_t = temp_max * f^t with
+// f = (temp_min / temp_max)^(1 / (k - 1)); f^t by binary exponentiation, the
+// way Rust's powi lowers it.  beta[t] = 1 / T_t.
+inline double powi_binary(double a, int b) {
+    const bool recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+inline std::vector<double> ultimate_ladder(double temp_max, double temp_min, uint32_t k) {
+    const double f = k > 1 ? std::pow(temp_min / temp_max, 1.0 / static_cast<double>(k - 1)) : 1.0;
+    std::vector<double> beta(k);
+    for (uint32_t t = 0; t < k; ++t) beta[t] = 1.0 / (temp_max * powi_binary(f, static_cast<int>(t)));
+    return beta;
+}
+
 // Host-side description of one instance once lowered to position space.
 struct LoweredInstance {
     uint32_t n = 0, ncol = 0, sweeps = 0, dtab = 0;
     uint32_t track_every = 0, polish = 0;  // serving: incumbent cadence, 1-opt polish
+    uint32_t cells = 1, pt = 0;            // PT solve: cells on a fixed ladder
+    std::vector<double> ladder;            // its betas
     uint64_t seed = 0;
     double beta_hi = 0.0, c = 0.0;
     int64_t offset = 0;
@@ -46,7 +70,7 @@ struct LoweredInstance {
     std::vector<uint32_t> cs;   // ncol+1
     std::vector<uint32_t> order;  // position -> site
     std::vector<uint32_t> pos_of; // site -> position
-    uint64_t work() const { return (static_cast<uint64_t>(n) + ci.size()) * sweeps; }
+    uint64_t work() const { return (static_cast<uint64_t>(n) + ci.size()) * sweeps * cells; }
 };
 
 inline void check_schedule(double beta_lo, double beta_hi, const std::string& who) {
@@ -61,7 +85,16 @@ inline LoweredInstance lower_instance(const ige_instance_desc& d, const std::str
     require(d.n >= 1 && d.linear && (d.npairs == 0 || (d.pi && d.pj && d.q)),
             who + ": bad model arrays");
     require(d.sweeps >= 1, who + ": sweeps must be >= 1");
-    check_schedule(d.beta_lo, d.beta_hi, who);
+    if (d.num_temps == 0) {
+        check_schedule(d.beta_lo, d.beta_hi, who);
+    } else {
+        require(d.num_temps <= kMaxPtCells, who + ": num_temps must be in [1, 64]");
+        require(std::isfinite(d.temp_max) && std::isfinite(d.temp_min) && d.temp_max > 0.0 &&
+                    d.temp_min > 0.0,
+                who + ": temperatures must be finite and > 0");
+        require(d.track_every >= 1 && d.sweeps % d.track_every == 0,
+                who + ": track_every (sweeps per exchange round) must divide sweeps");
+    }
     ProblemIR ir = ProblemIR::from_pairs(d.n, d.offset, d.linear, d.npairs, d.pi, d.pj, d.q);
     const int64_t ml = ir.max_local_field();
     if (ml > 0x7FFFFFFFll) throw RangeError(who + ": max local field exceeds int32");
@@ -91,6 +124,13 @@ inline LoweredInstance lower_instance(const ige_instance_desc& d, const std::str
     L.pos_of = co.pos_of;
     L.track_every = d.track_every;
     L.polish = d.polish;
+    if (d.num_temps) {
+        L.pt = 1;
+        L.cells = d.num_temps;
+        L.ladder = ultimate_ladder(d.temp_max, d.temp_min, d.num_temps);
+        for (double b : L.ladder)
+            require(std::isfinite(b) && b > 0.0, who + ": ladder temperatures out of range");
+    }
     return L;
 }
 

@@ -1,8 +1,11 @@
-// persistent.hpp -- the persistent multi-instance anneal kernel shared by the
-// batch API and the serving loop.
+// persistent.hpp -- the persistent multi-instance kernel shared by the batch
+// API and the serving loop.
 //
-// Each instance is one cell of 64 replicas with its own CSR (position space),
-// seed and geometric beta schedule.  A workgroup pulls the next instance from
+// An instance is either an anneal (one cell of 64 replicas on a geometric beta
+// schedule) or a parallel-tempering solve (UltimateSolver's static-ladder
+// loop, src/solver/ultimate.rs:234-336: `cells` cells of 64 replicas on a fixed
+// ladder, a DEO exchange after every sweep, the incumbent updated every
+// track_every sweeps), each with its own CSR (position space) and seed.  A workgroup pulls the next instance from
 // a device work queue, advances it by up to `round` sweeps from its stored
 // progress, with workgroup barriers between colour classes, and pulls again:
 // no grid-wide synchronisation.  Per-sweep Metropolis thresholds are rebuilt
@@ -44,7 +47,17 @@ struct InstDesc {
     double beta_hi;
     double c;  // beta_k = beta_hi * det_exp(-(sweeps - 1 - k) * c)
     int64_t offset;
+    // Parallel tempering (pt != 0): spins [cells][n], energy [cells][64];
+    // lane r of temperature index t sits in slot slot_of[t * 64 + r], and
+    // temp_of is its inverse per lane, as in a single state.
+    uint32_t cells;         // 1 for an anneal
+    uint32_t pt;            // 1: fixed ladder, DEO exchange after every sweep
+    const double* ladder;   // [cells] betas, hottest first
+    uint32_t* temp_of;      // [cells][64]
+    uint32_t* slot_of;      // [cells][64]
 };
+
+constexpr uint32_t kMaxPtCells = 64;  // temperature cells per solve
 
 constexpr uint32_t kLdsThrEntries = 4096;
 constexpr uint32_t kMaxGang = 32;        // queue codes carry the member index in 5 bits
@@ -53,8 +66,8 @@ constexpr uint32_t kGangSites = 2048;    // sites per workgroup before a gang is
 constexpr uint32_t kThrLds = 0;   // dtab < kLdsThrEntries: per-sweep table in LDS
 constexpr uint32_t kThrBand = 1;  // larger ranges: threshold_band, no table
 
-__host__ __device__ __forceinline__ uint32_t choose_thr_mode(uint32_t dtab) {
-    return dtab < kLdsThrEntries ? kThrLds : kThrBand;
+__host__ __device__ __forceinline__ uint32_t choose_thr_mode(uint32_t dtab, uint32_t rows = 1) {
+    return (uint64_t)rows * ((uint64_t)dtab + 1) <= kLdsThrEntries ? kThrLds : kThrBand;
 }
 
 __host__ __device__ __forceinline__ double anneal_beta(double beta_hi, double c, uint32_t sweeps,
@@ -214,6 +227,184 @@ __device__ __forceinline__ void anneal_sweeps(const InstDesc& I, uint32_t k0, ui
     }
 }
 
+// Device-scope barrier of one workgroup (gang of one): every store and atomic
+// before it is visible to every wave after it, from L2.
+__device__ __forceinline__ void block_sync_device() {
+    __threadfence();
+    __syncthreads();
+    __threadfence();
+}
+
+__device__ __forceinline__ int64_t load_agent(const int64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The DEO exchange after sweep number `round` (engine.rs:713-818, the
+// exchange_kernel of a single state): pairs (t, t + 1) with t = round mod 2;
+// lane r of temperature t against lane r of t + 1; accept iff
+// delta = (beta_t - beta_{t+1}) (E_t - E_{t+1}) >= 0 or u < floor(e^delta 2^32),
+// u = Philox(t, r >> 2, round, TAG_EXCH)[r & 3]; accepted lanes swap
+// temperature indices.  One wave; the pairs are disjoint.
+__device__ __forceinline__ void pt_exchange(const InstDesc& I, uint32_t round, uint32_t lane) {
+    for (uint32_t t = round & 1u; t + 1 < I.cells; t += 2) {
+        const uint32_t s1 = I.slot_of[t * 64 + lane], s2 = I.slot_of[(t + 1) * 64 + lane];
+        const double delta =
+            (I.ladder[t] - I.ladder[t + 1]) *
+            (double)(load_agent(I.energy + s1 * 64 + lane) - load_agent(I.energy + s2 * 64 + lane));
+        const U32x4 u = philox4x32_10(t, lane >> 2, round, kTagExch << 24, I.key0, I.key1);
+        const uint32_t q = lane & 3u;
+        const uint32_t uu = q == 0 ? u.v[0] : q == 1 ? u.v[1] : q == 2 ? u.v[2] : u.v[3];
+        if (delta >= 0.0 || uu < exchange_threshold(delta)) {
+            I.slot_of[t * 64 + lane] = s2;
+            I.slot_of[(t + 1) * 64 + lane] = s1;
+            I.temp_of[s1 * 64 + lane] = t + 1;
+            I.temp_of[s2 * 64 + lane] = t;
+        }
+    }
+}
+
+// UltimateSolver::update_incumbent (ultimate.rs:397-431) after `sweep` sweeps
+// of a PT solve: the lowest tracked energy, ties to the lowest (temperature
+// index, lane), replaces the record when strictly lower.  Every member takes
+// the same decision from the same energies; a barrier separates the reads of
+// the record from member 0's write, and another one the copy of the winner's
+// bits (each member a share of the positions) from the next sweep.
+template <int WAVES>
+__device__ __forceinline__ void pt_track_point(const InstDesc& I, uint32_t sweep, uint32_t member,
+                                               uint32_t gang, uint32_t lane, uint32_t wave,
+                                               uint32_t& bar_target, uint32_t spin_limit,
+                                               uint32_t* err) {
+    int64_t e = 0;
+    uint32_t key = 0xFFFFFFFFu, q = 0;
+    for (uint32_t s = 0; s < I.cells; ++s) {
+        const int64_t es = load_agent(I.energy + s * 64 + lane);
+        const uint32_t ks = I.temp_of[s * 64 + lane] * 64 + lane;
+        if (key == 0xFFFFFFFFu || es < e || (es == e && ks < key)) {
+            e = es;
+            key = ks;
+            q = s * 64 + lane;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t elo = __shfl_xor((uint32_t)e, off), ehi = __shfl_xor((uint32_t)(e >> 32), off);
+        const int64_t e2 = (int64_t)(((uint64_t)ehi << 32) | elo);
+        const uint32_t k2 = __shfl_xor(key, off), q2 = __shfl_xor(q, off);
+        if (e2 < e || (e2 == e && k2 < key)) {
+            e = e2;
+            key = k2;
+            q = q2;
+        }
+    }
+    const bool take = !I.best->valid || e < I.best->energy;
+    if (gang > 1)
+        gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+    else
+        block_sync_device();
+    if (take) {
+        const uint64_t* src = I.spins + (uint64_t)(q / 64) * I.n;
+        const uint32_t l = q % 64;
+        for (uint32_t p = (member * WAVES + wave) * 64 + lane; p < I.n; p += gang * WAVES * 64)
+            I.best_x[p] = (uint8_t)((src[p] >> l) & 1u);
+        if (member == 0 && threadIdx.x == 0) {
+            I.best->energy = e;
+            I.best->valid = 1;
+            I.best->sweep = sweep;
+            I.best->temp = key / 64;
+            I.best->lane = l;
+        }
+    }
+    if (gang > 1)
+        gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+    else
+        block_sync_device();
+}
+
+// Sweeps [k0, k1) of a PT solve by one member of its gang: per colour class,
+// the class's groups of four positions in every cell, spread over the gang's
+// waves; each lane accepts with its own temperature (temp_of, ladder), read
+// once per item.  Energy changes go to the tracked energies by device-scope
+// atomics whenever a wave moves on to another cell; after each sweep a
+// barrier, the exchange (member 0, wave 0), another barrier, and every
+// track_every sweeps the incumbent.  The sweep's uniforms are keyed by
+// (position, cell, sweep), its exchange by the sweep number: a single state
+// with the same seed, ladder and IGE_RUN_EXCHANGE follows the same path.
+template <int WAVES, uint32_t MODE>
+__device__ __forceinline__ void pt_sweeps(const InstDesc& I, uint32_t k0, uint32_t k1,
+                                          uint32_t member, uint32_t gang, uint32_t lane,
+                                          uint32_t wave, uint32_t* thr_lds, uint32_t& bar_target,
+                                          uint32_t spin_limit, uint32_t* err, uint32_t& nacc) {
+    const uint32_t C = I.cells;
+    if constexpr (MODE == kThrLds) {
+        // the ladder is fixed: one table [C][dtab + 1] for the whole item
+        const uint32_t row = I.dtab + 1;
+        for (uint32_t i = threadIdx.x; i < C * row; i += WAVES * 64) {
+            const uint32_t t = i / row, d = i % row;
+            thr_lds[i] = d == 0 ? 0xFFFFFFFFu : metropolis_threshold(I.ladder[t], (int64_t)d);
+        }
+        __syncthreads();
+    }
+    const uint32_t stride = gang * WAVES;
+    const uint32_t lane_of_gang = member * WAVES + wave;
+    for (uint32_t k = k0; k < k1; ++k) {
+        for (uint32_t c = 0; c < I.ncol; ++c) {
+            const uint32_t cb = __builtin_amdgcn_readfirstlane(I.cs[c]);
+            const uint32_t ce = __builtin_amdgcn_readfirstlane(I.cs[c + 1]);
+            if (ce > cb) {
+                const uint32_t g0 = cb >> 2, ng = ((ce - 1) >> 2) - g0 + 1;
+                int64_t eacc = 0;
+                uint32_t cur = 0xFFFFFFFFu;
+                uint32_t tr = 0, dlim = 0;
+                double beta = 0.0, nbd = 0.0;
+                for (uint32_t it = lane_of_gang; it < C * ng; it += stride) {
+                    const uint32_t cell = it / ng, g = g0 + it % ng;
+                    if (cell != cur) {
+                        if (cur != 0xFFFFFFFFu && eacc != 0)
+                            atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + cur * 64 +
+                                                                            lane),
+                                      (unsigned long long)eacc);
+                        eacc = 0;
+                        cur = cell;
+                        tr = I.temp_of[cell * 64 + lane];
+                        beta = I.ladder[tr];
+                        nbd = -beta * 1.4426950408889634;
+                        dlim = threshold_limit(beta, I.dtab);
+                    }
+                    auto accept = [&](uint32_t uu, int32_t de) -> bool {
+                        if ((uint32_t)de > dlim) return false;
+                        if constexpr (MODE == kThrLds)
+                            return uu < thr_lds[tr * (I.dtab + 1) + (uint32_t)de];
+                        else
+                            return band_accept(uu, de, nbd, beta);
+                    };
+                    const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
+                    uint64_t* S = I.spins + (uint64_t)cell * I.n;
+                    generic_group(I.rp, I.ci, I.wt, I.lin, S, S, p0, p1, cell, k, I.key0, I.key1,
+                                  lane, accept, eacc, nacc);
+                }
+                if (cur != 0xFFFFFFFFu && eacc != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + cur * 64 + lane),
+                              (unsigned long long)eacc);
+            }
+            if (gang > 1)
+                gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+            else
+                __syncthreads();
+        }
+        // the energy atomics complete at device scope before anyone reads them
+        if (gang == 1) block_sync_device();
+        if (C > 1) {
+            if (member == 0 && wave == 0) pt_exchange(I, k, lane);
+            if (gang > 1)
+                gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+            else
+                block_sync_device();
+        }
+        if (I.track_every && ((k + 1) % I.track_every == 0 || k + 1 == I.sweeps))
+            pt_track_point<WAVES>(I, k + 1, member, gang, lane, wave, bar_target, spin_limit, err);
+    }
+}
+
 // queue[t] = instance << 5 | member.  A gang's tickets are consecutive, so at
 // most one gang is incomplete at any time and it is completed by the next
 // workgroups to finish their work: with every workgroup of the launch
@@ -244,26 +435,37 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
         int64_t eacc = 0;
         uint32_t nacc = 0;
-        // One code path per threshold mode, so neither carries the other's registers.
-        if (mode == kThrLds)
+        // One code path per kind and threshold mode, so none carries another's
+        // registers.  A PT solve keeps its energies up to date itself (eacc stays 0).
+        if (I.pt) {
+            if (mode == kThrLds)
+                pt_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds,
+                                          bar_target, spin_limit, err, nacc);
+            else
+                pt_sweeps<WAVES, kThrBand>(I, k0, k1, member, gang, lane, wave, thr_lds,
+                                           bar_target, spin_limit, err, nacc);
+        } else if (mode == kThrLds) {
             anneal_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds, part,
                                           bar_target, spin_limit, err, eacc, nacc);
-        else
+        } else {
             anneal_sweeps<WAVES, kThrBand>(I, k0, k1, member, gang, lane, wave, thr_lds, part,
                                            bar_target, spin_limit, err, eacc, nacc);
-        part[wave][lane] = eacc;
+        }
         // nacc is already the wave's total (a popcount of the accept ballot).
         if (lane == 0 && nacc) atomicAdd(I.accepted, (unsigned long long)nacc);
-        __syncthreads();
-        if (wave == 0) {
-            int64_t tot = 0;
+        if (!I.pt) {
+            part[wave][lane] = eacc;
+            __syncthreads();
+            if (wave == 0) {
+                int64_t tot = 0;
 #pragma unroll
-            for (int w = 0; w < WAVES; ++w) tot += part[w][lane];
-            if (gang > 1)
-                atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + lane),
-                          (unsigned long long)tot);
-            else
-                I.energy[lane] += tot;
+                for (int w = 0; w < WAVES; ++w) tot += part[w][lane];
+                if (gang > 1)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + lane),
+                              (unsigned long long)tot);
+                else
+                    I.energy[lane] += tot;
+            }
         }
         if (gang > 1) gang_barrier(I.bar, bar_target, gang, spin_limit, err);
         if (member == 0 && threadIdx.x == 0) {
@@ -281,10 +483,17 @@ static __global__ __launch_bounds__(256) void instance_init_kernel(const InstDes
                                                             const uint32_t* const* orders) {
     const InstDesc I = insts[list[blockIdx.x]];
     const uint32_t* order = orders[list[blockIdx.x]];
-    for (uint32_t p = threadIdx.x; p < I.n; p += blockDim.x) {
-        const U32x4 o = philox4x32_10(order[p], 0u, 0u, kTagInit << 24, I.key0, I.key1);
-        I.spins[p] = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
-    }
+    const uint32_t cells = I.pt ? I.cells : 1u;
+    for (uint32_t s = 0; s < cells; ++s)
+        for (uint32_t p = threadIdx.x; p < I.n; p += blockDim.x) {
+            const U32x4 o = philox4x32_10(order[p], s, 0u, kTagInit << 24, I.key0, I.key1);
+            I.spins[(uint64_t)s * I.n + p] = (uint64_t)o.v[0] | ((uint64_t)o.v[1] << 32);
+        }
+    if (I.pt)
+        for (uint32_t i = threadIdx.x; i < cells * 64; i += blockDim.x) {
+            I.temp_of[i] = i / 64;
+            I.slot_of[i] = i / 64;
+        }
     if (threadIdx.x == 0) {
         *I.progress = 0;
         *I.accepted = 0;
@@ -293,7 +502,8 @@ static __global__ __launch_bounds__(256) void instance_init_kernel(const InstDes
 }
 
 // Exact energies of the listed instances from their words: into their tracked
-// energies, or (out != NULL) into out[list index][64], leaving them untouched.
+// energies, or (out != NULL) into out[list index][cells][64], leaving them
+// untouched.
 static __global__ __launch_bounds__(256) void instance_energy_kernel(const InstDesc* __restrict__ insts,
                                                               const uint32_t* __restrict__ list,
                                                               int64_t* out) {
@@ -301,21 +511,26 @@ static __global__ __launch_bounds__(256) void instance_energy_kernel(const InstD
     const InstDesc I = insts[list[blockIdx.x]];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int64_t a = 0;
-    for (uint32_t p = wv; p < I.n; p += 4) {
-        int64_t f = 2 * (int64_t)I.lin[p];
-        for (uint32_t e = I.rp[p]; e < I.rp[p + 1]; ++e) f += lane_bit(I.spins[I.ci[e]]) ? I.wt[e] : 0;
-        a += lane_bit(I.spins[p]) ? f : 0;
-    }
-    part[wv][lane] = a;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int64_t e =
-            I.offset + (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / 2;
-        if (out)
-            out[(uint64_t)blockIdx.x * 64 + lane] = e;
-        else
-            I.energy[lane] = e;
+    const uint32_t cells = I.pt ? I.cells : 1u;
+    for (uint32_t s = 0; s < cells; ++s) {
+        const uint64_t* S = I.spins + (uint64_t)s * I.n;
+        int64_t a = 0;
+        for (uint32_t p = wv; p < I.n; p += 4) {
+            int64_t f = 2 * (int64_t)I.lin[p];
+            for (uint32_t e = I.rp[p]; e < I.rp[p + 1]; ++e) f += lane_bit(S[I.ci[e]]) ? I.wt[e] : 0;
+            a += lane_bit(S[p]) ? f : 0;
+        }
+        part[wv][lane] = a;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int64_t e =
+                I.offset + (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / 2;
+            if (out)
+                out[((uint64_t)blockIdx.x * cells + s) * 64 + lane] = e;
+            else
+                I.energy[s * 64 + lane] = e;
+        }
+        __syncthreads();
     }
 }
 

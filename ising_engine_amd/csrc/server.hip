@@ -97,10 +97,10 @@ class ArenaAllocator {
 };
 
 // Byte layout of one request inside the arena; each region 256-B aligned.
-// The model regions (rp .. pos_of) come first and are uploaded together.
+// The model regions (rp .. ladder) come first and are uploaded together.
 struct Layout {
-    uint64_t rp, ci, wt, lin, cs, order, pos_of, spins, energy, accepted, progress, best, best_x,
-        total;
+    uint64_t rp, ci, wt, lin, cs, order, pos_of, ladder, spins, energy, temp_of, slot_of,
+        accepted, progress, best, best_x, total;
     explicit Layout(const ige::LoweredInstance& L) {
         uint64_t o = 0;
         auto take = [&](uint64_t bytes) {
@@ -115,8 +115,11 @@ struct Layout {
         cs = take(L.cs.size() * 4ull);
         order = take(L.n * 4ull);
         pos_of = take(L.n * 4ull);
-        spins = take(L.n * 8ull);
-        energy = take(64 * 8);
+        ladder = take(L.cells * 8ull);
+        spins = take(static_cast<uint64_t>(L.cells) * L.n * 8ull);
+        energy = take(L.cells * 64ull * 8);
+        temp_of = take(L.cells * 64ull * 4);
+        slot_of = take(L.cells * 64ull * 4);
         accepted = take(8);
         progress = take(4);
         best = take(sizeof(ige::BestRecord));
@@ -206,7 +209,7 @@ namespace {
 void upload_request(ige_server* s, Request& r) {
     const ige::LoweredInstance& L = r.L;
     const Layout lay(L);
-    r.staging.assign(lay.spins, 0);  // model regions only: rp .. order
+    r.staging.assign(lay.spins, 0);  // model regions only: rp .. ladder
     auto put = [&](uint64_t at, const void* src, uint64_t bytes) {
         if (bytes) std::memcpy(r.staging.data() + at, src, bytes);
     };
@@ -217,6 +220,7 @@ void upload_request(ige_server* s, Request& r) {
     put(lay.cs, L.cs.data(), L.cs.size() * 4);
     put(lay.order, L.order.data(), L.order.size() * 4);
     put(lay.pos_of, L.pos_of.data(), L.pos_of.size() * 4);
+    put(lay.ladder, L.ladder.data(), L.ladder.size() * 8);
     uint8_t* base = s->d_arena + r.off;
     check(hipMemcpyAsync(base, r.staging.data(), r.staging.size(), hipMemcpyHostToDevice,
                          s->stream),
@@ -234,7 +238,12 @@ void upload_request(ige_server* s, Request& r) {
     D.best = reinterpret_cast<ige::BestRecord*>(base + lay.best);
     D.best_x = base + lay.best_x;
     D.track_every = L.track_every;
-    D.thr_mode = ige::choose_thr_mode(L.dtab);
+    D.thr_mode = ige::choose_thr_mode(L.dtab, L.pt ? L.cells : 1u);
+    D.cells = L.cells;
+    D.pt = L.pt;
+    D.ladder = reinterpret_cast<const double*>(base + lay.ladder);
+    D.temp_of = reinterpret_cast<uint32_t*>(base + lay.temp_of);
+    D.slot_of = reinterpret_cast<uint32_t*>(base + lay.slot_of);
     D.n = L.n;
     D.ncol = L.ncol;
     D.sweeps = L.sweeps;
@@ -314,12 +323,19 @@ void polish_state(ige_server* s, Request& r, std::vector<uint8_t>& x, int64_t& e
 
 void retire(ige_server* s, Request& r) {
     const ige::LoweredInstance& L = r.L;
-    std::vector<uint64_t> w(L.n);
-    r.energies.resize(64);
-    check(hipMemcpyAsync(w.data(), r.D.spins, L.n * 8ull, hipMemcpyDeviceToHost, s->stream),
+    const uint32_t C = L.cells;
+    std::vector<uint64_t> w(static_cast<size_t>(C) * L.n);
+    std::vector<uint32_t> tof(C * 64, 0);
+    r.energies.resize(C * 64);
+    check(hipMemcpyAsync(w.data(), r.D.spins, w.size() * 8, hipMemcpyDeviceToHost, s->stream),
           "result words");
-    check(hipMemcpyAsync(r.energies.data(), r.D.energy, 64 * 8, hipMemcpyDeviceToHost, s->stream),
+    check(hipMemcpyAsync(r.energies.data(), r.D.energy, C * 64 * 8, hipMemcpyDeviceToHost,
+                         s->stream),
           "result energies");
+    if (L.pt)
+        check(hipMemcpyAsync(tof.data(), r.D.temp_of, C * 64 * 4, hipMemcpyDeviceToHost,
+                             s->stream),
+              "result temperatures");
     unsigned long long acc = 0;
     check(hipMemcpyAsync(&acc, r.D.accepted, 8, hipMemcpyDeviceToHost, s->stream), "accepted");
     ige::BestRecord best{};
@@ -331,21 +347,34 @@ void retire(ige_server* s, Request& r) {
               "best state");
     }
     check(hipStreamSynchronize(s->stream), "retire");
-    r.words.assign(L.n, 0);
-    for (uint32_t p = 0; p < L.n; ++p) r.words[L.order[p]] = w[p];
+    r.words.assign(w.size(), 0);
+    for (uint32_t c = 0; c < C; ++c)
+        for (uint32_t p = 0; p < L.n; ++p)
+            r.words[static_cast<size_t>(c) * L.n + L.order[p]] = w[static_cast<size_t>(c) * L.n + p];
     r.accepted = acc;
-    // the solution: best seen at the incumbent updates, else the best final replica
-    int64_t e = 0;
+    // The solution (ultimate.rs:339-373): the best final replica, first in
+    // (temperature index, lane) order on an energy tie, replaced by the
+    // incumbent when that is strictly lower.
+    uint32_t q = 0;
+    uint64_t key_q = ~0ull;
+    for (uint32_t i = 0; i < C * 64; ++i) {
+        const uint64_t key = static_cast<uint64_t>(L.pt ? tof[i] : 0u) * 64 + i % 64;
+        if (key_q == ~0ull || r.energies[i] < r.energies[q] ||
+            (r.energies[i] == r.energies[q] && key < key_q)) {
+            q = i;
+            key_q = key;
+        }
+    }
+    int64_t e = r.energies[q];
     if (L.track_every) {
-        if (!best.valid) throw Failure(IGE_ERR_HIP, "incumbent record missing after the anneal");
+        if (!best.valid) throw Failure(IGE_ERR_HIP, "incumbent record missing after the solve");
+    }
+    if (L.track_every && best.energy < e) {
         e = best.energy;
         r.solution_sweep = best.sweep;
     } else {
-        uint32_t lane = 0;
-        for (uint32_t i = 1; i < 64; ++i)
-            if (r.energies[i] < r.energies[lane]) lane = i;
-        for (uint32_t p = 0; p < L.n; ++p) x[p] = static_cast<uint8_t>((w[p] >> lane) & 1u);
-        e = r.energies[lane];
+        const uint64_t* wc = w.data() + static_cast<size_t>(q / 64) * L.n;
+        for (uint32_t p = 0; p < L.n; ++p) x[p] = static_cast<uint8_t>((wc[p] >> (q % 64)) & 1u);
         r.solution_sweep = L.sweeps;
     }
     if (L.polish) polish_state(s, r, x, e);
@@ -400,7 +429,7 @@ void serve_loop(ige_server* s) {
             const double per_block = std::max(1.0, s->attempts_per_ms_per_block) * s->round_ms;
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
-                const uint64_t per_sweep = static_cast<uint64_t>(r.L.n) * 64;
+                const uint64_t per_sweep = static_cast<uint64_t>(r.L.n) * 64 * r.L.cells;
                 const double budget = r.attempts_per_tick > 0.0
                                           ? r.attempts_per_tick * s->ticks_per_ms * s->round_ms
                                           : per_block * r.gang;
@@ -414,8 +443,8 @@ void serve_loop(ige_server* s) {
             std::vector<uint32_t> gangs(L), ord(L);
             for (size_t i = 0; i < L; ++i) {
                 gangs[i] = s->live[i]->gang;
-                order_by_work[i] = {static_cast<uint64_t>(s->live[i]->L.n) * descs[i].round /
-                                        gangs[i],
+                order_by_work[i] = {static_cast<uint64_t>(s->live[i]->L.n) * s->live[i]->L.cells *
+                                        descs[i].round / gangs[i],
                                     static_cast<uint32_t>(i)};
             }
             std::sort(order_by_work.begin(), order_by_work.end(),
@@ -487,7 +516,7 @@ void serve_loop(ige_server* s) {
             check(hipStreamSynchronize(s->stream), "round ticks");
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
-                const double done = static_cast<double>(r.L.n) * 64.0 * r.D.round;
+                const double done = static_cast<double>(r.L.n) * 64.0 * r.L.cells * r.D.round;
                 if (s->h_ticks[i] > 0 && r.D.round > 0) {
                     const double rate = done / static_cast<double>(s->h_ticks[i]);
                     r.attempts_per_tick = r.attempts_per_tick > 0.0
@@ -553,9 +582,10 @@ void fill_info(const Request& r, ige_request_info* info) {
     info->id = r.id;
     info->status = r.status;
     info->n = r.L.n;
+    info->cells = r.L.cells;
     if (r.status == IGE_REQ_DONE) {
         uint32_t best = 0;
-        for (uint32_t i = 1; i < 64; ++i)
+        for (uint32_t i = 1; i < r.energies.size(); ++i)
             if (r.energies[i] < r.energies[best]) best = i;
         info->best_replica = best;
         info->best_energy = r.energies[best];
@@ -637,7 +667,15 @@ int32_t ige_server_submit(ige_server* s, const ige_instance_desc* d, uint64_t* i
     return guarded([&] {
         require(s && d && id, "NULL argument");
         require(d->n >= 1 && d->n <= kMaxVars, "num_vars must be in [1, 50000]");
-        require(d->sweeps >= 1 && d->sweeps <= kMaxSweeps, "sweeps must be in [1, 10000]");
+        if (d->num_temps == 0) {
+            require(d->sweeps >= 1 && d->sweeps <= kMaxSweeps, "sweeps must be in [1, 10000]");
+        } else {
+            // server_api.rs:16-17: sweeps (per exchange) and exchanges up to 10000 each
+            require(d->track_every >= 1 && d->track_every <= kMaxSweeps,
+                    "sweeps per exchange must be in [1, 10000]");
+            require(d->sweeps % d->track_every == 0 && d->sweeps / d->track_every <= kMaxSweeps,
+                    "exchanges must be in [1, 10000]");
+        }
         auto r = std::make_unique<Request>();
         r->t_submit = Clock::now();
         r->L = ige::lower_instance(*d, "request");
@@ -667,7 +705,7 @@ int32_t ige_server_poll(ige_server* s, uint64_t id, int32_t release, ige_request
         fill_info(r, info);
         if (r.status == IGE_REQ_DONE) {
             if (words) std::memcpy(words, r.words.data(), r.words.size() * 8);
-            if (energies) std::memcpy(energies, r.energies.data(), 64 * 8);
+            if (energies) std::memcpy(energies, r.energies.data(), r.energies.size() * 8);
             if (release) s->reqs.erase(it);
         }
     });

@@ -218,10 +218,15 @@ def workload_sk_anneal(n: int = 4096, sweeps: int = 5000, seed: int = 5) -> dict
 
 
 def serving_request(rng, n_lo: int = 128, n_hi: int = 16384, w_max: int = 1000,
-                    sweeps_lo: int = 500, sweeps_hi: int = 5000) -> dict:
+                    sweeps_lo: int = 500, sweeps_hi: int = 5000, num_temps: int = 10,
+                    sweeps_per_exchange: int = 100) -> dict:
     """One solve request: 50/50 Ising or QUBO, N log-uniform in [n_lo, n_hi], mean degree
     uniform in 4..16, integer weights uniform in [-w_max, w_max] (nonzero), sweep budget
-    uniform in [sweeps_lo, sweeps_hi], anneal between the suggested temperatures."""
+    uniform in [sweeps_lo, sweeps_hi].  As POST /api/v1/solve runs it
+    (src/bin/server_api.rs:111-169): UltimateSolver with num_temps temperatures between the
+    suggested temperatures, sweeps_per_exchange steps per exchange round and the budget
+    rounded to whole rounds (`sweeps`); beta_lo / beta_hi give the same range for a
+    one-cell anneal."""
     n = int(np.rint(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
     deg = int(rng.integers(4, 17))
     pi, pj = _er_pairs(n, deg, rng)
@@ -233,6 +238,9 @@ def serving_request(rng, n_lo: int = 128, n_hi: int = 16384, w_max: int = 1000,
         lin = rng.integers(-w_max, w_max + 1, size=n).astype(np.int64)
         qb = Qubo(n=n, offset=0, linear=lin, pi=pi, pj=pj, q=w, name=f"qubo_n{n}_d{deg}")
     t_max, t_min = suggested_temp_range(qb)
+    budget = int(rng.integers(sweeps_lo, sweeps_hi + 1))
+    spe = max(1, min(sweeps_per_exchange, budget))
     return dict(qubo=qb, seed=int(rng.integers(1, 1 << 62)),
-                sweeps=int(rng.integers(sweeps_lo, sweeps_hi + 1)),
+                sweeps=max(1, round(budget / spe)) * spe, sweeps_per_exchange=spe,
+                num_temps=num_temps, temp_max=t_max, temp_min=t_min,
                 beta_lo=1.0 / t_max, beta_hi=1.0 / t_min)
