@@ -305,6 +305,9 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
             D.ladder = nullptr;
             D.temp_of = nullptr;
             D.slot_of = nullptr;
+            D.round_ticks = 0;
+            D.stop = nullptr;
+            D.swept = nullptr;
             for (uint32_t r : L.rp) rp.push_back(e_base + r);
             ci.insert(ci.end(), L.ci.begin(), L.ci.end());
             wt.insert(wt.end(), L.wt.begin(), L.wt.end());

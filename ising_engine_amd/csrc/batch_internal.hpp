@@ -167,6 +167,21 @@ void launch_persistent_waves(const PersistentLaunch& L, hipStream_t st) {
     check(hipGetLastError(), "persistent launch");
 }
 
+// Workgroups of persistent_anneal_kernel<waves> resident per CU.
+inline int persistent_per_cu(uint32_t waves) {
+    int per_cu = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (waves) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<1>, 64, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<2>, 128, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<4>, 256, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<8>, 512, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<16>, 1024, 0); break;
+        default: break;
+    }
+    return e == hipSuccess && per_cu > 0 ? per_cu : 1;
+}
+
 inline void launch_persistent(uint32_t waves, const PersistentLaunch& L, hipStream_t st) {
     switch (waves) {
         case 1: launch_persistent_waves<1>(L, st); break;

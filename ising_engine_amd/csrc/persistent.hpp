@@ -55,6 +55,13 @@ struct InstDesc {
     const double* ladder;   // [cells] betas, hottest first
     uint32_t* temp_of;      // [cells][64]
     uint32_t* slot_of;      // [cells][64]
+    // Deadline rounds (PT, round_ticks > 0): the solve sweeps until its member
+    // 0 has spent round_ticks of device wall clock in this launch (at least
+    // one sweep, at most `round`), deciding after each sweep through *stop;
+    // *swept receives the sweeps done.
+    uint64_t round_ticks;
+    uint32_t* stop;
+    uint32_t* swept;
 };
 
 constexpr uint32_t kMaxPtCells = 64;  // temperature cells per solve
@@ -320,20 +327,23 @@ __device__ __forceinline__ void pt_track_point(const InstDesc& I, uint32_t sweep
         block_sync_device();
 }
 
-// Sweeps [k0, k1) of a PT solve by one member of its gang: per colour class,
-// the class's groups of four positions in every cell, spread over the gang's
-// waves; each lane accepts with its own temperature (temp_of, ladder), read
-// once per item.  Energy changes go to the tracked energies by device-scope
-// atomics whenever a wave moves on to another cell; after each sweep a
-// barrier, the exchange (member 0, wave 0), another barrier, and every
-// track_every sweeps the incumbent.  The sweep's uniforms are keyed by
-// (position, cell, sweep), its exchange by the sweep number: a single state
-// with the same seed, ladder and IGE_RUN_EXCHANGE follows the same path.
+// Sweeps [k0, k1) of a PT solve by one member of its gang.  The member owns
+// cells member, member + gang, ... (cells are independent within a sweep): it
+// sweeps each of them class by class with workgroup barriers, each lane
+// accepting with its own temperature (temp_of, ladder), and adds the cell's
+// energy changes to the tracked energies by device-scope atomics.  After the
+// sweep: a barrier across the gang, the exchange (member 0, wave 0), another
+// barrier, and every track_every sweeps the incumbent.  The sweep's uniforms
+// are keyed by (position, cell, sweep), its exchange by the sweep number: a
+// single state with the same seed, ladder and IGE_RUN_EXCHANGE follows the
+// same path.
+// Returns the end of the sweeps done (k1, or earlier at a deadline).
 template <int WAVES, uint32_t MODE>
-__device__ __forceinline__ void pt_sweeps(const InstDesc& I, uint32_t k0, uint32_t k1,
-                                          uint32_t member, uint32_t gang, uint32_t lane,
-                                          uint32_t wave, uint32_t* thr_lds, uint32_t& bar_target,
-                                          uint32_t spin_limit, uint32_t* err, uint32_t& nacc) {
+__device__ __forceinline__ uint32_t pt_sweeps(const InstDesc& I, uint32_t k0, uint32_t k1,
+                                              uint32_t member, uint32_t gang, uint32_t lane,
+                                              uint32_t wave, uint32_t* thr_lds,
+                                              uint32_t& bar_target, uint32_t spin_limit,
+                                              uint32_t* err, uint32_t& nacc, uint64_t t_start) {
     const uint32_t C = I.cells;
     if constexpr (MODE == kThrLds) {
         // the ladder is fixed: one table [C][dtab + 1] for the whole item
@@ -344,57 +354,51 @@ __device__ __forceinline__ void pt_sweeps(const InstDesc& I, uint32_t k0, uint32
         }
         __syncthreads();
     }
-    const uint32_t stride = gang * WAVES;
-    const uint32_t lane_of_gang = member * WAVES + wave;
     for (uint32_t k = k0; k < k1; ++k) {
-        for (uint32_t c = 0; c < I.ncol; ++c) {
-            const uint32_t cb = __builtin_amdgcn_readfirstlane(I.cs[c]);
-            const uint32_t ce = __builtin_amdgcn_readfirstlane(I.cs[c + 1]);
-            if (ce > cb) {
-                const uint32_t g0 = cb >> 2, ng = ((ce - 1) >> 2) - g0 + 1;
-                int64_t eacc = 0;
-                uint32_t cur = 0xFFFFFFFFu;
-                uint32_t tr = 0, dlim = 0;
-                double beta = 0.0, nbd = 0.0;
-                for (uint32_t it = lane_of_gang; it < C * ng; it += stride) {
-                    const uint32_t cell = it / ng, g = g0 + it % ng;
-                    if (cell != cur) {
-                        if (cur != 0xFFFFFFFFu && eacc != 0)
-                            atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + cur * 64 +
-                                                                            lane),
-                                      (unsigned long long)eacc);
-                        eacc = 0;
-                        cur = cell;
-                        tr = I.temp_of[cell * 64 + lane];
-                        beta = I.ladder[tr];
-                        nbd = -beta * 1.4426950408889634;
-                        dlim = threshold_limit(beta, I.dtab);
+        for (uint32_t cell = member; cell < C; cell += gang) {
+            const uint32_t tr = I.temp_of[cell * 64 + lane];
+            const double beta = I.ladder[tr];
+            const double nbd = -beta * 1.4426950408889634;
+            const uint32_t dlim = threshold_limit(beta, I.dtab);
+            auto accept = [&](uint32_t uu, int32_t de) -> bool {
+                if ((uint32_t)de > dlim) return false;
+                if constexpr (MODE == kThrLds)
+                    return uu < thr_lds[tr * (I.dtab + 1) + (uint32_t)de];
+                else
+                    return band_accept(uu, de, nbd, beta);
+            };
+            uint64_t* S = I.spins + (uint64_t)cell * I.n;
+            int64_t eacc = 0;
+            for (uint32_t c = 0; c < I.ncol; ++c) {
+                const uint32_t cb = __builtin_amdgcn_readfirstlane(I.cs[c]);
+                const uint32_t ce = __builtin_amdgcn_readfirstlane(I.cs[c + 1]);
+                if (ce > cb) {
+                    for (uint32_t g = (cb >> 2) + wave; g <= ((ce - 1) >> 2); g += WAVES) {
+                        const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
+                        generic_group(I.rp, I.ci, I.wt, I.lin, S, S, p0, p1, cell, k, I.key0,
+                                      I.key1, lane, accept, eacc, nacc);
                     }
-                    auto accept = [&](uint32_t uu, int32_t de) -> bool {
-                        if ((uint32_t)de > dlim) return false;
-                        if constexpr (MODE == kThrLds)
-                            return uu < thr_lds[tr * (I.dtab + 1) + (uint32_t)de];
-                        else
-                            return band_accept(uu, de, nbd, beta);
-                    };
-                    const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
-                    uint64_t* S = I.spins + (uint64_t)cell * I.n;
-                    generic_group(I.rp, I.ci, I.wt, I.lin, S, S, p0, p1, cell, k, I.key0, I.key1,
-                                  lane, accept, eacc, nacc);
                 }
-                if (cur != 0xFFFFFFFFu && eacc != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + cur * 64 + lane),
-                              (unsigned long long)eacc);
-            }
-            if (gang > 1)
-                gang_barrier(I.bar, bar_target, gang, spin_limit, err);
-            else
                 __syncthreads();
+            }
+            if (eacc != 0)
+                atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + cell * 64 + lane),
+                          (unsigned long long)eacc);
         }
         // the energy atomics complete at device scope before anyone reads them
-        if (gang == 1) block_sync_device();
-        if (C > 1) {
-            if (member == 0 && wave == 0) pt_exchange(I, k, lane);
+        if (gang > 1)
+            gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+        else
+            block_sync_device();
+        const bool timed = I.round_ticks != 0;
+        if (C > 1 || timed) {
+            if (member == 0 && wave == 0) {
+                if (C > 1) pt_exchange(I, k, lane);
+                if (timed && lane == 0)
+                    __hip_atomic_store(I.stop,
+                                       wall_clock64() - t_start >= I.round_ticks ? 1u : 0u,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             if (gang > 1)
                 gang_barrier(I.bar, bar_target, gang, spin_limit, err);
             else
@@ -402,7 +406,11 @@ __device__ __forceinline__ void pt_sweeps(const InstDesc& I, uint32_t k0, uint32
         }
         if (I.track_every && ((k + 1) % I.track_every == 0 || k + 1 == I.sweeps))
             pt_track_point<WAVES>(I, k + 1, member, gang, lane, wave, bar_target, spin_limit, err);
+        // every member reads the flag member 0 wrote before the last barrier
+        if (timed && __hip_atomic_load(I.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            return k + 1;
     }
+    return k1;
 }
 
 // queue[t] = instance << 5 | member.  A gang's tickets are consecutive, so at
@@ -435,15 +443,16 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         const uint32_t k1 = min(I.sweeps, k0 + (I.round ? I.round : round));
         int64_t eacc = 0;
         uint32_t nacc = 0;
+        uint32_t k_end = k1;
         // One code path per kind and threshold mode, so none carries another's
         // registers.  A PT solve keeps its energies up to date itself (eacc stays 0).
         if (I.pt) {
             if (mode == kThrLds)
-                pt_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds,
-                                          bar_target, spin_limit, err, nacc);
+                k_end = pt_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds,
+                                                  bar_target, spin_limit, err, nacc, t_start);
             else
-                pt_sweeps<WAVES, kThrBand>(I, k0, k1, member, gang, lane, wave, thr_lds,
-                                           bar_target, spin_limit, err, nacc);
+                k_end = pt_sweeps<WAVES, kThrBand>(I, k0, k1, member, gang, lane, wave, thr_lds,
+                                                   bar_target, spin_limit, err, nacc, t_start);
         } else if (mode == kThrLds) {
             anneal_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds, part,
                                           bar_target, spin_limit, err, eacc, nacc);
@@ -469,8 +478,9 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         }
         if (gang > 1) gang_barrier(I.bar, bar_target, gang, spin_limit, err);
         if (member == 0 && threadIdx.x == 0) {
-            *I.progress = k1;
+            *I.progress = k_end;
             if (I.ticks) *I.ticks = wall_clock64() - t_start;
+            if (I.swept) *I.swept = k_end - k0;
         }
         __syncthreads();
     }

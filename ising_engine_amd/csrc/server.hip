@@ -5,9 +5,11 @@
 // background loop works in rounds:
 //   1. admit queued requests, first come first served, while their HBM
 //      footprint fits the arena (one device allocation made up front);
-//   2. run one persistent launch in which every live request advances by its
-//      own number of sweeps, sized so that a round takes about the target
-//      round time whatever the request's size;
+//   2. run one persistent launch in which every live request advances: a
+//      parallel-tempering solve sweeps until round_ms of device time has
+//      passed (checked at sweep boundaries), an anneal by its own number of
+//      sweeps sized from its measured rate, so that a round takes about the
+//      target round time whatever the request's size;
 //   3. retire finished requests: copy the final replicas, their energies and
 //      the counters back, form the solution (best seen at the incumbent
 //      updates, or the best final replica; 1-opt polished on request) and
@@ -100,7 +102,7 @@ class ArenaAllocator {
 // The model regions (rp .. ladder) come first and are uploaded together.
 struct Layout {
     uint64_t rp, ci, wt, lin, cs, order, pos_of, ladder, spins, energy, temp_of, slot_of,
-        accepted, progress, best, best_x, total;
+        accepted, progress, stop, best, best_x, total;
     explicit Layout(const ige::LoweredInstance& L) {
         uint64_t o = 0;
         auto take = [&](uint64_t bytes) {
@@ -122,6 +124,7 @@ struct Layout {
         slot_of = take(L.cells * 64ull * 4);
         accepted = take(8);
         progress = take(4);
+        stop = take(4);
         best = take(sizeof(ige::BestRecord));
         best_x = take(L.n);
         total = o;
@@ -153,6 +156,7 @@ struct Request {
 struct ige_server {
     int device = 0;
     int num_cus = 0;
+    int per_cu = 1;  // persistent workgroups resident per CU (occupancy of `waves`)
     hipStream_t stream = nullptr;
     uint32_t waves = 8;
     double round_ms = 10.0;
@@ -166,6 +170,8 @@ struct ige_server {
     DevBuf<uint32_t> d_queue, d_list, d_bars, d_err;
     DevBuf<uint64_t> d_ticks;
     std::vector<uint64_t> h_ticks;
+    DevBuf<uint32_t> d_swept;  // sweeps each live request did in the last round
+    std::vector<uint32_t> h_swept;
     double ticks_per_ms = 1.0e5;  // device wall clock (hipDeviceAttributeWallClockRate)
     DevBuf<const uint32_t*> d_orders;
     DevBuf<unsigned int> d_next;
@@ -244,6 +250,9 @@ void upload_request(ige_server* s, Request& r) {
     D.ladder = reinterpret_cast<const double*>(base + lay.ladder);
     D.temp_of = reinterpret_cast<uint32_t*>(base + lay.temp_of);
     D.slot_of = reinterpret_cast<uint32_t*>(base + lay.slot_of);
+    D.stop = reinterpret_cast<uint32_t*>(base + lay.stop);
+    D.round_ticks = 0;   // set per round
+    D.swept = nullptr;   // set per round
     D.n = L.n;
     D.ncol = L.ncol;
     D.sweeps = L.sweeps;
@@ -259,19 +268,21 @@ void upload_request(ige_server* s, Request& r) {
     D.offset = L.offset;
 }
 
-// Workgroups per live request for the next round: n / s_eff, at most the
-// request's gang_cap, with s_eff = max(gang_sites, live sites / CUs): the
-// gangs together ask for about one workgroup per CU.  Lightly loaded,
-// requests spread over many CUs (latency); loaded, each keeps one workgroup
-// and no barrier is paid (throughput).  A request's trajectory does not depend
-// on its gang.
+// Workgroups per live request for the next round: its site-cells (n x cells)
+// over s_eff, at most the request's gang_cap, with s_eff = max(gang_sites,
+// live site-cells / resident workgroups): the gangs together ask for about as
+// many workgroups as the kernel keeps resident (per_cu on each CU).  Lightly
+// loaded, requests spread over many CUs (latency); loaded, each keeps one
+// workgroup and no barrier is paid (throughput).  A request's trajectory does
+// not depend on its gang.
 void plan_gangs(ige_server* s) {
-    uint64_t sites = 0;
-    for (Request* r : s->live) sites += r->L.n;
-    const double per_wg = std::max<double>(
-        s->gang_sites, static_cast<double>(sites) / std::max(1, s->num_cus));
+    uint64_t work = 0;
+    for (Request* r : s->live) work += static_cast<uint64_t>(r->L.n) * r->L.cells;
+    const double slots = static_cast<double>(std::max(1, s->num_cus) * std::max(1, s->per_cu));
+    const double per_wg = std::max<double>(s->gang_sites, static_cast<double>(work) / slots);
     for (Request* r : s->live) {
-        const uint32_t g = static_cast<uint32_t>(std::max(1.0, std::floor(r->L.n / per_wg)));
+        const double w = static_cast<double>(r->L.n) * r->L.cells;
+        const uint32_t g = static_cast<uint32_t>(std::max(1.0, std::floor(w / per_wg)));
         r->gang = std::min(g, r->gang_cap);
         r->D.gang = r->gang;
     }
@@ -427,17 +438,26 @@ void serve_loop(ige_server* s) {
             // member 0's device time), or of the per-workgroup estimate before
             // its first round, so that no request stretches the round.
             const double per_block = std::max(1.0, s->attempts_per_ms_per_block) * s->round_ms;
+            // A PT solve instead sweeps until round_ms of device time has passed
+            // (at sweep boundaries), up to its remaining sweeps: every workgroup
+            // stays busy until the round's end.
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
                 const uint64_t per_sweep = static_cast<uint64_t>(r.L.n) * 64 * r.L.cells;
-                const double budget = r.attempts_per_tick > 0.0
-                                          ? r.attempts_per_tick * s->ticks_per_ms * s->round_ms
-                                          : per_block * r.gang;
-                uint32_t k = static_cast<uint32_t>(std::max(1.0, budget / per_sweep));
-                k = std::min(k, r.L.sweeps - r.progress);
+                uint32_t k = r.L.sweeps - r.progress;
+                if (r.L.pt) {
+                    r.D.round_ticks =
+                        std::max<uint64_t>(1, static_cast<uint64_t>(s->round_ms * s->ticks_per_ms));
+                } else {
+                    const double budget = r.attempts_per_tick > 0.0
+                                              ? r.attempts_per_tick * s->ticks_per_ms * s->round_ms
+                                              : per_block * r.gang;
+                    k = std::min<uint32_t>(k, static_cast<uint32_t>(
+                                                  std::max(1.0, budget / per_sweep)));
+                }
                 r.D.round = k;
                 descs[i] = r.D;
-                round_attempts += per_sweep * k;
+                if (!r.L.pt) round_attempts += per_sweep * k;
             }
             std::vector<std::pair<uint64_t, uint32_t>> order_by_work(L);
             std::vector<uint32_t> gangs(L), ord(L);
@@ -457,11 +477,13 @@ void serve_loop(ige_server* s) {
                 s->d_orders.alloc(s->d_desc.count);
                 s->d_bars.alloc(s->d_desc.count);
                 s->d_ticks.alloc(s->d_desc.count);
+                s->d_swept.alloc(s->d_desc.count);
             }
             if (s->d_queue.count < queue.size()) s->d_queue.alloc(std::max<size_t>(queue.size(), 2 * s->d_queue.count));
             for (size_t i = 0; i < L; ++i) {
                 descs[i].bar = s->d_bars.p + i;
                 descs[i].ticks = s->d_ticks.p + i;
+                descs[i].swept = s->d_swept.p + i;
             }
             s->d_desc.upload(descs.data(), L, s->stream);
             s->d_queue.upload(queue.data(), queue.size(), s->stream);
@@ -510,12 +532,20 @@ void serve_loop(ige_server* s) {
             check(hipEventElapsedTime(&ms, e0, e1), "event");
             // per-request device time of this round -> its rate for the next one
             s->h_ticks.resize(L);
+            s->h_swept.resize(L);
             check(hipMemcpyAsync(s->h_ticks.data(), s->d_ticks.p, L * sizeof(uint64_t),
                                  hipMemcpyDeviceToHost, s->stream),
                   "round ticks");
+            check(hipMemcpyAsync(s->h_swept.data(), s->d_swept.p, L * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s->stream),
+                  "round sweeps");
             check(hipStreamSynchronize(s->stream), "round ticks");
             for (size_t i = 0; i < L; ++i) {
                 Request& r = *s->live[i];
+                if (r.L.pt) {
+                    r.D.round = s->h_swept[i];  // what the deadline left it
+                    round_attempts += static_cast<uint64_t>(r.L.n) * 64 * r.L.cells * r.D.round;
+                }
                 const double done = static_cast<double>(r.L.n) * 64.0 * r.L.cells * r.D.round;
                 if (s->h_ticks[i] > 0 && r.D.round > 0) {
                     const double rate = done / static_cast<double>(s->h_ticks[i]);
@@ -621,6 +651,7 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->arena = ArenaAllocator(hbm_bytes);
         s->waves = waves ? waves : 8;
         s->round_ms = round_ms;
+        s->per_cu = ige::persistent_per_cu(s->waves);
         s->d_next.alloc(1);
         s->d_desc.alloc(256);
         s->d_queue.alloc(256);
@@ -628,6 +659,7 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->d_orders.alloc(256);
         s->d_bars.alloc(256);
         s->d_ticks.alloc(256);
+        s->d_swept.alloc(256);
         {
             int khz = 0;
             if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) ==
@@ -679,7 +711,9 @@ int32_t ige_server_submit(ige_server* s, const ige_instance_desc* d, uint64_t* i
         auto r = std::make_unique<Request>();
         r->t_submit = Clock::now();
         r->L = ige::lower_instance(*d, "request");
-        r->gang_cap = ige::choose_gang(r->L.n, s->max_gang, s->gang_sites);
+        // a PT solve's gang members own whole temperature cells, at most one each
+        r->gang_cap = r->L.pt ? std::min(r->L.cells, s->max_gang)
+                              : ige::choose_gang(r->L.n, s->max_gang, s->gang_sites);
         r->gang = 1;
         const Layout lay(r->L);
         std::lock_guard<std::mutex> g(s->mu);
