@@ -577,46 +577,96 @@ def bench_batch(_lib, instances, ctx, count: int, cpu_s: float) -> dict:
             "cpu_baseline": cpu}
 
 
-def bench_serving(_lib, instances, ctx, rate: float, count: int, track_every: int = 100,
-                  polish: bool = True) -> dict:
-    """Poisson stream of mixed Ising/QUBO solve requests (N log-uniform 128..16384, degree
-    4..16, weights in [-1000, 1000], 500..5000 sweeps) into the continuous-batching loop.
-    Each request returns what UltimateSolver::solve returns: the best state seen at an
-    incumbent update every `track_every` sweeps (the reference's default sweeps per
-    exchange round), 1-opt polished."""
-    rng = np.random.default_rng(11)
-    reqs = [instances.serving_request(rng) for _ in range(count)]
-    srv = _lib.Server(ctx, hbm_bytes=16 << 30, round_ms=10.0)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
-    t0 = time.perf_counter()
-    ids = []
-    for a, r in zip(arrivals, reqs):
-        while time.perf_counter() - t0 < a:
-            time.sleep(0.0005)
-        ids.append(srv.submit(r["qubo"], r["seed"], r["sweeps"], r["beta_lo"], r["beta_hi"],
-                              track_every=track_every, polish=polish))
-    infos = [srv.wait(rid, 600000.0) for rid in ids]
-    if any(i.status != _lib.REQ_DONE for i in infos):
-        raise SystemExit("a serving request did not complete")
-    lat = np.array([i.latency_ms for i in infos])
-    # the solution is never worse than the final best replica, and its energy is exact
+def _serving_submit(srv, r, polish=True):
+    return srv.submit(r["qubo"], r["seed"], r["sweeps"], track_every=r["sweeps_per_exchange"],
+                      polish=polish, num_temps=r["num_temps"], temp_max=r["temp_max"],
+                      temp_min=r["temp_min"])
+
+
+def _serving_check(srv, reqs, ids, infos):
+    """Every result's solution energy is exact and never worse than its best final replica."""
     for k in (0, len(ids) // 2, len(ids) - 1):
         x, e = srv.solution(ids[k])
         if e != int(reqs[k]["qubo"].energy(x)) or e > infos[k].best_energy:
             raise SystemExit("serving solution energy check failed")
-    wall = time.perf_counter() - t0
+
+
+def bench_serving(_lib, instances, ctx, rates, burst: int, count: int) -> dict:
+    """Solve requests as POST /api/v1/solve runs them (UltimateSolver: 10 temperatures
+    between the suggested temperatures, a DEO exchange every sweep, the incumbent every
+    100-sweep exchange round, 1-opt polish; N log-uniform 128..16384, degree 4..16,
+    weights in [-1000, 1000], 500..5000 sweeps), into the continuous-batching loop.
+
+    Capacity: a burst of `burst` requests submitted at once; completions per second over
+    the burst (its drain, the largest solves finishing on few workgroups, included, so
+    this is a lower bound).  Latency: a Poisson stream of `count` requests at 80 % of that
+    capacity, where the queue stays bounded: p50 / p99 and the largest queue seen.  Each
+    rate in `rates` is reported as saturated (above capacity: the queue grows without
+    bound, so no p99 exists) or run like the 80 % stream."""
+    rng = np.random.default_rng(11)
+    pool = [instances.serving_request(rng) for _ in range(max(burst, count))]
+    per_req = np.array([r["qubo"].n * 64 * r["num_temps"] * r["sweeps"] for r in pool])
+    srv = _lib.Server(ctx, hbm_bytes=32 << 30, round_ms=20.0)
+    # warm-up: a few requests, outside the measurements
+    for rid in [_serving_submit(srv, r) for r in pool[:4]]:
+        srv.wait(rid, 600000.0)
+    t0 = time.perf_counter()
+    ids = [_serving_submit(srv, r) for r in pool[:burst]]
+    infos = [srv.wait(rid, 600000.0) for rid in ids]
+    t_burst = time.perf_counter() - t0
+    if any(i.status != _lib.REQ_DONE for i in infos):
+        raise SystemExit("a serving request did not complete")
+    _serving_check(srv, pool[:burst], ids, infos)
+    capacity = burst / t_burst
+    cap_attempts = float(per_req[:burst].sum()) / t_burst
+    log(f"[bench] serving capacity: {burst} requests in {t_burst:.2f} s -> {capacity:.1f} req/s, "
+        f"{cap_attempts:.3e} attempts/s")
+
+    def stream(rate, n):
+        arrivals = np.cumsum(np.random.default_rng(12).exponential(1.0 / rate, size=n))
+        reqs = pool[:n]
+        ids, max_q = [], 0
+        t0 = time.perf_counter()
+        for a, r in zip(arrivals, reqs):
+            while time.perf_counter() - t0 < a:
+                time.sleep(0.0005)
+            ids.append(_serving_submit(srv, r))
+            if len(ids) % 16 == 0:
+                max_q = max(max_q, int(srv.stats().queued))
+        infos = [srv.wait(rid, 600000.0) for rid in ids]
+        wall = time.perf_counter() - t0
+        if any(i.status != _lib.REQ_DONE for i in infos):
+            raise SystemExit("a serving request did not complete")
+        _serving_check(srv, reqs, ids, infos)
+        lat = np.array([i.latency_ms for i in infos])
+        return {"rate_req_s": rate, "requests": n, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                "max_queued": max_q, "wall_s": wall,
+                "attempts_per_s": float(per_req[:n].sum()) / wall,
+                "mean_solution_gain": float(np.mean([i.best_energy - i.solution_energy
+                                                     for i in infos]))}
+
+    held = stream(0.8 * capacity, count)
+    log(f"[bench] serving at {held['rate_req_s']:.1f} req/s x {count}: p50 {held['p50_ms']:.0f} ms "
+        f"p99 {held['p99_ms']:.0f} ms, max queued {held['max_queued']}")
+    at_rates = []
+    for rate in rates:
+        if rate > capacity:
+            at_rates.append({"rate_req_s": rate, "saturated": True,
+                             "note": f"above the measured capacity {capacity:.1f} req/s: the "
+                                     f"queue grows without bound"})
+        else:
+            at_rates.append(dict(stream(rate, count), saturated=False))
     st = srv.stats()
-    attempts = sum(r["qubo"].n * 64 * r["sweeps"] for r in reqs)
     srv.close()
-    out = {"rate_req_s": rate, "requests": count, "p50_ms": float(np.percentile(lat, 50)),
-           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
-           "attempts_per_s": attempts / wall, "wall_s": wall, "rounds": int(st.rounds),
-           "max_live": int(st.max_live), "track_every": track_every, "polish": polish,
-           "mean_polish_flips": float(np.mean([i.polish_flips for i in infos])),
-           "mean_solution_gain": float(np.mean([i.best_energy - i.solution_energy
-                                                for i in infos]))}
-    log(f"[bench] serving {rate}/s x {count}: p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms")
-    return out
+    return {"requests": "UltimateSolver solves: 10 temperatures, DEO every sweep, incumbent "
+                        "per 100-sweep round, 1-opt polish; N 128..16384, degree 4..16, "
+                        "weights +-1000, 500..5000 sweeps",
+            "mean_attempts_per_request": float(per_req.mean()),
+            "capacity_req_s": capacity, "capacity_attempts_per_s": cap_attempts,
+            "capacity_sample": f"burst of {burst} requests, {t_burst:.2f} s",
+            "held": held, "rates": at_rates, "rounds": int(st.rounds),
+            "max_live": int(st.max_live)}
 
 
 def main():
@@ -639,9 +689,13 @@ def main():
     ap.add_argument("--tts-runs", type=int, default=32)
     ap.add_argument("--sk-sweeps", type=int, default=200)
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--serving-rates", type=str, default="100,250",
-                    help="comma-separated Poisson request rates (req/s)")
-    ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--serving-rates", type=str, default="100,250,500,1000,2000",
+                    help="comma-separated Poisson request rates (req/s); those above the "
+                         "measured capacity are reported as saturated")
+    ap.add_argument("--serving-burst", type=int, default=200,
+                    help="requests in the capacity burst")
+    ap.add_argument("--serving-requests", type=int, default=300,
+                    help="requests per Poisson stream")
     ap.add_argument("--counter-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS steps of --probe-workload and a 1 GiB copy "
                          "(for rocprofv3 --pmc passes)")
@@ -730,8 +784,9 @@ def main():
         gset = bench_gset(_lib, ctx, cpu_s=0.0 if args.no_cpu_baseline else 2.0)
     serving = None
     if not args.no_serving:
-        serving = [bench_serving(_lib, instances, ctx, float(r), args.serving_requests)
-                   for r in args.serving_rates.split(",") if r]
+        serving = bench_serving(_lib, instances, ctx,
+                                [float(r) for r in args.serving_rates.split(",") if r],
+                                args.serving_burst, args.serving_requests)
 
     cpu = None
     if not args.no_cpu_baseline:
