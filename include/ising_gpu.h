@@ -405,13 +405,18 @@ int32_t ige_batch_run(ige_batch *batch, uint32_t mode, uint32_t waves, uint32_t 
 int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *waves_out,
                            double *ms_out /* 5 entries, or NULL */);
 /* Large instances run on a gang of workgroups (instance sites / gang_sites of
- * them, at most max_gang <= 32) that meet at a device-wide barrier after every
+ * them, at most max_gang <= 64) that meet at a device-wide barrier after every
  * colour class.  spin_limit bounds each barrier wait; an expired wait makes
  * the next synchronize/read fail with IGE_ERR_HIP.  Batch defaults: 1 (no gangs:
  * throughput), 2048, 2^24.  The server sizes gangs per round from its load:
- * n / max(gang_sites, live sites / CUs) members, at most max_gang and at most
- * n / gang_sites; server defaults: 16, 1024, 2^24 (a gang barrier per colour
- * class costs more than a few hundred sites of work on this GPU). */
+ * a PT request gets one workgroup per temperature cell (at most max_gang of
+ * them, owning several cells each when fewer), an anneal one; CUs left
+ * without any workgroup are shared out by sites to split requests further
+ * (an anneal up to min(max_gang, n / gang_sites) members, a PT request with
+ * one workgroup per cell up to s <= min(max_gang, n / gang_sites, 64 / cells)
+ * per cell); when the base gangs exceed the resident workgroups each request
+ * gets a share by sites instead.  Server defaults: 16, 1024, 2^24 (a gang
+ * barrier per colour class costs more than a few hundred sites of work). */
 int32_t ige_batch_configure(ige_batch *batch, uint32_t max_gang, uint32_t gang_sites,
                             uint32_t spin_limit);
 int32_t ige_batch_synchronize(ige_batch *batch);

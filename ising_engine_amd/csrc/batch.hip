@@ -131,6 +131,7 @@ void plan(ige_batch* b) {
         const auto& L = b->inst[i];
         ige::InstDesc& D = b->desc[i];
         D.gang = b->gang[i];
+        D.split = 1;
         D.bar = b->d_bars.p + i;
         D.ticks = b->d_ticks.p + i;
         D.thr_mode = ige::choose_thr_mode(L.dtab);
@@ -226,6 +227,7 @@ int32_t ige_batch_create(ige_ctx* ctx, uint32_t count, const ige_instance_desc* 
                          ige_batch** out) {
     return guarded([&] {
         require(ctx && desc && out && count >= 1, "bad arguments");
+        require(count < (1u << (32 - ige::kGangBits)), "too many instances in one batch");
         for (uint32_t i = 0; i < count; ++i)
             if (desc[i].num_temps != 0)
                 throw Failure(IGE_ERR_UNSUPPORTED, "a batch runs anneals (num_temps = 0)");
@@ -465,7 +467,7 @@ int32_t ige_batch_configure(ige_batch* b, uint32_t max_gang, uint32_t gang_sites
                             uint32_t spin_limit) {
     return guarded([&] {
         require(b != nullptr, "NULL batch");
-        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 32]");
+        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 64]");
         check(hipSetDevice(b->device), "hipSetDevice");
         check(hipStreamSynchronize(b->stream), "configure");
         b->max_gang = max_gang;

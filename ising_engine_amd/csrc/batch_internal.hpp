@@ -137,7 +137,7 @@ inline LoweredInstance lower_instance(const ige_instance_desc& d, const std::str
 // Everything a persistent launch needs besides the descriptors.
 struct PersistentLaunch {
     const InstDesc* insts = nullptr;
-    const uint32_t* queue = nullptr;  // tickets: instance << 5 | member
+    const uint32_t* queue = nullptr;  // tickets: instance << kGangBits | member
     uint32_t tickets = 0;
     unsigned int* next = nullptr;
     uint32_t* bars = nullptr;         // one counter per instance
@@ -198,7 +198,7 @@ inline void build_tickets(const std::vector<uint32_t>& order, const std::vector<
                           std::vector<uint32_t>& tickets) {
     tickets.clear();
     for (uint32_t id : order)
-        for (uint32_t m = 0; m < gang[id]; ++m) tickets.push_back((id << 5) | m);
+        for (uint32_t m = 0; m < gang[id]; ++m) tickets.push_back((id << kGangBits) | m);
 }
 
 }  // namespace ige

@@ -39,6 +39,7 @@ struct InstDesc {
     uint32_t round;        // sweeps to advance in this launch (0: the launch's default)
     uint32_t thr_mode;     // kThrLds or kThrBand
     uint32_t gang;         // workgroups cooperating on this instance (1..kMaxGang)
+    uint32_t split;        // PT: members per cell (1: each member owns whole cells)
     uint32_t* bar;         // gang barrier counter, zero at launch start (gang > 1)
     uint64_t* ticks;       // if set: wall-clock ticks member 0 spent on this launch's sweeps
     BestRecord* best;      // if set with track_every: the best-seen replica's record
@@ -67,7 +68,8 @@ struct InstDesc {
 constexpr uint32_t kMaxPtCells = 64;  // temperature cells per solve
 
 constexpr uint32_t kLdsThrEntries = 4096;
-constexpr uint32_t kMaxGang = 32;        // queue codes carry the member index in 5 bits
+constexpr uint32_t kMaxGang = 64;        // queue codes carry the member index in 6 bits
+constexpr uint32_t kGangBits = 6;
 constexpr uint32_t kGangSites = 2048;    // sites per workgroup before a gang is formed
 // How a workgroup decides u < thr(beta, dE) for dE <= dtab.
 constexpr uint32_t kThrLds = 0;   // dtab < kLdsThrEntries: per-sweep table in LDS
@@ -327,11 +329,15 @@ __device__ __forceinline__ void pt_track_point(const InstDesc& I, uint32_t sweep
         block_sync_device();
 }
 
-// Sweeps [k0, k1) of a PT solve by one member of its gang.  The member owns
-// cells member, member + gang, ... (cells are independent within a sweep): it
-// sweeps each of them class by class with workgroup barriers, each lane
-// accepting with its own temperature (temp_of, ladder), and adds the cell's
-// energy changes to the tracked energies by device-scope atomics.  After the
+// Sweeps [k0, k1) of a PT solve by one member of its gang.  The gang's
+// members form gang / split owners of split members each; owner o sweeps
+// cells o, o + owners, ... (cells are independent within a sweep) class by
+// class, its split members taking every split-th group of positions of a
+// class, with a workgroup barrier between classes (split 1) or a gang barrier
+// (split > 1: every owner then runs the same number of cell iterations, idle
+// on a missing cell, so that the gang's barriers line up).  Each lane accepts
+// with its own temperature (temp_of, ladder), and each member adds its energy
+// changes to the cell's tracked energies by device-scope atomics.  After the
 // sweep: a barrier across the gang, the exchange (member 0, wave 0), another
 // barrier, and every track_every sweeps the incumbent.  The sweep's uniforms
 // are keyed by (position, cell, sweep), its exchange by the sweep number: a
@@ -354,9 +360,15 @@ __device__ __forceinline__ uint32_t pt_sweeps(const InstDesc& I, uint32_t k0, ui
         }
         __syncthreads();
     }
+    const uint32_t split = I.split > 1 ? I.split : 1u;
+    const uint32_t owners = gang / split, owner = member / split, sub = member % split;
+    const uint32_t per_owner = (C + owners - 1) / owners;
     for (uint32_t k = k0; k < k1; ++k) {
-        for (uint32_t cell = member; cell < C; cell += gang) {
-            const uint32_t tr = I.temp_of[cell * 64 + lane];
+        for (uint32_t j = 0; j < per_owner; ++j) {
+            const uint32_t cell = owner + j * owners;
+            const bool active = cell < C;
+            if (!active && split == 1) break;
+            const uint32_t tr = active ? I.temp_of[cell * 64 + lane] : 0u;
             const double beta = I.ladder[tr];
             const double nbd = -beta * 1.4426950408889634;
             const uint32_t dlim = threshold_limit(beta, I.dtab);
@@ -372,14 +384,18 @@ __device__ __forceinline__ uint32_t pt_sweeps(const InstDesc& I, uint32_t k0, ui
             for (uint32_t c = 0; c < I.ncol; ++c) {
                 const uint32_t cb = __builtin_amdgcn_readfirstlane(I.cs[c]);
                 const uint32_t ce = __builtin_amdgcn_readfirstlane(I.cs[c + 1]);
-                if (ce > cb) {
-                    for (uint32_t g = (cb >> 2) + wave; g <= ((ce - 1) >> 2); g += WAVES) {
+                if (active && ce > cb) {
+                    for (uint32_t g = (cb >> 2) + sub * WAVES + wave; g <= ((ce - 1) >> 2);
+                         g += split * WAVES) {
                         const uint32_t p0 = max(g * 4u, cb), p1 = min(g * 4u + 4u, ce);
                         generic_group(I.rp, I.ci, I.wt, I.lin, S, S, p0, p1, cell, k, I.key0,
                                       I.key1, lane, accept, eacc, nacc);
                     }
                 }
-                __syncthreads();
+                if (split > 1)
+                    gang_barrier(I.bar, bar_target, gang, spin_limit, err);
+                else
+                    __syncthreads();
             }
             if (eacc != 0)
                 atomicAdd(reinterpret_cast<unsigned long long*>(I.energy + cell * 64 + lane),
@@ -413,7 +429,7 @@ __device__ __forceinline__ uint32_t pt_sweeps(const InstDesc& I, uint32_t k0, ui
     return k1;
 }
 
-// queue[t] = instance << 5 | member.  A gang's tickets are consecutive, so at
+// queue[t] = instance << kGangBits | member.  A gang's tickets are consecutive, so at
 // most one gang is incomplete at any time and it is completed by the next
 // workgroups to finish their work: with every workgroup of the launch
 // resident and gang <= grid size, every wait ends.
@@ -433,8 +449,8 @@ __global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
         __syncthreads();
         if (item >= count) break;
         const uint32_t code = queue[item];
-        const InstDesc I = insts[code >> 5];
-        const uint32_t member = code & 31u;
+        const InstDesc I = insts[code >> kGangBits];
+        const uint32_t member = code & (kMaxGang - 1u);
         const uint32_t gang = I.gang;
         uint32_t bar_target = 0;
         const uint32_t mode = I.thr_mode;

@@ -214,7 +214,12 @@ def ultimate_sequence(ctx, r, polish=True):
     return words, es, x, e, sw, flips
 
 
-@pytest.mark.parametrize("gangs", [False, True], ids=["one_workgroup", "gangs"])
+# (max_gang, gang_sites): the defaults (cells owned one each, splits of large
+# requests into idle CUs), fine splits, members owning several cells, one workgroup
+GANG_CONFIGS = {"default": None, "split": (16, 256), "owners": (4, 256), "single": (1, 256)}
+
+
+@pytest.mark.parametrize("gangs", list(GANG_CONFIGS), ids=list(GANG_CONFIGS))
 def test_pt_request_is_the_ultimate_solver_loop(ctx, gangs):
     rng = np.random.default_rng(21)
     shapes = [(10, 25, 3), (10, 7, 4), (2, 13, 2), (1, 10, 3), (5, 1, 9), (16, 20, 2)]
@@ -224,9 +229,15 @@ def test_pt_request_is_the_ultimate_solver_loop(ctx, gangs):
                                       sweeps_per_exchange=spe, sweeps_lo=spe, sweeps_hi=spe)
         r["sweeps"] = spe * rounds
         reqs.append(r)
+    # 2048 sites x 10 cells: split into 6 workgroups per cell (60 members) under
+    # "split", into 2 under the defaults
+    r = instances.serving_request(rng, n_lo=2048, n_hi=2048, num_temps=10,
+                                  sweeps_per_exchange=10, sweeps_lo=10, sweeps_hi=10)
+    r["sweeps"] = 30
+    reqs.append(r)
     srv = _lib.Server(ctx, hbm_bytes=512 << 20, round_ms=1.0)
-    if gangs:
-        srv.configure(max_gang=4, gang_sites=256)
+    if GANG_CONFIGS[gangs]:
+        srv.configure(*GANG_CONFIGS[gangs])
     print(f"[serving pt] gangs={gangs}", flush=True)
     ids = [submit_pt(srv, r, polish=(i % 3 != 1)) for i, r in enumerate(reqs)]
     for i, (r, rid) in enumerate(zip(reqs, ids)):
