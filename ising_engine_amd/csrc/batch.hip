@@ -382,7 +382,7 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
             } end_event{b};
             if (!ige::abi::graphs_enabled()) {
                 set_progress(b, k0);
-                ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+                ige::launch_persistent<true>(w, launch_args(b, k1 - k0), b->stream);
                 return;
             }
             if (!b->graph || !std::equal(key, key + 3, b->graph_key)) {
@@ -396,7 +396,7 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
                 if (recorded) {
                     try {
                         set_progress(b, k0);
-                        ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+                        ige::launch_persistent<true>(w, launch_args(b, k1 - k0), b->stream);
                     } catch (const std::exception&) {
                         recorded = false;
                     }
@@ -409,7 +409,7 @@ int32_t ige_batch_run(ige_batch* b, uint32_t mode, uint32_t waves, uint32_t k0, 
                 if (!exec) {
                     (void)hipGetLastError();
                     set_progress(b, k0);
-                    ige::launch_persistent(w, launch_args(b, k1 - k0), b->stream);
+                    ige::launch_persistent<true>(w, launch_args(b, k1 - k0), b->stream);
                     return;
                 }
                 b->graph = exec;
@@ -433,10 +433,10 @@ int32_t ige_batch_autotune(ige_batch* b, uint32_t probe_sweeps, uint32_t* waves_
         uint32_t best_w = 8;
         for (int i = 0; i < 5; ++i) {
             set_progress(b, 0);
-            ige::launch_persistent(cand[i], launch_args(b, probe_sweeps), b->stream);  // warm
+            ige::launch_persistent<true>(cand[i], launch_args(b, probe_sweeps), b->stream);  // warm
             set_progress(b, 0);
             check(hipEventRecord(e0, b->stream), "event");
-            ige::launch_persistent(cand[i], launch_args(b, probe_sweeps), b->stream);
+            ige::launch_persistent<true>(cand[i], launch_args(b, probe_sweeps), b->stream);
             check(hipEventRecord(e1, b->stream), "event");
             check(hipEventSynchronize(e1), "event");
             float ms = 0.f;

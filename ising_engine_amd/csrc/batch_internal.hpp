@@ -148,11 +148,11 @@ struct PersistentLaunch {
     int num_cus = 1;
 };
 
-template <int WAVES>
+template <int WAVES, bool kPacked>
 void launch_persistent_waves(const PersistentLaunch& L, hipStream_t st) {
     int per_cu = 0;
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<WAVES>,
-                                                       WAVES * 64, 0),
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, persistent_anneal_kernel<WAVES, kPacked>, WAVES * 64, 0),
           "occupancy");
     if (per_cu < 1) throw Failure(IGE_ERR_HIP, "persistent kernel does not fit on a CU");
     // Every workgroup of the launch must be resident (gang waits rely on it).
@@ -162,33 +162,35 @@ void launch_persistent_waves(const PersistentLaunch& L, hipStream_t st) {
     if (L.nbars)
         hipLaunchKernelGGL(fill_u32_kernel, dim3((L.nbars + 255) / 256), dim3(256), 0, st, L.bars,
                            0u, L.nbars);
-    hipLaunchKernelGGL(persistent_anneal_kernel<WAVES>, dim3(blocks), dim3(WAVES * 64), 0, st,
-                       L.insts, L.queue, L.tickets, L.next, L.round, L.spin_limit, L.err);
+    hipLaunchKernelGGL((persistent_anneal_kernel<WAVES, kPacked>), dim3(blocks), dim3(WAVES * 64),
+                       0, st, L.insts, L.queue, L.tickets, L.next, L.round, L.spin_limit, L.err);
     check(hipGetLastError(), "persistent launch");
 }
 
-// Workgroups of persistent_anneal_kernel<waves> resident per CU.
-inline int persistent_per_cu(uint32_t waves) {
+// Workgroups of persistent_anneal_kernel<waves, kPacked> resident per CU.
+template <bool kPacked>
+int persistent_per_cu(uint32_t waves) {
     int per_cu = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (waves) {
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<1>, 64, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<2>, 128, 0); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<4>, 256, 0); break;
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<8>, 512, 0); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<16>, 1024, 0); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<1, kPacked>, 64, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<2, kPacked>, 128, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<4, kPacked>, 256, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<8, kPacked>, 512, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_anneal_kernel<16, kPacked>, 1024, 0); break;
         default: break;
     }
     return e == hipSuccess && per_cu > 0 ? per_cu : 1;
 }
 
-inline void launch_persistent(uint32_t waves, const PersistentLaunch& L, hipStream_t st) {
+template <bool kPacked>
+void launch_persistent(uint32_t waves, const PersistentLaunch& L, hipStream_t st) {
     switch (waves) {
-        case 1: launch_persistent_waves<1>(L, st); break;
-        case 2: launch_persistent_waves<2>(L, st); break;
-        case 4: launch_persistent_waves<4>(L, st); break;
-        case 8: launch_persistent_waves<8>(L, st); break;
-        case 16: launch_persistent_waves<16>(L, st); break;
+        case 1: launch_persistent_waves<1, kPacked>(L, st); break;
+        case 2: launch_persistent_waves<2, kPacked>(L, st); break;
+        case 4: launch_persistent_waves<4, kPacked>(L, st); break;
+        case 8: launch_persistent_waves<8, kPacked>(L, st); break;
+        case 16: launch_persistent_waves<16, kPacked>(L, st); break;
         default: throw Failure(IGE_ERR_INVALID, "waves per instance must be 1, 2, 4, 8 or 16");
     }
 }

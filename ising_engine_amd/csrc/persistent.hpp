@@ -433,8 +433,13 @@ __device__ __forceinline__ uint32_t pt_sweeps(const InstDesc& I, uint32_t k0, ui
 // most one gang is incomplete at any time and it is completed by the next
 // workgroups to finish their work: with every workgroup of the launch
 // resident and gang <= grid size, every wait ends.
-template <int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void persistent_anneal_kernel(
+// kPacked (WAVES >= 4): registers capped for 8 waves per SIMD (4 workgroups
+// of 8 waves per CU instead of 3; SGPRs spill to VGPR lanes outside the
+// entry loops).  The batch runs this form (+7% on the 4096-instance QUBO
+// batch); the serving loop the other, whose PT gangs lost 7% with it.
+template <int WAVES, bool kPacked>
+__global__ __launch_bounds__(WAVES * 64)
+__attribute__((amdgpu_waves_per_eu(kPacked && WAVES >= 4 ? 8 : 1))) void persistent_anneal_kernel(
     const InstDesc* __restrict__ insts, const uint32_t* __restrict__ queue, uint32_t count,
     unsigned int* next, uint32_t round, uint32_t spin_limit, uint32_t* err) {
     __shared__ uint32_t thr_lds[kLdsThrEntries];

@@ -601,7 +601,7 @@ void serve_loop(ige_server* s) {
             pl.round = 1;
             pl.spin_limit = s->spin_limit;
             pl.num_cus = s->num_cus;
-            ige::launch_persistent(s->waves, pl, s->stream);
+            ige::launch_persistent<false>(s->waves, pl, s->stream);
             check(hipEventRecord(e1, s->stream), "event");
             check(hipEventSynchronize(e1), "round");
             {
@@ -639,11 +639,8 @@ void serve_loop(ige_server* s) {
             }
             // refine the per-workgroup rate used to size the next round
             {
-                int per_cu = 0;
-                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, ige::persistent_anneal_kernel<8>, 512, 0);
                 const double blocks = std::max(
-                    1.0, std::min<double>(queue.size(), std::max(1, per_cu) * s->num_cus));
+                    1.0, std::min<double>(queue.size(), std::max(1, s->per_cu) * s->num_cus));
                 if (ms > 0.05)
                     s->attempts_per_ms_per_block =
                         0.5 * s->attempts_per_ms_per_block + 0.5 * (round_attempts / ms / blocks);
@@ -734,7 +731,7 @@ int32_t ige_server_create(ige_ctx* ctx, uint64_t hbm_bytes, double round_ms, uin
         s->arena = ArenaAllocator(hbm_bytes);
         s->waves = waves ? waves : 8;
         s->round_ms = round_ms;
-        s->per_cu = ige::persistent_per_cu(s->waves);
+        s->per_cu = ige::persistent_per_cu<false>(s->waves);
         s->d_next.alloc(1);
         s->d_desc.alloc(256);
         s->d_queue.alloc(256);
