@@ -324,6 +324,22 @@ int32_t ige_pa_parents(const uint32_t *counts, uint32_t count, uint64_t seed, ui
                        uint32_t *parents);
 int32_t ige_pa_family_entropy(const uint32_t *family, uint32_t count, double *rho,
                               uint32_t *families);
+/* adapt_spacings (population_annealing.rs:137-148): out_t = d_t sqrt(max(A_t,
+ * 0.02)) rescaled so that the spacings keep their total (sums in index order);
+ * uniform acceptance is a fixed point. */
+int32_t ige_pa_adapt_spacings(const double *spacings, const double *acceptance, uint32_t count,
+                              double *out);
+/* The acceptance-uniform ladder tuning of the PA-PT solve (ultimate.rs:599-660,
+ * LadderMode::AcceptanceUniform), on a throwaway state whose populations are
+ * set: `iters` times, the ladder T_t = temp_max (temp_min / temp_max)^alpha_t
+ * (t < T = cells / pops, the same for every population), `steps` sweeps with a
+ * DEO exchange each, the swap acceptance of pair (t, t + 1) summed over the
+ * populations, and alpha_1.. = the running sums of ige_pa_adapt_spacings of
+ * the spacings alpha_{t+1} - alpha_t, alpha_{T-1} = 1.  alphas: [T], in (the
+ * starting profile, normally t / (T - 1)) and out.  T < 3, iters == 0 or
+ * steps == 0 leave it unchanged, as the reference does. */
+int32_t ige_pa_tune_ladder(ige_state *state, double temp_max, double temp_min, uint32_t iters,
+                           uint32_t steps, double *alphas);
 
 /* Incumbent: the replica with the lowest tracked energy, ties to the lowest
  * (temperature index, lane), replaces the best-seen state when strictly lower

@@ -50,6 +50,8 @@ EXPORTED = (
     "ige_icm", "ige_set_populations", "ige_resample", "ige_pa_normalize",
     "ige_pa_free_energy_increment", "ige_pa_systematic_resample", "ige_pa_parents",
     "ige_pa_family_entropy",
+    "ige_pa_adapt_spacings",
+    "ige_pa_tune_ladder",
 )
 
 FORMAT_RUDY = 0
@@ -221,6 +223,10 @@ def load() -> ctypes.CDLL:
                                      POINTER(c_uint32)]),
         "ige_pa_family_entropy": (c_int32, [POINTER(c_uint32), c_uint32, POINTER(c_double),
                                             POINTER(c_uint32)]),
+        "ige_pa_adapt_spacings": (c_int32, [POINTER(c_double), POINTER(c_double), c_uint32,
+                                            POINTER(c_double)]),
+        "ige_pa_tune_ladder": (c_int32, [c_void_p, c_double, c_double, c_uint32, c_uint32,
+                                         POINTER(c_double)]),
         "ige_problem_parse": (c_int32, [c_char_p, c_uint64, c_uint32, c_uint32, POINTER(vp)]),
         "ige_problem_destroy": (c_int32, [vp]),
         "ige_problem_get_info": (c_int32, [vp, POINTER(ProblemInfo)]),
@@ -466,6 +472,16 @@ class State:
         check(self._lib.ige_set_populations(self.handle, int(pops)))
         self.pops = int(pops)
 
+    def tune_ladder(self, temp_max: float, temp_min: float, iters: int, steps: int,
+                    alphas=None) -> np.ndarray:
+        """Acceptance-uniform ladder tuning on this (throwaway) state; the tuned alphas."""
+        T = self.cells // getattr(self, "pops", 1)
+        a = (np.array([t / max(T - 1, 1) for t in range(T)]) if alphas is None
+             else np.array(alphas, dtype=np.float64))
+        check(self._lib.ige_pa_tune_ladder(self.handle, float(temp_max), float(temp_min),
+                                           int(iters), int(steps), _ptr(a, c_double)))
+        return a
+
     def resample(self, parents) -> None:
         par = np.ascontiguousarray(parents, dtype=np.uint32)
         check(self._lib.ige_resample(self.handle, _ptr(par, c_uint32)))
@@ -591,6 +607,15 @@ def pa_family_entropy(family):
     check(load().ige_pa_family_entropy(_ptr(f, c_uint32), f.size, ctypes.byref(rho),
                                        ctypes.byref(k)))
     return rho.value, k.value
+
+
+def pa_adapt_spacings(spacings, acceptance) -> np.ndarray:
+    d = np.ascontiguousarray(spacings, dtype=np.float64)
+    a = np.ascontiguousarray(acceptance, dtype=np.float64)
+    out = np.empty_like(d)
+    check(load().ige_pa_adapt_spacings(_ptr(d, c_double), _ptr(a, c_double), d.size,
+                                       _ptr(out, c_double)))
+    return out
 
 
 def ultimate_ladder(temp_max: float, temp_min: float, num_temps: int) -> np.ndarray:

@@ -1049,6 +1049,57 @@ int32_t ige_set_populations(ige_state* s, uint32_t pops) {
     });
 }
 
+// The acceptance-uniform tuning phase of UltimateSolver's PA-PT solve
+// (src/solver/ultimate.rs:599-660, LadderMode::AcceptanceUniform): `iters`
+// rounds of `steps` sweep + exchange steps on this (throwaway) state at
+// T_t = temp_max (temp_min / temp_max)^alpha_t for every population, each
+// followed by ige_pa_adapt_spacings on the per-pair swap acceptance summed over
+// the populations.  alpha_0 stays, alpha_{T-1} is set to 1.
+int32_t ige_pa_tune_ladder(ige_state* s, double temp_max, double temp_min, uint32_t iters,
+                           uint32_t steps, double* alphas) {
+    auto call = [](int32_t rc) {
+        if (rc != IGE_OK) throw Failure(rc, ige::abi::g_last_error);
+    };
+    return guarded([&] {
+        require(s && alphas, "NULL argument");
+        require(!s->replica_view, "ladder tuning works on the slot ladder, not the replica view");
+        require(std::isfinite(temp_max) && std::isfinite(temp_min) && temp_max > 0.0 &&
+                    temp_min > 0.0,
+                "temperatures must be finite and > 0");
+        const uint32_t T = s->cells / s->pops;
+        if (T < 3 || iters == 0 || steps == 0) return;
+        std::vector<double> beta(s->cells), temps(T), spacings(T - 1), rates(T - 1),
+            adapted(T - 1);
+        std::vector<uint64_t> att0(s->cells), acc0(s->cells), att1(s->cells), acc1(s->cells);
+        ige_stats st{};
+        for (uint32_t it = 0; it < iters; ++it) {
+            for (uint32_t t = 0; t < T; ++t) temps[t] = temp_max * std::pow(temp_min / temp_max, alphas[t]);
+            for (uint32_t p = 0; p < s->pops; ++p)
+                for (uint32_t t = 0; t < T; ++t) beta[p * T + t] = 1.0 / temps[t];
+            call(ige_set_ladder(s, beta.data()));
+            call(ige_read_stats(s, &st, att0.data(), acc0.data()));
+            call(ige_run(s, steps, IGE_RUN_EXCHANGE, nullptr));
+            call(ige_read_stats(s, &st, att1.data(), acc1.data()));
+            for (uint32_t t = 0; t + 1 < T; ++t) {
+                uint64_t na = 0, nc = 0;
+                for (uint32_t p = 0; p < s->pops; ++p) {
+                    na += att1[p * T + t] - att0[p * T + t];
+                    nc += acc1[p * T + t] - acc0[p * T + t];
+                }
+                rates[t] = na == 0 ? 0.0 : static_cast<double>(nc) / static_cast<double>(na);
+                spacings[t] = alphas[t + 1] - alphas[t];
+            }
+            call(ige_pa_adapt_spacings(spacings.data(), rates.data(), T - 1, adapted.data()));
+            double acc = 0.0;
+            for (uint32_t t = 0; t + 1 < T; ++t) {
+                acc += adapted[t];
+                alphas[t + 1] = acc;
+            }
+            alphas[T - 1] = 1.0;
+        }
+    });
+}
+
 int32_t ige_resample(ige_state* s, const uint32_t* parents) {
     return guarded([&] {
         require(s && parents, "NULL argument");

@@ -126,4 +126,22 @@ int32_t ige_pa_family_entropy(const uint32_t* family, uint32_t count, double* rh
     });
 }
 
+// adapt_spacings (population_annealing.rs:137-148): d'_t = d_t sqrt(max(A_t,
+// 0.02)), rescaled to the old total span; sums taken in index order.
+int32_t ige_pa_adapt_spacings(const double* spacings, const double* acceptance, uint32_t count,
+                              double* out) {
+    return guarded([&] {
+        require(spacings && acceptance && out && count >= 1, "NULL argument or no spacings");
+        constexpr double kFloor = 0.02;
+        std::vector<double> raw(count);
+        double total_old = 0.0, total_new = 0.0;
+        for (uint32_t t = 0; t < count; ++t) {
+            raw[t] = spacings[t] * std::sqrt(std::fmax(acceptance[t], kFloor));
+            total_old += spacings[t];
+        }
+        for (uint32_t t = 0; t < count; ++t) total_new += raw[t];
+        for (uint32_t t = 0; t < count; ++t) out[t] = raw[t] * total_old / total_new;
+    });
+}
+
 }  // extern "C"

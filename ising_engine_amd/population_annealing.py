@@ -3,8 +3,9 @@ UltimateSolver::solve_population_annealing (src/solver/ultimate.rs:559-852).
 
 R = num_pops x 64 members; member m = p*64 + r is lane r of population p across that
 population's ladder of num_temps levels (ige_set_populations).  Stage k's ladder spans
-[T_cold(k), temp_max] geometrically, T_t = temp_max (T_cold / temp_max)^(t / (nt - 1)),
-level nt - 1 the coldest, and T_cold anneals from temp_max to temp_min across the stages.
+[T_cold(k), temp_max], T_t = temp_max (T_cold / temp_max)^alpha_t (alpha_t = t / (nt - 1)
+before tuning), level nt - 1 the coldest, and T_cold anneals from temp_max to temp_min
+across the stages.
 After each stage but the first, member m gains log-weight -(beta_cold - beta_cold_prev) E_m
 with E_m its coldest-level energy; when the effective sample size falls below
 ess_threshold x R the population is systematically resampled (one offset u, a shuffled
@@ -13,11 +14,18 @@ and post_resample_relaxation PT steps follow.  Each stage then runs sweeps_per_s
 steps (a sweep and a DEO exchange inside every population), optionally an ICM sweep, and
 an incumbent update; the best state seen is 1-opt polished at the end.
 
-The library does the sweeps, exchanges, resampling copies, cluster moves, incumbent and
-polish; this module holds the stage loop and the PA bookkeeping the reference keeps on the
-host.  The ladder is the uniform geometric profile (the reference's optional
-acceptance-tuned profile is not built), and the offset u of each resampling comes from a
-numpy PCG64 generator seeded with the solve's seed.
+Before the stages, the ladder shape is tuned as LadderMode::AcceptanceUniform does it
+(ultimate.rs:599-660, ige_pa_tune_ladder): on a throwaway state at the full span
+[temp_min, temp_max], ladder_tuning_iters rounds of ladder_tuning_steps PT steps, each
+followed by an acceptance-uniformizing update of the log-temperature positions alpha_t;
+the stage ladders then use the tuned profile, T_t = temp_max (T_cold / temp_max)^alpha_t.
+The throwaway state's seed is the solve's seed xor TUNE_SEED_XOR (the reference draws both
+fields from one generator stream).
+
+The library does the sweeps, exchanges, resampling copies, cluster moves, incumbent,
+polish and the tuning loop; this module holds the stage loop and the PA bookkeeping the
+reference keeps on the host.  The offset u of each resampling comes from a numpy PCG64
+generator seeded with the solve's seed.
 """
 from __future__ import annotations
 
@@ -26,6 +34,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
+
+TUNE_SEED_XOR = 0x74756E65 << 32
 
 
 def powi(a: float, b: int) -> float:
@@ -42,17 +52,24 @@ def powi(a: float, b: int) -> float:
     return 1.0 / r if recip else r
 
 
-def stage_temperatures(temp_max: float, temp_min: float, num_temps: int, stages: int):
-    """[(t_cold, temps[num_temps])] per stage, temps[0] = temp_max, temps[-1] = t_cold
-    (ultimate.rs:694-721 with the uniform profile alpha_t = t / (nt - 1))."""
+def uniform_alphas(num_temps: int) -> np.ndarray:
+    """alpha_t = t / max(nt - 1, 1) (ultimate.rs:604-605): the geometric profile; one
+    temperature gets alpha 0, i.e. temp_max, as in the reference."""
+    return np.array([t / max(num_temps - 1, 1) for t in range(num_temps)])
+
+
+def stage_temperatures(temp_max: float, temp_min: float, num_temps: int, stages: int,
+                       alphas=None):
+    """[(t_cold, temps[num_temps])] per stage, temps[t] = temp_max (t_cold /
+    temp_max)^alpha_t (ultimate.rs:708-721), uniform alphas unless given."""
     k_stages = max(stages, 1)
     cold_factor = (temp_min / temp_max) ** (1.0 / (k_stages - 1)) if k_stages > 1 else 1.0
-    alphas = ([t / (num_temps - 1) for t in range(num_temps)] if num_temps > 1 else [1.0])
+    alphas = uniform_alphas(num_temps) if alphas is None else alphas
     out = []
     for k in range(k_stages):
         t_cold = temp_max * powi(cold_factor, k) if k_stages > 1 else temp_min
         span = t_cold / temp_max
-        out.append((t_cold, np.array([temp_max * span ** a for a in alphas])))
+        out.append((t_cold, np.array([temp_max * span ** float(a) for a in alphas])))
     return out
 
 
@@ -66,6 +83,7 @@ class PaResult:
     resample_events: int = 0
     family: np.ndarray | None = None
     state: object = None  # the device state after the last stage
+    alphas: np.ndarray | None = None  # the ladder profile the stages used
 
 
 def coldest_energies(st: _lib.State, num_pops: int, num_temps: int) -> np.ndarray:
@@ -84,17 +102,25 @@ def solve(ctx: _lib.Context, model: _lib.Model, seed: int, num_pops: int = 4,
           num_temps: int = 8, stages: int = 50, sweeps_per_stage: int = 10,
           temp_max: float = 10.0, temp_min: float = 0.1, ess_threshold: float = 0.5,
           post_resample_relaxation: int = 5, use_icm: bool = False,
-          polish: bool = True) -> PaResult:
+          polish: bool = True, ladder_tuning_iters: int = 8,
+          ladder_tuning_steps: int = 10) -> PaResult:
     P, T = num_pops, num_temps
     R = P * 64
+    alphas = uniform_alphas(T)
+    if T >= 3 and ladder_tuning_iters > 0 and ladder_tuning_steps > 0:
+        tune = _lib.State(model, P * T, seed ^ TUNE_SEED_XOR)
+        tune.set_populations(P)
+        alphas = tune.tune_ladder(temp_max, temp_min, ladder_tuning_iters,
+                                  ladder_tuning_steps, alphas)
+        tune.close()
     st = _lib.State(model, P * T, seed)
     st.set_populations(P)
     rng = np.random.Generator(np.random.PCG64(seed))
     log_w = np.zeros(R)
     family = np.arange(R, dtype=np.uint32)
-    res = PaResult(x=np.zeros(model.n, np.uint8), energy=0)
+    res = PaResult(x=np.zeros(model.n, np.uint8), energy=0, alphas=alphas)
     prev = None
-    for t_cold, temps in stage_temperatures(temp_max, temp_min, T, stages):
+    for t_cold, temps in stage_temperatures(temp_max, temp_min, T, stages, alphas):
         st.set_ladder(np.tile(1.0 / temps, P))
         beta_cold = 1.0 / t_cold
         if prev is not None:

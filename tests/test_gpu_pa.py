@@ -45,7 +45,13 @@ def test_population_exchange_and_resample_parity(ctx):
         st.set_populations(4)
 
 
-def oracle_pa(csr, seed, P, T, stages, sweeps, tmax, tmin, thr, relax, use_icm):
+def oracle_pa(csr, seed, P, T, stages, sweeps, tmax, tmin, thr, relax, use_icm,
+              tune_iters=0, tune_steps=0):
+    alphas = pa_drv.uniform_alphas(T)
+    if T >= 3 and tune_iters and tune_steps:
+        tune = ob.OracleState(csr, P * T, seed ^ pa_drv.TUNE_SEED_XOR)
+        tune.pops = P
+        alphas = pa.tune_ladder(tune, tmax, tmin, tune_iters, tune_steps, alphas)
     st = ob.OracleState(csr, P * T, seed)
     st.pops = P
     R = P * 64
@@ -54,7 +60,7 @@ def oracle_pa(csr, seed, P, T, stages, sweeps, tmax, tmin, thr, relax, use_icm):
     family = np.arange(R, dtype=np.uint32)
     inc = ob.Incumbent(csr.n)
     ess_hist, events, prev = [], 0, None
-    for t_cold, temps in pa_drv.stage_temperatures(tmax, tmin, T, stages):
+    for t_cold, temps in pa_drv.stage_temperatures(tmax, tmin, T, stages, alphas):
         ladder = np.tile(1.0 / temps, P)
         beta_cold = 1.0 / t_cold
         if prev is not None:
@@ -82,21 +88,26 @@ def oracle_pa(csr, seed, P, T, stages, sweeps, tmax, tmin, thr, relax, use_icm):
         if use_icm:
             st.icm(1)
         inc.update(st.words, st.temp_of, st.energy)
-    return st, family, ess_hist, events, inc
+    return st, family, ess_hist, events, inc, alphas
 
 
-@pytest.mark.parametrize("use_icm", [False, True], ids=["pa_pt", "pa_pt_icm"])
-def test_pa_pt_solve_matches_oracle(ctx, use_icm):
-    print(f"[pa] icm={use_icm}", flush=True)
+@pytest.mark.parametrize("use_icm,tune", [(False, 0), (True, 0), (False, 3)],
+                         ids=["pa_pt", "pa_pt_icm", "pa_pt_tuned"])
+def test_pa_pt_solve_matches_oracle(ctx, use_icm, tune):
+    print(f"[pa] icm={use_icm} tune={tune}", flush=True)
     qb = instances.random_qubo(90, 4, 82, offdiag=20, diag=30)
     csr = ob.CSR.from_qubo(qb)
     model = make(ctx, qb)
     t_max, t_min = instances.suggested_temp_range(qb)
     args = dict(num_pops=2, num_temps=4, stages=10, sweeps_per_stage=3, temp_max=t_max,
-                temp_min=t_min, ess_threshold=0.9, post_resample_relaxation=2, use_icm=use_icm)
+                temp_min=t_min, ess_threshold=0.9, post_resample_relaxation=2, use_icm=use_icm,
+                ladder_tuning_iters=tune, ladder_tuning_steps=5)
     res = pa_drv.solve(ctx, model, 4321, **args)
-    st, family, ess_hist, events, inc = oracle_pa(csr, 4321, 2, 4, 10, 3, t_max, t_min, 0.9, 2,
-                                                  use_icm)
+    st, family, ess_hist, events, inc, alphas = oracle_pa(
+        csr, 4321, 2, 4, 10, 3, t_max, t_min, 0.9, 2, use_icm, tune, 5)
+    np.testing.assert_array_equal(res.alphas, alphas)
+    if tune:
+        assert not np.array_equal(alphas, pa_drv.uniform_alphas(4))
     assert res.resample_events == events > 0
     np.testing.assert_allclose(res.ess_history, ess_hist, rtol=1e-12)
     np.testing.assert_array_equal(res.family, family)
@@ -107,3 +118,50 @@ def test_pa_pt_solve_matches_oracle(ctx, use_icm):
     csr.polish_1opt(xo)
     np.testing.assert_array_equal(res.x, xo)
     assert res.energy == csr.energy(xo) <= inc.energy.value
+
+
+def test_ladder_tuning_matches_the_oracle(ctx):
+    # ige_pa_tune_ladder (ultimate.rs:599-660) against the oracle's loop: the same
+    # per-pair swap counts every round, so the same alphas bit for bit, and the same state
+    qb = instances.random_qubo(150, 6, 83, offdiag=30, diag=40)
+    csr = ob.CSR.from_qubo(qb)
+    t_max, t_min = instances.suggested_temp_range(qb)
+    P, T = 3, 6
+    st = _lib.State(make(ctx, qb), P * T, 99, kernel=_lib.KERNEL_GENERIC)
+    st.set_populations(P)
+    ost = ob.OracleState(csr, P * T, 99)
+    ost.pops = P
+    got = st.tune_ladder(t_max, t_min, 8, 10)
+    want = pa.tune_ladder(ost, t_max, t_min, 8, 10)
+    np.testing.assert_array_equal(got, want)
+    assert got[0] == 0.0 and got[-1] == 1.0 and np.all(np.diff(got) > 0)
+    np.testing.assert_array_equal(st.spins(), ost.words)
+    np.testing.assert_array_equal(st.temp_of(), ost.temp_of)
+    # fewer than 3 temperatures: unchanged, as in the reference
+    st2 = _lib.State(make(ctx, qb), 4, 5)
+    st2.set_populations(2)
+    np.testing.assert_array_equal(st2.tune_ladder(t_max, t_min, 8, 10), [0.0, 1.0])
+
+
+def test_device_swap_rates_on_the_two_level_system(ctx):
+    # one spin, E = h x: the device's per-pair DEO acceptance against the equilibrium closed
+    # form, and tuning rounds that track the closed-form iteration
+    h = 2
+    qb = instances.Qubo(n=1, offset=0, linear=np.array([h]), pi=np.zeros(0, np.uint32),
+                        pj=np.zeros(0, np.uint32), q=np.zeros(0, np.int64), name="two_level")
+    betas = 1.0 / np.array([4.0, 2.5, 1.6, 1.0, 0.6])
+    st = _lib.State(make(ctx, qb), 5, 17, kernel=_lib.KERNEL_GENERIC)
+    st.set_ladder(betas)
+    st.run(200, exchange=True)
+    _, a0, c0 = st.stats()
+    st.run(4000, exchange=True)
+    _, a1, c1 = st.stats()
+    rates = (c1.astype(np.int64) - c0) / (a1.astype(np.int64) - a0)
+    exact = pa.two_level_exchange_rates(h, betas)
+    assert np.all(np.abs(rates - exact) < 0.012), (rates, exact)
+    tune = _lib.State(make(ctx, qb), 20, 23, kernel=_lib.KERNEL_GENERIC)
+    tune.set_populations(4)
+    alphas = tune.tune_ladder(20.0, 0.1, 12, 200)
+    from test_pa import exact_tuning
+    closed, _ = exact_tuning(float(h), 20.0, 0.1, 5, 12)
+    assert np.max(np.abs(alphas - closed)) < 0.006, (alphas, closed)

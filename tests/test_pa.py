@@ -98,3 +98,96 @@ def test_resample_copies_whole_ladders():
             assert st.energy[d, r] == before_e[s, rs]
     np.testing.assert_array_equal(st.energy, csr.energies(st.words))
     np.testing.assert_array_equal(st.temp_of, tof)
+
+
+# ---- acceptance-uniform ladder tuning (population_annealing.rs:137-148,
+# ultimate.rs:599-660) ----
+
+def test_adapt_spacings_reference_cases():
+    # tests/test_population_annealing.rs:324-347, on the oracle and on the C ABI
+    for f in (pa.adapt_spacings, _lib.pa_adapt_spacings):
+        sp = [0.1, 0.3, 0.2, 0.4]
+        np.testing.assert_allclose(f(sp, [0.3] * 4), sp, rtol=0, atol=1e-12)
+        out = f([0.5, 0.5], [0.9, 0.1])
+        assert out[0] > 0.5 and out[1] < 0.5
+        assert abs(sum(out) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("seed", [4, 5, 6])
+def test_adapt_spacings_abi_equals_the_oracle(seed):
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(2, 40))
+    sp = rng.uniform(0.001, 0.3, size=k)
+    acc = rng.uniform(0.0, 1.0, size=k)
+    acc[rng.integers(0, k)] = 0.0           # below the 0.02 floor
+    got = _lib.pa_adapt_spacings(sp, acc)
+    np.testing.assert_array_equal(got, pa.adapt_spacings(sp, acc))
+    # closed form: d sqrt(max(a, 0.02)) times the span ratio
+    raw = sp * np.sqrt(np.maximum(acc, 0.02))
+    np.testing.assert_allclose(got, raw * sp.sum() / raw.sum(), rtol=1e-12)
+
+
+def two_level_csr(h):
+    return ob.CSR(1, 0, [h], [], [], [])
+
+
+def test_two_level_swap_rates_match_the_closed_form():
+    # one spin, E = h x, 5 temperatures, 64 lanes: the oracle's per-pair DEO acceptance over
+    # 4000 steps against the equilibrium closed form
+    h = 2
+    betas = 1.0 / np.array([4.0, 2.5, 1.6, 1.0, 0.6])
+    st = ob.OracleState(two_level_csr(h), 5, 17)
+    st.run(200, exchange=True, ladder=betas)           # equilibrate
+    before = st.stats.copy()
+    st.run(4000, exchange=True, ladder=betas)
+    rates = pa.pair_rates(before, st.stats, 1, 5)
+    exact = pa.two_level_exchange_rates(h, betas)
+    assert np.all(np.abs(rates - exact) < 0.012), (rates, exact)
+    # and a planted sign mistake in the closed form would not pass
+    wrong = 1.0 - (1.0 - np.exp(-betas * h) / (1 + np.exp(-betas * h)))[:-1] * \
+        (np.exp(-betas * h) / (1 + np.exp(-betas * h)))[1:] * \
+        (1.0 - np.exp(-(betas[1:] - betas[:-1]) * h))
+    assert np.max(np.abs(rates - wrong)) > 0.02
+
+
+def exact_tuning(h, t_max, t_min, T, iters, transpose=False):
+    """adapt_spacings iterated on the two-level closed-form rates; transpose plants an
+    index mistake (the rates of pair t applied to spacing T - 2 - t)."""
+    alphas = np.array([t / (T - 1) for t in range(T)])
+    for _ in range(iters):
+        betas = 1.0 / pa.ladder_temperatures(t_max, t_min, alphas)
+        rates = pa.two_level_exchange_rates(h, betas)
+        if transpose:
+            rates = rates[::-1]
+        new = pa.adapt_spacings(np.diff(alphas), rates)
+        alphas[1:] = np.cumsum(new)
+        alphas[-1] = 1.0
+    return alphas, pa.two_level_exchange_rates(h, 1.0 / pa.ladder_temperatures(t_max, t_min,
+                                                                                  alphas))
+
+
+def test_tuning_fixed_point_is_uniform_acceptance():
+    # the geometric ladder's acceptances on the two-level system are far from uniform; the
+    # iteration drives them to one value, and an index mistake does not
+    h, t_max, t_min, T = 2.0, 20.0, 0.1, 5
+    _, r0 = exact_tuning(h, t_max, t_min, T, 0)
+    alphas, r = exact_tuning(h, t_max, t_min, T, 60)
+    assert np.ptp(r0) > 0.15
+    assert np.ptp(r) < 3e-3
+    assert np.all(np.diff(alphas) > 0) and alphas[0] == 0.0 and alphas[-1] == 1.0
+    _, rbad = exact_tuning(h, t_max, t_min, T, 60, transpose=True)
+    assert np.ptp(rbad) > 0.05
+
+
+def test_oracle_tuning_on_the_two_level_system_follows_the_closed_form():
+    # the oracle's stochastic tuning (4 populations, 200 steps per round) tracks the same
+    # 12 iterations run on the closed-form rates, and shrinks the acceptance spread as the
+    # reference's tests/test_population_annealing.rs:430-490 requires
+    h, t_max, t_min, T, P = 2, 20.0, 0.1, 5, 4
+    st = ob.OracleState(two_level_csr(h), P * T, 23)
+    st.pops = P
+    alphas = pa.tune_ladder(st, t_max, t_min, iters=12, steps=200)
+    closed, r = exact_tuning(float(h), t_max, t_min, T, 12)
+    assert np.max(np.abs(alphas - closed)) < 0.006, (alphas, closed)
+    _, r0 = exact_tuning(float(h), t_max, t_min, T, 0)
+    assert np.ptp(r) < 0.5 * np.ptp(r0)
