@@ -5,12 +5,15 @@ acceptance within 99% intervals.
 Native order: the port of src/solver/engine.rs::step (oracle/cpu_baseline.c): every cell
 swept in ascending site order with per-cell Xoshiro streams and fast_exp acceptance, then a
 DEO exchange that swaps spins.  Device: the multispin kernel in colour order with Philox
-plane uniforms and temperature-index exchange.  Both run K independent seeds of a 2D +-J
-L=16 lattice on 8 temperatures geometric in T 0.5 -> 2.0 (the 3D workload's range), 2000
-burn-in steps and 4000 measured steps; each run contributes one mean per quantity.  A
-quantity passes when |mean_gpu - mean_cpu| <= t * sqrt(se_gpu^2 + se_cpu^2) with t the
-two-sided 99% Student quantile, Bonferroni-adjusted over the 15 quantities so that the whole
-check holds at 99% when the two laws agree.
+plane uniforms and temperature-index exchange.  Both run K = 12 independent seeds of
+- a 2D +-J L=16 lattice on 8 temperatures geometric in T 0.5 -> 2.0 (the 3D workload's
+  range), 2000 burn-in and 4000 measured steps;
+- a 3D +-J L=6 lattice on 16 temperatures geometric in T 0.5 -> 2.0, 1000 burn-in and
+  2000 measured steps;
+each run contributes one mean per quantity.  A quantity passes when |mean_gpu - mean_cpu|
+<= t * sqrt(se_gpu^2 + se_cpu^2) with t the two-sided 99% Student quantile (df = K - 1),
+Bonferroni-adjusted over the case's 2T - 1 quantities so that each case's check holds at
+99% when the two laws agree.
 """
 import ctypes
 import os
@@ -25,11 +28,16 @@ from oracle import binding as ob
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-K, BURN, MEASURE, EVERY = 6, 2000, 4000, 40
-TEMPS = np.geomspace(0.5, 2.0, 8)
+K, EVERY = 12, 40
+CASES = {
+    "2d_L16_8T": dict(L=16, dim=2, seed=11, temps=np.geomspace(0.5, 2.0, 8), burn=2000,
+                      measure=4000),
+    "3d_L6_16T": dict(L=6, dim=3, seed=12, temps=np.geomspace(0.5, 2.0, 16), burn=1000,
+                      measure=2000),
+}
 
 
-def cpu_run(lib, csr, seed):
+def cpu_run(lib, csr, seed, TEMPS, BURN, MEASURE):
     cells, n = len(TEMPS), csr.n
     rp, ci = csr.row_ptr, csr.col_idx
     w, lin = csr.w.astype(np.float64), csr.linear.astype(np.float64)
@@ -59,7 +67,7 @@ def cpu_run(lib, csr, seed):
     return np.mean(means, axis=0), acc
 
 
-def gpu_run(ctx, model, seed):
+def gpu_run(ctx, model, seed, TEMPS, BURN, MEASURE):
     cells = len(TEMPS)
     st = _lib.State(model, cells, seed)
     assert st.stats()[0].kernel == _lib.KERNEL_MULTISPIN
@@ -79,15 +87,20 @@ def gpu_run(ctx, model, seed):
     return np.mean(means, axis=0), (c1 - c0) / (a1 - a0)
 
 
-def test_pt_matches_native_sequential_order_statistically(ctx):
+@pytest.mark.parametrize("case", list(CASES), ids=list(CASES))
+def test_pt_matches_native_sequential_order_statistically(ctx, case):
+    cfg = CASES[case]
+    TEMPS = cfg["temps"]
     path = os.path.join(ROOT, "oracle", "build", "libcpu_baseline.so")
     lib = ctypes.CDLL(path)
     lib.cb_run_stats.restype = c_int
-    qb = instances.hypercubic_pm_j(16, 2, 11)
+    qb = instances.hypercubic_pm_j(cfg["L"], cfg["dim"], cfg["seed"])
     csr = ob.CSR.from_qubo(qb)
     model = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
-    g = [gpu_run(ctx, model, 1000 + k) for k in range(K)]
-    c = [cpu_run(lib, csr, 2000 + k) for k in range(K)]
+    print(f"[native-order] {case}: device runs", flush=True)
+    g = [gpu_run(ctx, model, 1000 + k, TEMPS, cfg["burn"], cfg["measure"]) for k in range(K)]
+    print(f"[native-order] {case}: native-order port runs", flush=True)
+    c = [cpu_run(lib, csr, 2000 + k, TEMPS, cfg["burn"], cfg["measure"]) for k in range(K)]
     ge, ga = np.array([x[0] for x in g]), np.array([x[1] for x in g])
     ce, ca = np.array([x[0] for x in c]), np.array([x[1] for x in c])
     q = 2 * len(TEMPS) - 1
