@@ -140,6 +140,10 @@ typedef struct {
     uint64_t resident_sweeps; /* sweeps run with the slot's words in LDS for
                                  the whole call (small regular lattices,
                                  no exchange)                               */
+    uint64_t chain_sweeps;    /* sweeps run by one launch over every colour
+                                 class of the call (one-slot multispin states
+                                 of irregular degree, no exchange,
+                                 incumbent, heat bath or ICM)               */
 } ige_stats;
 
 /* ---- context ---- */

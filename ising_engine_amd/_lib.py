@@ -84,6 +84,7 @@ class Stats(ctypes.Structure):
         ("proposed", c_uint64),
         ("phase", c_uint32), ("kernel", c_uint32),
         ("graph_sweeps", c_uint64), ("graph_fallbacks", c_uint64), ("resident_sweeps", c_uint64),
+        ("chain_sweeps", c_uint64),
     ]
 
 

@@ -49,6 +49,7 @@ struct ige_model {
     // Multispin sweeps run classes [fuse_from, num_colors) in one launch (each
     // at most kFuseMaxClassSites sites, at least two classes); num_colors: none.
     uint32_t fuse_from = 0;
+    uint32_t max_class = 0;   // positions in the largest class
     DevBuf<uint32_t> rp, ci, order, pos_of, sgn, cstart;  // cstart: colour class bounds
     DevBuf<int32_t> wt, lin;
     // Dense int8 matrix, built on the first dense state of this model.
@@ -161,7 +162,9 @@ struct ige_state {
     uint64_t graph_fallbacks = 0;  // captures that failed (those calls ran eagerly)
     uint64_t graph_sweeps = 0;     // sweeps run by graph replays
     uint64_t resident_sweeps = 0;  // sweeps run by resident (LDS) runs
+    uint64_t chain_sweeps = 0;     // sweeps run by chained launches
     DevBuf<uint32_t> lev_rows;     // per-sweep level thresholds of a resident run
+    DevBuf<uint32_t> chain_sync;   // chained runs: error flag, then a barrier counter per slot
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -245,6 +248,9 @@ void finish_model(ige_model* m) {
         uint32_t c = m->co.num_colors;
         while (c > 0 && cs[c] - cs[c - 1] <= ige::kFuseMaxClassSites) --c;
         m->fuse_from = m->co.num_colors - c >= 2 ? c : m->co.num_colors;
+        m->max_class = 0;
+        for (uint32_t k = 0; k < m->co.num_colors; ++k)
+            m->max_class = std::max(m->max_class, cs[k + 1] - cs[k]);
     }
     m->ms = ige::multispin_shape(m->ir);
     m->ds = ige::dense_shape(m->ir);
@@ -333,6 +339,26 @@ struct StepPlan {
 
 // Steps per replay of the captured sweep loop.
 constexpr uint32_t kGraphSteps = 32;
+// Polls of one device-wide barrier wait of a chained run before it gives up
+// (its workgroups are all resident, so a wait ends long before).
+constexpr uint32_t kChainSpinLimit = 1u << 24;
+
+// Raises the error a chained run's expired barrier wait left (and clears it).
+void check_chain_error(ige_state* s) {
+    if (!s->chain_sync.p) return;
+    uint32_t e = 0;
+    check(hipMemcpyAsync(&e, s->chain_sync.p, sizeof(e), hipMemcpyDeviceToHost, s->stream),
+          "chain error flag");
+    check(hipStreamSynchronize(s->stream), "chain error flag");
+    if (e) {
+        const uint32_t zero = 0;
+        check(hipMemcpyAsync(s->chain_sync.p, &zero, sizeof(zero), hipMemcpyHostToDevice,
+                             s->stream),
+              "chain error flag");
+        check(hipStreamSynchronize(s->stream), "chain error flag");
+        throw Failure(IGE_ERR_HIP, "a chained sweep's barrier wait exceeded its spin limit");
+    }
+}
 
 // One sweep step on s->stream: the thresholds (when `refresh`), the colour
 // sweeps, the exchange, an ICM sweep and the incumbent update.  sweep and
@@ -757,6 +783,47 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             s->resident_sweeps += sweeps;
             done = sweeps;
         }
+        // One-slot multispin states of irregular degree without exchange,
+        // incumbent, heat bath or ICM: every class of every sweep in one
+        // launch, workgroups meeting at a device-wide barrier per class (small
+        // regular lattices run resident above; with more slots the barriers'
+        // cache invalidations cost more than the launches they save).
+        const bool chain = !resident && !(flags & IGE_RUN_EAGER) && !s->profile && !P.exch &&
+                           !P.track && !P.heat && !P.icm && !s->replica_view && sweeps >= 2 &&
+                           s->cells == 1 && s->kernel == IGE_KERNEL_MULTISPIN &&
+                           s->model->regular_degree == 0 && s->model->dmax < 32 &&
+                           s->model->max_class <= ige::kChainMaxSites &&
+                           ige::chain_ms_capacity_blocks(s->model->dmax) >= 1;
+        if (chain) {
+            // [0]: error flag, [1 + slot]: barrier counters
+            if (s->chain_sync.count < s->cells + 1u) {
+                s->chain_sync.alloc(s->cells + 1u);
+                const std::vector<uint32_t> zero(s->cells + 1u, 0u);
+                s->chain_sync.upload(zero.data(), zero.size(), s->stream);
+            }
+            const auto m = s->model->view();
+            const uint32_t lrow = s->cells * (s->model->dmax + 1);
+            const uint32_t* lev = s->lev_thr.p;
+            uint32_t stride = 0;
+            if (sched) {
+                const size_t need = static_cast<size_t>(sweeps) * lrow;
+                if (s->lev_rows.count < need) s->lev_rows.alloc(need);
+                check(ige::launch_level_rows(m, s->cells, s->sched_buf.p, sweeps, s->lev_rows.p,
+                                             s->stream),
+                      "level rows");
+                lev = s->lev_rows.p;
+                stride = lrow;
+            }
+            check(ige::launch_sweep_chain_ms(m, s->view(), s->model->cstart.p,
+                                             s->model->co.num_colors, s->model->max_class, lev,
+                                             stride, s->sweep, sweeps, s->seed,
+                                             s->chain_sync.p + 1, s->chain_sync.p,
+                                             kChainSpinLimit, s->stream),
+                  "chained multispin sweeps");
+            s->sweep += sweeps;
+            s->chain_sweeps += sweeps;
+            done = sweeps;
+        }
         // Graph replays next: kGraphSteps steps per launch.
         const bool graphable = ige::abi::graphs_enabled() && !s->profile && !P.icm &&
                                !(flags & IGE_RUN_EAGER) &&
@@ -799,6 +866,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             }
             // the caller may free `sched` once we return
             check(hipStreamSynchronize(s->stream), "run");
+            if (chain) check_chain_error(s);
             if (sweeps > 0) refresh_thresholds(s, s->beta.p);
         }
     });
@@ -864,6 +932,7 @@ int32_t ige_synchronize(ige_state* s) {
         require(s != nullptr, "NULL state");
         set_device(s->ctx);
         check(hipStreamSynchronize(s->stream), "synchronize");
+        check_chain_error(s);
     });
 }
 
@@ -962,6 +1031,7 @@ int32_t ige_read_stats(ige_state* s, ige_stats* stats, uint64_t* pair_attempts,
         stats->graph_sweeps = s->graph_sweeps;
         stats->graph_fallbacks = s->graph_fallbacks;
         stats->resident_sweeps = s->resident_sweeps;
+        stats->chain_sweeps = s->chain_sweeps;
         for (uint32_t t = 0; t + 1 < s->temps(); ++t) {
             if (pair_attempts) pair_attempts[t] = c[1 + 2 * t];
             if (pair_accepts) pair_accepts[t] = c[2 + 2 * t];

@@ -1,0 +1,201 @@
+// chain.hip -- many Metropolis sweeps of a one-slot multispin state of
+// irregular degree in one launch: every colour class of every sweep, the
+// launch's workgroups meeting at a device-wide barrier after each class.
+//
+// For graphs whose classes are too large for one workgroup but too small to
+// fill the GPU in a launch each (ER N=20000: 7 classes of about 2900 sites),
+// the per-class launches cost more than the class itself.  Here the launch's
+// waves keep a fixed share of every class (wave w takes positions cb + 64 w +
+// lane, + 64 W, ...), so a class costs one site update per lane plus one
+// barrier.  The updates are those of launch_sweep_multispin (the same
+// irregular_site, uniforms keyed by position, slot and sweep, thresholds and
+// tie resolution), so the chain is bit-exact against the per-class launches.
+//
+// Measured on MI355X (ER N=20000, degree 6, 7 classes): 69 us a sweep against
+// 91 us for the captured per-class launches.  Each barrier's device-scope
+// acquire invalidates the caches of every workgroup taking part, so the chain
+// pays only with few workgroups: with 16 slots (192 workgroups) it took
+// 369 us a sweep against 107 us for the launches, and a lane-per-replica
+// form with one group of four positions per wave (182 workgroups) 181 us.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+
+#include "gang_sync.h"
+#include "ige_math.h"
+#include "kernels.hpp"
+#include "multispin_ops.h"
+
+namespace ige {
+namespace {
+
+constexpr uint32_t kChainBlock = 256;  // 4 waves
+constexpr uint32_t kChainWaves = kChainBlock / 64;
+}  // namespace
+
+constexpr uint32_t kChainMaxBlocks = kChainMaxSites / (64 * 4);  // workgroups per slot
+
+namespace {
+// Bit-sliced counts and plane-view uniforms as sweep_multispin_kernel:
+// blockIdx.y = slot, gridDim.x workgroups per slot meeting at the slot's own
+// barrier bar[slot] after each class.  Wave w of a slot takes positions
+// cb + 64 w + lane, + 64 W, ... of every class (W waves per slot), queueing
+// its tied replicas (entries relative to the class start) and draining them
+// before the barrier, as classes_sweep_kernel does in one workgroup.  Sweep k
+// reads its level thresholds from lev + k * lev_stride.
+constexpr uint32_t kChainQueueCap = 1024;  // tied replicas per wave and class
+
+template <int B, int W>
+__global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
+    const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+    const uint32_t* __restrict__ sgn, const uint32_t* __restrict__ cstart, uint32_t ncol,
+    uint64_t* spins, uint32_t n, uint32_t dmax, int32_t w0, const uint32_t* __restrict__ lev,
+    uint32_t lev_stride, const uint32_t* __restrict__ temp_of, int64_t* energy,
+    unsigned long long* accepted, uint32_t sweep0, uint32_t sweeps, uint32_t key0,
+    uint32_t key1, uint32_t* bar, uint32_t spin_limit, uint32_t* err) {
+    using namespace msc;
+    static_assert(W >= B + 3, "counters too narrow");
+    constexpr uint32_t kFlush = 8;  // sites per lane between counter flushes
+    extern __shared__ unsigned long long recs[];  // (dmax + 1) * kQStride
+    __shared__ uint32_t queue[kChainWaves][kChainQueueCap];
+    __shared__ uint32_t qcount[kChainWaves];
+    __shared__ unsigned long long edef[kChainWaves][64];
+    __shared__ unsigned long long blk_e[64];
+    __shared__ uint32_t blk_acc;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t slot = blockIdx.y;
+    const uint32_t gang = gridDim.x;
+    const uint32_t gw = blockIdx.x * kChainWaves + wv, tw = gang * kChainWaves;
+    uint64_t* S = spins + (uint64_t)slot * n;
+    uint32_t* sbar = bar + slot;
+    const uint32_t tr = temp_of[slot * 64 + lane];
+    edef[wv][lane] = 0;
+    if (wv == 0) blk_e[lane] = 0;
+    if (tid == 0) blk_acc = 0;
+    uint64_t kacc[W], dacc[W];
+#pragma unroll
+    for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
+    int64_t e_lane = 0;
+    uint32_t nacc = 0, pending = 0, target = 0;
+    auto flush = [&]() {
+        const int64_t ksum = wave_column_sum<W>(kacc, lane);
+        const int64_t dsum = wave_column_sum<W>(dacc, lane);
+        e_lane += ((int64_t)w0 * (2 * ksum - dsum)) / 2;
+#pragma unroll
+        for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
+        pending = 0;
+    };
+    for (uint32_t k = 0; k < sweeps; ++k) {
+        const uint32_t sweep = sweep0 + k;
+        build_records(recs, lev + (uint64_t)k * lev_stride, tr, dmax, wv, kChainWaves, lane);
+        __syncthreads();
+        for (uint32_t c = 0; c < ncol; ++c) {
+            const uint32_t cb = __builtin_amdgcn_readfirstlane(cstart[c]);
+            const uint32_t ce = __builtin_amdgcn_readfirstlane(cstart[c + 1]);
+            if (lane == 0) qcount[wv] = 0;
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t base = cb + gw * 64; base < ce; base += tw * 64) {
+                const uint32_t p = base + lane;
+                if (p < ce)
+                    irregular_site<B, W>(S, S, p, p - cb, rp, ci, sgn, recs, slot, sweep, key0,
+                                         key1, w0, kacc, dacc, nacc, queue[wv], &qcount[wv],
+                                         kChainQueueCap, edef[wv]);
+                if (++pending == kFlush) flush();  // wave-uniform
+            }
+            __threadfence_block();
+            __builtin_amdgcn_wave_barrier();
+            drain_queue(queue[wv], &qcount[wv], kChainQueueCap, cb, S, recs, slot, sweep, key0,
+                        key1, w0, edef[wv], lane, nacc);
+            if (gang > 1)
+                gang_barrier(sbar, target, gang, spin_limit, err);
+            else
+                __syncthreads();
+        }
+    }
+    flush();
+    e_lane += (int64_t)edef[wv][lane];
+    for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
+    atomicAdd(&blk_e[lane], (unsigned long long)e_lane);
+    if (lane == 0) atomicAdd(&blk_acc, nacc);
+    __syncthreads();
+    if (wv == 0) {
+        if (blk_e[lane])
+            atomicAdd(reinterpret_cast<unsigned long long*>(energy + slot * 64 + lane), blk_e[lane]);
+        if (lane == 0 && blk_acc) atomicAdd(accepted, (unsigned long long)blk_acc);
+    }
+}
+
+__global__ void chain_fill_kernel(uint32_t* p, uint32_t count) {
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0u;
+}
+
+}  // namespace
+
+namespace {
+
+template <int B, int W>
+int chain_ms_per_cu(uint32_t lds) {
+    int pc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &pc, reinterpret_cast<const void*>(sweep_chain_ms_kernel<B, W>), kChainBlock, lds) !=
+        hipSuccess)
+        return 0;
+    return pc;
+}
+
+}  // namespace
+
+uint32_t chain_ms_capacity_blocks(uint32_t dmax) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const uint32_t lds = (dmax + 1) * kQStride * sizeof(unsigned long long);
+    const int pc = dmax < 8 ? chain_ms_per_cu<3, 6>(lds)
+                 : dmax < 16 ? chain_ms_per_cu<4, 7>(lds)
+                 : dmax < 32 ? chain_ms_per_cu<5, 8>(lds) : 0;
+    return pc > 0 && ncu > 0 ? (uint32_t)(pc * ncu) : 0u;
+}
+
+hipError_t launch_sweep_chain_ms(const DevModel& m, const DevState& s, const uint32_t* cstart,
+                                 uint32_t ncol, uint32_t max_class, const uint32_t* lev,
+                                 uint32_t lev_stride, uint32_t sweep0, uint32_t sweeps,
+                                 uint64_t seed, uint32_t* bar, uint32_t* err, uint32_t spin_limit,
+                                 hipStream_t stream) {
+    const uint32_t cap = chain_ms_capacity_blocks(m.dmax);
+    if (cap < s.cells || m.dmax >= 32) return hipErrorInvalidValue;
+    // one site per lane of the largest class, at most kChainMaxBlocks
+    // workgroups per slot and what the GPU holds for every slot at once
+    const uint32_t want = (max_class + 64 * kChainWaves - 1) / (64 * kChainWaves);
+    uint32_t per_slot = cap / s.cells;
+    if (per_slot > kChainMaxBlocks) per_slot = kChainMaxBlocks;
+    if (want < per_slot) per_slot = want ? want : 1u;
+    const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
+    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
+    const uint64_t per_launch = ((1ull << 31) / per_slot) / (ncol ? ncol : 1u);
+    for (uint32_t done = 0; done < sweeps;) {
+        const uint32_t k =
+            (uint32_t)((uint64_t)(sweeps - done) < per_launch ? sweeps - done : per_launch);
+        hipLaunchKernelGGL(chain_fill_kernel, dim3(1), dim3(256), 0, stream, bar, s.cells);
+        const uint32_t* L = lev + (uint64_t)done * lev_stride;
+#define IGE_CHAIN_MS(B, W)                                                                       \
+    hipLaunchKernelGGL((sweep_chain_ms_kernel<B, W>), dim3(per_slot, s.cells), dim3(kChainBlock), \
+                       lds, stream, m.rp, m.ci, m.sgn, cstart, ncol, s.spins, m.n, m.dmax, m.w0, \
+                       L, lev_stride, s.temp_of, s.energy, s.accepted, sweep0 + done, k, key0,   \
+                       key1, bar, spin_limit, err)
+        if (m.dmax < 8)
+            IGE_CHAIN_MS(3, 6);
+        else if (m.dmax < 16)
+            IGE_CHAIN_MS(4, 7);
+        else
+            IGE_CHAIN_MS(5, 8);
+#undef IGE_CHAIN_MS
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        done += k;
+    }
+    return hipSuccess;
+}
+
+}  // namespace ige

@@ -1,0 +1,41 @@
+// gang_sync.h -- device-wide barriers among co-resident workgroups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ige {
+
+// Barrier across the `gang` workgroups of one instance (or of one launch).
+// The workgroup's waves meet first (their stores reach L2); then one thread
+// releases at device scope (L2 write-back), arrives on the counter, waits
+// with relaxed polls (bounded by spin_limit polls; on expiry the launch's
+// error flag is raised and every later wait is skipped) and acquires at
+// device scope (L1/L2 invalidation), before the waves meet again: neighbour
+// words written by the other workgroups are then read afresh.  One thread
+// per workgroup fences, and the polls are relaxed: fences issued by every
+// wave and acquire polls made a barrier of 64 workgroups cost 9.4 us, this
+// form 2.4 us (256 workgroups: 40 and 10 us; MI355X, 2000 barriers timed).
+__device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
+                                             uint32_t spin_limit, uint32_t* err) {
+    __syncthreads();
+    target += gang;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        atomicAdd(bar, 1u);
+        uint32_t polls = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+            if (++polls > spin_limit) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+}  // namespace ige

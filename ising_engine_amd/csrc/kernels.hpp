@@ -125,6 +125,23 @@ hipError_t launch_resident_sweeps(const DevModel& m, const DevState& s,
 // dependent memory round trips per pass and runs no faster fused).
 // lev_thr as for launch_sweep_multispin.
 constexpr uint32_t kFuseMaxClassSites = 512;
+
+// Sweeps [sweep0, sweep0 + sweeps) of a multispin state of irregular degree
+// (dmax < 32) without exchange in one launch (a few when a barrier counter
+// would overflow): every class of every sweep, gridDim.x workgroups per slot
+// meeting at the slot's barrier bars[slot] (zeroed here; err raised when a
+// wait expires) after each class; the same updates as launch_sweep_multispin
+// class by class.  Sweep k's level thresholds are lev + k * lev_stride
+// (lev_thr layout).  max_class: positions in the largest class, at most
+// kChainMaxSites (one site per lane of kChainMaxSites / 256 workgroups).
+// Needs cells <= chain_ms_capacity_blocks(dmax).
+constexpr uint32_t kChainMaxSites = 32768;
+hipError_t launch_sweep_chain_ms(const DevModel& m, const DevState& s, const uint32_t* cstart,
+                                 uint32_t ncol, uint32_t max_class, const uint32_t* lev,
+                                 uint32_t lev_stride, uint32_t sweep0, uint32_t sweeps,
+                                 uint64_t seed, uint32_t* bars, uint32_t* err, uint32_t spin_limit,
+                                 hipStream_t stream);
+uint32_t chain_ms_capacity_blocks(uint32_t dmax);
 hipError_t launch_classes_sweep(const DevModel& m, const DevState& s,
                                 const uint32_t* class_start, uint32_t c0, uint32_t c1,
                                 uint32_t sweep, uint64_t seed, const uint32_t* lev_thr,

@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "gang_sync.h"
 #include "ige_math.h"
 #include "kernels.hpp"
 #include "sweep_generic.h"
@@ -89,33 +90,6 @@ __host__ __device__ __forceinline__ uint32_t choose_gang(uint32_t n, uint32_t ma
     const uint32_t g = n / (gang_sites ? gang_sites : kGangSites);
     const uint32_t cap = max_gang < kMaxGang ? max_gang : kMaxGang;
     return g < 2 ? 1u : (g < cap ? g : cap);
-}
-
-// Barrier across the `gang` workgroups of one instance: every wave releases
-// its stores at device scope, one thread per workgroup arrives on the
-// counter and waits (bounded by spin_limit polls; on expiry the launch's
-// error flag is raised and every later wait is skipped), then every wave
-// acquires at device scope so that neighbour words written by the other
-// workgroups are read from L2, not from a stale L1 line.
-__device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
-                                             uint32_t spin_limit, uint32_t* err) {
-    __threadfence();
-    __syncthreads();
-    target += gang;
-    if (threadIdx.x == 0) {
-        atomicAdd(bar, 1u);
-        uint32_t polls = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-            if (++polls > spin_limit) {
-                atomicOr(err, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    __threadfence();
 }
 
 // Incumbent update of UltimateSolver::update_incumbent (src/solver/ultimate.rs:397-431)
@@ -236,13 +210,13 @@ __device__ __forceinline__ void anneal_sweeps(const InstDesc& I, uint32_t k0, ui
     }
 }
 
-// Device-scope barrier of one workgroup (gang of one): every store and atomic
-// before it is visible to every wave after it, from L2.
-__device__ __forceinline__ void block_sync_device() {
-    __threadfence();
-    __syncthreads();
-    __threadfence();
-}
+// Barrier of a gang of one workgroup: the solve's data is written and read by
+// this workgroup only within a launch (the waves share the CU's L1, and the
+// energy atomics and load_agent reads meet at L2), so the workgroup barrier
+// orders it; device-scope fences here (an L2 write-back and invalidation per
+// wave, twice a sweep) cost the serving loop's PT solves several microseconds
+// a sweep.
+__device__ __forceinline__ void block_sync_device() { __syncthreads(); }
 
 __device__ __forceinline__ int64_t load_agent(const int64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

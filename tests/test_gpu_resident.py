@@ -1,5 +1,7 @@
 """Resident runs (a small regular lattice's whole ige_run call in one workgroup per slot, words
-in LDS) and fused small colour classes (trailing classes of a sweep in one launch) against the
+in LDS), chained runs (a one-slot irregular state's whole call in one launch, a device-wide
+barrier per class) and fused small colour classes (trailing classes of a sweep in one launch)
+against the
 colour-class launches of the same sweeps (IGE_RUN_EAGER), bit for bit after every call: words,
 int64 energies, temperatures, flip counts.  Covers both degrees and irregular graphs, schedules
 and fixed ladders, several slots, the 4096-site limit, calls mixed with exchange runs, and runs
@@ -129,9 +131,52 @@ def test_fused_tail_matches_the_oracle(ctx):
     ost = ob.OracleState(csr, 1, 0x5E51_3000, view=ob.VIEW_PLANE)
     t_max, t_min = instances.suggested_temp_range(qb)
     sched = instances.anneal_schedule(1.0 / t_max, 1.0 / t_min, 6)
-    st.run(6, beta_schedule=sched)
+    for k in range(6):          # one sweep per call: the colour-class launches and fused tail
+        st.run(1, beta_schedule=sched[k:k + 1])
     ost.run(6, exchange=False, schedule=sched)
     np.testing.assert_array_equal(st.spins(), ost.words)
     np.testing.assert_array_equal(st.energies(), ost.energy)
     s = st.stats()[0]
     assert s.accepted == int(ost.stats[0])
+    assert s.chain_sweeps == 0
+
+
+@pytest.mark.parametrize("n,degree", [(20000, 6), (3000, 1), (3000, 5), (3000, 12)],
+                         ids=["er20000_d6", "er3000_d1", "er3000_d5", "er3000_d12"])
+def test_chained_sweeps_match_the_oracle(ctx, n, degree):
+    # the whole call in one launch, every class behind a device-wide barrier (degree < 8,
+    # < 16 and < 32 counter widths): against the oracle on a schedule, then on a fixed
+    # ladder, and against the same sweeps run one call each
+    qb = instances.erdos_renyi_pm_j(n, degree, 90 + degree)
+    dmax = int(np.bincount(np.concatenate([qb.pi, qb.pj]), minlength=n).max())
+    assert dmax < 32
+    csr = ob.CSR.from_qubo(qb)
+    model = make(ctx, qb)
+    print(f"[chain] n={n} degree={degree} dmax={model.info.max_degree} "
+          f"colours={model.info.num_colors}", flush=True)
+    seed = 0xC4A1 + n
+    st = _lib.State(model, 1, seed, kernel=_lib.KERNEL_MULTISPIN)
+    ost = ob.OracleState(csr, 1, seed, view=ob.VIEW_PLANE)
+    t_max, t_min = instances.suggested_temp_range(qb)
+    sched = instances.anneal_schedule(1.0 / t_max, 1.0 / t_min, 12)
+    st.run(12, beta_schedule=sched)
+    ost.run(12, exchange=False, schedule=sched)
+    np.testing.assert_array_equal(st.spins(), ost.words)
+    np.testing.assert_array_equal(st.energies(), ost.energy)
+    assert st.stats()[0].chain_sweeps == 12
+    assert st.stats()[0].accepted == int(ost.stats[0])
+    beta = np.array([1.0 / (0.5 * (t_max + t_min))])
+    st.set_ladder(beta)
+    st.run(9)
+    ost.run(9, exchange=False, ladder=beta)
+    np.testing.assert_array_equal(st.spins(), ost.words)
+    np.testing.assert_array_equal(st.energies(), ost.energy)
+    np.testing.assert_array_equal(st.recompute_energies(), st.energies())
+    one = _lib.State(model, 1, seed, kernel=_lib.KERNEL_MULTISPIN)
+    for k in range(12):
+        one.run(1, beta_schedule=sched[k:k + 1])
+    one.set_ladder(beta)
+    for _ in range(9):
+        one.run(1)
+    same(st, one)
+    assert one.stats()[0].chain_sweeps == 0
