@@ -425,7 +425,7 @@ int32_t ige_batch_run(ige_batch *batch, uint32_t mode, uint32_t waves, uint32_t 
 int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *waves_out,
                            double *ms_out /* 5 entries, or NULL */);
 /* Large instances run on a gang of workgroups (instance sites / gang_sites of
- * them, at most max_gang <= 64) that meet at a device-wide barrier after every
+ * them, at most max_gang <= 128) that meet at a device-wide barrier after every
  * colour class.  spin_limit bounds each barrier wait; an expired wait makes
  * the next synchronize/read fail with IGE_ERR_HIP.  Batch defaults: 1 (no gangs:
  * throughput), 2048, 2^24.  The server sizes gangs per round from its load:
@@ -433,7 +433,7 @@ int32_t ige_batch_autotune(ige_batch *batch, uint32_t probe_sweeps, uint32_t *wa
  * them, owning several cells each when fewer), an anneal one; CUs left
  * without any workgroup are shared out by sites to split requests further
  * (an anneal up to min(max_gang, n / gang_sites) members, a PT request with
- * one workgroup per cell up to s <= min(max_gang, n / gang_sites, 64 / cells)
+ * one workgroup per cell up to s <= min(max_gang, n / gang_sites, 128 / cells)
  * per cell); when the base gangs exceed the resident workgroups each request
  * gets a share by sites instead.  Server defaults: 16, 1024, 2^24 (a gang
  * barrier per colour class costs more than a few hundred sites of work). */

@@ -467,7 +467,7 @@ int32_t ige_batch_configure(ige_batch* b, uint32_t max_gang, uint32_t gang_sites
                             uint32_t spin_limit) {
     return guarded([&] {
         require(b != nullptr, "NULL batch");
-        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 64]");
+        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 128]");
         check(hipSetDevice(b->device), "hipSetDevice");
         check(hipStreamSynchronize(b->stream), "configure");
         b->max_gang = max_gang;

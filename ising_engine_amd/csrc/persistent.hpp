@@ -69,8 +69,8 @@ struct InstDesc {
 constexpr uint32_t kMaxPtCells = 64;  // temperature cells per solve
 
 constexpr uint32_t kLdsThrEntries = 4096;
-constexpr uint32_t kMaxGang = 64;        // queue codes carry the member index in 6 bits
-constexpr uint32_t kGangBits = 6;
+constexpr uint32_t kMaxGang = 128;       // queue codes carry the member index in 7 bits
+constexpr uint32_t kGangBits = 7;
 constexpr uint32_t kGangSites = 2048;    // sites per workgroup before a gang is formed
 // How a workgroup decides u < thr(beta, dE) for dE <= dtab.
 constexpr uint32_t kThrLds = 0;   // dtab < kLdsThrEntries: per-sweep table in LDS

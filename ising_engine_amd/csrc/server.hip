@@ -761,7 +761,7 @@ int32_t ige_server_configure(ige_server* s, uint32_t max_gang, uint32_t gang_sit
                              uint32_t spin_limit) {
     return guarded([&] {
         require(s != nullptr, "NULL server");
-        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 64]");
+        require(max_gang >= 1 && max_gang <= ige::kMaxGang, "max_gang must be in [1, 128]");
         std::lock_guard<std::mutex> g(s->mu);
         require(s->reqs.empty(), "configure before submitting requests");
         s->max_gang = max_gang;
