@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -54,7 +55,7 @@ struct ige_model {
     DevBuf<int32_t> wt, lin;
     // Dense int8 matrix, built on the first dense state of this model.
     mutable std::mutex dense_mu;
-    mutable DevBuf<int8_t> dense_a;
+    mutable DevBuf<int8_t> dense_a, dense_hdiag;
     mutable DevBuf<int32_t> dense_lin, dense_wdiag;
     mutable uint32_t n_pad = 0;
     bool dense_ready() const {
@@ -74,6 +75,13 @@ struct ige_model {
                 for (uint32_t k = 0; k < 64; ++k)
                     wd[(static_cast<size_t>(b) * 64 + p) * 64 + k] =
                         ds.scale * a[static_cast<size_t>(b * 64 + p) * np + b * 64 + k];
+        // diagonal 32 x 32 tiles, row-major: the accept-bound chain's coupling rows
+        std::vector<int8_t> hd(static_cast<size_t>(np / 32) * 1024);
+        for (uint32_t t = 0; t < np / 32; ++t)
+            for (uint32_t p = 0; p < 32; ++p)
+                for (uint32_t k = 0; k < 32; ++k)
+                    hd[(static_cast<size_t>(t) * 32 + p) * 32 + k] =
+                        a[static_cast<size_t>(t * 32 + p) * np + t * 32 + k];
         // matrix-core operand order: tile (g, c) = rows 32g.., columns 32c.., byte
         // 16 l + b = row 32g + l % 32, column 32c + 16 (l / 32) + b
         std::vector<int8_t> at(a.size());
@@ -87,6 +95,8 @@ struct ige_model {
         dense_a.alloc(at.size());
         dense_lin.alloc(np);
         dense_wdiag.alloc(wd.size());
+        dense_hdiag.alloc(hd.size());
+        dense_hdiag.upload(hd.data(), hd.size(), nullptr);
         dense_a.upload(at.data(), at.size(), nullptr);
         dense_lin.upload(l.data(), np, nullptr);
         dense_wdiag.upload(wd.data(), wd.size(), nullptr);
@@ -99,6 +109,7 @@ struct ige_model {
         d.n_pad = n_pad;
         d.a = dense_a.p;
         d.wdiag = dense_wdiag.p;
+        d.hdiag = dense_hdiag.p;
         d.lin = dense_lin.p;
         d.order = order.p;
         d.scale = ds.scale;
@@ -146,6 +157,7 @@ struct ige_state {
     DevBuf<unsigned long long> counters;  // accepted, then pair stats
     DevBuf<uint32_t> lev_thr;             // multispin thresholds per temperature and level
     DevBuf<double> beta;                             // current ladder (+ schedule rows)
+    DevBuf<int32_t> dense_bounds;                    // accept bounds of a dense sweep
     DevBuf<uint64_t> hb;                             // heat-bath thresholds, on first use
     DevBuf<int32_t> fields;
     DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
@@ -334,6 +346,7 @@ void icm_rounds(ige_state* s, uint32_t rounds, unsigned long long* flipped);
 struct StepPlan {
     bool exch = false, track = false, heat = false, icm = false, sched = false;
     bool per_class = false;  // one launch per colour class (no fused tail)
+    bool dense_bounds = false;  // dense sweeps decide by accept bounds (sweep_dense_bound_kernel)
     uint32_t hb_dtab = 0;
 };
 
@@ -389,7 +402,11 @@ void issue_step(ige_state* s, const StepPlan& P, const double* beta, uint32_t sw
                   "sweep_heatbath");
     } else if (s->kernel == IGE_KERNEL_DENSE) {
         require(clock == nullptr, "the dense sweep is not captured");
-        check(ige::launch_sweep_dense(s->model->dense_view(), st, sweep, s->seed, beta, s->stream),
+        const auto dv = s->model->dense_view();
+        if (P.dense_bounds && !s->dense_bounds.p)
+            s->dense_bounds.alloc(ige::dense_bounds_words(dv, s->cells));
+        check(ige::launch_sweep_dense(dv, st, sweep, s->seed, beta,
+                                      P.dense_bounds ? s->dense_bounds.p : nullptr, s->stream),
               "sweep_dense");
     } else if (s->kernel == IGE_KERNEL_GENERIC) {
         for (uint32_t c = 0; c < s->model->co.num_colors; ++c)
@@ -753,6 +770,18 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
         const auto row = [&](uint32_t k) {
             return sched ? s->sched_buf.p + static_cast<size_t>(k) * s->cells : s->beta.p;
         };
+        if (s->kernel == IGE_KERNEL_DENSE && !P.heat) {
+            // Accept bounds need every beta of the call at or above their minimum.
+            double bmin = std::numeric_limits<double>::infinity();
+            if (sched) {
+                for (size_t i = 0; i < static_cast<size_t>(sweeps) * s->cells; ++i)
+                    bmin = std::min(bmin, sched[i]);
+            } else {
+                for (const double b : s->ladder) bmin = std::min(bmin, b);
+            }
+            P.dense_bounds = ige::dense_bounds_eligible(s->model->dense_view()) &&
+                             bmin >= ige::kDenseBoundMinBeta;
+        }
         uint32_t done = 0;
         // Small regular lattices without exchange: the whole call in one
         // resident launch (words in LDS, one workgroup per slot).

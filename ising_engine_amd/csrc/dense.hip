@@ -2,8 +2,12 @@
 // dense QUBO) with the local fields recomputed on int8 matrix cores.
 //
 // Model: q_ij = scale * a_ij with a_ij int8, stored as an n_pad x n_pad int8
-// matrix in position space.  One workgroup (4 waves) per slot walks the
-// positions in blocks of 64:
+// matrix in position space.  Two sweeps share the field recompute below:
+// sweep_dense_bound_kernel (further down; decisions by accept bounds built
+// before the sweep) whenever every beta is >= kDenseBoundMinBeta and the
+// field range fits its 24-bit arithmetic, and sweep_dense_kernel otherwise.
+// sweep_dense_kernel: one workgroup (4 waves) per slot walks the positions in
+// blocks of 64:
 //   1. every wave draws the block's uniforms (word view, the same Philox
 //      words as the lane-per-replica kernel) into LDS, and the block's own
 //      couplings scale * a[p0+p][p0+k] are staged in LDS;
@@ -382,6 +386,453 @@ __global__ __launch_bounds__(kDenseThreads) void dense_fields_kernel(DevDense d,
         out[((size_t)slot * n + d.order[p0 + i]) * 64 + lane] = h[i * kHStride + lane];
 }
 
+// ---------------------------------------------------------------------------
+// Accept-bound chain: sweep_dense_bound_kernel.
+//
+// The kernel above decides every proposal inside the sequential loop (a
+// double-precision threshold band) and pushes every flip into all later
+// fields of its block, so its one sequential wave issues about 120
+// instructions per position.  Here that wave keeps only what is sequential:
+//  * Accept bounds.  Before the sweep, dense_bounds_kernel turns the uniform
+//    u of every (position, replica) into D = the largest dE in [0, dlim] with
+//    dE == 0 or u < metropolis_threshold(beta, dE), on the whole GPU (the
+//    sweep itself holds one CU per slot).  In the sweep a helper wave folds D
+//    with the spin's sign into one key K = -(sgn + D + 1) (sgn = -x); the
+//    chain accepts iff (h ^ sgn) + K < 0, i.e. dE = (1 - 2x) h <= D.
+//    metropolis_threshold is non-increasing in dE (det_exp is within a few ulp
+//    of exp, and beta >= kDenseBoundMinBeta separates neighbouring dE by far
+//    more), so the accepted dE form the prefix [.., D] and the decision
+//    equals the threshold's.
+//  * Pulled fields.  The chain keeps the flips of the current half block (32
+//    positions) as int8 bytes (+1, -1 or 0 in byte q of word q / 4) and forms
+//    the field of position i as h0_i + scale * sum_q a_iq dx_q with v_dot4
+//    over the words written so far; the coupling rows are read from LDS.
+//  * Between the two halves of a block, two 32x32x32 int8 MFMAs add the first
+//    half's flips to the second half's fields.
+// Helper waves meanwhile compute the next block's fields from the spins on
+// the matrix cores (as above) and stage its keys and coupling rows.  Waves:
+// 0 the chain; 4 the keys and rows (it shares a SIMD with the chain, so it
+// gets the light work); 1-3 and 5-7 the fields (all 64 rows, a sixth of the
+// columns each, so that every expansion of spin bits into operand bytes feeds
+// both row halves); 1, 2, 3 and 5 also combine the fields' tiles.
+
+constexpr uint32_t kBoundThreads = 512;  // 8 waves
+constexpr int kFieldPrefetch = 4;        // chunks of coupling rows in flight per field wave
+constexpr uint32_t kRS = 68;             // dwords per replica row of the [replica][position] tiles
+
+struct BoundTiles {
+    int32_t h[64 * kRS];         // fields of the next block at its start, [replica][position]
+    int32_t key[2][64 * kRS];    // accept keys of blocks blk and blk + 1, [replica][position]
+    int32_t hacc[2][2][16][64];  // sums a.x of the next block: row half, replica half,
+                                 // element, lane (added to by the six field waves,
+                                 // cleared when read)
+    uint4 rows[2][128];          // coupling rows (hdiag) of blocks blk and blk + 1
+    v4i corr[2][2][64];          // coupling tiles (row half, column tile) of the next
+                                 // block's rows against the current block's columns
+    int32_t lin[64];             // linear terms of the next block
+    v4i diag[2][64];             // coupling tile of rows 32.. against columns 0..31 of
+                                 // blocks blk and blk + 1 (the chain's half correction)
+    uint64_t xold[64];           // the finished block's spins per replica, before
+    uint64_t xnew[64];           // and after the chain
+};
+
+struct BoundLane {
+    double beta;  // this lane's inverse temperature
+    double cd;    // ln 2 / beta
+    double eps;   // error bound of y below (without its 1e-15 y part)
+    uint32_t dlim;
+};
+
+// D = the largest d in [0, dlim] with d == 0 or u < metropolis_threshold(beta, d).
+// u < thr(d) <=> u + 1 <= 2^32 det_exp(-beta d) <=> d <= y + theta with
+// y = (32 - log2(u + 1)) ln 2 / beta and |theta| <= 1.5 eta / beta for a
+// relative error eta of det_exp (a few ulp).  y is formed from u + 1 = 2^e m
+// with log2(m) on v_log_f32 (absolute error below 2^-21 on [1, 2], checked
+// over every float there; m itself is rounded to 24 bits, 8.6e-8 more), the
+// rest in double: |y - y_true| <= 5.7e-7 ln2/beta + 1e-15 y.  When the
+// fractional part of y is farther than eps = 6e-7 ln2/beta + 1e-12 + 1e-15 y
+// from an integer, D = floor(y) (capped at dlim); otherwise D is one of two
+// neighbours and one exact threshold decides (eps < 1/2 for
+// beta >= kDenseBoundMinBeta).
+__device__ __forceinline__ uint32_t accept_bound(uint32_t u, const BoundLane& bp) {
+    if (u == 0xFFFFFFFFu) return 0u;
+    const uint32_t v = u + 1u;
+    const uint32_t z = __builtin_clz(v);
+    const float mf = (float)(v << z) * 0x1p-31f;  // u + 1 = 2^(31 - z) mf, mf in [1, 2]
+    const double y = ((double)(z + 1u) - (double)__builtin_amdgcn_logf(mf)) * bp.cd;
+    if (y >= (double)bp.dlim + 2.0) return bp.dlim;
+    const double d0 = floor(y);
+    const double fr = y - d0;
+    const double ep = bp.eps + 1e-15 * y;
+    uint32_t D = (uint32_t)d0;
+    if (fr < ep) {
+        if (D > 0u && !exact_accept(u, (int32_t)D, bp.beta)) D -= 1u;
+    } else if (fr > 1.0 - ep) {
+        if (exact_accept(u, (int32_t)D + 1, bp.beta)) D += 1u;
+    }
+    return D < bp.dlim ? D : bp.dlim;
+}
+
+// Accept bounds of one sweep: D[((slot * nblk + blk) * 16 + g) * 64 + lane][q]
+// for position 64 blk + 4 g + q of replica lane (one Philox call per thread:
+// the word-view uniforms), -0x40000001 past n (a key that rejects).  One
+// wave per (slot, block, group).
+__global__ __launch_bounds__(256) void dense_bounds_kernel(DevDense d, DevState s,
+                                                           const double* __restrict__ beta,
+                                                           uint32_t sweep, uint32_t key0,
+                                                           uint32_t key1, int4* __restrict__ D,
+                                                           uint32_t waves) {
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= waves) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nblk = (d.n + 63) / 64;
+    const uint32_t g = w % 16, blk = (w / 16) % nblk, slot = w / (16 * nblk);
+    BoundLane bp;
+    bp.beta = beta[s.temp_of[slot * 64 + lane]];
+    bp.cd = 0.6931471805599453 / bp.beta;
+    bp.eps = 6e-7 * bp.cd + 1e-12;
+    bp.dlim = threshold_limit(bp.beta, d.dtab);
+    const uint32_t p = blk * 64 + 4 * g;
+    const U32x4 u = philox4x32_10(p >> 2, slot, sweep, (kTagWord << 24) | lane, key0, key1);
+    int32_t v[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q)
+        v[q] = p + q < d.n ? (int32_t)accept_bound(u.v[q], bp) : -0x40000001;
+    D[(size_t)w * 64 + lane] = int4{v[0], v[1], v[2], v[3]};
+}
+
+// Keys of block nb into key[replica][position] from its accept bounds Db
+// (this lane = replica: K = -(sgn + D + 1), sgn = -x at the block's start),
+// the block's coupling rows (hdiag tiles 2 nb, 2 nb + 1) into rows, its tile
+// of rows 32.. against columns 0..31 into diag, and what combine_fields
+// reads for block nb: the tiles of its rows against block nb - 1's columns
+// and its linear terms.  One wave.
+__device__ __forceinline__ void stage_keys(const DevDense& d, uint32_t nb, const int4* Db,
+                                           const uint64_t* xt, uint32_t xs, int32_t* key,
+                                           uint4* rows, v4i* diag, v4i (*corr)[2][64],
+                                           int32_t* lin, uint32_t lane) {
+    const uint32_t ntile = d.n_pad / 32;
+    const uint4* src = reinterpret_cast<const uint4*>(d.hdiag + (size_t)nb * 2048);
+    rows[lane] = src[lane];
+    rows[64 + lane] = src[64 + lane];
+    lin[lane] = d.lin[nb * 64 + lane];
+    diag[lane] = *reinterpret_cast<const v4i*>(
+        d.a + ((size_t)(2 * nb + 1) * ntile + 2 * nb) * 1024 + 16 * lane);
+    if (nb > 0) {
+#pragma unroll
+        for (uint32_t r = 0; r < 2; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < 2; ++c)
+                corr[r][c][lane] = *reinterpret_cast<const v4i*>(
+                    d.a + ((size_t)(2 * nb + r) * ntile + 2 * (nb - 1) + c) * 1024 + 16 * lane);
+    }
+    const uint64_t xb = xt[lane * xs + nb];
+    int4 dv[16];
+#pragma unroll
+    for (int g = 0; g < 16; ++g) dv[g] = Db[g * 64 + lane];
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+        const int32_t k[4] = {dv[g].x, dv[g].y, dv[g].z, dv[g].w};
+        int32_t kv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t sgn = -(int32_t)((xb >> (4 * g + q)) & 1u);
+            kv[q] = -(sgn + k[q] + 1);
+        }
+        *reinterpret_cast<int4*>(key + lane * kRS + 4 * g) = int4{kv[0], kv[1], kv[2], kv[3]};
+    }
+}
+
+// a.x of the 64 rows of the block at p0 over the 64-column chunks [c0, c1)
+// (a multiple of kFieldPrefetch chunks, possibly none), all 64 replicas, added
+// into hacc; one wave.  Each
+// chunk's spin bytes (two replica halves x two 32-column tiles) feed the MFMAs
+// of both 32-row halves.
+__device__ __forceinline__ void block_partial(const DevDense& d, const uint64_t* xt, uint32_t xs,
+                                              uint32_t p0, uint32_t c0, uint32_t c1,
+                                              int32_t (*hacc)[2][16][64], uint32_t lane) {
+    // Chunks of coupling rows in flight: the rows stream from the memory
+    // behind L2, so a wave keeps P * 4 KB in flight.  With whole groups of P
+    // chunks the loop has no early exit, and the loads of the next group stay
+    // in flight across the loop's back edge.
+    constexpr int P = kFieldPrefetch;
+    if (c1 <= c0) return;
+    const uint32_t kh = 16 * (lane >> 5);
+    const uint32_t ntile = d.n_pad / 32;
+    const int8_t* ar0 = d.a + ((size_t)(p0 / 32) * ntile) * 1024 + 16 * lane;
+    const int8_t* ar1 = ar0 + (size_t)ntile * 1024;
+    const uint64_t* xr0 = xt + (size_t)(lane & 31) * xs;
+    const uint64_t* xr1 = xt + (size_t)(32 + (lane & 31)) * xs;
+    v16i acc[2][2] = {};  // [row half][replica half]
+    v4i pa[P][4];         // [chunk][row half * 2 + tile]
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const size_t o = (size_t)min(c0 + i, c1 - 1) * 2048;
+        pa[i][0] = *reinterpret_cast<const v4i*>(ar0 + o);
+        pa[i][1] = *reinterpret_cast<const v4i*>(ar0 + o + 1024);
+        pa[i][2] = *reinterpret_cast<const v4i*>(ar1 + o);
+        pa[i][3] = *reinterpret_cast<const v4i*>(ar1 + o + 1024);
+    }
+    for (uint32_t kb = c0; kb < c1; kb += P) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const uint32_t kc = kb + i;
+            v4i a[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[t] = pa[i][t];
+            const size_t o = (size_t)min(kc + (uint32_t)P, c1 - 1) * 2048;
+            pa[i][0] = *reinterpret_cast<const v4i*>(ar0 + o);
+            pa[i][1] = *reinterpret_cast<const v4i*>(ar0 + o + 1024);
+            pa[i][2] = *reinterpret_cast<const v4i*>(ar1 + o);
+            pa[i][3] = *reinterpret_cast<const v4i*>(ar1 + o + 1024);
+            const uint64_t x0 = xr0[kc], x1 = xr1[kc];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {  // 32-column tile of the chunk
+                const v4i b0 = spread16((uint32_t)(x0 >> (32 * t + kh)));
+                const v4i b1 = spread16((uint32_t)(x1 >> (32 * t + kh)));
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    acc[r][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * r + t], b0, acc[r][0],
+                                                                      0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * r + t], b1, acc[r][1],
+                                                                      0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) atomicAdd(&hacc[r][j][v][lane], acc[r][j][v]);
+}
+
+// Fields of block nb at its start, one 32 x 32 tile (rows 32 rh.., replicas
+// 32 jh..) per wave: lin + scale * (the field waves' sums + block nb - 1's
+// flips applied to rows of block nb), into h[replica][position]; the sums are
+// cleared for the next block.  nb == 0: no flips yet.  The coupling tiles and
+// linear terms were staged in LDS (stage_keys).
+__device__ __forceinline__ void combine_fields(const DevDense& d, BoundTiles& T, uint32_t nb,
+                                               uint32_t rh, uint32_t jh, uint32_t lane) {
+    const uint32_t j = 32 * jh + (lane & 31);
+    v16i acc = {};
+    if (nb > 0) {
+        const uint32_t kh = 16 * (lane >> 5);
+        const uint64_t o = T.xold[j], nw = T.xnew[j];
+        const uint64_t up = nw & ~o, dn = o & ~nw;
+#pragma unroll
+        for (uint32_t c = 0; c < 2; ++c) {
+            const uint32_t sh = 32 * c + kh;
+            const v4i bd = spread16((uint32_t)(up >> sh)) | (spread16((uint32_t)(dn >> sh)) * 0xFF);
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.corr[rh][c][lane], bd, acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const uint32_t i = 32 * rh + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int32_t sum = T.hacc[rh][jh][v][lane] + acc[v];
+        T.hacc[rh][jh][v][lane] = 0;
+        T.h[j * kRS + i] = T.lin[i] + d.scale * sum;
+    }
+}
+
+// Writes the low byte of e into byte k of w, keeping the other bytes (SDWA);
+// k is a constant once the caller's loop is unrolled.
+__device__ __forceinline__ void put_byte(uint32_t& w, int32_t e, int k) {
+    switch (k) {
+    case 0:
+        asm("v_mov_b32_sdwa %0, %1 dst_sel:BYTE_0 dst_unused:UNUSED_PRESERVE src0_sel:DWORD"
+            : "+v"(w) : "v"(e));
+        break;
+    case 1:
+        asm("v_mov_b32_sdwa %0, %1 dst_sel:BYTE_1 dst_unused:UNUSED_PRESERVE src0_sel:DWORD"
+            : "+v"(w) : "v"(e));
+        break;
+    case 2:
+        asm("v_mov_b32_sdwa %0, %1 dst_sel:BYTE_2 dst_unused:UNUSED_PRESERVE src0_sel:DWORD"
+            : "+v"(w) : "v"(e));
+        break;
+    default:
+        asm("v_mov_b32_sdwa %0, %1 dst_sel:BYTE_3 dst_unused:UNUSED_PRESERVE src0_sel:DWORD"
+            : "+v"(w) : "v"(e));
+        break;
+    }
+}
+
+// Bit q of the result = (byte q of the flip words is nonzero), q < 32: the
+// bytes are 0x00, 0x01 or 0xFF, so bit 0 of each byte tells; the multiply
+// moves bit 8k of a word to bit 24 + k with no overlapping terms.
+__device__ __forceinline__ uint32_t flip_bits(const uint32_t (&dxp)[8]) {
+    uint32_t f = 0;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) f |= (((dxp[g] & 0x01010101u) * 0x01020408u) >> 24) << (4 * g);
+    return f;
+}
+
+// Positions OFF .. OFF + 31 of the block in order (lane = replica).  xh: this
+// replica's spins there at the block start (bit i); rows: row i's 32
+// couplings a (int8, columns OFF .. OFF + 31) at rows[2 i], rows[2 i + 1]
+// (LDS, read as broadcasts).  dxp receives the flips as bytes; eacc the
+// accepted dE.
+// kScale1: scale == 1, so the dot products accumulate onto h0 directly.
+// The rows are read from LDS kRowAhead positions ahead of their use.
+constexpr int kRowAhead = 3;
+
+template <int OFF, bool kScale1>
+__device__ __forceinline__ void chain_half(int32_t (&hr)[64], const int32_t (&kr)[64],
+                                           uint32_t xh, const uint4* rows, int32_t scale,
+                                           uint32_t (&dxp)[8], int32_t& eacc) {
+#pragma unroll
+    for (int g = 0; g < 8; ++g) dxp[g] = 0u;
+    // groups holding flips of positions before i: 0 .. (i - 1) / 4
+    auto groups = [](int i) { return (i + 3) >> 2; };
+    // ring slot k % (kRowAhead + 1) holds row k; row i + kRowAhead is loaded at
+    // position i into the slot of row i - 1, which is no longer needed
+    uint4 ring[kRowAhead + 1][2];
+#pragma unroll
+    for (int i = 1; i < kRowAhead; ++i) {
+        ring[i][0] = rows[2 * i];
+        if (groups(i) > 4) ring[i][1] = rows[2 * i + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        const int ng = groups(i);
+        const int nx = i + kRowAhead;
+        if (nx < 32) {
+            ring[nx % (kRowAhead + 1)][0] = rows[2 * nx];
+            if (groups(nx) > 4) ring[nx % (kRowAhead + 1)][1] = rows[2 * nx + 1];
+        }
+        const uint4 r0 = ring[i % (kRowAhead + 1)][0], r1 = ring[i % (kRowAhead + 1)][1];
+        const uint32_t w[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+        int32_t s = kScale1 ? hr[OFF + i] : 0;
+#pragma unroll
+        for (int g = 0; g < ng; ++g)
+            s = __builtin_amdgcn_sdot4((int32_t)dxp[g], (int32_t)w[g], s, false);
+        const int32_t h = kScale1 ? s : hr[OFF + i] + __mul24(s, scale);
+        const int32_t sgn = (int32_t)(xh << (31 - i)) >> 31;
+        const uint32_t t = ((uint32_t)h ^ (uint32_t)sgn) + (uint32_t)kr[OFF + i];
+        const int32_t m = (int32_t)t >> 31;
+        const int32_t e = m & (sgn | 1);
+        put_byte(dxp[i >> 2], e, i & 3);
+        eacc = __mul24(e, h) + eacc;
+    }
+}
+
+template <bool kScale1>
+__global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
+    DevDense d, DevState s, const int4* __restrict__ bounds) {
+    extern __shared__ __align__(16) uint64_t xt[];
+    __shared__ __align__(16) BoundTiles T;
+    const uint32_t nchunk = d.n_pad / 64, xs = nchunk + 1, nblk = (d.n + 63) / 64;
+    const uint32_t slot = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t n = d.n;
+    uint64_t* words = s.spins + (size_t)slot * n;
+    const int4* Dslot = bounds + (size_t)slot * nblk * 1024;
+    for (uint32_t c = wave; c < nchunk; c += kBoundThreads / 64) {
+        const uint32_t p = c * 64 + lane;
+        xt[lane * xs + c] = wave_transpose(p < n ? words[p] : 0ull, lane);
+    }
+    // Field waves 1, 2, 3, 5, 6, 7 are numbered fw = 0..5 and take columns
+    // [ch0, ch1) in whole prefetch groups; the first four also combine tile
+    // (rh, jh).
+    const bool fieldw = wave != 0 && wave != 4;
+    const uint32_t fw = wave < 4 ? wave - 1 : wave - 2;
+    const uint32_t ngrp = nchunk / kFieldPrefetch;  // n_pad is a multiple of 512
+    const uint32_t ch0 = fw * ngrp / 6 * kFieldPrefetch, ch1 = (fw + 1) * ngrp / 6 * kFieldPrefetch;
+    const bool combw = fieldw && fw < 4;
+    const uint32_t rh = fw >> 1, jh = fw & 1u;
+    for (uint32_t i = threadIdx.x; i < 2 * 2 * 16 * 64; i += kBoundThreads)
+        (&T.hacc[0][0][0][0])[i] = 0;
+    __syncthreads();
+    // Block 0: fields, keys and rows.
+    if (fieldw) {
+        block_partial(d, xt, xs, 0, ch0, ch1, T.hacc, lane);
+    } else if (wave == 4) {
+        stage_keys(d, 0, Dslot, xt, xs, T.key[0], T.rows[0], T.diag[0], T.corr, T.lin, lane);
+    }
+    __syncthreads();
+    if (combw) combine_fields(d, T, 0, rh, jh, lane);
+    __syncthreads();
+    int64_t eacc = 0;
+    uint32_t nacc = 0;
+    for (uint32_t blk = 0; blk < nblk; ++blk) {
+        const bool more = blk + 1 < nblk;
+        if (wave == 0) {
+            int32_t hr[64], kr[64];
+            const int32_t* hs = T.h + lane * kRS;
+            const int32_t* ks = T.key[blk & 1u] + lane * kRS;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int4 a = *reinterpret_cast<const int4*>(hs + 4 * q);
+                const int4 b = *reinterpret_cast<const int4*>(ks + 4 * q);
+                hr[4 * q] = a.x; hr[4 * q + 1] = a.y; hr[4 * q + 2] = a.z; hr[4 * q + 3] = a.w;
+                kr[4 * q] = b.x; kr[4 * q + 1] = b.y; kr[4 * q + 2] = b.z; kr[4 * q + 3] = b.w;
+            }
+            const uint64_t myx = xt[lane * xs + blk];
+            const v4i atile = T.diag[blk & 1u][lane];
+            const uint4* rows = T.rows[blk & 1u];
+            uint32_t dxp[8];
+            int32_t eb = 0;
+            chain_half<0, kScale1>(hr, kr, (uint32_t)myx, rows, d.scale, dxp, eb);
+            const uint32_t f0 = flip_bits(dxp);
+            // The first half's flips into the second half's fields.
+            {
+                v4i b0, b1;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const auto p = __builtin_amdgcn_permlane32_swap(dxp[g], dxp[4 + g], false, false);
+                    b0[g] = (int32_t)p[0];
+                    b1[g] = (int32_t)p[1];
+                }
+                const v16i c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(atile, b0, v16i{}, 0, 0, 0);
+                const v16i c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(atile, b1, v16i{}, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const auto p = __builtin_amdgcn_permlane32_swap((uint32_t)c0[v], (uint32_t)c1[v],
+                                                                    false, false);
+                    const int r = 32 + 8 * (v >> 2) + (v & 3);
+                    hr[r] += kScale1 ? (int32_t)p[0] : __mul24((int32_t)p[0], d.scale);
+                    hr[r + 4] += kScale1 ? (int32_t)p[1] : __mul24((int32_t)p[1], d.scale);
+                }
+            }
+            chain_half<32, kScale1>(hr, kr, (uint32_t)(myx >> 32), rows + 64, d.scale, dxp, eb);
+            const uint32_t f1 = flip_bits(dxp);
+            const uint64_t xn = myx ^ ((uint64_t)f1 << 32 | f0);
+            nacc += (uint32_t)__builtin_popcount(f0) + (uint32_t)__builtin_popcount(f1);
+            eacc += eb;
+            T.xold[lane] = myx;
+            T.xnew[lane] = xn;
+        } else if (more && fieldw) {
+            block_partial(d, xt, xs, (blk + 1) * 64, ch0, ch1, T.hacc, lane);
+        } else if (more) {
+            stage_keys(d, blk + 1, Dslot + (size_t)(blk + 1) * 1024, xt, xs,
+                       T.key[(blk + 1) & 1u], T.rows[(blk + 1) & 1u], T.diag[(blk + 1) & 1u],
+                       T.corr, T.lin, lane);
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const uint64_t xn = T.xnew[lane];
+            xt[lane * xs + blk] = xn;
+            const uint64_t w = wave_transpose(xn, lane);
+            const uint32_t p = blk * 64 + lane;
+            if (p < n) words[p] = w;
+        } else if (more && combw) {
+            combine_fields(d, T, blk + 1, rh, jh, lane);
+        }
+        __syncthreads();
+    }
+    if (wave == 0) {
+        s.energy[slot * 64 + lane] += eacc;
+        uint32_t tot = nacc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+        if (lane == 0 && tot) atomicAdd(s.accepted, (unsigned long long)tot);
+    }
+}
+
 bool set_lds_limit(const void* fn, size_t bytes) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
            hipSuccess;
@@ -391,9 +842,35 @@ bool set_lds_limit(const void* fn, size_t bytes) {
 
 size_t dense_lds_bytes_host(uint32_t n_pad) { return dense_lds_bytes(n_pad); }
 
+bool dense_bounds_eligible(const DevDense& d) {
+    return d.hdiag != nullptr && d.dtab < (1u << 22) && d.scale > 0 && d.scale < (1 << 22) &&
+           sizeof(BoundTiles) + dense_lds_bytes(d.n_pad) <= 160u * 1024u;
+}
+
+size_t dense_bounds_words(const DevDense& d, uint32_t cells) {
+    return (size_t)cells * ((d.n + 63) / 64) * 4096;
+}
+
 hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t sweep, uint64_t seed,
-                              const double* beta, hipStream_t stream) {
+                              const double* beta, int32_t* bounds, hipStream_t stream) {
     const size_t lds = dense_lds_bytes(d.n_pad);
+    if (bounds) {
+        const void* fn = d.scale == 1
+                             ? reinterpret_cast<const void*>(&sweep_dense_bound_kernel<true>)
+                             : reinterpret_cast<const void*>(&sweep_dense_bound_kernel<false>);
+        if (!dense_bounds_eligible(d) || !set_lds_limit(fn, lds)) return hipErrorInvalidValue;
+        const uint32_t waves = s.cells * ((d.n + 63) / 64) * 16;
+        int4* D = reinterpret_cast<int4*>(bounds);
+        hipLaunchKernelGGL(dense_bounds_kernel, dim3((waves + 3) / 4), dim3(256), 0, stream, d, s,
+                           beta, sweep, (uint32_t)seed, (uint32_t)(seed >> 32), D, waves);
+        if (d.scale == 1)
+            hipLaunchKernelGGL(sweep_dense_bound_kernel<true>, dim3(s.cells), dim3(kBoundThreads),
+                               lds, stream, d, s, D);
+        else
+            hipLaunchKernelGGL(sweep_dense_bound_kernel<false>, dim3(s.cells), dim3(kBoundThreads),
+                               lds, stream, d, s, D);
+        return hipGetLastError();
+    }
     if (!set_lds_limit(reinterpret_cast<const void*>(&sweep_dense_kernel), lds))
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(sweep_dense_kernel, dim3(s.cells), dim3(kDenseThreads), lds, stream, d, s,

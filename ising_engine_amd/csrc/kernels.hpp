@@ -29,6 +29,8 @@ struct DevDense {
     const int8_t* a = nullptr;      // n_pad^2 position space, zero padded, in 32x32
                                     // operand tiles (see block_fields)
     const int32_t* wdiag = nullptr; // [n_pad / 64][64][64] scale * a inside each diagonal block
+    const int8_t* hdiag = nullptr;  // [n_pad / 32][32][32] a inside each diagonal 32 x 32
+                                    // tile, row-major (32 bytes per row)
     const int32_t* lin = nullptr;   // n_pad linear terms, zero padded
     const uint32_t* order = nullptr;  // n: position -> site
     int32_t scale = 1;
@@ -169,9 +171,19 @@ hipError_t launch_exchange_replicas(const DevState& s, const double* beta, uint3
                                     uint32_t round, uint64_t seed, hipStream_t stream);
 
 // Dense sweep: all positions in order, one workgroup per slot, fields of each
-// block of 64 positions recomputed on int8 matrix cores.
+// block of 64 positions recomputed on int8 matrix cores.  bounds != NULL
+// (dense_bounds_words(d, cells) int32 of scratch): proposals are decided by
+// per-(position, replica) accept bounds computed before the sweep on the whole
+// GPU (dense_bounds_kernel, then sweep_dense_bound_kernel; requires
+// dense_bounds_eligible and every beta the sweep reads >= kDenseBoundMinBeta);
+// NULL: by the threshold band inside the sequential chain.
 hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t sweep, uint64_t seed,
-                              const double* beta, hipStream_t stream);
+                              const double* beta, int32_t* bounds, hipStream_t stream);
+size_t dense_bounds_words(const DevDense& d, uint32_t cells);
+// The accept-bound kernel keeps local fields, accept bounds and their sums in
+// 24-bit multiplies and needs beta >= kDenseBoundMinBeta for its error bound.
+constexpr double kDenseBoundMinBeta = 1e-6;
+bool dense_bounds_eligible(const DevDense& d);
 // Local fields of every (slot, site, lane) on the matrix cores, layout as
 // launch_fields.
 hipError_t launch_dense_fields(const DevDense& d, const DevState& s, int32_t* out,

@@ -85,6 +85,33 @@ def test_dense_parity_with_oracle(ctx, case):
     assert 0 < s.accepted < s.proposed
 
 
+@pytest.mark.parametrize("factory", [lambda: instances.sk_int8(200, 11),
+                                     lambda: instances.random_qubo(130, 7, 12)],
+                         ids=["sk_n200_scale4", "qubo_n130_scale1"])
+def test_dense_band_fallback_and_accept_bounds(ctx, factory):
+    """A ladder holding a beta below the accept bounds' minimum runs the whole call on the
+    in-chain threshold band; the same ladder without it runs on accept bounds.  Both equal
+    the oracle, and the two kernels continue each other's state bit for bit."""
+    qb = factory()
+    csr = ob.CSR.from_qubo(qb)
+    model = make(ctx, qb)
+    base = sk_betas(qb, 3) if qb.name.startswith("SK") else \
+        instances.geometric_betas(3.0, 0.3, 3) / max(1, int(np.abs(qb.q).max()) // 8)
+    cold = np.array(base, dtype=np.float64)
+    hot = cold.copy()
+    hot[0] = 5e-7  # below the accept bounds' minimum beta (1e-6): the band kernel runs
+    seed = 0xBA4D_0000_0000_0003
+    st = _lib.State(model, 3, seed, kernel=_lib.KERNEL_DENSE)
+    ost = ob.OracleState(csr, 3, seed, view=ob.VIEW_WORD)
+    for ladder in (hot, cold, hot, cold):
+        print(f"[dense] {qb.name} ladder beta0={ladder[0]:.3g}", flush=True)
+        st.set_ladder(ladder)
+        st.run(3, exchange=True)
+        ost.run(3, exchange=True, ladder=ladder)
+        compare(st, ost, csr, fields=False)
+    compare(st, ost, csr)
+
+
 def test_dense_equals_generic_kernel(ctx):
     """Same seed, same schedule: the dense and the lane-per-replica kernels walk the same
     trajectory (one position at a time vs a colour class at a time)."""
