@@ -176,7 +176,8 @@ struct ige_state {
     uint64_t resident_sweeps = 0;  // sweeps run by resident (LDS) runs
     uint64_t chain_sweeps = 0;     // sweeps run by chained launches
     DevBuf<uint32_t> lev_rows;     // per-sweep level thresholds of a resident run
-    DevBuf<uint32_t> chain_sync;   // chained runs: error flag, then a barrier counter per slot
+    DevBuf<uint32_t> chain_sync;   // chained runs: error flag, a barrier counter per slot, the
+                                   // XCDs of each slot's gang
     bool profile = false;
     double prof_ms = 0.0, prof_other_ms = 0.0;
     uint64_t prof_launches = 0, prof_other_launches = 0;
@@ -824,10 +825,10 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                            s->model->max_class <= ige::kChainMaxSites &&
                            ige::chain_ms_capacity_blocks(s->model->dmax) >= 1;
         if (chain) {
-            // [0]: error flag, [1 + slot]: barrier counters
-            if (s->chain_sync.count < s->cells + 1u) {
-                s->chain_sync.alloc(s->cells + 1u);
-                const std::vector<uint32_t> zero(s->cells + 1u, 0u);
+            // [0]: error flag, [1 + slot]: barrier counters, [1 + cells + slot]: XCDs
+            if (s->chain_sync.count < 2 * s->cells + 1u) {
+                s->chain_sync.alloc(2 * s->cells + 1u);
+                const std::vector<uint32_t> zero(2 * s->cells + 1u, 0u);
                 s->chain_sync.upload(zero.data(), zero.size(), s->stream);
             }
             const auto m = s->model->view();

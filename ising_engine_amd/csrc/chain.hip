@@ -11,12 +11,26 @@
 // irregular_site, uniforms keyed by position, slot and sweep, thresholds and
 // tie resolution), so the chain is bit-exact against the per-class launches.
 //
-// Measured on MI355X (ER N=20000, degree 6, 7 classes): 69 us a sweep against
-// 91 us for the captured per-class launches.  Each barrier's device-scope
-// acquire invalidates the caches of every workgroup taking part, so the chain
-// pays only with few workgroups: with 16 slots (192 workgroups) it took
-// 369 us a sweep against 107 us for the launches, and a lane-per-replica
-// form with one group of four positions per wave (182 workgroups) 181 us.
+// A device-scope barrier on a GPU of several XCDs writes back and
+// invalidates the L2 of every XCD taking part, so the following class reads
+// its rows and words again from memory.  The launch therefore places a
+// slot's workgroups on one XCD: it launches kXcdStride times as many
+// workgroups, of which those with blockIdx.x % kXcdStride == 0 take part
+// (workgroups are dispatched to the XCDs in turn), and the members check that
+// they share an XCD (the XCC_ID register) at a first device-scope barrier.
+// If they do, the class barriers neither write back nor invalidate any cache
+// (xcd_barrier: the words meet in the XCD's L2, and the site updates read
+// them with device-coherent loads, which go to that L2), so the model's rows
+// stay cached from class to class; otherwise every barrier is the
+// device-scope gang_barrier.
+//
+// Measured on MI355X (ER N=20000, degree 6, 7 classes): with device-scope
+// barriers 69-75 us a sweep against 91 us for the captured per-class
+// launches; on one XCD with coherent word loads 64.5 us, and with each
+// wave's sites counted on the narrowest planes their degrees allow 59.9 us.
+// With 16 slots (192 workgroups, device-scope barriers) 369 us against 107 us,
+// and a lane-per-replica form with one group of four positions per wave (182
+// workgroups) 181 us.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -31,14 +45,17 @@ namespace {
 
 constexpr uint32_t kChainBlock = 256;  // 4 waves
 constexpr uint32_t kChainWaves = kChainBlock / 64;
+constexpr uint32_t kXcdStride = 8;     // XCDs of the GPU: workgroups per member
 }  // namespace
 
 constexpr uint32_t kChainMaxBlocks = kChainMaxSites / (64 * 4);  // workgroups per slot
 
 namespace {
 // Bit-sliced counts and plane-view uniforms as sweep_multispin_kernel:
-// blockIdx.y = slot, gridDim.x workgroups per slot meeting at the slot's own
-// barrier bar[slot] after each class.  Wave w of a slot takes positions
+// blockIdx.y = slot; of the gridDim.x workgroups per slot, the gang of
+// gridDim.x / kXcdStride with blockIdx.x % kXcdStride == 0 takes part and
+// meets at the slot's own barrier bar[slot] after each class (bar[cells +
+// slot]: the XCDs its members run on).  Wave w of a slot takes positions
 // cb + 64 w + lane, + 64 W, ... of every class (W waves per slot), queueing
 // its tied replicas (entries relative to the class start) and draining them
 // before the barrier, as classes_sweep_kernel does in one workgroup.  Sweep k
@@ -62,13 +79,15 @@ __global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
     __shared__ unsigned long long edef[kChainWaves][64];
     __shared__ unsigned long long blk_e[64];
     __shared__ uint32_t blk_acc;
+    if (blockIdx.x % kXcdStride != 0) return;  // not a member: on another XCD
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t slot = blockIdx.y;
-    const uint32_t gang = gridDim.x;
-    const uint32_t gw = blockIdx.x * kChainWaves + wv, tw = gang * kChainWaves;
+    const uint32_t gang = gridDim.x / kXcdStride;
+    const uint32_t gw = (blockIdx.x / kXcdStride) * kChainWaves + wv, tw = gang * kChainWaves;
     uint64_t* S = spins + (uint64_t)slot * n;
     uint32_t* sbar = bar + slot;
+    uint32_t* sxcd = bar + gridDim.y + slot;
     const uint32_t tr = temp_of[slot * 64 + lane];
     edef[wv][lane] = 0;
     if (wv == 0) blk_e[lane] = 0;
@@ -78,6 +97,13 @@ __global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
     for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
     int64_t e_lane = 0;
     uint32_t nacc = 0, pending = 0, target = 0;
+    bool one_xcd = false;
+    if (gang > 1) {
+        if (tid == 0) atomicOr(sxcd, 1u << xcc_id());
+        gang_barrier(sbar, target, gang, spin_limit, err);
+        one_xcd = __builtin_popcount(
+                      __hip_atomic_load(sxcd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 1;
+    }
     auto flush = [&]() {
         const int64_t ksum = wave_column_sum<W>(kacc, lane);
         const int64_t dsum = wave_column_sum<W>(dacc, lane);
@@ -97,17 +123,21 @@ __global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
             __builtin_amdgcn_wave_barrier();
             for (uint32_t base = cb + gw * 64; base < ce; base += tw * 64) {
                 const uint32_t p = base + lane;
+                const uint32_t dw = wave_max_u32(p < ce ? rp[p + 1] - rp[p] : 0u);
                 if (p < ce)
-                    irregular_site<B, W>(S, S, p, p - cb, rp, ci, sgn, recs, slot, sweep, key0,
-                                         key1, w0, kacc, dacc, nacc, queue[wv], &qcount[wv],
-                                         kChainQueueCap, edef[wv]);
+                    irregular_site_fit<B, W, true>(dw, S, S, p, p - cb, rp, ci, sgn, recs, slot,
+                                                   sweep, key0, key1, w0, kacc, dacc, nacc,
+                                                   queue[wv], &qcount[wv], kChainQueueCap,
+                                                   edef[wv]);
                 if (++pending == kFlush) flush();  // wave-uniform
             }
             __threadfence_block();
             __builtin_amdgcn_wave_barrier();
             drain_queue(queue[wv], &qcount[wv], kChainQueueCap, cb, S, recs, slot, sweep, key0,
                         key1, w0, edef[wv], lane, nacc);
-            if (gang > 1)
+            if (gang > 1 && one_xcd)
+                xcd_barrier(sbar, target, gang, spin_limit, err);
+            else if (gang > 1)
                 gang_barrier(sbar, target, gang, spin_limit, err);
             else
                 __syncthreads();
@@ -166,21 +196,23 @@ hipError_t launch_sweep_chain_ms(const DevModel& m, const DevState& s, const uin
     const uint32_t cap = chain_ms_capacity_blocks(m.dmax);
     if (cap < s.cells || m.dmax >= 32) return hipErrorInvalidValue;
     // one site per lane of the largest class, at most kChainMaxBlocks
-    // workgroups per slot and what the GPU holds for every slot at once
+    // workgroups per slot and what one XCD holds for every slot at once
     const uint32_t want = (max_class + 64 * kChainWaves - 1) / (64 * kChainWaves);
-    uint32_t per_slot = cap / s.cells;
+    uint32_t per_slot = cap / kXcdStride / s.cells;
     if (per_slot > kChainMaxBlocks) per_slot = kChainMaxBlocks;
     if (want < per_slot) per_slot = want ? want : 1u;
+    if (per_slot == 0) return hipErrorInvalidValue;
     const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
     const uint64_t per_launch = ((1ull << 31) / per_slot) / (ncol ? ncol : 1u);
     for (uint32_t done = 0; done < sweeps;) {
         const uint32_t k =
             (uint32_t)((uint64_t)(sweeps - done) < per_launch ? sweeps - done : per_launch);
-        hipLaunchKernelGGL(chain_fill_kernel, dim3(1), dim3(256), 0, stream, bar, s.cells);
+        hipLaunchKernelGGL(chain_fill_kernel, dim3(1), dim3(256), 0, stream, bar, 2 * s.cells);
         const uint32_t* L = lev + (uint64_t)done * lev_stride;
 #define IGE_CHAIN_MS(B, W)                                                                       \
-    hipLaunchKernelGGL((sweep_chain_ms_kernel<B, W>), dim3(per_slot, s.cells), dim3(kChainBlock), \
+    hipLaunchKernelGGL((sweep_chain_ms_kernel<B, W>), dim3(per_slot * kXcdStride, s.cells),     \
+                       dim3(kChainBlock),                                                        \
                        lds, stream, m.rp, m.ci, m.sgn, cstart, ncol, s.spins, m.n, m.dmax, m.w0, \
                        L, lev_stride, s.temp_of, s.energy, s.accepted, sweep0 + done, k, key0,   \
                        key1, bar, spin_limit, err)

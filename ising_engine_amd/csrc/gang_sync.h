@@ -38,4 +38,42 @@ __device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, ui
     __syncthreads();
 }
 
+// Barrier across `gang` workgroups that all run on one XCD, so share one L2,
+// for data the members read with device-coherent loads (which go to the L2,
+// not the CU's L1).  Every wave waits for its own stores and atomics (they
+// then sit in that L2), the waves meet, and one thread arrives on the counter
+// and polls it as above.  Nothing is written back or invalidated, so data
+// the members only read stays cached.  Measured on MI355X with 12 workgroups
+// of one XCD (20000 rounds of write, barrier, coherent read of another
+// member's words): 0.56 us per barrier, no stale read; gang_barrier 2.05 us.
+// Plain loads after it can be stale (they were in that test even after a
+// workgroup-scope L1 invalidation).  Callers must have checked that every
+// member runs on the same XCD (xcc_id below).
+__device__ __forceinline__ void xcd_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
+                                            uint32_t spin_limit, uint32_t* err) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    target += gang;
+    if (threadIdx.x == 0) {
+        atomicAdd(bar, 1u);
+        uint32_t polls = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+            if (++polls > spin_limit) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+// The XCD (accelerator complex die) this wave runs on.
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+    return x;
+}
+
 }  // namespace ige

@@ -130,9 +130,10 @@ constexpr uint32_t kFuseMaxClassSites = 512;
 
 // Sweeps [sweep0, sweep0 + sweeps) of a multispin state of irregular degree
 // (dmax < 32) without exchange in one launch (a few when a barrier counter
-// would overflow): every class of every sweep, gridDim.x workgroups per slot
-// meeting at the slot's barrier bars[slot] (zeroed here; err raised when a
-// wait expires) after each class; the same updates as launch_sweep_multispin
+// would overflow): every class of every sweep, a gang of workgroups per slot
+// meeting at the slot's barrier bars[slot] (bars: 2 * cells words, zeroed
+// here; err raised when a wait expires) after each class, on one XCD when
+// dispatch puts them there; the same updates as launch_sweep_multispin
 // class by class.  Sweep k's level thresholds are lev + k * lev_stride
 // (lev_thr layout).  max_class: positions in the largest class, at most
 // kChainMaxSites (one site per lane of kChainMaxSites / 256 workgroups).

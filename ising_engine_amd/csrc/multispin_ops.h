@@ -74,6 +74,19 @@ __device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
+// Word i of S: a plain load, or (kCoherent) a device-coherent one that reads
+// the L2 and not the CU's L1, for words other workgroups may have written
+// since this CU last read them with no cache invalidation in between.
+template <bool kCoherent>
+__device__ __forceinline__ uint64_t ld_word(const uint64_t* S, uint32_t i) {
+    if constexpr (kCoherent)
+        return __hip_atomic_load(const_cast<uint64_t*>(reinterpret_cast<const uint64_t*>(
+                                     reinterpret_cast<const char*>(S) + i * 8u)),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return ld32(S, i);
+}
+
 // All ones if bit e of sg is set, else zero.
 __device__ __forceinline__ uint32_t bit_mask(uint32_t sg, uint32_t e) {
     return (uint32_t)((int32_t)(sg << (31u - e)) >> 31);
@@ -341,8 +354,8 @@ __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, ui
 // planes as regular_site, unsigned counters kacc += k flip and dacc += d flip,
 // and one queue entry per tied replica, (qoff << 11) | (r << 5) | m with qoff
 // = p - (the queue's base position) < 2^21, resolved in place when the queue
-// (qcap entries) is full.
-template <int B, int W>
+// (qcap entries) is full.  kCoherent: words are read with ld_word<true>.
+template <int B, int W, bool kCoherent = false>
 __device__ __forceinline__ void irregular_site(
     const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t qoff, const uint32_t* rp,
     const uint32_t* ci, const uint32_t* sgn, const unsigned long long* recs, uint32_t slot,
@@ -350,7 +363,7 @@ __device__ __forceinline__ void irregular_site(
     uint64_t (&dacc)[W], uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t qcap,
     unsigned long long* edef) {
     constexpr int LMAX = 1 << (B - 1);
-    const uint64_t x = ld32(S, p);
+    const uint64_t x = ld_word<kCoherent>(S, p);
     const uint32_t e0 = ld32(rp, p);
     const uint32_t d = ld32(rp, p + 1) - e0;
     const uint32_t sg = ld32(sgn, p);
@@ -369,7 +382,7 @@ __device__ __forceinline__ void irregular_site(
 #pragma unroll
         for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ld32(ci, e0 + min(g + q, d - 1));
 #pragma unroll
-        for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld32(S, j[q]);
+        for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld_word<kCoherent>(S, j[q]);
 #pragma unroll
         for (uint32_t q = 0; q < kRowGroup; ++q) {
             const uint32_t e = g + q;
@@ -463,6 +476,39 @@ __device__ __forceinline__ void irregular_site(
             nacc += 1;
         }
     }
+}
+
+// Largest value of v over the wave's lanes (lanes that left early may add
+// their stale values, which can only make the result larger).
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    return v;
+}
+
+// irregular_site on the narrowest count planes the wave's sites allow: a
+// site's update depends on its degree, its counts, uniforms and thresholds,
+// not on the planes' width, so when every degree dw of the wave is below
+// 2^(B-1) the wave counts on B - 1 planes, with half the levels to compare.
+// (An ER graph of mean degree 6 and N = 20000 has a maximum degree near 20,
+// so B = 5, but almost every wave of 64 sites stays below 16.)
+template <int B, int W, bool kCoherent = false>
+__device__ __forceinline__ void irregular_site_fit(
+    uint32_t dw, const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t qoff, const uint32_t* rp,
+    const uint32_t* ci, const uint32_t* sgn, const unsigned long long* recs, uint32_t slot,
+    uint32_t sweep, uint32_t key0, uint32_t key1, int32_t w0, uint64_t (&kacc)[W],
+    uint64_t (&dacc)[W], uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t qcap,
+    unsigned long long* edef) {
+    if constexpr (B > 3) {
+        if (dw < (1u << (B - 1))) {
+            irregular_site<B - 1, W, kCoherent>(S, Wr, p, qoff, rp, ci, sgn, recs, slot, sweep,
+                                                key0, key1, w0, kacc, dacc, nacc, queue, qcount,
+                                                qcap, edef);
+            return;
+        }
+    }
+    irregular_site<B, W, kCoherent>(S, Wr, p, qoff, rp, ci, sgn, recs, slot, sweep, key0, key1,
+                                    w0, kacc, dacc, nacc, queue, qcount, qcap, edef);
 }
 
 // Drains a wave's per-replica tie queue (irregular_site entries relative to
