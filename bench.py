@@ -71,7 +71,8 @@ def measured_counters(kernel: str, workload: str) -> dict:
 def counter_probe(_lib, instances, L: int, temps: int, sweeps: int, workload: str = "pt",
                   waves: int = 8):
     """The program the counter passes run: `sweeps` PT steps of the workload (for workload
-    "sk", sweeps of the dense SK N=4096 anneal; for "batch", sweeps [0, sweeps) of the
+    "sk", sweeps of the dense SK N=4096 anneal; for "er", the first sweeps of the ER
+    N=20000 anneal in one call (one chained launch); for "batch", sweeps [0, sweeps) of the
     4096-instance QUBO batch in one persistent launch of `waves` waves per workgroup), then
     one 1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE readings calibrate the
     counters' units."""
@@ -82,6 +83,13 @@ def counter_probe(_lib, instances, L: int, temps: int, sweeps: int, workload: st
         st = _lib.Batch(ctx, wl["qubos"], wl["seeds"], wl["sweeps"], wl["beta_lo"],
                         wl["beta_hi"])
         st.run(_lib.BATCH_PERSISTENT, waves=waves, k0=0, k1=sweeps)
+    elif workload == "er":
+        wl = instances.workload_er_anneal()
+        qb = wl["qubo"]
+        m = _lib.Model(ctx, qb.n, qb.offset, qb.linear, qb.pi, qb.pj, qb.q)
+        st = _lib.State(m, 1, 3, kernel=_lib.KERNEL_MULTISPIN)
+        sched = _lib.anneal_schedule(wl["beta_lo"], wl["beta_hi"], wl["sweeps"])
+        st.run(sweeps, beta_schedule=sched[:sweeps, None])
     elif workload == "sk":
         wl = instances.workload_sk_anneal()
         qb = wl["qubo"]
@@ -699,7 +707,7 @@ def main():
     ap.add_argument("--counter-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS steps of --probe-workload and a 1 GiB copy "
                          "(for rocprofv3 --pmc passes)")
-    ap.add_argument("--probe-workload", choices=["pt", "sk", "batch"], default="pt")
+    ap.add_argument("--probe-workload", choices=["pt", "sk", "er", "batch"], default="pt")
     ap.add_argument("--probe-waves", type=int, default=8,
                     help="waves per workgroup of the batch counter probe")
     args = ap.parse_args()
