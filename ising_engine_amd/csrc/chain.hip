@@ -5,11 +5,20 @@
 // For graphs whose classes are too large for one workgroup but too small to
 // fill the GPU in a launch each (ER N=20000: 7 classes of about 2900 sites),
 // the per-class launches cost more than the class itself.  Here the launch's
-// waves keep a fixed share of every class (wave w takes positions cb + 64 w +
-// lane, + 64 W, ...), so a class costs one site update per lane plus one
-// barrier.  The updates are those of launch_sweep_multispin (the same
-// irregular_site, uniforms keyed by position, slot and sweep, thresholds and
-// tie resolution), so the chain is bit-exact against the per-class launches.
+// waves keep a fixed share of every class (wave w takes positions cb + 32 w +
+// lane / 2, + 32 W, ..., a lane pair per position: irregular_site_pair), so a
+// class costs one site update per lane plus one barrier.  The updates are
+// those of launch_sweep_multispin (uniforms keyed by position, slot and
+// sweep, the same thresholds and tie resolution), so the chain is bit-exact
+// against the per-class launches.
+//
+// A class's time is the latency of its slowest site update, so the update
+// is cut for latency: a position's 64 replicas on two lanes (half the bit
+// operations and Philox calls per lane), its row (degree, signs, the first
+// neighbour positions) loaded while the wave waits at the barrier before the
+// class, all of its neighbour words loaded at once (only up to the degree: a
+// class's gathers are bound by the L2's request rate), and the counts from a
+// carry-save tree.
 //
 // A device-scope barrier on a GPU of several XCDs writes back and
 // invalidates the L2 of every XCD taking part, so the following class reads
@@ -28,9 +37,12 @@
 // barriers 69-75 us a sweep against 91 us for the captured per-class
 // launches; on one XCD with coherent word loads 64.5 us, and with each
 // wave's sites counted on the narrowest planes their degrees allow 59.9 us.
-// With 16 slots (192 workgroups, device-scope barriers) 369 us against 107 us,
-// and a lane-per-replica form with one group of four positions per wave (182
-// workgroups) 181 us.
+// A lane per position: 60.1 us; lane pairs, rows loaded at the barrier and
+// the words in one batch of masked loads 50.0 us; carry-save counts 48.8 us;
+// the next sweep's records built at the last barrier 49.4 against 50.4 us
+// (same GPU, alternating runs).  With 16 slots (192 workgroups,
+// device-scope barriers) 369 us against 107 us, and a lane-per-replica form
+// with one group of four positions per wave (182 workgroups) 181 us.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -46,9 +58,12 @@ namespace {
 constexpr uint32_t kChainBlock = 256;  // 4 waves
 constexpr uint32_t kChainWaves = kChainBlock / 64;
 constexpr uint32_t kXcdStride = 8;     // XCDs of the GPU: workgroups per member
+constexpr uint32_t kPairSites = 32;    // positions per wave and pass: a lane pair each
 }  // namespace
 
 constexpr uint32_t kChainMaxBlocks = kChainMaxSites / (64 * 4);  // workgroups per slot
+static_assert(kChainMaxBlocks * kChainWaves * kPairSites * 2 >= kChainMaxSites,
+              "a wave takes at most two passes of a class");
 
 namespace {
 // Bit-sliced counts and plane-view uniforms as sweep_multispin_kernel:
@@ -56,14 +71,15 @@ namespace {
 // gridDim.x / kXcdStride with blockIdx.x % kXcdStride == 0 takes part and
 // meets at the slot's own barrier bar[slot] after each class (bar[cells +
 // slot]: the XCDs its members run on).  Wave w of a slot takes positions
-// cb + 64 w + lane, + 64 W, ... of every class (W waves per slot), queueing
-// its tied replicas (entries relative to the class start) and draining them
-// before the barrier, as classes_sweep_kernel does in one workgroup.  Sweep k
-// reads its level thresholds from lev + k * lev_stride.
+// cb + 32 w + lane / 2, + 32 W, ... of every class (W waves per slot; lane &
+// 1 picks the replica half), queueing its tied replicas (entries relative to
+// the class start) and draining them before the barrier, as
+// classes_sweep_kernel does in one workgroup.  Sweep k reads its level
+// thresholds from lev + k * lev_stride.
 constexpr uint32_t kChainQueueCap = 1024;  // tied replicas per wave and class
 
 template <int B, int W>
-__global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
+__global__ __launch_bounds__(kChainBlock, 2) void sweep_chain_ms_kernel(
     const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
     const uint32_t* __restrict__ sgn, const uint32_t* __restrict__ cstart, uint32_t ncol,
     uint64_t* spins, uint32_t n, uint32_t dmax, int32_t w0, const uint32_t* __restrict__ lev,
@@ -73,7 +89,7 @@ __global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
     using namespace msc;
     static_assert(W >= B + 3, "counters too narrow");
     constexpr uint32_t kFlush = 8;  // sites per lane between counter flushes
-    extern __shared__ unsigned long long recs[];  // (dmax + 1) * kQStride
+    extern __shared__ unsigned long long recs[];  // 2 buffers of (2^B + 1) * kQStride
     __shared__ uint32_t queue[kChainWaves][kChainQueueCap];
     __shared__ uint32_t qcount[kChainWaves];
     __shared__ unsigned long long edef[kChainWaves][64];
@@ -92,7 +108,7 @@ __global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
     edef[wv][lane] = 0;
     if (wv == 0) blk_e[lane] = 0;
     if (tid == 0) blk_acc = 0;
-    uint64_t kacc[W], dacc[W];
+    uint32_t kacc[W], dacc[W];
 #pragma unroll
     for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
     int64_t e_lane = 0;
@@ -105,40 +121,70 @@ __global__ __launch_bounds__(kChainBlock) void sweep_chain_ms_kernel(
                       __hip_atomic_load(sxcd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 1;
     }
     auto flush = [&]() {
-        const int64_t ksum = wave_column_sum<W>(kacc, lane);
-        const int64_t dsum = wave_column_sum<W>(dacc, lane);
+        const int64_t ksum = wave_column_sum_pair<W>(kacc, lane);
+        const int64_t dsum = wave_column_sum_pair<W>(dacc, lane);
         e_lane += ((int64_t)w0 * (2 * ksum - dsum)) / 2;
 #pragma unroll
         for (int b = 0; b < W; ++b) kacc[b] = dacc[b] = 0;
         pending = 0;
     };
+    // The row of this wave's next position and the wave's largest degree
+    // there.  Rows are read-only, so the first pass of a class loads them
+    // while the wave waits at the barrier before it.
+    PairRow row;
+    uint32_t dw = 0;
+    auto load_row = [&](uint32_t base, uint32_t ce) {
+        const uint32_t p = base + (lane >> 1);
+        if (p < ce) row = load_pair_row<B>(p, rp, ci, sgn);
+        dw = __builtin_amdgcn_readfirstlane(wave_max_u32(p < ce ? row.d : 0u));
+    };
+    auto first_row = [&](uint32_t c) {
+        load_row(__builtin_amdgcn_readfirstlane(cstart[c]) + gw * kPairSites,
+                 __builtin_amdgcn_readfirstlane(cstart[c + 1]));
+    };
+    first_row(0);
+    // Two record buffers: sweep k reads buffer k & 1, and the next sweep's
+    // records are built while the last class's barrier is pending.
+    constexpr uint32_t kRecWords = ((1u << B) + 1) * kQStride;
+    if (sweeps > 0)
+        build_records(recs, lev, tr, dmax, wv, kChainWaves, lane);
+    __syncthreads();
     for (uint32_t k = 0; k < sweeps; ++k) {
         const uint32_t sweep = sweep0 + k;
-        build_records(recs, lev + (uint64_t)k * lev_stride, tr, dmax, wv, kChainWaves, lane);
-        __syncthreads();
+        const unsigned long long* R = recs + (k & 1u) * kRecWords;
         for (uint32_t c = 0; c < ncol; ++c) {
             const uint32_t cb = __builtin_amdgcn_readfirstlane(cstart[c]);
             const uint32_t ce = __builtin_amdgcn_readfirstlane(cstart[c + 1]);
             if (lane == 0) qcount[wv] = 0;
             __builtin_amdgcn_wave_barrier();
-            for (uint32_t base = cb + gw * 64; base < ce; base += tw * 64) {
-                const uint32_t p = base + lane;
-                const uint32_t dw = wave_max_u32(p < ce ? rp[p + 1] - rp[p] : 0u);
+            for (uint32_t base = cb + gw * kPairSites; base < ce; base += tw * kPairSites) {
+                const uint32_t p = base + (lane >> 1);
                 if (p < ce)
-                    irregular_site_fit<B, W, true>(dw, S, S, p, p - cb, rp, ci, sgn, recs, slot,
-                                                   sweep, key0, key1, w0, kacc, dacc, nacc,
-                                                   queue[wv], &qcount[wv], kChainQueueCap,
-                                                   edef[wv]);
+                    irregular_site_pair<B, W>(dw, S, S, p, lane & 1u, p - cb, row, ci, R,
+                                                  slot, sweep, key0, key1, w0, kacc, dacc, nacc,
+                                                  queue[wv], &qcount[wv], kChainQueueCap,
+                                                  edef[wv]);
                 if (++pending == kFlush) flush();  // wave-uniform
+                if (base + tw * kPairSites < ce) load_row(base + tw * kPairSites, ce);
             }
             __threadfence_block();
             __builtin_amdgcn_wave_barrier();
-            drain_queue(queue[wv], &qcount[wv], kChainQueueCap, cb, S, recs, slot, sweep, key0,
+            drain_queue(queue[wv], &qcount[wv], kChainQueueCap, cb, S, R, slot, sweep, key0,
                         key1, w0, edef[wv], lane, nacc);
+            // Arrive, load the next class's rows, build the next sweep's
+            // records after the last class and flush counters, then wait.
             if (gang > 1 && one_xcd)
-                xcd_barrier(sbar, target, gang, spin_limit, err);
+                xcd_arrive(sbar, target, gang);
             else if (gang > 1)
-                gang_barrier(sbar, target, gang, spin_limit, err);
+                gang_arrive(sbar, target, gang);
+            first_row(c + 1 < ncol ? c + 1 : 0);
+            if (c + 1 == ncol && k + 1 < sweeps)
+                build_records(recs + ((k + 1) & 1u) * kRecWords,
+                              lev + (uint64_t)(k + 1) * lev_stride, tr, dmax, wv, kChainWaves,
+                              lane);
+            if (2 * pending >= kFlush) flush();
+            if (gang > 1)
+                gang_wait(sbar, target, spin_limit, err, !one_xcd);
             else
                 __syncthreads();
         }
@@ -164,6 +210,13 @@ __global__ void chain_fill_kernel(uint32_t* p, uint32_t count) {
 
 namespace {
 
+// Two buffers of threshold records of levels 0 .. 2^B for the count planes
+// B of degree dmax (irregular_site_pair reads levels up to 2^B).
+uint32_t chain_lds(uint32_t dmax) {
+    const uint32_t planes = dmax < 8 ? 3 : dmax < 16 ? 4 : 5;
+    return 2 * ((1u << planes) + 1) * kQStride * sizeof(unsigned long long);
+}
+
 template <int B, int W>
 int chain_ms_per_cu(uint32_t lds) {
     int pc = 0;
@@ -181,7 +234,7 @@ uint32_t chain_ms_capacity_blocks(uint32_t dmax) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    const uint32_t lds = (dmax + 1) * kQStride * sizeof(unsigned long long);
+    const uint32_t lds = chain_lds(dmax);
     const int pc = dmax < 8 ? chain_ms_per_cu<3, 6>(lds)
                  : dmax < 16 ? chain_ms_per_cu<4, 7>(lds)
                  : dmax < 32 ? chain_ms_per_cu<5, 8>(lds) : 0;
@@ -197,12 +250,12 @@ hipError_t launch_sweep_chain_ms(const DevModel& m, const DevState& s, const uin
     if (cap < s.cells || m.dmax >= 32) return hipErrorInvalidValue;
     // one site per lane of the largest class, at most kChainMaxBlocks
     // workgroups per slot and what one XCD holds for every slot at once
-    const uint32_t want = (max_class + 64 * kChainWaves - 1) / (64 * kChainWaves);
+    const uint32_t want = (max_class + kPairSites * kChainWaves - 1) / (kPairSites * kChainWaves);
     uint32_t per_slot = cap / kXcdStride / s.cells;
     if (per_slot > kChainMaxBlocks) per_slot = kChainMaxBlocks;
     if (want < per_slot) per_slot = want ? want : 1u;
     if (per_slot == 0) return hipErrorInvalidValue;
-    const uint32_t lds = (m.dmax + 1) * kQStride * sizeof(unsigned long long);
+    const uint32_t lds = chain_lds(m.dmax);
     const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
     const uint64_t per_launch = ((1ull << 31) / per_slot) / (ncol ? ncol : 1u);
     for (uint32_t done = 0; done < sweeps;) {

@@ -17,13 +17,23 @@ namespace ige {
 // per workgroup fences, and the polls are relaxed: fences issued by every
 // wave and acquire polls made a barrier of 64 workgroups cost 9.4 us, this
 // form 2.4 us (256 workgroups: 40 and 10 us; MI355X, 2000 barriers timed).
-__device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
-                                             uint32_t spin_limit, uint32_t* err) {
+//
+// gang_arrive and gang_wait are its two halves, for a caller with work
+// between them that reads nothing the other members write.
+__device__ __forceinline__ void gang_arrive(uint32_t* bar, uint32_t& target, uint32_t gang) {
     __syncthreads();
     target += gang;
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         atomicAdd(bar, 1u);
+    }
+}
+
+// acquire: invalidate at device scope after the wait (gang_arrive's
+// partner); xcd_arrive's partner waits without it.
+__device__ __forceinline__ void gang_wait(uint32_t* bar, uint32_t target, uint32_t spin_limit,
+                                          uint32_t* err, bool acquire) {
+    if (threadIdx.x == 0) {
         uint32_t polls = 0;
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
@@ -33,40 +43,34 @@ __device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, ui
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (acquire) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
 
-// Barrier across `gang` workgroups that all run on one XCD, so share one L2,
-// for data the members read with device-coherent loads (which go to the L2,
-// not the CU's L1).  Every wave waits for its own stores and atomics (they
-// then sit in that L2), the waves meet, and one thread arrives on the counter
-// and polls it as above.  Nothing is written back or invalidated, so data
-// the members only read stays cached.  Measured on MI355X with 12 workgroups
-// of one XCD (20000 rounds of write, barrier, coherent read of another
-// member's words): 0.56 us per barrier, no stale read; gang_barrier 2.05 us.
-// Plain loads after it can be stale (they were in that test even after a
-// workgroup-scope L1 invalidation).  Callers must have checked that every
-// member runs on the same XCD (xcc_id below).
-__device__ __forceinline__ void xcd_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
-                                            uint32_t spin_limit, uint32_t* err) {
+__device__ __forceinline__ void gang_barrier(uint32_t* bar, uint32_t& target, uint32_t gang,
+                                             uint32_t spin_limit, uint32_t* err) {
+    gang_arrive(bar, target, gang);
+    gang_wait(bar, target, spin_limit, err, true);
+}
+
+// Arrival at a barrier across `gang` workgroups that all run on one XCD, so
+// share one L2, for data the members read with device-coherent loads (which
+// go to the L2, not the CU's L1); gang_wait(..., false) completes it.  Every
+// wave waits for its own stores and atomics (they then sit in that L2), the
+// waves meet, and one thread arrives on the counter.  Nothing is written
+// back or invalidated, so data the members only read stays cached.
+// Measured on MI355X with 12 workgroups of one XCD (20000 rounds of write,
+// barrier, coherent read of another member's words): 0.56 us per barrier,
+// no stale read; gang_barrier 2.05 us.  Plain loads after it can be stale
+// (they were in that test even after a workgroup-scope L1 invalidation).
+// Callers must have checked that every member runs on the same XCD (xcc_id
+// below).
+__device__ __forceinline__ void xcd_arrive(uint32_t* bar, uint32_t& target, uint32_t gang) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     target += gang;
-    if (threadIdx.x == 0) {
-        atomicAdd(bar, 1u);
-        uint32_t polls = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-            if (++polls > spin_limit) {
-                atomicOr(err, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(bar, 1u);
 }
 
 // The XCD (accelerator complex die) this wave runs on.

@@ -136,7 +136,8 @@ constexpr uint32_t kFuseMaxClassSites = 512;
 // dispatch puts them there; the same updates as launch_sweep_multispin
 // class by class.  Sweep k's level thresholds are lev + k * lev_stride
 // (lev_thr layout).  max_class: positions in the largest class, at most
-// kChainMaxSites (one site per lane of kChainMaxSites / 256 workgroups).
+// kChainMaxSites (a lane pair per position: two passes of kChainMaxSites /
+// 256 workgroups).
 // Needs cells <= chain_ms_capacity_blocks(dmax).
 constexpr uint32_t kChainMaxSites = 32768;
 hipError_t launch_sweep_chain_ms(const DevModel& m, const DevState& s, const uint32_t* cstart,

@@ -74,19 +74,6 @@ __device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
-// Word i of S: a plain load, or (kCoherent) a device-coherent one that reads
-// the L2 and not the CU's L1, for words other workgroups may have written
-// since this CU last read them with no cache invalidation in between.
-template <bool kCoherent>
-__device__ __forceinline__ uint64_t ld_word(const uint64_t* S, uint32_t i) {
-    if constexpr (kCoherent)
-        return __hip_atomic_load(const_cast<uint64_t*>(reinterpret_cast<const uint64_t*>(
-                                     reinterpret_cast<const char*>(S) + i * 8u)),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else
-        return ld32(S, i);
-}
-
 // All ones if bit e of sg is set, else zero.
 __device__ __forceinline__ uint32_t bit_mask(uint32_t sg, uint32_t e) {
     return (uint32_t)((int32_t)(sg << (31u - e)) >> 31);
@@ -139,22 +126,34 @@ __device__ __forceinline__ uint32_t record_thr(const unsigned long long* recs, u
 // Builds a slot's threshold records in LDS for levels m = m0, m0 + step, ...,
 // <= dmax: lane r reads thr(temperature of lane r, m) from lev_thr ([cells][dmax
 // + 1]) and nine ballots make the planes.  Every lane of the wave takes part.
+// The reads of eight levels are issued before their ballots.
 __device__ __forceinline__ void build_records(unsigned long long* recs, const uint32_t* lev_thr,
                                               uint32_t tr, uint32_t dmax, uint32_t m0,
                                               uint32_t step, uint32_t lane) {
-    for (uint32_t m = m0; m <= dmax; m += step) {
-        const uint32_t th = lev_thr[tr * (dmax + 1) + m];
-        unsigned long long* out = recs + m * kQStride;
-        uint64_t mine = 0;
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t m1 = m0; m1 <= dmax; m1 += kBatch * step) {
+        uint32_t th[kBatch];
 #pragma unroll
-        for (uint32_t b = 0; b < kEagerPlanes; ++b) {
-            const uint64_t plane = wave_ballot((th >> (31u - b)) & 1u);
-            if (lane == b) mine = plane;
+        for (uint32_t i = 0; i < kBatch; ++i) {
+            const uint32_t m = m1 + i * step;
+            th[i] = m <= dmax ? lev_thr[tr * (dmax + 1) + m] : 0u;
         }
-        const uint64_t nz = wave_ballot(th != 0);
-        if (lane < kEagerPlanes) out[lane] = mine;
-        if (lane == kEagerPlanes) out[kEagerPlanes] = nz;
-        reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[lane] = th;
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) {
+            const uint32_t m = m1 + i * step;
+            if (m > dmax) break;  // uniform
+            unsigned long long* out = recs + m * kQStride;
+            uint64_t mine = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < kEagerPlanes; ++b) {
+                const uint64_t plane = wave_ballot((th[i] >> (31u - b)) & 1u);
+                if (lane == b) mine = plane;
+            }
+            const uint64_t nz = wave_ballot(th[i] != 0);
+            if (lane < kEagerPlanes) out[lane] = mine;
+            if (lane == kEagerPlanes) out[kEagerPlanes] = nz;
+            reinterpret_cast<uint32_t*>(out + kEagerPlanes + 1)[lane] = th[i];
+        }
     }
 }
 
@@ -354,8 +353,8 @@ __device__ __forceinline__ void regular_site(const uint64_t* S, uint64_t* Wr, ui
 // planes as regular_site, unsigned counters kacc += k flip and dacc += d flip,
 // and one queue entry per tied replica, (qoff << 11) | (r << 5) | m with qoff
 // = p - (the queue's base position) < 2^21, resolved in place when the queue
-// (qcap entries) is full.  kCoherent: words are read with ld_word<true>.
-template <int B, int W, bool kCoherent = false>
+// (qcap entries) is full.
+template <int B, int W>
 __device__ __forceinline__ void irregular_site(
     const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t qoff, const uint32_t* rp,
     const uint32_t* ci, const uint32_t* sgn, const unsigned long long* recs, uint32_t slot,
@@ -363,7 +362,7 @@ __device__ __forceinline__ void irregular_site(
     uint64_t (&dacc)[W], uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t qcap,
     unsigned long long* edef) {
     constexpr int LMAX = 1 << (B - 1);
-    const uint64_t x = ld_word<kCoherent>(S, p);
+    const uint64_t x = ld32(S, p);
     const uint32_t e0 = ld32(rp, p);
     const uint32_t d = ld32(rp, p + 1) - e0;
     const uint32_t sg = ld32(sgn, p);
@@ -382,7 +381,7 @@ __device__ __forceinline__ void irregular_site(
 #pragma unroll
         for (uint32_t q = 0; q < kRowGroup; ++q) j[q] = ld32(ci, e0 + min(g + q, d - 1));
 #pragma unroll
-        for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld_word<kCoherent>(S, j[q]);
+        for (uint32_t q = 0; q < kRowGroup; ++q) wj[q] = ld32(S, j[q]);
 #pragma unroll
         for (uint32_t q = 0; q < kRowGroup; ++q) {
             const uint32_t e = g + q;
@@ -486,29 +485,318 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// irregular_site on the narrowest count planes the wave's sites allow: a
-// site's update depends on its degree, its counts, uniforms and thresholds,
-// not on the planes' width, so when every degree dw of the wave is below
-// 2^(B-1) the wave counts on B - 1 planes, with half the levels to compare.
-// (An ER graph of mean degree 6 and N = 20000 has a maximum degree near 20,
-// so B = 5, but almost every wave of 64 sites stays below 16.)
-template <int B, int W, bool kCoherent = false>
-__device__ __forceinline__ void irregular_site_fit(
-    uint32_t dw, const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t qoff, const uint32_t* rp,
-    const uint32_t* ci, const uint32_t* sgn, const unsigned long long* recs, uint32_t slot,
-    uint32_t sweep, uint32_t key0, uint32_t key1, int32_t w0, uint64_t (&kacc)[W],
-    uint64_t (&dacc)[W], uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t qcap,
+// Half h of word i of S (replicas 32 h .. 32 h + 31), read device-coherently:
+// the load goes to the L2, not the CU's L1, so it sees words other
+// workgroups wrote since this CU last read them, with no cache invalidation.
+__device__ __forceinline__ uint32_t ld_half(const uint64_t* S, uint32_t i, uint32_t h) {
+    return __hip_atomic_load(
+        reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(const_cast<uint64_t*>(S)) + i * 8u +
+                                    h * 4u),
+        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// v of the other lane of this lane's pair (lanes 2 i and 2 i + 1): DPP
+// quad_perm [1, 0, 3, 2].  Both lanes of the pair must be active.
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+
+// As wave_column_sum for the 32-replica counters of irregular_site_pair
+// (lane l holds replicas 32 (l & 1) ..).
+template <int W>
+__device__ __forceinline__ int64_t wave_column_sum_pair(const uint32_t* acc, uint32_t lane) {
+    int64_t tot = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b)
+        tot += (int64_t)__builtin_popcountll(
+                   wave_transpose((uint64_t)acc[b] << (32u * (lane & 1u)), lane))
+               << b;
+    return tot;
+}
+
+// What irregular_site_pair reads of a position's row before any spin word:
+// entry start, degree, coupling sign bits and the first kPairPre neighbour
+// positions (entries past the degree repeat the last one; zero for degree
+// 0; load_pair_row<B> fills pair_pre<B>() of them).  A caller loads it
+// ahead, while it waits at a barrier.
+constexpr uint32_t kPairPre = 16;
+struct PairRow {
+    uint32_t e0, d, sg;
+    uint32_t j[kPairPre];
+};
+
+// Entries of a row read ahead for count planes B: 8 when every degree is
+// below 8 (B = 3), else kPairPre.
+template <int B>
+constexpr uint32_t pair_pre() {
+    return B == 3 ? kPairPre / 2 : kPairPre;
+}
+
+template <int B>
+__device__ __forceinline__ PairRow load_pair_row(uint32_t p, const uint32_t* rp,
+                                                 const uint32_t* ci, const uint32_t* sgn) {
+    PairRow r;
+    r.e0 = ld32(rp, p);
+    r.d = ld32(rp, p + 1) - r.e0;
+    r.sg = ld32(sgn, p);
+#pragma unroll
+    for (uint32_t q = 0; q < kPairPre; ++q) r.j[q] = 0;
+    if (r.d) {
+#pragma unroll
+        for (uint32_t q = 0; q < pair_pre<B>(); ++q) r.j[q] = ld32(ci, r.e0 + min(q, r.d - 1));
+    }
+    return r;
+}
+
+// Full adder on bit planes: s = a ^ b ^ c, k = majority(a, b, c).
+__device__ __forceinline__ void full_add(uint32_t a, uint32_t b, uint32_t c, uint32_t& s,
+                                         uint32_t& k) {
+    s = __builtin_amdgcn_bitop3_b32(a, b, c, kXor3);
+    k = __builtin_amdgcn_bitop3_b32(a, b, c, kMaj);
+}
+
+// Carry-save trees: bit r of o[i] is bit i of the number of inputs with bit
+// r set (8 inputs: 4 full and 3 half adders; 16: 11 and 4), where a ripple
+// add per input costs two operations per plane.
+__device__ __forceinline__ void count8(const uint32_t* b, uint32_t (&o)[4]) {
+    uint32_t s0, s1, s2, k0, k1, k2, u, q;
+    full_add(b[0], b[1], b[2], s0, k0);
+    full_add(b[3], b[4], b[5], s1, k1);
+    full_add(s0, s1, b[6], s2, k2);
+    o[0] = s2 ^ b[7];
+    const uint32_t k3 = s2 & b[7];
+    full_add(k0, k1, k2, u, q);
+    o[1] = u ^ k3;
+    const uint32_t q1 = u & k3;
+    o[2] = q ^ q1;
+    o[3] = q & q1;
+}
+
+__device__ __forceinline__ void count16(const uint32_t* b, uint32_t (&o)[5]) {
+    uint32_t s[5], k[8], t0, t1, u0, u1, u2, q[4], v, r0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) full_add(b[3 * i], b[3 * i + 1], b[3 * i + 2], s[i], k[i]);
+    full_add(s[0], s[1], s[2], t0, k[5]);
+    full_add(s[3], s[4], b[15], t1, k[6]);
+    o[0] = t0 ^ t1;
+    k[7] = t0 & t1;
+    full_add(k[0], k[1], k[2], u0, q[0]);
+    full_add(k[3], k[4], k[5], u1, q[1]);
+    full_add(k[6], k[7], u0, u2, q[2]);
+    o[1] = u1 ^ u2;
+    q[3] = u1 & u2;
+    full_add(q[0], q[1], q[2], v, r0);
+    o[2] = v ^ q[3];
+    const uint32_t r1 = v & q[3];
+    o[3] = r0 ^ r1;
+    o[4] = r0 & r1;
+}
+
+// The count planes of position p for replicas 32 h .. 32 h + 31 (h = lane &
+// 1; bit r of c[b] = bit b of k_r), from the row prefetched in `row`; returns
+// this lane's half of p's word.  The words of the first pair_pre<B>()
+// entries load together before any is used, so a site of such a degree
+// costs one round trip to the L2.
+template <int B>
+__device__ __forceinline__ uint32_t pair_counts(const uint64_t* S, uint32_t p, uint32_t h,
+                                                const PairRow& row, const uint32_t* ci,
+                                                uint32_t (&c)[B]) {
+    static_assert(B >= 3 && B <= 5, "count planes");
+    const uint32_t d = row.d, sg = row.sg;
+    const uint32_t x = ld_half(S, p, h);
+    // ripple add of entry e (past kPre)
+    auto add = [&](uint32_t e, uint32_t w) {
+        uint32_t carry = __builtin_amdgcn_bitop3_b32(x, w, bit_mask(sg, e & 31u), kXor3) &
+                         (e < d ? ~0u : 0u);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const uint32_t s = c[b] ^ carry;
+            carry &= c[b];
+            c[b] = s;
+        }
+    };
+    // The first kPre entries' words, each loaded only by the lanes whose
+    // degree reaches it (a masked lane sends no request: the loads bypass
+    // the L1, and the L2's request rate bounds a class of random gathers).
+    // No branch on the wave's degrees: the compiler would hoist the first
+    // loads and count them before issuing the rest.
+    constexpr uint32_t kPre = pair_pre<B>(), kG = kPairPre / 2;
+    uint32_t wj[kPre];
+#pragma unroll
+    for (uint32_t q = 0; q < kPre; ++q) {
+        wj[q] = 0;
+        if (q < d) wj[q] = ld_half(S, row.j[q], h);
+    }
+    // bit r of dis[q]: replica r disagrees with entry q's bond (x ^ w ^ sign),
+    // zero past the degree
+    uint32_t dis[kPre];
+#pragma unroll
+    for (uint32_t q = 0; q < kPre; ++q)
+        dis[q] = __builtin_amdgcn_bitop3_b32(wj[q] ^ bit_mask(sg, q), x, q < d ? ~0u : 0u, kXorAnd);
+    if constexpr (kPre == 16) {
+        uint32_t o[5];
+        count16(dis, o);
+#pragma unroll
+        for (int b = 0; b < B; ++b) c[b] = o[b];  // o[4] = 0 when B = 4 (d < 16)
+    } else {
+        uint32_t o[4];
+        count8(dis, o);
+#pragma unroll
+        for (int b = 0; b < B; ++b) c[b] = o[b];  // B = 3: d < 8
+    }
+    // entries past kPre (degrees above 16), in groups of kG
+    for (uint32_t g = kPre; g < d; g += kG) {
+        uint32_t j[kG], wj[kG];
+#pragma unroll
+        for (uint32_t q = 0; q < kG; ++q) j[q] = ld32(ci, row.e0 + min(g + q, d - 1));
+#pragma unroll
+        for (uint32_t q = 0; q < kG; ++q) wj[q] = ld_half(S, j[q], h);
+#pragma unroll
+        for (uint32_t q = 0; q < kG; ++q) add(g + q, wj[q]);
+    }
+    return x;
+}
+
+// The rest of the update after pair_counts, on the first BF count planes
+// (the wave's degrees are below 2^BF, so planes BF .. B - 1 are zero).
+// Every level up to 2^(BF-1) is compared: skipping the levels past the
+// wave's largest degree with uniform branches measured 14 % slower (the
+// branches split the batches of record reads).
+template <int BF, int B, int W>
+__device__ __forceinline__ void pair_update(
+    const uint32_t (&c)[B], uint32_t x, uint64_t* Wr, uint32_t p, uint32_t h, uint32_t qoff,
+    uint32_t d, const unsigned long long* recs, uint32_t slot, uint32_t sweep, uint32_t key0,
+    uint32_t key1, int32_t w0, uint32_t (&kacc)[W], uint32_t (&dacc)[W], uint32_t& nacc,
+    uint32_t* queue, uint32_t* qcount, uint32_t qcap, unsigned long long* edef) {
+    static_assert(kEagerPlanes == 8, "a pair's lanes compute two Philox calls each");
+    constexpr int LMAX = 1 << (BF - 1);
+    // recs as 32-bit halves: half h of record word i at rh[2 i + h].  Level m
+    // = 2 k - d of k = floor(d/2) + 1 + j is 2 j + 2 - (d & 1): its record
+    // starts 2 j kQStride words after rl's.  Levels of k > d (up to 2^BF) are
+    // read but masked out by lev[j] = 0.
+    const uint32_t m0 = 2u - (d & 1u);
+    const uint32_t* rl = reinterpret_cast<const uint32_t*>(recs) + h + 2 * m0 * kQStride;
+    uint32_t lev[LMAX];
+    uint32_t pos = 0, und = 0;
+#pragma unroll
+    for (int j = 0; j < LMAX; ++j) {
+        const uint32_t k = (d >> 1) + 1 + (uint32_t)j;
+        uint32_t eq = k <= d ? ~0u : 0u;
+#pragma unroll
+        for (int b = 0; b < BF; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
+        lev[j] = eq;
+        pos |= eq;
+        und = __builtin_amdgcn_bitop3_b32(eq, rl[2 * (2 * j * kQStride + kEagerPlanes)], und,
+                                          kAndOr);
+    }
+    // Plane 2 j + t of call j: replicas 0..31 in word 2 t, 32..63 in 2 t + 1.
+    uint32_t keep[4], send[4];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 2; ++jj) {
+        const U32x4 o = philox4x32_10(p, slot, sweep, (kTagPlane << 24) | (2 * h + jj), key0, key1);
+#pragma unroll
+        for (uint32_t t = 0; t < 2; ++t) {
+            keep[2 * jj + t] = h ? o.v[2 * t + 1] : o.v[2 * t];
+            send[2 * jj + t] = h ? o.v[2 * t] : o.v[2 * t + 1];
+        }
+    }
+    uint32_t P[kEagerPlanes];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t recv = pair_swap(send[k]);
+        P[k] = h ? recv : keep[k];
+        P[4 + k] = h ? keep[k] : recv;
+    }
+    uint32_t lt = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < kEagerPlanes; ++b) {
+        uint32_t T = lev[0] & rl[2 * b];
+#pragma unroll
+        for (int l = 1; l < LMAX; ++l)
+            T = __builtin_amdgcn_bitop3_b32(lev[l], rl[2 * (2 * l * kQStride + b)], T, kAndOr);
+        lt |= __builtin_amdgcn_bitop3_b32(und, T, P[b], kAndAndNot);
+        und = __builtin_amdgcn_bitop3_b32(und, P[b], T, kAndXnor);
+    }
+    const uint32_t flip = ~pos | lt;
+    uint32_t* Wh = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Wr) + p * 8u + h * 4u);
+    *Wh = x ^ flip;
+    nacc += (uint32_t)__builtin_popcount(flip);
+    uint32_t ck = 0, cd = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+        if (b == 0) {
+            const uint32_t ak = c[0] & flip;
+            ck = kacc[0] & ak;
+            kacc[0] ^= ak;
+        } else if (b < BF) {
+            const uint32_t ak = c[b] & flip;
+            const uint32_t sk = __builtin_amdgcn_bitop3_b32(kacc[b], ak, ck, kXor3);
+            ck = __builtin_amdgcn_bitop3_b32(kacc[b], ak, ck, kMaj);
+            kacc[b] = sk;
+        } else {
+            const uint32_t sk = kacc[b] ^ ck;
+            ck &= kacc[b];
+            kacc[b] = sk;
+        }
+        const uint32_t ad = (b < BF && ((d >> b) & 1u)) ? flip : 0u;
+        const uint32_t sd = __builtin_amdgcn_bitop3_b32(dacc[b], ad, cd, kXor3);
+        cd = __builtin_amdgcn_bitop3_b32(dacc[b], ad, cd, kMaj);
+        dacc[b] = sd;
+    }
+    while (und) {
+        const uint32_t rb = (uint32_t)__builtin_ctz(und);
+        und &= und - 1;
+        const uint32_t r = 32 * h + rb;
+        uint32_t m = 0;
+#pragma unroll
+        for (int l = 0; l < LMAX; ++l)
+            if ((lev[l] >> rb) & 1u) m = m0 + 2 * l;
+        const uint32_t at = atomicAdd(qcount, 1u);
+        if (at < qcap) {
+            queue[at] = (qoff << 11) | (r << 5) | m;
+        } else if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) {
+            __threadfence_block();
+            atomicXor(Wh, 1u << rb);
+            atomicAdd(&edef[r], (unsigned long long)(((int64_t)m * w0) / 2));
+            nacc += 1;
+        }
+    }
+}
+
+// irregular_site with a position's 64 replicas split over a lane pair: lane
+// 2 i + h (h = lane & 1) updates replicas 32 h .. 32 h + 31 on 32-bit planes,
+// so each lane does half the bit operations, and the pair shares the Philox
+// calls of the position: lane h computes calls 2 h and 2 h + 1 (bit planes
+// 4 h .. 4 h + 3 of all 64 replicas) and trades the other half's words with
+// its partner.  The uniforms, thresholds, decisions and queue entries are
+// those of irregular_site; kacc and dacc count this lane's 32 replicas
+// (wave_column_sum_pair).  Both lanes of a pair take the same p, whose row
+// is `row` (load_pair_row); dw is the largest degree of the wave (uniform).
+// A site's update depends on its degree, counts, uniforms and thresholds,
+// not on the planes' width, so when dw < 2^(B-1) the levels are compared on
+// B - 1 planes, half as many (an ER graph of mean degree 6 and N = 20000 has
+// a largest degree near 20, so B = 5, but almost every wave of 32 sites
+// stays below 16).  Words are read device-coherently.  recs must hold
+// levels 0 .. 2^B (those past the graph's largest degree are read, not
+// used).
+template <int B, int W>
+__device__ __forceinline__ void irregular_site_pair(
+    uint32_t dw, const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t h, uint32_t qoff,
+    const PairRow& row, const uint32_t* ci, const unsigned long long* recs, uint32_t slot,
+    uint32_t sweep, uint32_t key0, uint32_t key1, int32_t w0, uint32_t (&kacc)[W],
+    uint32_t (&dacc)[W], uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t qcap,
     unsigned long long* edef) {
+    uint32_t c[B];
+    const uint32_t x = pair_counts<B>(S, p, h, row, ci, c);
     if constexpr (B > 3) {
         if (dw < (1u << (B - 1))) {
-            irregular_site<B - 1, W, kCoherent>(S, Wr, p, qoff, rp, ci, sgn, recs, slot, sweep,
-                                                key0, key1, w0, kacc, dacc, nacc, queue, qcount,
-                                                qcap, edef);
+            pair_update<B - 1, B, W>(c, x, Wr, p, h, qoff, row.d, recs, slot, sweep, key0, key1,
+                                     w0, kacc, dacc, nacc, queue, qcount, qcap, edef);
             return;
         }
     }
-    irregular_site<B, W, kCoherent>(S, Wr, p, qoff, rp, ci, sgn, recs, slot, sweep, key0, key1,
-                                    w0, kacc, dacc, nacc, queue, qcount, qcap, edef);
+    pair_update<B, B, W>(c, x, Wr, p, h, qoff, row.d, recs, slot, sweep, key0, key1, w0, kacc,
+                         dacc, nacc, queue, qcount, qcap, edef);
 }
 
 // Drains a wave's per-replica tie queue (irregular_site entries relative to
