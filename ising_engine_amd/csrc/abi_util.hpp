@@ -99,4 +99,14 @@ inline bool graphs_enabled() {
     return on;
 }
 
+// IGE_DENSE_HELPERS=0 keeps a dense sweep's fields in its own workgroup (the
+// helpers' sums are the same integers, so the result is the same).
+inline bool dense_helpers_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("IGE_DENSE_HELPERS");
+        return !(v && v[0] == '0' && v[1] == '\0');
+    }();
+    return on;
+}
+
 }  // namespace ige::abi

@@ -158,6 +158,11 @@ struct ige_state {
     DevBuf<uint32_t> lev_thr;             // multispin thresholds per temperature and level
     DevBuf<double> beta;                             // current ladder (+ schedule rows)
     DevBuf<int32_t> dense_bounds;                    // accept bounds of a dense sweep
+    // Dense field helpers: [0] error flag, then kDenseGangSyncWords per slot;
+    // finished blocks' spins; the helpers' partial sums (on first use).
+    DevBuf<uint32_t> dense_sync;
+    DevBuf<uint64_t> dense_xpub;
+    DevBuf<int32_t> dense_part;
     DevBuf<uint64_t> hb;                             // heat-bath thresholds, on first use
     DevBuf<int32_t> fields;
     DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
@@ -357,21 +362,25 @@ constexpr uint32_t kGraphSteps = 32;
 // (its workgroups are all resident, so a wait ends long before).
 constexpr uint32_t kChainSpinLimit = 1u << 24;
 
-// Raises the error a chained run's expired barrier wait left (and clears it).
-void check_chain_error(ige_state* s) {
-    if (!s->chain_sync.p) return;
+// Raises the error an expired wait left in flag word *f (and clears it).
+void check_wait_flag(ige_state* s, uint32_t* f, const char* what) {
+    if (!f) return;
     uint32_t e = 0;
-    check(hipMemcpyAsync(&e, s->chain_sync.p, sizeof(e), hipMemcpyDeviceToHost, s->stream),
-          "chain error flag");
-    check(hipStreamSynchronize(s->stream), "chain error flag");
+    check(hipMemcpyAsync(&e, f, sizeof(e), hipMemcpyDeviceToHost, s->stream), what);
+    check(hipStreamSynchronize(s->stream), what);
     if (e) {
         const uint32_t zero = 0;
-        check(hipMemcpyAsync(s->chain_sync.p, &zero, sizeof(zero), hipMemcpyHostToDevice,
-                             s->stream),
-              "chain error flag");
-        check(hipStreamSynchronize(s->stream), "chain error flag");
-        throw Failure(IGE_ERR_HIP, "a chained sweep's barrier wait exceeded its spin limit");
+        check(hipMemcpyAsync(f, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream), what);
+        check(hipStreamSynchronize(s->stream), what);
+        throw Failure(IGE_ERR_HIP, std::string(what) + ": a wait exceeded its spin limit");
     }
+}
+
+// Raises the error a chained run's barrier or a dense sweep's field helpers
+// left (and clears it).
+void check_chain_error(ige_state* s) {
+    check_wait_flag(s, s->chain_sync.p, "chained sweep barrier");
+    check_wait_flag(s, s->dense_sync.p, "dense sweep field helpers");
 }
 
 // One sweep step on s->stream: the thresholds (when `refresh`), the colour
@@ -406,8 +415,27 @@ void issue_step(ige_state* s, const StepPlan& P, const double* beta, uint32_t sw
         const auto dv = s->model->dense_view();
         if (P.dense_bounds && !s->dense_bounds.p)
             s->dense_bounds.alloc(ige::dense_bounds_words(dv, s->cells));
+        ige::DevDenseGang gang;
+        if (P.dense_bounds && ige::abi::dense_helpers_enabled())
+            gang.helpers = ige::dense_gang_helpers(dv, s->cells);
+        if (gang.helpers) {
+            if (!s->dense_sync.p) {
+                const size_t words = 1 + static_cast<size_t>(s->cells) * ige::kDenseGangSyncWords;
+                s->dense_sync.alloc(words);
+                const std::vector<uint32_t> zero(words, 0u);
+                s->dense_sync.upload(zero.data(), zero.size(), s->stream);
+                s->dense_xpub.alloc(ige::dense_gang_xpub_words(dv, s->cells));
+                s->dense_part.alloc(ige::dense_gang_part_words(s->cells, gang.helpers));
+            }
+            gang.err = s->dense_sync.p;
+            gang.sync = s->dense_sync.p + 1;
+            gang.xpub = s->dense_xpub.p;
+            gang.part = s->dense_part.p;
+            gang.spin_limit = kChainSpinLimit;
+        }
         check(ige::launch_sweep_dense(dv, st, sweep, s->seed, beta,
-                                      P.dense_bounds ? s->dense_bounds.p : nullptr, s->stream),
+                                      P.dense_bounds ? s->dense_bounds.p : nullptr, &gang,
+                                      s->stream),
               "sweep_dense");
     } else if (s->kernel == IGE_KERNEL_GENERIC) {
         for (uint32_t c = 0; c < s->model->co.num_colors; ++c)
@@ -896,7 +924,7 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
             }
             // the caller may free `sched` once we return
             check(hipStreamSynchronize(s->stream), "run");
-            if (chain) check_chain_error(s);
+            if (chain || (s->kernel == IGE_KERNEL_DENSE && s->dense_sync.p)) check_chain_error(s);
             if (sweeps > 0) refresh_thresholds(s, s->beta.p);
         }
     });

@@ -31,6 +31,7 @@
 // decisions are those of metropolis_threshold bit for bit.
 #include <hip/hip_runtime.h>
 
+#include "gang_sync.h"
 #include "ige_math.h"
 #include "kernels.hpp"
 #include "sweep_generic.h"
@@ -434,6 +435,10 @@ struct BoundTiles {
                                  // blocks blk and blk + 1 (the chain's half correction)
     uint64_t xold[64];           // the finished block's spins per replica, before
     uint64_t xnew[64];           // and after the chain
+    v4i corr2[2][2][64];         // tiles of the next block's rows against the columns
+                                 // of the block before the current one
+    uint64_t xold2[64];          // xold and xnew of the block before
+    uint64_t xnew2[64];
 };
 
 struct BoundLane {
@@ -506,11 +511,12 @@ __global__ __launch_bounds__(256) void dense_bounds_kernel(DevDense d, DevState 
 // the block's coupling rows (hdiag tiles 2 nb, 2 nb + 1) into rows, its tile
 // of rows 32.. against columns 0..31 into diag, and what combine_fields
 // reads for block nb: the tiles of its rows against block nb - 1's columns
-// and its linear terms.  One wave.
+// (and, when `stale`, block nb - 2's) and its linear terms.  One wave.
 __device__ __forceinline__ void stage_keys(const DevDense& d, uint32_t nb, const int4* Db,
                                            const uint64_t* xt, uint32_t xs, int32_t* key,
                                            uint4* rows, v4i* diag, v4i (*corr)[2][64],
-                                           int32_t* lin, uint32_t lane) {
+                                           v4i (*corr2)[2][64], bool stale, int32_t* lin,
+                                           uint32_t lane) {
     const uint32_t ntile = d.n_pad / 32;
     const uint4* src = reinterpret_cast<const uint4*>(d.hdiag + (size_t)nb * 2048);
     rows[lane] = src[lane];
@@ -525,6 +531,14 @@ __device__ __forceinline__ void stage_keys(const DevDense& d, uint32_t nb, const
             for (uint32_t c = 0; c < 2; ++c)
                 corr[r][c][lane] = *reinterpret_cast<const v4i*>(
                     d.a + ((size_t)(2 * nb + r) * ntile + 2 * (nb - 1) + c) * 1024 + 16 * lane);
+    }
+    if (stale) {
+#pragma unroll
+        for (uint32_t r = 0; r < 2; ++r)
+#pragma unroll
+            for (uint32_t c = 0; c < 2; ++c)
+                corr2[r][c][lane] = *reinterpret_cast<const v4i*>(
+                    d.a + ((size_t)(2 * nb + r) * ntile + 2 * (nb - 2) + c) * 1024 + 16 * lane);
     }
     const uint64_t xb = xt[lane * xs + nb];
     int4 dv[16];
@@ -543,18 +557,42 @@ __device__ __forceinline__ void stage_keys(const DevDense& d, uint32_t nb, const
     }
 }
 
+// The coupling rows of a wave's first kFieldPrefetch chunks of [c0, c1)
+// (clamped to c1 - 1), loaded ahead of field_sums: they do not depend on the
+// spins, so a caller can issue them before it waits for spins.
+struct FieldRows {
+    v4i pa[kFieldPrefetch][4];  // [chunk][row half * 2 + tile]
+};
+
+__device__ __forceinline__ void field_rows(const DevDense& d, uint32_t p0, uint32_t c0,
+                                           uint32_t c1, FieldRows& r, uint32_t lane) {
+    if (c1 <= c0) return;
+    const uint32_t ntile = d.n_pad / 32;
+    const int8_t* ar0 = d.a + ((size_t)(p0 / 32) * ntile) * 1024 + 16 * lane;
+    const int8_t* ar1 = ar0 + (size_t)ntile * 1024;
+#pragma unroll
+    for (int i = 0; i < kFieldPrefetch; ++i) {
+        const size_t o = (size_t)min(c0 + i, c1 - 1) * 2048;
+        r.pa[i][0] = *reinterpret_cast<const v4i*>(ar0 + o);
+        r.pa[i][1] = *reinterpret_cast<const v4i*>(ar0 + o + 1024);
+        r.pa[i][2] = *reinterpret_cast<const v4i*>(ar1 + o);
+        r.pa[i][3] = *reinterpret_cast<const v4i*>(ar1 + o + 1024);
+    }
+}
+
 // a.x of the 64 rows of the block at p0 over the 64-column chunks [c0, c1)
 // (a multiple of kFieldPrefetch chunks, possibly none), all 64 replicas, added
-// into hacc; one wave.  Each
+// into hacc; one wave, with the first chunks' rows in r (field_rows).  Each
 // chunk's spin bytes (two replica halves x two 32-column tiles) feed the MFMAs
 // of both 32-row halves.
-__device__ __forceinline__ void block_partial(const DevDense& d, const uint64_t* xt, uint32_t xs,
-                                              uint32_t p0, uint32_t c0, uint32_t c1,
-                                              int32_t (*hacc)[2][16][64], uint32_t lane) {
+__device__ __forceinline__ void field_sums(const DevDense& d, const uint64_t* xt, uint32_t xs,
+                                           uint32_t p0, uint32_t c0, uint32_t c1, FieldRows& r,
+                                           int32_t (*hacc)[2][16][64], uint32_t lane) {
     // Chunks of coupling rows in flight: the rows stream from the memory
     // behind L2, so a wave keeps P * 4 KB in flight.  With whole groups of P
     // chunks the loop has no early exit, and the loads of the next group stay
-    // in flight across the loop's back edge.
+    // in flight across the loop's back edge (the last group reloads its last
+    // chunk: a branch around those loads measured slower).
     constexpr int P = kFieldPrefetch;
     if (c1 <= c0) return;
     const uint32_t kh = 16 * (lane >> 5);
@@ -564,76 +602,147 @@ __device__ __forceinline__ void block_partial(const DevDense& d, const uint64_t*
     const uint64_t* xr0 = xt + (size_t)(lane & 31) * xs;
     const uint64_t* xr1 = xt + (size_t)(32 + (lane & 31)) * xs;
     v16i acc[2][2] = {};  // [row half][replica half]
-    v4i pa[P][4];         // [chunk][row half * 2 + tile]
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-        const size_t o = (size_t)min(c0 + i, c1 - 1) * 2048;
-        pa[i][0] = *reinterpret_cast<const v4i*>(ar0 + o);
-        pa[i][1] = *reinterpret_cast<const v4i*>(ar0 + o + 1024);
-        pa[i][2] = *reinterpret_cast<const v4i*>(ar1 + o);
-        pa[i][3] = *reinterpret_cast<const v4i*>(ar1 + o + 1024);
-    }
     for (uint32_t kb = c0; kb < c1; kb += P) {
 #pragma unroll
         for (int i = 0; i < P; ++i) {
             const uint32_t kc = kb + i;
             v4i a[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) a[t] = pa[i][t];
-            const size_t o = (size_t)min(kc + (uint32_t)P, c1 - 1) * 2048;
-            pa[i][0] = *reinterpret_cast<const v4i*>(ar0 + o);
-            pa[i][1] = *reinterpret_cast<const v4i*>(ar0 + o + 1024);
-            pa[i][2] = *reinterpret_cast<const v4i*>(ar1 + o);
-            pa[i][3] = *reinterpret_cast<const v4i*>(ar1 + o + 1024);
+            for (int t = 0; t < 4; ++t) a[t] = r.pa[i][t];
+            {
+                const size_t o = (size_t)min(kc + (uint32_t)P, c1 - 1) * 2048;
+                r.pa[i][0] = *reinterpret_cast<const v4i*>(ar0 + o);
+                r.pa[i][1] = *reinterpret_cast<const v4i*>(ar0 + o + 1024);
+                r.pa[i][2] = *reinterpret_cast<const v4i*>(ar1 + o);
+                r.pa[i][3] = *reinterpret_cast<const v4i*>(ar1 + o + 1024);
+            }
             const uint64_t x0 = xr0[kc], x1 = xr1[kc];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {  // 32-column tile of the chunk
                 const v4i b0 = spread16((uint32_t)(x0 >> (32 * t + kh)));
                 const v4i b1 = spread16((uint32_t)(x1 >> (32 * t + kh)));
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    acc[r][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * r + t], b0, acc[r][0],
-                                                                      0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * r + t], b1, acc[r][1],
-                                                                      0, 0, 0);
+                for (int rr = 0; rr < 2; ++rr) {
+                    acc[rr][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * rr + t], b0,
+                                                                       acc[rr][0], 0, 0, 0);
+                    acc[rr][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[2 * rr + t], b1,
+                                                                       acc[rr][1], 0, 0, 0);
                 }
             }
         }
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) atomicAdd(&hacc[r][j][v][lane], acc[r][j][v]);
+            for (int v = 0; v < 16; ++v) atomicAdd(&hacc[rr][j][v][lane], acc[rr][j][v]);
 }
 
+__device__ __forceinline__ void block_partial(const DevDense& d, const uint64_t* xt, uint32_t xs,
+                                              uint32_t p0, uint32_t c0, uint32_t c1,
+                                              int32_t (*hacc)[2][16][64], uint32_t lane) {
+    FieldRows r;
+    field_rows(d, p0, c0, c1, r, lane);
+    field_sums(d, xt, xs, p0, c0, c1, r, hacc, lane);
+}
+
+constexpr uint32_t kPartWords = 2 * 2 * 16 * 64;  // one partial sum, BoundTiles::hacc layout
+// Buffers of partial sums per helper: block b's in buffer b % kPartBufs.  A
+// helper sums blocks 0, 1 and 2 without waiting and block b >= 3 once block
+// b - 3 is published, which the main workgroup does after it has read block
+// b - 3's sums, so three buffers are never overwritten before they are read.
+constexpr uint32_t kPartBufs = 3;
+
 // Fields of block nb at its start, one 32 x 32 tile (rows 32 rh.., replicas
-// 32 jh..) per wave: lin + scale * (the field waves' sums + block nb - 1's
-// flips applied to rows of block nb), into h[replica][position]; the sums are
-// cleared for the next block.  nb == 0: no flips yet.  The coupling tiles and
-// linear terms were staged in LDS (stage_keys).
+// 32 jh..) per wave: lin + scale * (the sums in hacc + block nb - 1's flips
+// applied to rows of block nb, and block nb - 2's when `stale`: a helper
+// summed its columns from the spins before them), into h[replica][position];
+// the sums are cleared for the next block.  nb == 0: no flips yet.  The
+// coupling tiles and linear terms were staged in LDS (stage_keys).
 __device__ __forceinline__ void combine_fields(const DevDense& d, BoundTiles& T, uint32_t nb,
-                                               uint32_t rh, uint32_t jh, uint32_t lane) {
+                                               uint32_t rh, uint32_t jh, uint32_t lane,
+                                               bool stale) {
     const uint32_t j = 32 * jh + (lane & 31);
+    const uint32_t kh = 16 * (lane >> 5);
     v16i acc = {};
-    if (nb > 0) {
-        const uint32_t kh = 16 * (lane >> 5);
-        const uint64_t o = T.xold[j], nw = T.xnew[j];
+    // flips o -> nw of a block's columns through coupling tiles ct
+    auto flips = [&](const v4i (*ct)[64], uint64_t o, uint64_t nw) {
         const uint64_t up = nw & ~o, dn = o & ~nw;
 #pragma unroll
         for (uint32_t c = 0; c < 2; ++c) {
             const uint32_t sh = 32 * c + kh;
             const v4i bd = spread16((uint32_t)(up >> sh)) | (spread16((uint32_t)(dn >> sh)) * 0xFF);
-            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.corr[rh][c][lane], bd, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(ct[c][lane], bd, acc, 0, 0, 0);
         }
-    }
+    };
+    if (nb > 0) flips(T.corr[rh], T.xold[j], T.xnew[j]);
+    if (stale) flips(T.corr2[rh], T.xold2[j], T.xnew2[j]);
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const uint32_t i = 32 * rh + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         const int32_t sum = T.hacc[rh][jh][v][lane] + acc[v];
         T.hacc[rh][jh][v][lane] = 0;
         T.h[j * kRS + i] = T.lin[i] + d.scale * sum;
+    }
+}
+
+// Flags of a slot's field gang (one XCD; flags and data meet in its L2 and
+// are read device-coherently): word 0 counts the blocks whose spins the main
+// workgroup has published, word 1 + k the partial sums helper k has
+// published, word 5 the XCDs of the members, word 6 their first barrier.
+constexpr uint32_t kGangPublished = 0, kGangPartial = 1, kGangXcds = 5, kGangBarrier = 6;
+constexpr uint32_t kGangXcdStride = 8;  // XCDs: workgroups launched per member
+
+__device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
+    return __hip_atomic_load(const_cast<uint32_t*>(f), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Polls until *f >= v, at most spin_limit polls; on expiry raises *err (and
+// every later wait then returns at once).  One lane calls it.
+__device__ __forceinline__ void wait_flag(const uint32_t* f, uint32_t v, uint32_t spin_limit,
+                                          uint32_t* err) {
+    uint32_t polls = 0;
+    while (load_flag(f) < v) {
+        if (load_flag(err) != 0u) break;
+        if (++polls > spin_limit) {
+            atomicOr(err, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Adds the H helpers' sums for tile (rh, jh) (hp + k * hstride, helper k,
+// read device-coherently once its flag reaches nb + 1) into hacc; one wave.
+// Every helper's sums load before any is added (one round trip to the L2).
+__device__ __forceinline__ void add_helper_sums(int32_t (*hacc)[2][16][64], const uint32_t* sy,
+                                                uint32_t nb, const int32_t* hp, size_t hstride,
+                                                uint32_t H, uint32_t rh, uint32_t jh,
+                                                uint32_t lane, uint32_t spin_limit,
+                                                uint32_t* err) {
+    if (lane == 0)
+        for (uint32_t k = 0; k < H; ++k) wait_flag(&sy[kGangPartial + k], nb + 1, spin_limit, err);
+    __builtin_amdgcn_wave_barrier();
+    int32_t part[kMaxDenseHelpers][16];
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxDenseHelpers; ++k) {
+        if (k < H) {
+            const int32_t* src = hp + k * hstride + (rh * 2 + jh) * 16 * 64 + lane;
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                part[k][v] = __hip_atomic_load(const_cast<int32_t*>(src + v * 64),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        int32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxDenseHelpers; ++k)
+            if (k < H) sum += part[k][v];
+        atomicAdd(&hacc[rh][jh][v][lane], sum);
     }
 }
 
@@ -719,42 +828,142 @@ __device__ __forceinline__ void chain_half(int32_t (&hr)[64], const int32_t (&kr
     }
 }
 
+// Member m of a slot's gang takes these groups of kFieldPrefetch column chunks
+// of every block's fields.  With helpers the main workgroup keeps two (for
+// its two field waves that do not combine: the four that do wait for and add
+// the helpers' sums while the chain runs) and the helpers share the rest.
+__device__ __forceinline__ void gang_groups(uint32_t ngrp, uint32_t H, uint32_t m, uint32_t& gb,
+                                            uint32_t& ge) {
+    const uint32_t g0 = H ? 2u : ngrp;
+    if (m == 0) {
+        gb = 0;
+        ge = g0;
+    } else {
+        gb = g0 + (m - 1) * (ngrp - g0) / H;
+        ge = g0 + m * (ngrp - g0) / H;
+    }
+}
+
+// One slot's sweep.  Without helpers (g.helpers == 0) workgroup blockIdx.x
+// is slot blockIdx.x.  With helpers the grid is (kGangXcdStride * (1 + H),
+// cells): the workgroups with blockIdx.x % kGangXcdStride == 0 are the
+// slot's gang (workgroups are dispatched to the XCDs in turn, so they share
+// one), member 0 the main workgroup and 1..H the helpers.  They check the XCD
+// at a first barrier; if they do not share one, the helpers leave and the
+// main workgroup computes every field itself.  A helper computes its columns'
+// share of block b's fields from the spins at the start of block b - 2 (the
+// main workgroup publishes every finished block's spins), so it has two
+// blocks' time, and publishes the sums; the main workgroup's combine waves
+// add them at the end of block b - 1's chain and apply block b - 2's flips in
+// the helpers' columns with the other flip corrections.  Every wait is
+// bounded by g.spin_limit polls.
 template <bool kScale1>
 __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
-    DevDense d, DevState s, const int4* __restrict__ bounds) {
+    DevDense d, DevState s, const int4* __restrict__ bounds, DevDenseGang g) {
     extern __shared__ __align__(16) uint64_t xt[];
     __shared__ __align__(16) BoundTiles T;
     const uint32_t nchunk = d.n_pad / 64, xs = nchunk + 1, nblk = (d.n + 63) / 64;
-    const uint32_t slot = blockIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t n = d.n;
+    uint32_t slot, m = 0, H = 0;
+    uint32_t* sy = nullptr;
+    if (g.helpers) {
+        if (blockIdx.x % kGangXcdStride != 0) return;  // not a member: on another XCD
+        m = blockIdx.x / kGangXcdStride;
+        slot = blockIdx.y;
+        sy = g.sync + slot * kDenseGangSyncWords;
+        if (threadIdx.x == 0) atomicOr(&sy[kGangXcds], 1u << xcc_id());
+        uint32_t target = 0;
+        gang_barrier(&sy[kGangBarrier], target, 1 + g.helpers, g.spin_limit, g.err);
+        const bool one_xcd = __builtin_popcount(load_flag(&sy[kGangXcds])) == 1;
+        if (!one_xcd && m > 0) return;  // the main workgroup computes every field
+        H = one_xcd ? g.helpers : 0u;
+    } else {
+        slot = blockIdx.x;
+    }
     uint64_t* words = s.spins + (size_t)slot * n;
-    const int4* Dslot = bounds + (size_t)slot * nblk * 1024;
     for (uint32_t c = wave; c < nchunk; c += kBoundThreads / 64) {
         const uint32_t p = c * 64 + lane;
         xt[lane * xs + c] = wave_transpose(p < n ? words[p] : 0ull, lane);
     }
-    // Field waves 1, 2, 3, 5, 6, 7 are numbered fw = 0..5 and take columns
-    // [ch0, ch1) in whole prefetch groups; the first four also combine tile
-    // (rh, jh).
-    const bool fieldw = wave != 0 && wave != 4;
-    const uint32_t fw = wave < 4 ? wave - 1 : wave - 2;
-    const uint32_t ngrp = nchunk / kFieldPrefetch;  // n_pad is a multiple of 512
-    const uint32_t ch0 = fw * ngrp / 6 * kFieldPrefetch, ch1 = (fw + 1) * ngrp / 6 * kFieldPrefetch;
-    const bool combw = fieldw && fw < 4;
-    const uint32_t rh = fw >> 1, jh = fw & 1u;
-    for (uint32_t i = threadIdx.x; i < 2 * 2 * 16 * 64; i += kBoundThreads)
+    for (uint32_t i = threadIdx.x; i < kPartWords; i += kBoundThreads)
         (&T.hacc[0][0][0][0])[i] = 0;
     __syncthreads();
+    const uint32_t ngrp = nchunk / kFieldPrefetch;  // n_pad is a multiple of 512
+    uint32_t gb, ge;
+    gang_groups(ngrp, H, m, gb, ge);
+    uint64_t* xpub = g.xpub + (size_t)slot * nblk * 64;
+    int32_t* part = g.part + (size_t)slot * g.helpers * kPartBufs * kPartWords;
+    if (m > 0) {
+        // A helper: all eight waves take columns; block b's sums go to buffer b % kPartBufs.
+        const uint32_t c0 = (gb + wave * (ge - gb) / 8) * kFieldPrefetch;
+        const uint32_t c1 = (gb + (wave + 1) * (ge - gb) / 8) * kFieldPrefetch;
+        int32_t* mine = part + (size_t)(m - 1) * kPartBufs * kPartWords;
+        // Block b's sums from the spins at the start of block b - 2 (two blocks of
+        // slack; the main workgroup corrects block b - 2's flips in these columns).
+        for (uint32_t b = 0; b < nblk; ++b) {
+            // the coupling rows load while the wave waits for the spins
+            FieldRows r;
+            field_rows(d, b * 64, c0, c1, r, lane);
+            if (b >= 3) {  // the spins of block b - 3, the last before block b - 2
+                if (wave == 0) {
+                    if (lane == 0) wait_flag(&sy[kGangPublished], b - 2, g.spin_limit, g.err);
+                    __builtin_amdgcn_wave_barrier();
+                    xt[lane * xs + (b - 3)] = __hip_atomic_load(xpub + (b - 3) * 64 + lane,
+                                                                __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __syncthreads();
+            }
+            field_sums(d, xt, xs, b * 64, c0, c1, r, T.hacc, lane);
+            __syncthreads();
+            int32_t* dst = mine + (b % kPartBufs) * kPartWords;
+            for (uint32_t i = threadIdx.x; i < kPartWords; i += kBoundThreads) {
+                dst[i] = (&T.hacc[0][0][0][0])[i];
+                (&T.hacc[0][0][0][0])[i] = 0;
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&sy[kGangPartial + m - 1], b + 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    const int4* Dslot = bounds + (size_t)slot * nblk * 1024;
+    // Field waves 1, 2, 3, 5, 6, 7 are numbered fw = 0..5 and take the main
+    // workgroup's groups [gb, ge) in whole prefetch groups (with helpers only
+    // fw 4 and 5 do); the first four also combine tile (rh, jh).
+    const bool fieldw = wave != 0 && wave != 4;
+    const uint32_t fw = wave < 4 ? wave - 1 : wave - 2;
+    const bool combw = fieldw && fw < 4;
+    // with helpers only fw 4 and 5 take the main workgroup's columns
+    const uint32_t nfw = H ? 2u : 6u, ifw = H ? fw - 4 : fw;
+    const uint32_t ch0 = (!H || !combw) ? (gb + ifw * (ge - gb) / nfw) * kFieldPrefetch : 0u;
+    const uint32_t ch1 = (!H || !combw) ? (gb + (ifw + 1) * (ge - gb) / nfw) * kFieldPrefetch : 0u;
+    const uint32_t rh = fw >> 1, jh = fw & 1u;
+    // block nb's fields need block nb - 2's flips in its columns when a helper
+    // owns that chunk (the main workgroup owns chunks [0, ge * kFieldPrefetch))
+    auto stale = [&](uint32_t nb) { return H && nb >= 2 && nb - 2 >= ge * kFieldPrefetch; };
+    // a field wave's share of block nb's sums; the combine waves then add the
+    // helpers' sums for their tile (still while the chain runs)
+    auto fields = [&](uint32_t nb) {
+        block_partial(d, xt, xs, nb * 64, ch0, ch1, T.hacc, lane);
+        if (combw && H)
+            add_helper_sums(T.hacc, sy, nb, part + (nb % kPartBufs) * kPartWords,
+                            kPartBufs * kPartWords, H,
+                            rh, jh, lane, g.spin_limit, g.err);
+    };
     // Block 0: fields, keys and rows.
     if (fieldw) {
-        block_partial(d, xt, xs, 0, ch0, ch1, T.hacc, lane);
+        fields(0);
     } else if (wave == 4) {
-        stage_keys(d, 0, Dslot, xt, xs, T.key[0], T.rows[0], T.diag[0], T.corr, T.lin, lane);
+        stage_keys(d, 0, Dslot, xt, xs, T.key[0], T.rows[0], T.diag[0], T.corr, T.corr2, false,
+                   T.lin, lane);
     }
     __syncthreads();
-    if (combw) combine_fields(d, T, 0, rh, jh, lane);
+    if (combw) combine_fields(d, T, 0, rh, jh, lane, false);
     __syncthreads();
     int64_t eacc = 0;
     uint32_t nacc = 0;
@@ -782,10 +991,10 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
             {
                 v4i b0, b1;
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const auto p = __builtin_amdgcn_permlane32_swap(dxp[g], dxp[4 + g], false, false);
-                    b0[g] = (int32_t)p[0];
-                    b1[g] = (int32_t)p[1];
+                for (int q = 0; q < 4; ++q) {
+                    const auto p = __builtin_amdgcn_permlane32_swap(dxp[q], dxp[4 + q], false, false);
+                    b0[q] = (int32_t)p[0];
+                    b1[q] = (int32_t)p[1];
                 }
                 const v16i c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(atile, b0, v16i{}, 0, 0, 0);
                 const v16i c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(atile, b1, v16i{}, 0, 0, 0);
@@ -803,14 +1012,16 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
             const uint64_t xn = myx ^ ((uint64_t)f1 << 32 | f0);
             nacc += (uint32_t)__builtin_popcount(f0) + (uint32_t)__builtin_popcount(f1);
             eacc += eb;
+            T.xold2[lane] = T.xold[lane];
+            T.xnew2[lane] = T.xnew[lane];
             T.xold[lane] = myx;
             T.xnew[lane] = xn;
         } else if (more && fieldw) {
-            block_partial(d, xt, xs, (blk + 1) * 64, ch0, ch1, T.hacc, lane);
+            fields(blk + 1);
         } else if (more) {
             stage_keys(d, blk + 1, Dslot + (size_t)(blk + 1) * 1024, xt, xs,
                        T.key[(blk + 1) & 1u], T.rows[(blk + 1) & 1u], T.diag[(blk + 1) & 1u],
-                       T.corr, T.lin, lane);
+                       T.corr, T.corr2, stale(blk + 1), T.lin, lane);
         }
         __syncthreads();
         if (wave == 0) {
@@ -819,8 +1030,15 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
             const uint64_t w = wave_transpose(xn, lane);
             const uint32_t p = blk * 64 + lane;
             if (p < n) words[p] = w;
+            if (H && blk + 4 <= nblk) {  // the helpers read blocks up to nblk - 4
+                xpub[blk * 64 + lane] = xn;
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                if (lane == 0)
+                    __hip_atomic_store(&sy[kGangPublished], blk + 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
         } else if (more && combw) {
-            combine_fields(d, T, blk + 1, rh, jh, lane);
+            combine_fields(d, T, blk + 1, rh, jh, lane, stale(blk + 1));
         }
         __syncthreads();
     }
@@ -831,6 +1049,10 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
         if (lane == 0 && tot) atomicAdd(s.accepted, (unsigned long long)tot);
     }
+}
+
+__global__ void dense_gang_fill_kernel(uint32_t* p, uint32_t count) {
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0u;
 }
 
 bool set_lds_limit(const void* fn, size_t bytes) {
@@ -851,8 +1073,27 @@ size_t dense_bounds_words(const DevDense& d, uint32_t cells) {
     return (size_t)cells * ((d.n + 63) / 64) * 4096;
 }
 
+// Helpers only while every slot's gang fits one XCD (one workgroup per CU:
+// the kernel's LDS) and has columns to share; a slot's fields take a few
+// groups of kFieldPrefetch chunks, so small models keep fewer helpers.
+uint32_t dense_gang_helpers(const DevDense& d, uint32_t cells) {
+    constexpr uint32_t kXcdCus = 32;
+    const uint32_t ngrp = d.n_pad / 64 / kFieldPrefetch;
+    if (cells == 0 || cells * (1 + kMaxDenseHelpers) > kXcdCus || ngrp < 8) return 0;
+    return kMaxDenseHelpers;
+}
+
+size_t dense_gang_xpub_words(const DevDense& d, uint32_t cells) {
+    return (size_t)cells * ((d.n + 63) / 64) * 64;
+}
+
+size_t dense_gang_part_words(uint32_t cells, uint32_t helpers) {
+    return (size_t)cells * helpers * kPartBufs * kPartWords;
+}
+
 hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t sweep, uint64_t seed,
-                              const double* beta, int32_t* bounds, hipStream_t stream) {
+                              const double* beta, int32_t* bounds, const DevDenseGang* gang,
+                              hipStream_t stream) {
     const size_t lds = dense_lds_bytes(d.n_pad);
     if (bounds) {
         const void* fn = d.scale == 1
@@ -863,12 +1104,23 @@ hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t swe
         int4* D = reinterpret_cast<int4*>(bounds);
         hipLaunchKernelGGL(dense_bounds_kernel, dim3((waves + 3) / 4), dim3(256), 0, stream, d, s,
                            beta, sweep, (uint32_t)seed, (uint32_t)(seed >> 32), D, waves);
+        DevDenseGang g;
+        dim3 grid(s.cells);
+        if (gang && gang->helpers) {
+            if (gang->helpers > kMaxDenseHelpers || !gang->sync || !gang->err || !gang->xpub ||
+                !gang->part)
+                return hipErrorInvalidValue;
+            g = *gang;
+            grid = dim3(kGangXcdStride * (1 + g.helpers), s.cells);
+            hipLaunchKernelGGL(dense_gang_fill_kernel, dim3(1), dim3(256), 0, stream, g.sync,
+                               s.cells * kDenseGangSyncWords);
+        }
         if (d.scale == 1)
-            hipLaunchKernelGGL(sweep_dense_bound_kernel<true>, dim3(s.cells), dim3(kBoundThreads),
-                               lds, stream, d, s, D);
+            hipLaunchKernelGGL(sweep_dense_bound_kernel<true>, grid, dim3(kBoundThreads), lds,
+                               stream, d, s, D, g);
         else
-            hipLaunchKernelGGL(sweep_dense_bound_kernel<false>, dim3(s.cells), dim3(kBoundThreads),
-                               lds, stream, d, s, D);
+            hipLaunchKernelGGL(sweep_dense_bound_kernel<false>, grid, dim3(kBoundThreads), lds,
+                               stream, d, s, D, g);
         return hipGetLastError();
     }
     if (!set_lds_limit(reinterpret_cast<const void*>(&sweep_dense_kernel), lds))

@@ -179,9 +179,27 @@ hipError_t launch_exchange_replicas(const DevState& s, const double* beta, uint3
 // GPU (dense_bounds_kernel, then sweep_dense_bound_kernel; requires
 // dense_bounds_eligible and every beta the sweep reads >= kDenseBoundMinBeta);
 // NULL: by the threshold band inside the sequential chain.
+// gang: NULL, or buffers for field helpers (dense_gang_helpers): other
+// workgroups on the slot's XCD that compute part of each block's fields.
+struct DevDenseGang {
+    uint32_t helpers = 0;       // helper workgroups per slot (<= kMaxDenseHelpers)
+    uint32_t* sync = nullptr;   // kDenseGangSyncWords per slot, zeroed by the launch
+    uint32_t* err = nullptr;    // raised when a wait exceeds spin_limit polls
+    uint64_t* xpub = nullptr;   // dense_gang_xpub_words: finished blocks' spins per replica
+    int32_t* part = nullptr;    // dense_gang_part_words: the helpers' partial fields
+    uint32_t spin_limit = 0;
+};
+constexpr uint32_t kMaxDenseHelpers = 3;
+constexpr uint32_t kDenseGangSyncWords = 8;
 hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t sweep, uint64_t seed,
-                              const double* beta, int32_t* bounds, hipStream_t stream);
+                              const double* beta, int32_t* bounds, const DevDenseGang* gang,
+                              hipStream_t stream);
 size_t dense_bounds_words(const DevDense& d, uint32_t cells);
+// Field helpers per slot for the accept-bound sweep of `cells` slots (0: the
+// slot's workgroup computes every field itself).
+uint32_t dense_gang_helpers(const DevDense& d, uint32_t cells);
+size_t dense_gang_xpub_words(const DevDense& d, uint32_t cells);
+size_t dense_gang_part_words(uint32_t cells, uint32_t helpers);
 // The accept-bound kernel keeps local fields, accept bounds and their sums in
 // 24-bit multiplies and needs beta >= kDenseBoundMinBeta for its error bound.
 constexpr double kDenseBoundMinBeta = 1e-6;

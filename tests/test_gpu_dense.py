@@ -112,6 +112,32 @@ def test_dense_band_fallback_and_accept_bounds(ctx, factory):
     compare(st, ost, csr)
 
 
+@pytest.mark.parametrize("cells", [1, 3], ids=["one_slot", "three_slots"])
+def test_dense_field_helpers(ctx, cells):
+    """N=1600 (2048 padded columns, 25 blocks): each slot's gang of the accept-bound sweep
+    shares every block's fields with three helper workgroups on its XCD (the smaller models
+    above keep them in one workgroup).  Equal to the oracle after exchange rounds and after
+    an annealing schedule."""
+    qb = instances.sk_int8(1600, 21)
+    csr = ob.CSR.from_qubo(qb)
+    model = make(ctx, qb)
+    seed = 0x6A46_0000_0000_0005 + cells
+    st = _lib.State(model, cells, seed, kernel=_lib.KERNEL_DENSE)
+    ost = ob.OracleState(csr, cells, seed, view=ob.VIEW_WORD)
+    ladder = sk_betas(qb, cells)
+    assert ladder.min() >= 1e-6  # the accept-bound kernel runs
+    print(f"[dense] helpers n={qb.n} cells={cells}", flush=True)
+    st.set_ladder(ladder)
+    st.run(2, exchange=True)
+    ost.run(2, exchange=True, ladder=ladder)
+    compare(st, ost, csr, fields=False)
+    t_max, t_min = instances.suggested_temp_range(qb)
+    sched = np.repeat(instances.anneal_schedule(1.0 / t_max, 1.0 / t_min, 3), cells, axis=1)
+    st.run(3, beta_schedule=sched)
+    ost.run(3, exchange=False, schedule=sched)
+    compare(st, ost, csr)
+
+
 def test_dense_equals_generic_kernel(ctx):
     """Same seed, same schedule: the dense and the lane-per-replica kernels walk the same
     trajectory (one position at a time vs a colour class at a time)."""
