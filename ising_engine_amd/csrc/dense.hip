@@ -481,12 +481,16 @@ __device__ __forceinline__ uint32_t accept_bound(uint32_t u, const BoundLane& bp
 // Accept bounds of one sweep: D[((slot * nblk + blk) * 16 + g) * 64 + lane][q]
 // for position 64 blk + 4 g + q of replica lane (one Philox call per thread:
 // the word-view uniforms), -0x40000001 past n (a key that rejects).  One
-// wave per (slot, block, group).
+// wave per (slot, block, group).  Workgroup 0 also zeroes the sweep's gang
+// flags (zero: sync_count words at sync).
 __global__ __launch_bounds__(256) void dense_bounds_kernel(DevDense d, DevState s,
                                                            const double* __restrict__ beta,
                                                            uint32_t sweep, uint32_t key0,
                                                            uint32_t key1, int4* __restrict__ D,
-                                                           uint32_t waves) {
+                                                           uint32_t waves, uint32_t* sync,
+                                                           uint32_t sync_count) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < sync_count; i += blockDim.x) sync[i] = 0u;
     const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= waves) return;
     const uint32_t lane = threadIdx.x & 63u;
@@ -660,24 +664,38 @@ constexpr uint32_t kPartBufs = 3;
 // summed its columns from the spins before them), into h[replica][position];
 // the sums are cleared for the next block.  nb == 0: no flips yet.  The
 // coupling tiles and linear terms were staged in LDS (stage_keys).
+template <bool kGang>
 __device__ __forceinline__ void combine_fields(const DevDense& d, BoundTiles& T, uint32_t nb,
                                                uint32_t rh, uint32_t jh, uint32_t lane,
                                                bool stale) {
     const uint32_t j = 32 * jh + (lane & 31);
-    const uint32_t kh = 16 * (lane >> 5);
     v16i acc = {};
-    // flips o -> nw of a block's columns through coupling tiles ct
-    auto flips = [&](const v4i (*ct)[64], uint64_t o, uint64_t nw) {
+    if (nb > 0) {
+        const uint32_t kh = 16 * (lane >> 5);
+        const uint64_t o = T.xold[j], nw = T.xnew[j];
         const uint64_t up = nw & ~o, dn = o & ~nw;
 #pragma unroll
         for (uint32_t c = 0; c < 2; ++c) {
             const uint32_t sh = 32 * c + kh;
             const v4i bd = spread16((uint32_t)(up >> sh)) | (spread16((uint32_t)(dn >> sh)) * 0xFF);
-            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(ct[c][lane], bd, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.corr[rh][c][lane], bd, acc, 0, 0, 0);
         }
-    };
-    if (nb > 0) flips(T.corr[rh], T.xold[j], T.xnew[j]);
-    if (stale) flips(T.corr2[rh], T.xold2[j], T.xnew2[j]);
+    }
+    if constexpr (kGang) {
+        if (stale) {
+            const uint32_t kh = 16 * (lane >> 5);
+            const uint64_t o = T.xold2[j], nw = T.xnew2[j];
+            const uint64_t up = nw & ~o, dn = o & ~nw;
+#pragma unroll
+            for (uint32_t c = 0; c < 2; ++c) {
+                const uint32_t sh = 32 * c + kh;
+                const v4i bd =
+                    spread16((uint32_t)(up >> sh)) | (spread16((uint32_t)(dn >> sh)) * 0xFF);
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(T.corr2[rh][c][lane], bd, acc, 0, 0,
+                                                            0);
+            }
+        }
+    }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const uint32_t i = 32 * rh + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
@@ -857,7 +875,9 @@ __device__ __forceinline__ void gang_groups(uint32_t ngrp, uint32_t H, uint32_t 
 // add them at the end of block b - 1's chain and apply block b - 2's flips in
 // the helpers' columns with the other flip corrections.  Every wait is
 // bounded by g.spin_limit polls.
-template <bool kScale1>
+// kGang: the launch has helpers (the grid above); without, no helper code is
+// compiled in.
+template <bool kScale1, bool kGang>
 __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
     DevDense d, DevState s, const int4* __restrict__ bounds, DevDenseGang g) {
     extern __shared__ __align__(16) uint64_t xt[];
@@ -868,7 +888,7 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
     const uint32_t n = d.n;
     uint32_t slot, m = 0, H = 0;
     uint32_t* sy = nullptr;
-    if (g.helpers) {
+    if constexpr (kGang) {
         if (blockIdx.x % kGangXcdStride != 0) return;  // not a member: on another XCD
         m = blockIdx.x / kGangXcdStride;
         slot = blockIdx.y;
@@ -895,7 +915,7 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
     gang_groups(ngrp, H, m, gb, ge);
     uint64_t* xpub = g.xpub + (size_t)slot * nblk * 64;
     int32_t* part = g.part + (size_t)slot * g.helpers * kPartBufs * kPartWords;
-    if (m > 0) {
+    if (kGang && m > 0) {
         // A helper: all eight waves take columns; block b's sums go to buffer b % kPartBufs.
         const uint32_t c0 = (gb + wave * (ge - gb) / 8) * kFieldPrefetch;
         const uint32_t c1 = (gb + (wave + 1) * (ge - gb) / 8) * kFieldPrefetch;
@@ -945,12 +965,14 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
     const uint32_t rh = fw >> 1, jh = fw & 1u;
     // block nb's fields need block nb - 2's flips in its columns when a helper
     // owns that chunk (the main workgroup owns chunks [0, ge * kFieldPrefetch))
-    auto stale = [&](uint32_t nb) { return H && nb >= 2 && nb - 2 >= ge * kFieldPrefetch; };
+    auto stale = [&](uint32_t nb) {
+        return kGang && H && nb >= 2 && nb - 2 >= ge * kFieldPrefetch;
+    };
     // a field wave's share of block nb's sums; the combine waves then add the
     // helpers' sums for their tile (still while the chain runs)
     auto fields = [&](uint32_t nb) {
         block_partial(d, xt, xs, nb * 64, ch0, ch1, T.hacc, lane);
-        if (combw && H)
+        if (kGang && combw && H)
             add_helper_sums(T.hacc, sy, nb, part + (nb % kPartBufs) * kPartWords,
                             kPartBufs * kPartWords, H,
                             rh, jh, lane, g.spin_limit, g.err);
@@ -963,7 +985,7 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
                    T.lin, lane);
     }
     __syncthreads();
-    if (combw) combine_fields(d, T, 0, rh, jh, lane, false);
+    if (combw) combine_fields<kGang>(d, T, 0, rh, jh, lane, false);
     __syncthreads();
     int64_t eacc = 0;
     uint32_t nacc = 0;
@@ -1012,8 +1034,10 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
             const uint64_t xn = myx ^ ((uint64_t)f1 << 32 | f0);
             nacc += (uint32_t)__builtin_popcount(f0) + (uint32_t)__builtin_popcount(f1);
             eacc += eb;
-            T.xold2[lane] = T.xold[lane];
-            T.xnew2[lane] = T.xnew[lane];
+            if constexpr (kGang) {
+                T.xold2[lane] = T.xold[lane];
+                T.xnew2[lane] = T.xnew[lane];
+            }
             T.xold[lane] = myx;
             T.xnew[lane] = xn;
         } else if (more && fieldw) {
@@ -1030,7 +1054,7 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
             const uint64_t w = wave_transpose(xn, lane);
             const uint32_t p = blk * 64 + lane;
             if (p < n) words[p] = w;
-            if (H && blk + 4 <= nblk) {  // the helpers read blocks up to nblk - 4
+            if (kGang && H && blk + 4 <= nblk) {  // the helpers read blocks up to nblk - 4
                 xpub[blk * 64 + lane] = xn;
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 if (lane == 0)
@@ -1038,7 +1062,7 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
                                        __HIP_MEMORY_SCOPE_AGENT);
             }
         } else if (more && combw) {
-            combine_fields(d, T, blk + 1, rh, jh, lane, stale(blk + 1));
+            combine_fields<kGang>(d, T, blk + 1, rh, jh, lane, stale(blk + 1));
         }
         __syncthreads();
     }
@@ -1049,10 +1073,6 @@ __global__ __launch_bounds__(kBoundThreads) void sweep_dense_bound_kernel(
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
         if (lane == 0 && tot) atomicAdd(s.accepted, (unsigned long long)tot);
     }
-}
-
-__global__ void dense_gang_fill_kernel(uint32_t* p, uint32_t count) {
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0u;
 }
 
 bool set_lds_limit(const void* fn, size_t bytes) {
@@ -1096,14 +1116,14 @@ hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t swe
                               hipStream_t stream) {
     const size_t lds = dense_lds_bytes(d.n_pad);
     if (bounds) {
-        const void* fn = d.scale == 1
-                             ? reinterpret_cast<const void*>(&sweep_dense_bound_kernel<true>)
-                             : reinterpret_cast<const void*>(&sweep_dense_bound_kernel<false>);
+        const bool gk = gang && gang->helpers;
+        const void* fn =
+            d.scale == 1
+                ? (gk ? reinterpret_cast<const void*>(&sweep_dense_bound_kernel<true, true>)
+                      : reinterpret_cast<const void*>(&sweep_dense_bound_kernel<true, false>))
+                : (gk ? reinterpret_cast<const void*>(&sweep_dense_bound_kernel<false, true>)
+                      : reinterpret_cast<const void*>(&sweep_dense_bound_kernel<false, false>));
         if (!dense_bounds_eligible(d) || !set_lds_limit(fn, lds)) return hipErrorInvalidValue;
-        const uint32_t waves = s.cells * ((d.n + 63) / 64) * 16;
-        int4* D = reinterpret_cast<int4*>(bounds);
-        hipLaunchKernelGGL(dense_bounds_kernel, dim3((waves + 3) / 4), dim3(256), 0, stream, d, s,
-                           beta, sweep, (uint32_t)seed, (uint32_t)(seed >> 32), D, waves);
         DevDenseGang g;
         dim3 grid(s.cells);
         if (gang && gang->helpers) {
@@ -1112,15 +1132,24 @@ hipError_t launch_sweep_dense(const DevDense& d, const DevState& s, uint32_t swe
                 return hipErrorInvalidValue;
             g = *gang;
             grid = dim3(kGangXcdStride * (1 + g.helpers), s.cells);
-            hipLaunchKernelGGL(dense_gang_fill_kernel, dim3(1), dim3(256), 0, stream, g.sync,
-                               s.cells * kDenseGangSyncWords);
         }
-        if (d.scale == 1)
-            hipLaunchKernelGGL(sweep_dense_bound_kernel<true>, grid, dim3(kBoundThreads), lds,
-                               stream, d, s, D, g);
+        const uint32_t waves = s.cells * ((d.n + 63) / 64) * 16;
+        int4* D = reinterpret_cast<int4*>(bounds);
+        hipLaunchKernelGGL(dense_bounds_kernel, dim3((waves + 3) / 4), dim3(256), 0, stream, d, s,
+                           beta, sweep, (uint32_t)seed, (uint32_t)(seed >> 32), D, waves, g.sync,
+                           g.helpers ? s.cells * kDenseGangSyncWords : 0u);
+        if (d.scale == 1 && gk)
+            hipLaunchKernelGGL((sweep_dense_bound_kernel<true, true>), grid, dim3(kBoundThreads),
+                               lds, stream, d, s, D, g);
+        else if (d.scale == 1)
+            hipLaunchKernelGGL((sweep_dense_bound_kernel<true, false>), grid, dim3(kBoundThreads),
+                               lds, stream, d, s, D, g);
+        else if (gk)
+            hipLaunchKernelGGL((sweep_dense_bound_kernel<false, true>), grid, dim3(kBoundThreads),
+                               lds, stream, d, s, D, g);
         else
-            hipLaunchKernelGGL(sweep_dense_bound_kernel<false>, grid, dim3(kBoundThreads), lds,
-                               stream, d, s, D, g);
+            hipLaunchKernelGGL((sweep_dense_bound_kernel<false, false>), grid, dim3(kBoundThreads),
+                               lds, stream, d, s, D, g);
         return hipGetLastError();
     }
     if (!set_lds_limit(reinterpret_cast<const void*>(&sweep_dense_kernel), lds))
