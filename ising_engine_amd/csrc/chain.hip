@@ -72,11 +72,8 @@ namespace {
 // meets at the slot's own barrier bar[slot] after each class (bar[cells +
 // slot]: the XCDs its members run on).  Wave w of a slot takes positions
 // cb + 32 w + lane / 2, + 32 W, ... of every class (W waves per slot; lane &
-// 1 picks the replica half), queueing its tied replicas (entries relative to
-// the class start) and draining them before the barrier, as
-// classes_sweep_kernel does in one workgroup.  Sweep k reads its level
-// thresholds from lev + k * lev_stride.
-constexpr uint32_t kChainQueueCap = 1024;  // tied replicas per wave and class
+// 1 picks the replica half), resolving its tied replicas in place.  Sweep k
+// reads its level thresholds from lev + k * lev_stride.
 
 template <int B, int W>
 __global__ __launch_bounds__(kChainBlock, 2) void sweep_chain_ms_kernel(
@@ -90,9 +87,6 @@ __global__ __launch_bounds__(kChainBlock, 2) void sweep_chain_ms_kernel(
     static_assert(W >= B + 3, "counters too narrow");
     constexpr uint32_t kFlush = 8;  // sites per lane between counter flushes
     extern __shared__ unsigned long long recs[];  // 2 buffers of (2^B + 1) * kQStride
-    __shared__ uint32_t queue[kChainWaves][kChainQueueCap];
-    __shared__ uint32_t qcount[kChainWaves];
-    __shared__ unsigned long long edef[kChainWaves][64];
     __shared__ unsigned long long blk_e[64];
     __shared__ uint32_t blk_acc;
     if (blockIdx.x % kXcdStride != 0) return;  // not a member: on another XCD
@@ -105,7 +99,6 @@ __global__ __launch_bounds__(kChainBlock, 2) void sweep_chain_ms_kernel(
     uint32_t* sbar = bar + slot;
     uint32_t* sxcd = bar + gridDim.y + slot;
     const uint32_t tr = temp_of[slot * 64 + lane];
-    edef[wv][lane] = 0;
     if (wv == 0) blk_e[lane] = 0;
     if (tid == 0) blk_acc = 0;
     uint32_t kacc[W], dacc[W];
@@ -155,22 +148,14 @@ __global__ __launch_bounds__(kChainBlock, 2) void sweep_chain_ms_kernel(
         for (uint32_t c = 0; c < ncol; ++c) {
             const uint32_t cb = __builtin_amdgcn_readfirstlane(cstart[c]);
             const uint32_t ce = __builtin_amdgcn_readfirstlane(cstart[c + 1]);
-            if (lane == 0) qcount[wv] = 0;
-            __builtin_amdgcn_wave_barrier();
             for (uint32_t base = cb + gw * kPairSites; base < ce; base += tw * kPairSites) {
                 const uint32_t p = base + (lane >> 1);
                 if (p < ce)
-                    irregular_site_pair<B, W>(dw, S, S, p, lane & 1u, p - cb, row, ci, R,
-                                                  slot, sweep, key0, key1, w0, kacc, dacc, nacc,
-                                                  queue[wv], &qcount[wv], kChainQueueCap,
-                                                  edef[wv]);
+                    irregular_site_pair<B, W>(dw, S, S, p, lane & 1u, row, ci, R, slot, sweep,
+                                              key0, key1, w0, kacc, dacc, nacc);
                 if (++pending == kFlush) flush();  // wave-uniform
                 if (base + tw * kPairSites < ce) load_row(base + tw * kPairSites, ce);
             }
-            __threadfence_block();
-            __builtin_amdgcn_wave_barrier();
-            drain_queue(queue[wv], &qcount[wv], kChainQueueCap, cb, S, R, slot, sweep, key0,
-                        key1, w0, edef[wv], lane, nacc);
             // Arrive, load the next class's rows, build the next sweep's
             // records after the last class and flush counters, then wait.
             if (gang > 1 && one_xcd)
@@ -190,7 +175,6 @@ __global__ __launch_bounds__(kChainBlock, 2) void sweep_chain_ms_kernel(
         }
     }
     flush();
-    e_lane += (int64_t)edef[wv][lane];
     for (uint32_t off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, (int)off, 64);
     atomicAdd(&blk_e[lane], (unsigned long long)e_lane);
     if (lane == 0) atomicAdd(&blk_acc, nacc);

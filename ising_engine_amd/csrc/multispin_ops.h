@@ -664,10 +664,9 @@ __device__ __forceinline__ uint32_t pair_counts(const uint64_t* S, uint32_t p, u
 // branches split the batches of record reads).
 template <int BF, int B, int W>
 __device__ __forceinline__ void pair_update(
-    const uint32_t (&c)[B], uint32_t x, uint64_t* Wr, uint32_t p, uint32_t h, uint32_t qoff,
-    uint32_t d, const unsigned long long* recs, uint32_t slot, uint32_t sweep, uint32_t key0,
-    uint32_t key1, int32_t w0, uint32_t (&kacc)[W], uint32_t (&dacc)[W], uint32_t& nacc,
-    uint32_t* queue, uint32_t* qcount, uint32_t qcap, unsigned long long* edef) {
+    const uint32_t (&c)[B], uint32_t x, uint64_t* Wr, uint32_t p, uint32_t h, uint32_t d,
+    const unsigned long long* recs, uint32_t slot, uint32_t sweep, uint32_t key0,
+    uint32_t key1, int32_t w0, uint32_t (&kacc)[W], uint32_t (&dacc)[W], uint32_t& nacc) {
     static_assert(kEagerPlanes == 8, "a pair's lanes compute two Philox calls each");
     constexpr int LMAX = 1 << (BF - 1);
     // recs as 32-bit halves: half h of record word i at rh[2 i + h].  Level m
@@ -717,6 +716,23 @@ __device__ __forceinline__ void pair_update(
         lt |= __builtin_amdgcn_bitop3_b32(und, T, P[b], kAndAndNot);
         und = __builtin_amdgcn_bitop3_b32(und, P[b], T, kAndXnor);
     }
+    // Tied replicas are resolved here from their tail words, before the word
+    // is written, so their flips and energy changes go through the same
+    // store and counters as the others.  A compared replica ties with
+    // probability 1/256, so a lane of 32 replicas has one with probability
+    // below 1/8 and a wave usually makes a single pass of the loop; a queue
+    // drained after the class added its atomics' round trip to the latency
+    // of every class.
+    while (und) {
+        const uint32_t rb = (uint32_t)__builtin_ctz(und);
+        und &= und - 1;
+        uint32_t m = 0;
+#pragma unroll
+        for (int l = 0; l < LMAX; ++l)
+            if ((lev[l] >> rb) & 1u) m = m0 + 2 * l;
+        const uint32_t r = 32 * h + rb;
+        if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) lt |= 1u << rb;
+    }
     const uint32_t flip = ~pos | lt;
     uint32_t* Wh = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Wr) + p * 8u + h * 4u);
     *Wh = x ^ flip;
@@ -743,24 +759,6 @@ __device__ __forceinline__ void pair_update(
         cd = __builtin_amdgcn_bitop3_b32(dacc[b], ad, cd, kMaj);
         dacc[b] = sd;
     }
-    while (und) {
-        const uint32_t rb = (uint32_t)__builtin_ctz(und);
-        und &= und - 1;
-        const uint32_t r = 32 * h + rb;
-        uint32_t m = 0;
-#pragma unroll
-        for (int l = 0; l < LMAX; ++l)
-            if ((lev[l] >> rb) & 1u) m = m0 + 2 * l;
-        const uint32_t at = atomicAdd(qcount, 1u);
-        if (at < qcap) {
-            queue[at] = (qoff << 11) | (r << 5) | m;
-        } else if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) {
-            __threadfence_block();
-            atomicXor(Wh, 1u << rb);
-            atomicAdd(&edef[r], (unsigned long long)(((int64_t)m * w0) / 2));
-            nacc += 1;
-        }
-    }
 }
 
 // irregular_site with a position's 64 replicas split over a lane pair: lane
@@ -768,9 +766,10 @@ __device__ __forceinline__ void pair_update(
 // so each lane does half the bit operations, and the pair shares the Philox
 // calls of the position: lane h computes calls 2 h and 2 h + 1 (bit planes
 // 4 h .. 4 h + 3 of all 64 replicas) and trades the other half's words with
-// its partner.  The uniforms, thresholds, decisions and queue entries are
-// those of irregular_site; kacc and dacc count this lane's 32 replicas
-// (wave_column_sum_pair).  Both lanes of a pair take the same p, whose row
+// its partner.  The uniforms, thresholds and decisions are those of
+// irregular_site (tied replicas resolved in place from the same tail words);
+// kacc and dacc count this lane's 32 replicas (wave_column_sum_pair), tied
+// flips included.  Both lanes of a pair take the same p, whose row
 // is `row` (load_pair_row); dw is the largest degree of the wave (uniform).
 // A site's update depends on its degree, counts, uniforms and thresholds,
 // not on the planes' width, so when dw < 2^(B-1) the levels are compared on
@@ -781,22 +780,21 @@ __device__ __forceinline__ void pair_update(
 // used).
 template <int B, int W>
 __device__ __forceinline__ void irregular_site_pair(
-    uint32_t dw, const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t h, uint32_t qoff,
-    const PairRow& row, const uint32_t* ci, const unsigned long long* recs, uint32_t slot,
-    uint32_t sweep, uint32_t key0, uint32_t key1, int32_t w0, uint32_t (&kacc)[W],
-    uint32_t (&dacc)[W], uint32_t& nacc, uint32_t* queue, uint32_t* qcount, uint32_t qcap,
-    unsigned long long* edef) {
+    uint32_t dw, const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t h, const PairRow& row,
+    const uint32_t* ci, const unsigned long long* recs, uint32_t slot, uint32_t sweep,
+    uint32_t key0, uint32_t key1, int32_t w0, uint32_t (&kacc)[W], uint32_t (&dacc)[W],
+    uint32_t& nacc) {
     uint32_t c[B];
     const uint32_t x = pair_counts<B>(S, p, h, row, ci, c);
     if constexpr (B > 3) {
         if (dw < (1u << (B - 1))) {
-            pair_update<B - 1, B, W>(c, x, Wr, p, h, qoff, row.d, recs, slot, sweep, key0, key1,
-                                     w0, kacc, dacc, nacc, queue, qcount, qcap, edef);
+            pair_update<B - 1, B, W>(c, x, Wr, p, h, row.d, recs, slot, sweep, key0, key1, w0,
+                                     kacc, dacc, nacc);
             return;
         }
     }
-    pair_update<B, B, W>(c, x, Wr, p, h, qoff, row.d, recs, slot, sweep, key0, key1, w0, kacc,
-                         dacc, nacc, queue, qcount, qcap, edef);
+    pair_update<B, B, W>(c, x, Wr, p, h, row.d, recs, slot, sweep, key0, key1, w0, kacc, dacc,
+                         nacc);
 }
 
 // Drains a wave's per-replica tie queue (irregular_site entries relative to
