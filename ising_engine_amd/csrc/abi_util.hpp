@@ -109,4 +109,12 @@ inline bool dense_helpers_enabled() {
     return on;
 }
 
+// IGE_CHAIN_BARRIERS=1 runs a one-slot chained run with a gang barrier after
+// every colour class instead of per-site waits (the same bits); read at each
+// call.
+inline bool chain_barriers_forced() {
+    const char* v = std::getenv("IGE_CHAIN_BARRIERS");
+    return v && v[0] == '1' && v[1] == '\0';
+}
+
 }  // namespace ige::abi

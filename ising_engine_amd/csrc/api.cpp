@@ -164,6 +164,7 @@ struct ige_state {
     DevBuf<uint64_t> dense_xpub;
     DevBuf<int32_t> dense_part;
     DevBuf<uint64_t> hb;                             // heat-bath thresholds, on first use
+    DevBuf<uint64_t> flow_units;  // words with per-site sweep counts (chained runs)
     DevBuf<int32_t> fields;
     DevBuf<ige::BestRecord> best;  // allocated by the first incumbent update
     DevBuf<uint32_t> icm_scratch;  // ICM bitmaps and queues, on first use
@@ -361,6 +362,9 @@ constexpr uint32_t kGraphSteps = 32;
 // Polls of one device-wide barrier wait of a chained run before it gives up
 // (its workgroups are all resident, so a wait ends long before).
 constexpr uint32_t kChainSpinLimit = 1u << 24;
+// Polls of one wait in a chained run without barriers (each poll reloads the
+// awaited words, about a microsecond with its sleep).
+constexpr uint32_t kFlowSpinLimit = 1u << 21;
 
 // Raises the error an expired wait left in flag word *f (and clears it).
 void check_wait_flag(ige_state* s, uint32_t* f, const char* what) {
@@ -872,12 +876,23 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                 lev = s->lev_rows.p;
                 stride = lrow;
             }
-            check(ige::launch_sweep_chain_ms(m, s->view(), s->model->cstart.p,
-                                             s->model->co.num_colors, s->model->max_class, lev,
-                                             stride, s->sweep, sweeps, s->seed,
-                                             s->chain_sync.p + 1, s->chain_sync.p,
-                                             kChainSpinLimit, s->stream),
-                  "chained multispin sweeps");
+            if (s->cells == 1 && !ige::abi::chain_barriers_forced() &&
+                ige::flow_ms_capacity_blocks(s->model->dmax) >= 1) {
+                if (s->flow_units.count < 2u * m.n) s->flow_units.alloc(2u * m.n);
+                check(ige::launch_sweep_flow_ms(m, s->view(), s->model->cstart.p,
+                                                s->model->co.num_colors, s->model->max_class,
+                                                lev, stride, s->sweep, sweeps, s->seed,
+                                                s->flow_units.p, s->chain_sync.p, kFlowSpinLimit,
+                                                s->stream),
+                      "chained multispin sweeps");
+            } else {
+                check(ige::launch_sweep_chain_ms(m, s->view(), s->model->cstart.p,
+                                                 s->model->co.num_colors, s->model->max_class,
+                                                 lev, stride, s->sweep, sweeps, s->seed,
+                                                 s->chain_sync.p + 1, s->chain_sync.p,
+                                                 kChainSpinLimit, s->stream),
+                      "chained multispin sweeps");
+            }
             s->sweep += sweeps;
             s->chain_sweeps += sweeps;
             done = sweeps;

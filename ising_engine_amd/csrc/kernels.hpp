@@ -146,6 +146,16 @@ hipError_t launch_sweep_chain_ms(const DevModel& m, const DevState& s, const uin
                                  uint64_t seed, uint32_t* bars, uint32_t* err, uint32_t spin_limit,
                                  hipStream_t stream);
 uint32_t chain_ms_capacity_blocks(uint32_t dmax);
+// launch_sweep_chain_ms without barriers, for one slot: a site waits for
+// its neighbours' per-site sweep counts.  units: 2 n u64 of scratch (the
+// words with their counts during the launch); err raised when a wait
+// expires.  The same bits as launch_sweep_chain_ms.
+hipError_t launch_sweep_flow_ms(const DevModel& m, const DevState& s, const uint32_t* cstart,
+                                uint32_t ncol, uint32_t max_class, const uint32_t* lev,
+                                uint32_t lev_stride, uint32_t sweep0, uint32_t sweeps,
+                                uint64_t seed, uint64_t* units, uint32_t* err,
+                                uint32_t spin_limit, hipStream_t stream);
+uint32_t flow_ms_capacity_blocks(uint32_t dmax);
 hipError_t launch_classes_sweep(const DevModel& m, const DevState& s,
                                 const uint32_t* class_start, uint32_t c0, uint32_t c1,
                                 uint32_t sweep, uint64_t seed, const uint32_t* lev_thr,

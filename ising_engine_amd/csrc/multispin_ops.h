@@ -658,23 +658,26 @@ __device__ __forceinline__ uint32_t pair_counts(const uint64_t* S, uint32_t p, u
 }
 
 // The rest of the update after pair_counts, on the first BF count planes
-// (the wave's degrees are below 2^BF, so planes BF .. B - 1 are zero).
-// Every level up to 2^(BF-1) is compared: skipping the levels past the
-// wave's largest degree with uniform branches measured 14 % slower (the
-// branches split the batches of record reads).
-template <int BF, int B, int W>
-__device__ __forceinline__ void pair_update(
-    const uint32_t (&c)[B], uint32_t x, uint64_t* Wr, uint32_t p, uint32_t h, uint32_t d,
-    const unsigned long long* recs, uint32_t slot, uint32_t sweep, uint32_t key0,
-    uint32_t key1, int32_t w0, uint32_t (&kacc)[W], uint32_t (&dacc)[W], uint32_t& nacc) {
+// (the wave's degrees are below 2^BF, so planes BF .. B - 1 are zero):
+// returns the lane's flip mask (bit rb: replica 32 h + rb flips) and adds the
+// flips to the counters.  Every level up to 2^(BF-1) is compared: skipping
+// the levels past the wave's largest degree with uniform branches measured
+// 14 % slower (the branches split the batches of record reads).  recs holds
+// a record every RS u64 (the planes and the nonzero mask first);
+// tie_thr(m, r) is replica r's full threshold at level m.
+template <int BF, int B, int W, uint32_t RS = kQStride, class TieThr>
+__device__ __forceinline__ uint32_t pair_update(
+    const uint32_t (&c)[B], uint32_t p, uint32_t h, uint32_t d, const unsigned long long* recs,
+    uint32_t slot, uint32_t sweep, uint32_t key0, uint32_t key1, uint32_t (&kacc)[W],
+    uint32_t (&dacc)[W], uint32_t& nacc, TieThr&& tie_thr) {
     static_assert(kEagerPlanes == 8, "a pair's lanes compute two Philox calls each");
     constexpr int LMAX = 1 << (BF - 1);
     // recs as 32-bit halves: half h of record word i at rh[2 i + h].  Level m
     // = 2 k - d of k = floor(d/2) + 1 + j is 2 j + 2 - (d & 1): its record
-    // starts 2 j kQStride words after rl's.  Levels of k > d (up to 2^BF) are
-    // read but masked out by lev[j] = 0.
+    // starts 2 j RS words after rl's.  Levels of k > d (up to 2^BF) are read
+    // but masked out by lev[j] = 0.
     const uint32_t m0 = 2u - (d & 1u);
-    const uint32_t* rl = reinterpret_cast<const uint32_t*>(recs) + h + 2 * m0 * kQStride;
+    const uint32_t* rl = reinterpret_cast<const uint32_t*>(recs) + h + 2 * m0 * RS;
     uint32_t lev[LMAX];
     uint32_t pos = 0, und = 0;
 #pragma unroll
@@ -685,8 +688,7 @@ __device__ __forceinline__ void pair_update(
         for (int b = 0; b < BF; ++b) eq &= ((k >> b) & 1u) ? c[b] : ~c[b];
         lev[j] = eq;
         pos |= eq;
-        und = __builtin_amdgcn_bitop3_b32(eq, rl[2 * (2 * j * kQStride + kEagerPlanes)], und,
-                                          kAndOr);
+        und = __builtin_amdgcn_bitop3_b32(eq, rl[2 * (2 * j * RS + kEagerPlanes)], und, kAndOr);
     }
     // Plane 2 j + t of call j: replicas 0..31 in word 2 t, 32..63 in 2 t + 1.
     uint32_t keep[4], send[4];
@@ -712,7 +714,7 @@ __device__ __forceinline__ void pair_update(
         uint32_t T = lev[0] & rl[2 * b];
 #pragma unroll
         for (int l = 1; l < LMAX; ++l)
-            T = __builtin_amdgcn_bitop3_b32(lev[l], rl[2 * (2 * l * kQStride + b)], T, kAndOr);
+            T = __builtin_amdgcn_bitop3_b32(lev[l], rl[2 * (2 * l * RS + b)], T, kAndOr);
         lt |= __builtin_amdgcn_bitop3_b32(und, T, P[b], kAndAndNot);
         und = __builtin_amdgcn_bitop3_b32(und, P[b], T, kAndXnor);
     }
@@ -731,11 +733,9 @@ __device__ __forceinline__ void pair_update(
         for (int l = 0; l < LMAX; ++l)
             if ((lev[l] >> rb) & 1u) m = m0 + 2 * l;
         const uint32_t r = 32 * h + rb;
-        if (resolve_tail(p, slot, sweep, key0, key1, r, record_thr(recs, m, r))) lt |= 1u << rb;
+        if (resolve_tail(p, slot, sweep, key0, key1, r, tie_thr(m, r))) lt |= 1u << rb;
     }
     const uint32_t flip = ~pos | lt;
-    uint32_t* Wh = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Wr) + p * 8u + h * 4u);
-    *Wh = x ^ flip;
     nacc += (uint32_t)__builtin_popcount(flip);
     uint32_t ck = 0, cd = 0;
 #pragma unroll
@@ -759,6 +759,7 @@ __device__ __forceinline__ void pair_update(
         cd = __builtin_amdgcn_bitop3_b32(dacc[b], ad, cd, kMaj);
         dacc[b] = sd;
     }
+    return flip;
 }
 
 // irregular_site with a position's 64 replicas split over a lane pair: lane
@@ -782,19 +783,19 @@ template <int B, int W>
 __device__ __forceinline__ void irregular_site_pair(
     uint32_t dw, const uint64_t* S, uint64_t* Wr, uint32_t p, uint32_t h, const PairRow& row,
     const uint32_t* ci, const unsigned long long* recs, uint32_t slot, uint32_t sweep,
-    uint32_t key0, uint32_t key1, int32_t w0, uint32_t (&kacc)[W], uint32_t (&dacc)[W],
-    uint32_t& nacc) {
+    uint32_t key0, uint32_t key1, uint32_t (&kacc)[W], uint32_t (&dacc)[W], uint32_t& nacc) {
     uint32_t c[B];
     const uint32_t x = pair_counts<B>(S, p, h, row, ci, c);
-    if constexpr (B > 3) {
-        if (dw < (1u << (B - 1))) {
-            pair_update<B - 1, B, W>(c, x, Wr, p, h, row.d, recs, slot, sweep, key0, key1, w0,
-                                     kacc, dacc, nacc);
-            return;
-        }
-    }
-    pair_update<B, B, W>(c, x, Wr, p, h, row.d, recs, slot, sweep, key0, key1, w0, kacc, dacc,
-                         nacc);
+    auto thr = [&](uint32_t m, uint32_t r) { return record_thr(recs, m, r); };
+    uint32_t flip;
+    if (B > 3 && dw < (1u << (B - 1)))
+        flip = pair_update<(B > 3 ? B - 1 : B), B, W>(c, p, h, row.d, recs, slot, sweep, key0,
+                                                      key1, kacc, dacc, nacc, thr);
+    else
+        flip = pair_update<B, B, W>(c, p, h, row.d, recs, slot, sweep, key0, key1, kacc, dacc,
+                                    nacc, thr);
+    uint32_t* Wh = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Wr) + p * 8u + h * 4u);
+    *Wh = x ^ flip;
 }
 
 // Drains a wave's per-replica tie queue (irregular_site entries relative to

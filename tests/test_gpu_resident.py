@@ -141,19 +141,25 @@ def test_fused_tail_matches_the_oracle(ctx):
     assert s.chain_sweeps == 0
 
 
+@pytest.mark.parametrize("barriers", [False, True], ids=["site_waits", "class_barriers"])
 @pytest.mark.parametrize("n,degree", [(20000, 6), (3000, 1), (3000, 5), (3000, 12)],
                          ids=["er20000_d6", "er3000_d1", "er3000_d5", "er3000_d12"])
-def test_chained_sweeps_match_the_oracle(ctx, n, degree):
-    # the whole call in one launch, every class behind a device-wide barrier (degree < 8,
-    # < 16 and < 32 counter widths): against the oracle on a schedule, then on a fixed
-    # ladder, and against the same sweeps run one call each
+def test_chained_sweeps_match_the_oracle(ctx, n, degree, barriers, monkeypatch):
+    # the whole call in one launch (degree < 8, < 16 and < 32 counter widths), each site
+    # waiting for its neighbours' sweep counts, or every class behind a gang barrier:
+    # against the oracle on a schedule, then on a fixed ladder, and against the same
+    # sweeps run one call each
+    if barriers:
+        monkeypatch.setenv("IGE_CHAIN_BARRIERS", "1")
+    else:
+        monkeypatch.delenv("IGE_CHAIN_BARRIERS", raising=False)
     qb = instances.erdos_renyi_pm_j(n, degree, 90 + degree)
     dmax = int(np.bincount(np.concatenate([qb.pi, qb.pj]), minlength=n).max())
     assert dmax < 32
     csr = ob.CSR.from_qubo(qb)
     model = make(ctx, qb)
     print(f"[chain] n={n} degree={degree} dmax={model.info.max_degree} "
-          f"colours={model.info.num_colors}", flush=True)
+          f"colours={model.info.num_colors} barriers={barriers}", flush=True)
     seed = 0xC4A1 + n
     st = _lib.State(model, 1, seed, kernel=_lib.KERNEL_MULTISPIN)
     ost = ob.OracleState(csr, 1, seed, view=ob.VIEW_PLANE)
