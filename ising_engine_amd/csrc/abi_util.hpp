@@ -117,4 +117,15 @@ inline bool chain_barriers_forced() {
     return v && v[0] == '1' && v[1] == '\0';
 }
 
+// Polls a chained run's waits may make before they give up and raise the
+// state's error flag: `fallback`, or IGE_CHAIN_SPIN_LIMIT when set (a test
+// sets 0 so that waits expire at once); read at each call.
+inline uint32_t chain_spin_limit(uint32_t fallback) {
+    const char* v = std::getenv("IGE_CHAIN_SPIN_LIMIT");
+    if (!v || !*v) return fallback;
+    char* end = nullptr;
+    const unsigned long x = std::strtoul(v, &end, 10);
+    return (end && *end == '\0' && x <= 0xFFFFFFFFul) ? (uint32_t)x : fallback;
+}
+
 }  // namespace ige::abi

@@ -882,7 +882,8 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                 check(ige::launch_sweep_flow_ms(m, s->view(), s->model->cstart.p,
                                                 s->model->co.num_colors, s->model->max_class,
                                                 lev, stride, s->sweep, sweeps, s->seed,
-                                                s->flow_units.p, s->chain_sync.p, kFlowSpinLimit,
+                                                s->flow_units.p, s->chain_sync.p,
+                                                ige::abi::chain_spin_limit(kFlowSpinLimit),
                                                 s->stream),
                       "chained multispin sweeps");
             } else {
@@ -890,7 +891,8 @@ int32_t ige_run(ige_state* s, uint32_t sweeps, uint32_t flags, const double* sch
                                                  s->model->co.num_colors, s->model->max_class,
                                                  lev, stride, s->sweep, sweeps, s->seed,
                                                  s->chain_sync.p + 1, s->chain_sync.p,
-                                                 kChainSpinLimit, s->stream),
+                                                 ige::abi::chain_spin_limit(kChainSpinLimit),
+                                                 s->stream),
                       "chained multispin sweeps");
             }
             s->sweep += sweeps;

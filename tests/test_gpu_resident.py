@@ -186,3 +186,29 @@ def test_chained_sweeps_match_the_oracle(ctx, n, degree, barriers, monkeypatch):
         one.run(1)
     same(st, one)
     assert one.stats()[0].chain_sweeps == 0
+
+
+@pytest.mark.parametrize("barriers", [False, True], ids=["site_waits", "class_barriers"])
+def test_chained_wait_expiry_is_reported(ctx, barriers, monkeypatch):
+    # a spin limit of 0 polls makes the first wait that is not already satisfied give up:
+    # the launch still ends, the state reports the expired wait when it is next
+    # synchronised (a run on a fixed ladder returns before its launch ends), and the next
+    # call runs
+    if barriers:
+        monkeypatch.setenv("IGE_CHAIN_BARRIERS", "1")
+    else:
+        monkeypatch.delenv("IGE_CHAIN_BARRIERS", raising=False)
+    qb = instances.erdos_renyi_pm_j(20000, 6, 96)
+    model = make(ctx, qb)
+    st = _lib.State(model, 1, 0xE7, kernel=_lib.KERNEL_MULTISPIN)
+    st.set_ladder(np.array([0.7]))
+    print(f"[chain-expiry] barriers={barriers}", flush=True)
+    monkeypatch.setenv("IGE_CHAIN_SPIN_LIMIT", "0")
+    with pytest.raises(_lib.IgeError, match="spin limit"):
+        st.run(50)
+        st.synchronize()
+    monkeypatch.delenv("IGE_CHAIN_SPIN_LIMIT")
+    before = st.stats()[0].chain_sweeps
+    st.run(10)
+    st.synchronize()
+    assert st.stats()[0].chain_sweeps == before + 10
