@@ -410,7 +410,9 @@ __device__ __forceinline__ uint32_t pt_sweeps(const InstDesc& I, uint32_t k0, ui
 // kPacked (WAVES >= 4): registers capped for 8 waves per SIMD (4 workgroups
 // of 8 waves per CU instead of 3; SGPRs spill to VGPR lanes outside the
 // entry loops).  The batch runs this form (+7% on the 4096-instance QUBO
-// batch); the serving loop the other, whose PT gangs lost 7% with it.
+// batch); the serving loop the other, whose PT gangs lost 7% with it.  The
+// packed form runs anneals only (the batch has no PT solves), so it compiles
+// no PT path and carries none of its uniform values.
 template <int WAVES, bool kPacked>
 __global__ __launch_bounds__(WAVES * 64)
 __attribute__((amdgpu_waves_per_eu(kPacked && WAVES >= 4 ? 8 : 1))) void persistent_anneal_kernel(
@@ -441,7 +443,7 @@ __attribute__((amdgpu_waves_per_eu(kPacked && WAVES >= 4 ? 8 : 1))) void persist
         uint32_t k_end = k1;
         // One code path per kind and threshold mode, so none carries another's
         // registers.  A PT solve keeps its energies up to date itself (eacc stays 0).
-        if (I.pt) {
+        if (!kPacked && I.pt) {
             if (mode == kThrLds)
                 k_end = pt_sweeps<WAVES, kThrLds>(I, k0, k1, member, gang, lane, wave, thr_lds,
                                                   bar_target, spin_limit, err, nacc, t_start);

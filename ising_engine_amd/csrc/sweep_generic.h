@@ -79,6 +79,23 @@ __device__ __forceinline__ uint64_t wave_transpose(uint64_t x, uint32_t lane) {
     return x;
 }
 
+// base[i] as a global-memory access with a 32-bit byte offset (i * sizeof(T)
+// < 2^32): pointers that come from a descriptor in memory are generic, and a
+// generic access is a flat instruction that needs a 64-bit address per lane
+// and waits on the LDS counter too; a global one takes the uniform base in
+// SGPRs and one offset VGPR.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, uint32_t i) {
+    // C-style casts: the address-space conversion has no C++ cast
+    return *(gptr<const T>)((gptr<const char>)base + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, uint32_t i, T v) {
+    *(gptr<T>)((gptr<char>)base + i * (uint32_t)sizeof(T)) = v;
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
     const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
     const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
@@ -135,18 +152,18 @@ __device__ __forceinline__ void generic_group(
     uint32_t rb[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q)
-        rb[q] = __builtin_amdgcn_readfirstlane(rp[p0 + min((uint32_t)q, cnt)]);
-    const uint64_t xw = lane < cnt ? S[p0 + lane] : 0ull;
+        rb[q] = __builtin_amdgcn_readfirstlane(gld(rp, p0 + min((uint32_t)q, cnt)));
+    const uint64_t xw = lane < cnt ? gld(S, p0 + lane) : 0ull;
     int32_t h[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) h[q] = (uint32_t)q < cnt ? lin[p0 + q] : 0;
+    for (int q = 0; q < 4; ++q) h[q] = (uint32_t)q < cnt ? gld(lin, p0 + q) : 0;
     for (uint32_t base = rb[0]; base < rb[4]; base += 64) {
         const uint32_t e = base + lane;
         uint64_t nw = 0;
         int32_t nwt = 0;
         if (e < rb[4]) {
-            nwt = wt[e];
-            nw = S[ci[e]];
+            nwt = gld(wt, e);
+            nw = gld(S, gld(ci, e));
         }
         const uint32_t top = min(base + 64, rb[4]);
 #pragma unroll
@@ -179,7 +196,7 @@ __device__ __forceinline__ void generic_group(
             if (!acc) acc = accept(uu, de);
         }
         const uint64_t m = wave_ballot(acc);
-        if (lane == 0) W[p0 + q] = x ^ m;
+        if (lane == 0) gst(W, p0 + q, x ^ m);
         eacc += acc ? de : 0;
         nacc += (uint32_t)__builtin_popcountll(m);
     }
