@@ -166,19 +166,30 @@ __device__ __forceinline__ void generic_group(
             nw = gld(S, gld(ci, e));
         }
         const uint32_t top = min(base + 64, rb[4]);
+        // h += (bit ? w : 0) per entry, as a select of h + w: an empty
+        // statement keeps each sum opaque, since the folded form needs w
+        // copied into a VGPR first (one more VALU instruction per entry).
+        // Two entries a pass, into two sums: half the loop's scalar
+        // bookkeeping, and the second entry's readlanes fill the wait the
+        // first one's select has on its lane mask.
+        auto add = [&](int32_t& acc, uint32_t k) {
+            const uint64_t word = readlane64(nw, k - base);
+            const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
+            int32_t hw = acc + w;
+            asm volatile("" : "+v"(hw));
+            acc = lane_bit(word) ? hw : acc;
+        };
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t a = max(rb[q], base), b = min(rb[q + 1], top);
-            for (uint32_t k = a; k < b; ++k) {
-                const uint64_t word = readlane64(nw, k - base);
-                const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
-                int32_t hw = h[q] + w;
-                // An empty statement that keeps hw opaque: without it the select
-                // below is folded into h + (bit ? w : 0), which needs w copied
-                // into a VGPR first (one more VALU instruction per entry).
-                asm volatile("" : "+v"(hw));
-                h[q] = lane_bit(word) ? hw : h[q];
+            int32_t h2 = 0;
+            uint32_t k = a;
+            for (; k + 1 < b; k += 2) {
+                add(h[q], k);
+                add(h2, k + 1);
             }
+            if (k < b) add(h[q], k);
+            h[q] += h2;
         }
     }
 #pragma unroll
