@@ -219,14 +219,25 @@ template <int G>
 __device__ __forceinline__ void walk_entries(const uint32_t (&rb)[G + 1], uint32_t base,
                                              uint32_t top, uint64_t nw, int32_t nwt,
                                              int32_t (&h)[G]) {
+    // as generic_group's walk: selects of opaque sums, two entries a pass
+    auto add = [&](int32_t& acc, uint32_t k) {
+        const uint64_t word = readlane64(nw, k - base);
+        const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
+        int32_t hw = acc + w;
+        asm volatile("" : "+v"(hw));
+        acc = lane_bit(word) ? hw : acc;
+    };
 #pragma unroll
     for (int q = 0; q < G; ++q) {
         const uint32_t a = max(rb[q], base), b = min(rb[q + 1], top);
-        for (uint32_t k = a; k < b; ++k) {
-            const uint64_t word = readlane64(nw, k - base);
-            const int32_t w = __builtin_amdgcn_readlane(nwt, k - base);
-            h[q] += lane_bit(word) ? w : 0;
+        int32_t h2 = 0;
+        uint32_t k = a;
+        for (; k + 1 < b; k += 2) {
+            add(h[q], k);
+            add(h2, k + 1);
         }
+        if (k < b) add(h[q], k);
+        h[q] += h2;
     }
 }
 
